@@ -1,0 +1,251 @@
+/*
+ * krr_wavefront.h - C ABI of the MI355X (gfx950) wavefront path tracer.
+ *
+ * This is the drop-in boundary for KiRaRay's WavefrontPathTracer render pass.
+ * Every entry point is a plain C function over plain pointers and sizes; the
+ * library owns every device allocation it makes, the caller owns the film
+ * buffer and the HIP stream it passes in.  Calls on one handle are not
+ * thread-safe; separate handles are independent.
+ *
+ * All functions return 0 (KRR_OK) on success and a negative KRR_ERR_* code on
+ * failure; krr_last_error() then describes the failure (thread-local).  The
+ * reference logs and exits on every error (src/util/check.h:8-42,
+ * src/core/logger.cpp:73-100); this ABI reports instead.
+ */
+#ifndef KRR_WAVEFRONT_H
+#define KRR_WAVEFRONT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KRR_ABI_VERSION 1
+
+enum krr_status {
+	KRR_OK				  = 0,
+	KRR_ERR_INVALID_ARG	  = -1, /* null handle/pointer, negative size, malformed data */
+	KRR_ERR_BAD_CONFIG	  = -2, /* JSON that does not parse or has a wrongly typed key */
+	KRR_ERR_NO_SCENE	  = -3, /* render hook called before set_scene */
+	KRR_ERR_HIP			  = -4, /* a HIP runtime call failed */
+	KRR_ERR_OUT_OF_MEMORY = -5,
+	KRR_ERR_CAPACITY	  = -6, /* a size exceeds a documented maximum */
+	KRR_ERR_UNSUPPORTED	  = -7, /* valid input the library does not implement */
+	KRR_ERR_NO_DEVICE	  = -8, /* no HIP device is visible */
+};
+
+/* Material kinds: same numbering as the reference MaterialType
+ * (src/core/raytracing.h:24-31).  Disney is not implemented. */
+enum krr_material_type {
+	KRR_MAT_NULL	   = 0,
+	KRR_MAT_DIFFUSE	   = 1,
+	KRR_MAT_DIELECTRIC = 2,
+	KRR_MAT_CONDUCTOR  = 3,
+};
+
+/* Material::MaterialParams under the SpecularGlossiness shading model that the
+ * JSON importer always selects (src/scene/krrscene.cpp:204-228):
+ * roughness = 1 - specular[3]. */
+typedef struct krr_material {
+	int32_t type;			   /* krr_material_type */
+	float diffuse[3];		   /* "diffuse", default (1,1,1) */
+	float specular[4];		   /* "specular" rgb, a = 1 - "roughness" */
+	float ior;				   /* "eta", default 1.5 */
+	float specular_transmission; /* "specular_transmission", default 0 */
+	float anisotropic;		   /* "anisotropic", default 0 */
+} krr_material;
+
+/* One mesh instance with identity transform: a contiguous triangle range of
+ * the scene arrays (rt::MeshData + rt::InstanceData, src/core/mesh.h:25-56). */
+typedef struct krr_mesh {
+	int32_t first_triangle;
+	int32_t n_triangles;
+	int32_t material;	   /* index into materials; -1 = null material (medium interface) */
+	int32_t medium_inside;  /* index into media or -1 */
+	int32_t medium_outside; /* index into media or -1 */
+	int32_t has_normals;	   /* per-vertex normals in krr_scene_desc.normals are valid */
+	float emission[3];	   /* Le; any component > 0 makes every triangle a DiffuseAreaLight */
+} krr_mesh;
+
+enum krr_medium_type {
+	KRR_MEDIUM_HOMOGENEOUS = 0, /* HomogeneousMedium (src/render/media.h:108-142) */
+	KRR_MEDIUM_GRID		   = 1, /* dense fp32 density grid, stands for NanoVDBMedium<float> */
+};
+
+/* Participating medium.  For KRR_MEDIUM_GRID the density value of voxel
+ * (i,j,k) sits at index-space point (i,j,k); world p maps to index space as
+ * (p - bounds_min) / (bounds_max - bounds_min) * res; lookups are trilinear
+ * with zero outside the grid (the NanoVDB SampleFromVoxels<...,1,false>
+ * convention used by src/util/volume.h:62,82-86). */
+typedef struct krr_medium {
+	int32_t type;
+	float sigma_t[3];
+	float albedo[3];
+	float Le[3];		   /* homogeneous emission */
+	float g;			   /* Henyey-Greenstein asymmetry */
+	float scale;		   /* grid density multiplier */
+	int32_t res[3];		   /* grid resolution, x fastest */
+	const float *density;  /* host pointer, res[0]*res[1]*res[2] floats */
+	float bounds_min[3];
+	float bounds_max[3];
+} krr_medium;
+
+/* Constant-radiance infinite light (rt::InfiniteLight without an image,
+ * src/core/light.h:208-259). */
+typedef struct krr_infinite_light {
+	float tint[3];
+	float scale;
+} krr_infinite_light;
+
+/* Scene handed to the tracer: flattened world-space triangle soup. */
+typedef struct krr_scene_desc {
+	int64_t n_vertices;
+	const float *positions; /* 3 * n_vertices */
+	const float *normals;	/* 3 * n_vertices, may be NULL if no mesh has normals */
+	int64_t n_triangles;
+	const int32_t *indices; /* 3 * n_triangles, absolute vertex indices */
+	int32_t n_meshes;
+	const krr_mesh *meshes;
+	int32_t n_materials;
+	const krr_material *materials;
+	int32_t n_media;
+	const krr_medium *media;
+	int32_t n_infinite_lights;
+	const krr_infinite_light *infinite_lights;
+} krr_scene_desc;
+
+/* rt::CameraData (src/core/camera.h:20-30).  transform is the camera-to-world
+ * affine map, row-major 3x4 (rotation | translation). */
+typedef struct krr_camera {
+	float film_size[2];	 /* mm, default 42.666667 x 24 */
+	float focal_length;	 /* mm, default 21 */
+	float focal_distance; /* default 10 */
+	float lens_radius;	 /* default 0 (pinhole) */
+	float aspect_ratio;	 /* default 1.777777 */
+	float shutter_open;
+	float shutter_time;
+	float transform[12];
+	int32_t medium;		 /* medium the camera sits in, -1 = none */
+} krr_camera;
+
+/* Counters of the last render() call, read back from the device. */
+typedef struct krr_stats {
+	uint64_t closest_rays;	 /* rays traced for closest hits, summed over depth */
+	uint64_t shadow_rays;	 /* shadow rays traced */
+	uint64_t paths;			 /* camera samples started: W*H*spp */
+	uint64_t node_visits;	 /* BVH nodes visited (only when "count_traversal" is set) */
+	uint64_t tri_tests;		 /* ray-triangle tests (only when "count_traversal" is set) */
+	uint64_t closest_launches; /* number of closest-hit trace kernels launched */
+} krr_stats;
+
+/* Closest hit record: t along the (unnormalised) ray direction, primitive
+ * index into the scene triangle array (-1 = miss), barycentrics of vertex 1
+ * and 2 (vertex 0 weight is 1 - b1 - b2), as optixGetTriangleBarycentrics. */
+typedef struct krr_hit {
+	float t;
+	int32_t prim;
+	float b1, b2;
+} krr_hit;
+
+/* ---------------------------------------------------------------------- */
+/* Library                                                                 */
+/* ---------------------------------------------------------------------- */
+
+/* Last error message of the calling thread ("" if none). */
+const char *krr_last_error(void);
+/* KRR_ABI_VERSION the library was built with. */
+int krr_abi_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int krr_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Procedural scenes (SceneImporter::import counterpart,                   */
+/* src/scene/krrscene.cpp:253-305).  Host-only, no device needed.          */
+/* ---------------------------------------------------------------------- */
+
+typedef struct krr_scene krr_scene;
+
+/* json: {"model": [...], "camera": {...}, "media": [...], ...}; see
+ * kiraray_amd/csrc/scene.cpp for the procedural model types. */
+int krr_scene_create(const char *json, krr_scene **out);
+int krr_scene_get_desc(const krr_scene *scene, krr_scene_desc *out);
+int krr_scene_get_camera(const krr_scene *scene, int view, krr_camera *out);
+/* Number of cameras ("views") the scene defines (multi-view scenes > 1). */
+int krr_scene_view_count(const krr_scene *scene);
+/* Per-view resolution (width, height) stored in the scene, 0 if unset. */
+int krr_scene_view_size(const krr_scene *scene, int view, int32_t *width, int32_t *height);
+int krr_scene_destroy(krr_scene *scene);
+
+/* ---------------------------------------------------------------------- */
+/* Acceleration structure + launch (OptixBackend counterpart:              */
+/* src/core/device/optix.cpp:143-250 build, src/core/device/optix.h:174-187 */
+/* launch).                                                                 */
+/* ---------------------------------------------------------------------- */
+
+typedef struct krr_accel krr_accel;
+
+int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_accel **out);
+/* rays_dev: n records of 8 floats {ox,oy,oz,tmax,dx,dy,dz,0} in device memory.
+ * Closest hit in (0, tmax); ties in t go to the lowest primitive index. */
+int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev,
+							void *hip_stream);
+/* visible_dev[i] = 1 if no triangle of a non-null material is hit in (0, tmax). */
+int krr_accel_trace_any(krr_accel *accel, const float *rays_dev, int64_t n, int32_t *visible_dev,
+						void *hip_stream);
+/* Structure statistics: node count, max depth, leaf triangle count, bytes. */
+int krr_accel_info(const krr_accel *accel, int64_t *n_nodes, int32_t *max_depth,
+				   int64_t *n_leaf_tris, int64_t *bytes);
+int krr_accel_destroy(krr_accel *accel);
+
+/* ---------------------------------------------------------------------- */
+/* WavefrontPathTracer render pass (src/render/wavefront/integrator.h:24-104) */
+/* ---------------------------------------------------------------------- */
+
+typedef struct krr_wf krr_wf;
+
+/* json_params: the pass "params" object of the reference (integrator.h:96-103):
+ *   nee (true), enable_medium (true), max_depth (10), rr (0.8),
+ *   enable_clamp (false), clamp_max (1e3)
+ * plus keys this library adds:
+ *   spp (1)              samples per pixel per frame (UI-only in the reference)
+ *   rr_min_depth (0)     first path depth at which Russian roulette applies
+ *   legacy_spp_accumulate (false)  keep the reference's running-sum spp loop
+ *   count_traversal (false)  count node visits / triangle tests
+ *   block_size (256), trace_block_size (128)  launch shapes
+ * NULL or "" means all defaults. */
+int krr_wf_create(const char *json_params, krr_wf **out);
+/* Uploads the scene and builds the acceleration structure (setScene). */
+int krr_wf_set_scene(krr_wf *wf, const krr_scene_desc *scene);
+/* Reallocates the per-pixel queues for a width x height frame (resize). */
+int krr_wf_resize(krr_wf *wf, int32_t width, int32_t height);
+/* Copies the camera and resets per-pixel state; frame_index is the 1-based
+ * frame counter of the host loop (beginFrame). */
+int krr_wf_begin_frame(krr_wf *wf, const krr_camera *camera, uint64_t frame_index);
+/* Traces spp samples per pixel and writes RGBA32F into film_dev, W x H
+ * float4, row (H-1-y) for pixel row y (render). */
+int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream);
+int krr_wf_end_frame(krr_wf *wf);
+/* Counters of the last render; synchronises the stream of that render. */
+int krr_wf_stats(krr_wf *wf, krr_stats *out);
+/* Current params as JSON (to_json); buffer of size cap, returns length. */
+int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap);
+int krr_wf_destroy(krr_wf *wf);
+
+/* ---------------------------------------------------------------------- */
+/* Error metric (ErrorMeasurePass, src/render/passes/errormeasure/metrics.cu) */
+/* ---------------------------------------------------------------------- */
+
+enum krr_metric { KRR_METRIC_MSE = 0, KRR_METRIC_MAPE = 1, KRR_METRIC_SMAPE = 2, KRR_METRIC_RELMSE = 3 };
+
+/* img_dev, ref_dev: n_pixels float4 in device memory; result accumulated in
+ * FP64 and written to *out (synchronises hip_stream). */
+int krr_error_metric(int32_t metric, const float *img_dev, const float *ref_dev, int64_t n_pixels,
+					 double *out, void *hip_stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KRR_WAVEFRONT_H */

@@ -1,0 +1,56 @@
+// bvh.h - 4-wide BVH layout shared by the host builder and the traversal
+// kernels.  Replaces the OptiX GAS/IAS (src/core/device/optix.cpp:143-250,
+// 357-398) with a structure the gfx950 kernels traverse in software.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+namespace krr {
+
+// One 128-byte node = one gfx950 L2 line.  Child boxes are stored as
+// structure-of-arrays so that a lane reads them with eight 16-byte loads.
+// child[c] >= 0: inner node index.  child[c] < 0: leaf, ~child[c] =
+// first * 8 + (count - 1) over the leaf triangle array (count 1..8).
+// Unused slots carry an inverted box (lo = +inf, hi = -inf) that no ray hits.
+struct alignas(128) Bvh4Node {
+	float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+	int32_t child[4];
+	int32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node must be 128 bytes");
+
+// Leaf triangle: three 16-byte records {v0, prim}, {v1, flags}, {v2, 0};
+// flags bit 0 = null material (skipped by shadow rays).
+struct LeafTri {
+	float v0[3];
+	int32_t prim;
+	float v1[3];
+	int32_t flags;
+	float v2[3];
+	int32_t pad;
+};
+static_assert(sizeof(LeafTri) == 48, "LeafTri must be 48 bytes");
+
+struct Bvh4Host {
+	std::vector<Bvh4Node> nodes;
+	std::vector<LeafTri> tris;
+	int maxDepth = 0;	// depth of the deepest inner node (root = 1)
+	int64_t nLeaves = 0;
+	double buildSeconds = 0;
+};
+
+struct BvhBuildOptions {
+	int maxLeafSize = 4;  // triangles per leaf (<= 8)
+	int bins		= 16; // SAH bins per axis
+	int threads		= 0;  // 0 = hardware concurrency (capped at 16)
+};
+
+// pos: 3 floats per vertex; idx: 3 ints per triangle; nullFlag may be null.
+void buildBvh4(const float *pos, const int32_t *idx, int64_t nTris, const uint8_t *nullFlag, const BvhBuildOptions &opt,
+			   Bvh4Host &out);
+
+// Stack entries a traversal needs for this BVH.
+inline int bvhStackSize(const Bvh4Host &b) { return 3 * b.maxDepth + 4; }
+
+} // namespace krr

@@ -1,0 +1,143 @@
+// kernels.h - device-side state and wave-level helpers of the wavefront tracer.
+//
+// Queue layout (array-of-structures in 16-byte records, so that one lane reads
+// one item with 16-byte loads and a wave reads consecutive cache lines):
+//   ray queue item (96 B):  ray[2i]   = {o.xyz, tmax}
+//                           ray[2i+1] = {d.xyz, pixelId}
+//                           st[4i]    = {thp.xyz, pu.x}
+//                           st[4i+1]  = {pu.y, pu.z, pl.x, pl.y}
+//                           st[4i+2]  = {pl.z, ctx.p.xyz}
+//                           st[4i+3]  = {ctx.n.xyz, meta}
+//   meta = depth | bsdfType << 8 | (medium + 1) << 16
+//   hit record (16 B):      {t, prim, b1, b2}
+//   shadow item (48/80 B):  ray[2i], ray[2i+1] = {o, tmax}, {d, pixelId}
+//                           c[i] = {Ld / mean(pu + pl), medium + 1}   (no media)
+//                           ext[3i..3i+2] = {Ld, pu, pl}               (media)
+// The reference keeps one SoA array per field (src/render/wavefront/
+// workitem.soa:79-152) and pushes with one atomicAdd per item
+// (workqueue.h:54-70); here a wave reserves its slots with one atomic.
+#pragma once
+
+#include "shading.h"
+#include "traverse.h"
+
+namespace krr {
+
+struct RayQueueDev {
+	float4 *ray; // 2 per item
+	float4 *st;	 // 4 per item
+};
+
+struct ShadowQueueDev {
+	float4 *ray; // 2 per item
+	float4 *c;	 // 1 per item
+	float4 *ext; // 3 per item (media only)
+};
+
+struct PixelStateDev {
+	float4 *L;		// radiance of the current sample
+	float4 *pixel;	// sum over samples
+	uint64_t *rng;	// PCG32 state
+	int *channel;	// hero channel of the medium code (SampledWavelengths::mainIndex)
+};
+
+struct Counters {
+	int rayCount[2];
+	int shadowCount;
+	int pad0;
+	unsigned long long closestTotal;
+	unsigned long long shadowTotal;
+	unsigned long long nodeVisits;
+	unsigned long long triTests;
+	unsigned long long closestLaunches;
+	unsigned long long pad1[2];
+};
+
+struct CameraDev {
+	float filmH, focalLength, focalDistance, lensRadius, aspect;
+	float M[12];
+	int medium;
+};
+
+struct WfParams {
+	int W, H, N;
+	int spp;
+	int maxDepth;
+	float rr;
+	int rrMinDepth;
+	int nee;
+	int enableMedium;
+	int enableClamp;
+	float clampMax;
+	int legacySpp;
+	uint64_t inc;		  // PCG32 increment of this frame
+	uint32_t sampleIndex; // frameIndex * spp (as uint)
+};
+
+KRR_DEV int laneId() { return __lane_id(); }
+
+// Wave-aggregated queue push: one atomic per wave per queue, each active lane
+// that wants a slot gets base + (number of wanting lanes below it).
+KRR_DEV int wavePush(int *counter, bool want) {
+	const uint64_t mask = __ballot(want);
+	if (mask == 0) return -1;
+	const int leader = __ffsll((unsigned long long) mask) - 1;
+	int base		 = 0;
+	if (laneId() == leader) base = atomicAdd(counter, __popcll(mask));
+	base = __shfl(base, leader);
+	if (!want) return -1;
+	const uint64_t below = laneId() == 0 ? 0ull : (mask & (~0ull >> (64 - laneId())));
+	return base + __popcll(below);
+}
+
+// Sum of a per-lane counter over the wave, added to a 64-bit global counter
+// by one lane.  Must be called by every lane of the wave.
+KRR_DEV void waveAddU64(unsigned long long *dst, uint32_t v) {
+	unsigned long long s = v;
+	for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+	if (laneId() == 0 && s) atomicAdd(dst, s);
+}
+
+KRR_DEV uint32_t packMeta(int depth, int bsdfType, int medium) {
+	return (uint32_t) depth | ((uint32_t) bsdfType << 8) | ((uint32_t) (medium + 1) << 16);
+}
+KRR_DEV int metaDepth(uint32_t m) { return (int) (m & 0xff); }
+KRR_DEV int metaBsdf(uint32_t m) { return (int) ((m >> 8) & 0xff); }
+KRR_DEV int metaMedium(uint32_t m) { return (int) (m >> 16) - 1; }
+
+struct RayItem {
+	f3 o, d;
+	f3 thp, pu, pl, cp, cn;
+	int depth, bsdfType, medium;
+	int pixel;
+};
+
+KRR_DEV RayItem loadRayItem(const RayQueueDev &q, int i) {
+	RayItem r;
+	const float4 a = q.ray[2 * i], b = q.ray[2 * i + 1];
+	const float4 s0 = q.st[4 * i], s1 = q.st[4 * i + 1], s2 = q.st[4 * i + 2], s3 = q.st[4 * i + 3];
+	r.o		 = f3(a.x, a.y, a.z);
+	r.d		 = f3(b.x, b.y, b.z);
+	r.pixel	 = __float_as_int(b.w);
+	r.thp	 = f3(s0.x, s0.y, s0.z);
+	r.pu	 = f3(s0.w, s1.x, s1.y);
+	r.pl	 = f3(s1.z, s1.w, s2.x);
+	r.cp	 = f3(s2.y, s2.z, s2.w);
+	r.cn	 = f3(s3.x, s3.y, s3.z);
+	uint32_t m = (uint32_t) __float_as_int(s3.w);
+	r.depth	   = metaDepth(m);
+	r.bsdfType = metaBsdf(m);
+	r.medium   = metaMedium(m);
+	return r;
+}
+
+KRR_DEV void storeRayItem(const RayQueueDev &q, int i, const RayItem &r) {
+	q.ray[2 * i]	 = make_float4(r.o.x, r.o.y, r.o.z, __builtin_inff());
+	q.ray[2 * i + 1] = make_float4(r.d.x, r.d.y, r.d.z, __int_as_float(r.pixel));
+	q.st[4 * i]		 = make_float4(r.thp.x, r.thp.y, r.thp.z, r.pu.x);
+	q.st[4 * i + 1]	 = make_float4(r.pu.y, r.pu.z, r.pl.x, r.pl.y);
+	q.st[4 * i + 2]	 = make_float4(r.pl.z, r.cp.x, r.cp.y, r.cp.z);
+	q.st[4 * i + 3]	 = make_float4(r.cn.x, r.cn.y, r.cn.z, __int_as_float((int) packMeta(r.depth, r.bsdfType, r.medium)));
+}
+
+} // namespace krr

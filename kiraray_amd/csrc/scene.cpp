@@ -1,0 +1,606 @@
+// scene.cpp - procedural scenes behind krr_scene_create (host only).
+//
+// Stands in for SceneImporter::import (src/scene/krrscene.cpp:253-305): a
+// JSON document lists "model" entries, "materials", "media", "camera" and
+// "environment".  Models are generated procedurally instead of loaded from
+// OBJ/glTF files:
+//   {"type": "cornell_box"}
+//   {"type": "icosphere_field", "count", "level", "seed", "radius": [lo,hi],
+//    "fractions": [diffuse, conductor], "ground": bool, "lights": n}
+//   {"type": "quad", "corners": [[x,y,z] x4], "material": i, "emission": [r,g,b]}
+//   {"type": "mesh", "positions": [...], "indices": [...], "normals": [...],
+//    "material": i, "emission": [r,g,b]}
+// Media: {"type": "value_noise_grid", "res", "octaves", "seed",
+//         "sigma_t_max", "albedo", "g", "threshold", "bounds": [[lo],[hi]]}
+//        {"type": "homogeneous", "sigma_t", "albedo", "g", "Le"}
+// Camera: {"eye", "target", "up", "focalLength", "focalDistance",
+//          "lensRadius", "filmSize", "medium"} with the aspect ratio taken
+// from "resolution" as the renderer does (src/main/renderer.cpp:28).
+// Views: {"count", "seed", "radius", "center", "sizes": [...]} creates many
+// cameras on a sphere looking at "center", each with a square resolution
+// drawn from "sizes".
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "errors.h"
+#include "json.h"
+#include "krr_wavefront.h"
+
+struct krr_scene {
+	std::vector<float> positions, normals;
+	std::vector<int32_t> indices;
+	std::vector<krr_mesh> meshes;
+	std::vector<krr_material> materials;
+	std::vector<krr_medium> media;
+	std::vector<std::unique_ptr<std::vector<float>>> densities;
+	std::vector<krr_infinite_light> infinite;
+	std::vector<krr_camera> cameras;
+	std::vector<std::pair<int32_t, int32_t>> viewSizes;
+	bool anyNormals = false;
+};
+
+namespace krr {
+
+namespace {
+
+// Small PCG32 for scene randomness (seeded per purpose).
+struct Rng {
+	uint64_t state, inc;
+	Rng(uint64_t seed, uint64_t stream) {
+		state = 0;
+		inc	  = (stream << 1u) | 1u;
+		next();
+		state += seed;
+		next();
+	}
+	uint32_t next() {
+		uint64_t old = state;
+		state		 = old * 0x5851f42d4c957f2dULL + inc;
+		uint32_t xs	 = (uint32_t) (((old >> 18u) ^ old) >> 27u);
+		uint32_t rot = (uint32_t) (old >> 59u);
+		return (xs >> rot) | (xs << ((~rot + 1u) & 31));
+	}
+	float uniform() { return (float) (next() >> 8) * (1.0f / 16777216.0f); }
+	float range(float lo, float hi) { return lo + (hi - lo) * uniform(); }
+};
+
+struct V { float x, y, z; };
+inline V sub(V a, V b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V add(V a, V b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V mul(V a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V crs(V a, V b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline float dt(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V nrmz(V a) {
+	float l = std::sqrt(dt(a, a));
+	return l > 0 ? mul(a, 1.f / l) : a;
+}
+
+krr_material makeMaterial(int type, V diffuse, float roughness, float ior, float specTrans) {
+	krr_material m;
+	std::memset(&m, 0, sizeof(m));
+	m.type		  = type;
+	m.diffuse[0]  = diffuse.x;
+	m.diffuse[1]  = diffuse.y;
+	m.diffuse[2]  = diffuse.z;
+	m.specular[3] = 1.f - roughness; // SpecularGlossiness: a = 1 - roughness (krrscene.cpp:207)
+	m.ior		  = ior;
+	m.specular_transmission = specTrans;
+	return m;
+}
+
+struct Builder {
+	krr_scene &s;
+	int addMaterial(const krr_material &m) {
+		s.materials.push_back(m);
+		return (int) s.materials.size() - 1;
+	}
+	// Appends a mesh; normals may be empty.
+	int addMesh(const std::vector<V> &verts, const std::vector<V> &nrms, const std::vector<int> &tris, int material,
+				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1) {
+		krr_mesh m;
+		std::memset(&m, 0, sizeof(m));
+		int32_t base	 = (int32_t) (s.positions.size() / 3);
+		m.first_triangle = (int32_t) (s.indices.size() / 3);
+		m.n_triangles	 = (int32_t) (tris.size() / 3);
+		m.material		 = material;
+		m.medium_inside	 = mIn;
+		m.medium_outside = mOut;
+		m.has_normals	 = nrms.empty() ? 0 : 1;
+		m.emission[0]	 = emission.x;
+		m.emission[1]	 = emission.y;
+		m.emission[2]	 = emission.z;
+		for (size_t i = 0; i < verts.size(); i++) {
+			s.positions.push_back(verts[i].x);
+			s.positions.push_back(verts[i].y);
+			s.positions.push_back(verts[i].z);
+			V n = nrms.empty() ? V{0, 0, 0} : nrms[i];
+			s.normals.push_back(n.x);
+			s.normals.push_back(n.y);
+			s.normals.push_back(n.z);
+		}
+		if (!nrms.empty()) s.anyNormals = true;
+		for (int t : tris) s.indices.push_back(base + t);
+		s.meshes.push_back(m);
+		return (int) s.meshes.size() - 1;
+	}
+	// Quad p0 p1 p2 p3 (counter-clockwise seen from the side its normal faces).
+	int addQuad(V p0, V p1, V p2, V p3, int material, V emission = {0, 0, 0}, bool withNormals = false) {
+		std::vector<V> v = {p0, p1, p2, p3};
+		std::vector<V> n;
+		if (withNormals) {
+			V fn = nrmz(crs(sub(p1, p0), sub(p2, p0)));
+			n	 = {fn, fn, fn, fn};
+		}
+		return addMesh(v, n, {0, 1, 2, 0, 2, 3}, material, emission);
+	}
+	// Axis-aligned box [lo,hi] rotated about the vertical axis through its centre.
+	int addBox(V lo, V hi, float angleDeg, int material) {
+		V c		= mul(add(lo, hi), 0.5f);
+		float a = angleDeg * 3.14159265358979f / 180.f;
+		float ca = std::cos(a), sa = std::sin(a);
+		auto rot = [&](V p) {
+			V q = sub(p, c);
+			return V{c.x + ca * q.x + sa * q.z, p.y, c.z - sa * q.x + ca * q.z};
+		};
+		std::vector<V> v;
+		for (int i = 0; i < 8; i++) v.push_back(rot({(i & 1) ? hi.x : lo.x, (i & 2) ? hi.y : lo.y, (i & 4) ? hi.z : lo.z}));
+		// outward counter-clockwise faces
+		std::vector<int> t = {0, 4, 6, 0, 6, 2,  1, 3, 7, 1, 7, 5,  0, 1, 5, 0, 5, 4,
+							  2, 6, 7, 2, 7, 3,  0, 2, 3, 0, 3, 1,  4, 5, 7, 4, 7, 6};
+		return addMesh(v, {}, t, material);
+	}
+};
+
+// Cornell-style box: floor, ceiling, back, left and right walls (10 tris),
+// two rotated boxes (24 tris) and a ceiling light quad (2 tris) -> 36.
+void cornellBox(Builder &b, const Json &) {
+	int white = b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.73f, 0.73f, 0.73f}, 1.f, 1.5f, 0.f));
+	int red	  = b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.63f, 0.065f, 0.05f}, 1.f, 1.5f, 0.f));
+	int green = b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.14f, 0.45f, 0.09f}, 1.f, 1.5f, 0.f));
+	int lightMat = b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.78f, 0.78f, 0.78f}, 1.f, 1.5f, 0.f));
+	const float L = -1, R = 1, B = 0, T = 2, Bk = -1, F = 1;
+	b.addQuad({L, B, F}, {R, B, F}, {R, B, Bk}, {L, B, Bk}, white);		 // floor (+y)
+	b.addQuad({L, T, Bk}, {R, T, Bk}, {R, T, F}, {L, T, F}, white);		 // ceiling (-y)
+	b.addQuad({L, B, Bk}, {R, B, Bk}, {R, T, Bk}, {L, T, Bk}, white);	 // back (+z)
+	b.addQuad({L, B, F}, {L, B, Bk}, {L, T, Bk}, {L, T, F}, red);		 // left (+x)
+	b.addQuad({R, B, Bk}, {R, B, F}, {R, T, F}, {R, T, Bk}, green);		 // right (-x)
+	b.addBox({0.05f, 0.f, -0.05f}, {0.65f, 0.6f, 0.55f}, -18.f, white);	 // short box
+	b.addBox({-0.7f, 0.f, -0.75f}, {-0.1f, 1.2f, -0.15f}, 15.f, white);	 // tall box
+	const float h = 1.98f, e = 0.25f;
+	b.addQuad({-e, h, -e}, {e, h, -e}, {e, h, e}, {-e, h, e}, lightMat, {15.f, 15.f, 15.f}, true); // faces -y
+}
+
+// Icosphere with `level` subdivisions (20 * 4^level triangles).
+void icosphere(int level, std::vector<V> &verts, std::vector<int> &tris) {
+	const float t = (1.f + std::sqrt(5.f)) / 2.f;
+	verts		  = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
+					 {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};
+	for (auto &v : verts) v = nrmz(v);
+	tris = {0, 11, 5, 0, 5, 1, 0, 1, 7, 0, 7, 10, 0, 10, 11, 1, 5, 9, 5, 11, 4, 11, 10, 2, 10, 7, 6, 7, 1, 8,
+			3, 9, 4, 3, 4, 2, 3, 2, 6, 3, 6, 8, 3, 8, 9, 4, 9, 5, 2, 4, 11, 6, 2, 10, 8, 6, 7, 9, 8, 1};
+	for (int l = 0; l < level; l++) {
+		std::map<uint64_t, int> mid;
+		auto midpoint = [&](int a, int c) {
+			uint64_t key = a < c ? ((uint64_t) a << 32 | (uint32_t) c) : ((uint64_t) c << 32 | (uint32_t) a);
+			auto it		 = mid.find(key);
+			if (it != mid.end()) return it->second;
+			V m = nrmz(mul(add(verts[a], verts[c]), 0.5f));
+			verts.push_back(m);
+			int idx	 = (int) verts.size() - 1;
+			mid[key] = idx;
+			return idx;
+		};
+		std::vector<int> nt;
+		nt.reserve(tris.size() * 4);
+		for (size_t i = 0; i < tris.size(); i += 3) {
+			int a = tris[i], bb = tris[i + 1], c = tris[i + 2];
+			int ab = midpoint(a, bb), bc = midpoint(bb, c), ca = midpoint(c, a);
+			int add4[12] = {a, ab, ca, bb, bc, ab, c, ca, bc, ab, bc, ca};
+			nt.insert(nt.end(), add4, add4 + 12);
+		}
+		tris.swap(nt);
+	}
+}
+
+// 1024 icospheres in the unit cube over a ground quad with 16 quad lights.
+void icosphereField(Builder &b, const Json &j) {
+	int count		= (int) j.value("count", 1024.0);
+	int level		= (int) j.value("level", 3.0);
+	uint64_t seed	= (uint64_t) j.value("seed", 7.0);
+	float rad[2]	= {0.01f, 0.05f};
+	j.vec<2>("radius", rad);
+	float frac[2] = {0.5f, 0.3f};
+	j.vec<2>("fractions", frac);
+	bool ground = j.value("ground", true);
+	int nLights = (int) j.value("lights", 16.0);
+	float Le	= (float) j.value("light_radiance", 12.0);
+	if (count < 0 || level < 0 || level > 6) throw std::runtime_error("icosphere_field: bad count/level");
+	std::vector<V> unitV;
+	std::vector<int> unitT;
+	icosphere(level, unitV, unitT);
+	Rng rng(seed, 1);
+	for (int s = 0; s < count; s++) {
+		V c		= {rng.uniform(), rng.uniform(), rng.uniform()};
+		float r = rng.range(rad[0], rad[1]);
+		float u = rng.uniform();
+		krr_material m;
+		if (u < frac[0]) m = makeMaterial(KRR_MAT_DIFFUSE, {rng.range(0.1f, 0.9f), rng.range(0.1f, 0.9f), rng.range(0.1f, 0.9f)}, 1.f, 1.5f, 0.f);
+		else if (u < frac[0] + frac[1]) {
+			V refl	= {rng.range(0.5f, 0.95f), rng.range(0.5f, 0.95f), rng.range(0.5f, 0.95f)};
+			m		= makeMaterial(KRR_MAT_CONDUCTOR, refl, rng.range(0.05f, 0.5f), 1.5f, 0.f);
+		} else m = makeMaterial(KRR_MAT_DIELECTRIC, {1.f, 1.f, 1.f}, 0.f, 1.5f, 1.f);
+		int mat = b.addMaterial(m);
+		std::vector<V> verts(unitV.size()), nrms(unitV.size());
+		for (size_t i = 0; i < unitV.size(); i++) {
+			verts[i] = add(c, mul(unitV[i], r));
+			nrms[i]	 = unitV[i];
+		}
+		b.addMesh(verts, nrms, unitT, mat);
+	}
+	if (ground) {
+		int g = b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.5f, 0.5f, 0.5f}, 1.f, 1.5f, 0.f));
+		const float lo = -8.f, hi = 9.f;
+		b.addQuad({lo, 0, hi}, {hi, 0, hi}, {hi, 0, lo}, {lo, 0, lo}, g);
+	}
+	if (nLights > 0) {
+		int lm	= b.addMaterial(makeMaterial(KRR_MAT_DIFFUSE, {0.8f, 0.8f, 0.8f}, 1.f, 1.5f, 0.f));
+		int n	= (int) std::ceil(std::sqrt((double) nLights));
+		float h = 1.8f, e = 0.075f;
+		for (int i = 0; i < nLights; i++) {
+			float cx = (i % n + 0.5f) / n, cz = (i / n + 0.5f) / n;
+			b.addQuad({cx - e, h, cz - e}, {cx + e, h, cz - e}, {cx + e, h, cz + e}, {cx - e, h, cz + e}, lm, {Le, Le, Le}, true);
+		}
+	}
+}
+
+void explicitMesh(Builder &b, const Json &j) {
+	const auto &P = j.at("positions").items();
+	const auto &I = j.at("indices").items();
+	if (P.size() % 3 || I.size() % 3) throw std::runtime_error("mesh: positions/indices must be multiples of 3");
+	std::vector<V> verts, nrms;
+	for (size_t i = 0; i < P.size(); i += 3) verts.push_back({(float) P[i].asNumber(), (float) P[i + 1].asNumber(), (float) P[i + 2].asNumber()});
+	if (j.contains("normals")) {
+		const auto &N = j.at("normals").items();
+		if (N.size() != P.size()) throw std::runtime_error("mesh: normals must match positions");
+		for (size_t i = 0; i < N.size(); i += 3) nrms.push_back({(float) N[i].asNumber(), (float) N[i + 1].asNumber(), (float) N[i + 2].asNumber()});
+	}
+	std::vector<int> tris;
+	for (auto &x : I) {
+		int v = (int) x.asNumber();
+		if (v < 0 || (size_t) v >= verts.size()) throw std::runtime_error("mesh: index out of range");
+		tris.push_back(v);
+	}
+	float em[3] = {0, 0, 0};
+	j.vec<3>("emission", em);
+	int mat	 = (int) j.value("material", 0.0);
+	int mIn	 = (int) j.value("medium_inside", -1.0);
+	int mOut = (int) j.value("medium_outside", -1.0);
+	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut);
+}
+
+void quadModel(Builder &b, const Json &j) {
+	const auto &C = j.at("corners").items();
+	if (C.size() != 4) throw std::runtime_error("quad: corners must hold 4 points");
+	V p[4];
+	for (int i = 0; i < 4; i++) {
+		const auto &c = C[i].items();
+		if (c.size() != 3) throw std::runtime_error("quad: corner must be [x,y,z]");
+		p[i] = {(float) c[0].asNumber(), (float) c[1].asNumber(), (float) c[2].asNumber()};
+	}
+	float em[3] = {0, 0, 0};
+	j.vec<3>("emission", em);
+	bool emissive = em[0] > 0 || em[1] > 0 || em[2] > 0;
+	b.addQuad(p[0], p[1], p[2], p[3], (int) j.value("material", 0.0), {em[0], em[1], em[2]}, emissive || j.value("normals", false));
+}
+
+// Hash of an integer lattice point to [0,1).
+inline float latticeValue(int x, int y, int z, uint32_t salt) {
+	uint32_t h = (uint32_t) x * 0x8da6b343u ^ (uint32_t) y * 0xd8163841u ^ (uint32_t) z * 0xcb1ab31fu ^ salt * 0x9e3779b9u;
+	h ^= h >> 16;
+	h *= 0x7feb352du;
+	h ^= h >> 15;
+	h *= 0x846ca68bu;
+	h ^= h >> 16;
+	return (float) (h >> 8) * (1.0f / 16777216.0f);
+}
+
+inline float valueNoise(float x, float y, float z, uint32_t salt) {
+	int ix = (int) std::floor(x), iy = (int) std::floor(y), iz = (int) std::floor(z);
+	float fx = x - ix, fy = y - iy, fz = z - iz;
+	auto sm = [](float t) { return t * t * (3.f - 2.f * t); };
+	fx = sm(fx);
+	fy = sm(fy);
+	fz = sm(fz);
+	float v[2][2][2];
+	for (int a = 0; a < 2; a++)
+		for (int bb = 0; bb < 2; bb++)
+			for (int c = 0; c < 2; c++) v[a][bb][c] = latticeValue(ix + a, iy + bb, iz + c, salt);
+	auto lerp = [](float a, float c, float t) { return a + t * (c - a); };
+	float x00 = lerp(v[0][0][0], v[1][0][0], fx), x10 = lerp(v[0][1][0], v[1][1][0], fx);
+	float x01 = lerp(v[0][0][1], v[1][0][1], fx), x11 = lerp(v[0][1][1], v[1][1][1], fx);
+	return lerp(lerp(x00, x10, fy), lerp(x01, x11, fy), fz);
+}
+
+void valueNoiseMedium(krr_scene &s, const Json &j) {
+	int res		  = (int) j.value("res", 256.0);
+	int octaves	  = (int) j.value("octaves", 5.0);
+	uint32_t seed = (uint32_t) j.value("seed", 11.0);
+	float smax	  = (float) j.value("sigma_t_max", 20.0);
+	float albedo  = (float) j.value("albedo", 0.8);
+	float g		  = (float) j.value("g", 0.3);
+	float thresh  = (float) j.value("threshold", 0.45);
+	float bounds[2][3] = {{0, 0, 0}, {1, 1, 1}};
+	if (j.contains("bounds")) {
+		const auto &B = j.at("bounds").items();
+		if (B.size() != 2) throw std::runtime_error("bounds must be [[lo],[hi]]");
+		for (int k = 0; k < 2; k++)
+			for (int a = 0; a < 3; a++) bounds[k][a] = (float) B[k][a].asNumber();
+	}
+	if (res < 2 || res > 1024 || octaves < 1 || octaves > 12) throw std::runtime_error("value_noise_grid: bad res/octaves");
+	auto grid = std::make_unique<std::vector<float>>((size_t) res * res * res);
+	float *d  = grid->data();
+	unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+	std::vector<std::thread> th;
+	for (unsigned t = 0; t < nt; t++)
+		th.emplace_back([=]() {
+			for (int z = (int) t; z < res; z += (int) nt)
+				for (int y = 0; y < res; y++)
+					for (int x = 0; x < res; x++) {
+						float sum = 0, amp = 1, norm = 0, freq = 4;
+						for (int o = 0; o < octaves; o++) {
+							sum += amp * valueNoise((x + 0.5f) / res * freq, (y + 0.5f) / res * freq, (z + 0.5f) / res * freq, seed + 101u * (uint32_t) o);
+							norm += amp;
+							amp *= 0.5f;
+							freq *= 2.f;
+						}
+						d[(size_t) x + (size_t) res * ((size_t) y + (size_t) res * z)] = std::max(0.f, sum / norm - thresh);
+					}
+		});
+	for (auto &t : th) t.join();
+	float mx = 0;
+	for (size_t i = 0; i < grid->size(); i++) mx = std::max(mx, d[i]);
+	if (mx > 0)
+		for (size_t i = 0; i < grid->size(); i++) d[i] /= mx;
+	krr_medium m;
+	std::memset(&m, 0, sizeof(m));
+	m.type = KRR_MEDIUM_GRID;
+	for (int a = 0; a < 3; a++) {
+		m.sigma_t[a]	= smax;
+		m.albedo[a]		= albedo;
+		m.res[a]		= res;
+		m.bounds_min[a] = bounds[0][a];
+		m.bounds_max[a] = bounds[1][a];
+	}
+	m.g		  = g;
+	m.scale	  = 1.f;
+	m.density = grid->data();
+	s.densities.push_back(std::move(grid));
+	s.media.push_back(m);
+}
+
+void homogeneousMedium(krr_scene &s, const Json &j) {
+	krr_medium m;
+	std::memset(&m, 0, sizeof(m));
+	m.type = KRR_MEDIUM_HOMOGENEOUS;
+	float st[3] = {1, 1, 1}, al[3] = {0.8f, 0.8f, 0.8f}, le[3] = {0, 0, 0};
+	j.vec<3>("sigma_t", st);
+	j.vec<3>("albedo", al);
+	j.vec<3>("Le", le);
+	for (int a = 0; a < 3; a++) {
+		m.sigma_t[a] = st[a];
+		m.albedo[a]	 = al[a];
+		m.Le[a]		 = le[a];
+	}
+	m.g		= (float) j.value("g", 0.0);
+	m.scale = 1.f;
+	s.media.push_back(m);
+}
+
+krr_camera lookAtCamera(V eye, V target, V up, int w, int h, const Json &j) {
+	krr_camera c;
+	std::memset(&c, 0, sizeof(c));
+	c.film_size[0]	 = 42.666667f;
+	c.film_size[1]	 = 24.0f;
+	c.focal_length	 = 21.f;
+	c.focal_distance = 10.f;
+	c.lens_radius	 = 0.f;
+	c.aspect_ratio	 = (w > 0 && h > 0) ? (float) w / (float) h : 1.777777f;
+	c.medium		 = -1;
+	if (j.isObject()) {
+		c.focal_length	 = (float) j.value("focalLength", (double) c.focal_length);
+		c.focal_distance = (float) j.value("focalDistance", (double) c.focal_distance);
+		c.lens_radius	 = (float) j.value("lensRadius", (double) c.lens_radius);
+		c.aspect_ratio	 = (float) j.value("aspectRatio", (double) c.aspect_ratio);
+		c.medium		 = (int) j.value("medium", -1.0);
+		j.vec<2>("filmSize", c.film_size);
+	}
+	V f	 = nrmz(sub(target, eye));
+	V r	 = nrmz(crs(f, up));
+	V u	 = crs(r, f);
+	V bk = mul(f, -1.f);
+	// columns: camera x -> r, y -> u, z -> -f; translation eye
+	float M[12] = {r.x, u.x, bk.x, eye.x, r.y, u.y, bk.y, eye.y, r.z, u.z, bk.z, eye.z};
+	std::memcpy(c.transform, M, sizeof(M));
+	return c;
+}
+
+void buildScene(krr_scene &s, const Json &j) {
+	Builder b{s};
+	if (j.contains("materials")) {
+		for (const Json &m : j.at("materials").items()) {
+			const Json &p = m.contains("params") ? m.at("params") : m;
+			std::string bsdf = m.value("bsdf", "diffuse");
+			int type		 = bsdf == "diffuse" ? KRR_MAT_DIFFUSE : bsdf == "dielectric" ? KRR_MAT_DIELECTRIC
+												: bsdf == "conductor" ? KRR_MAT_CONDUCTOR : bsdf == "null" ? KRR_MAT_NULL : -1;
+			if (type < 0) throw std::runtime_error("unsupported bsdf \"" + bsdf + "\"");
+			krr_material mm;
+			std::memset(&mm, 0, sizeof(mm));
+			mm.type		= type;
+			float dif[3] = {1, 1, 1}, spec[3] = {0, 0, 0};
+			p.vec<3>("diffuse", dif);
+			p.vec<3>("specular", spec);
+			for (int a = 0; a < 3; a++) {
+				mm.diffuse[a]  = dif[a];
+				mm.specular[a] = spec[a];
+			}
+			mm.specular[3]			 = 1.f - (float) p.value("roughness", 1.0);
+			mm.specular_transmission = (float) p.value("specular_transmission", 0.0);
+			mm.anisotropic			 = (float) p.value("anisotropic", 0.0);
+			mm.ior					 = (float) p.value("eta", 1.5);
+			s.materials.push_back(mm);
+		}
+	}
+	if (j.contains("media")) {
+		for (const Json &m : j.at("media").items()) {
+			std::string t = m.value("type", "homogeneous");
+			const Json &p = m.contains("params") ? m.at("params") : m;
+			if (t == "value_noise_grid" || t == "heterogeneous") valueNoiseMedium(s, p);
+			else if (t == "homogeneous") homogeneousMedium(s, p);
+			else throw std::runtime_error("unsupported medium type \"" + t + "\"");
+		}
+	}
+	if (j.contains("model")) {
+		const Json &models = j.at("model");
+		std::vector<Json> list;
+		if (models.isArray()) list = models.items();
+		else list.push_back(models);
+		for (const Json &m : list) {
+			std::string t = m.value("type", "mesh");
+			if (t == "cornell_box") cornellBox(b, m);
+			else if (t == "icosphere_field") icosphereField(b, m);
+			else if (t == "mesh") explicitMesh(b, m);
+			else if (t == "quad") quadModel(b, m);
+			else throw std::runtime_error("unsupported model type \"" + t + "\"");
+		}
+	}
+	for (const krr_mesh &m : s.meshes) {
+		if (m.material >= (int) s.materials.size()) throw std::runtime_error("mesh material index out of range");
+		if (m.medium_inside >= (int) s.media.size() || m.medium_outside >= (int) s.media.size())
+			throw std::runtime_error("mesh medium index out of range");
+		bool emissive = m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0;
+		if (emissive && !m.has_normals) throw std::runtime_error("emissive meshes need vertex normals (Triangle::sample reads them)");
+	}
+	if (j.contains("environment")) {
+		const Json &e = j.at("environment");
+		krr_infinite_light il;
+		float tint[3] = {1, 1, 1};
+		e.vec<3>("tint", tint);
+		for (int a = 0; a < 3; a++) il.tint[a] = tint[a];
+		il.scale = (float) e.value("scale", 1.0);
+		s.infinite.push_back(il);
+	}
+	int w = 0, h = 0;
+	if (j.contains("resolution")) {
+		const auto &r = j.at("resolution").items();
+		if (r.size() != 2) throw std::runtime_error("resolution must be [w, h]");
+		w = (int) r[0].asNumber();
+		h = (int) r[1].asNumber();
+	}
+	Json cj		 = j.contains("camera") ? j.at("camera") : Json::object();
+	float eye[3] = {0, 1, 3.4f}, target[3] = {0, 1, 0}, up[3] = {0, 1, 0};
+	cj.vec<3>("eye", eye);
+	cj.vec<3>("target", target);
+	cj.vec<3>("up", up);
+	if (j.contains("views")) {
+		const Json &vj = j.at("views");
+		int count	   = (int) vj.value("count", 256.0);
+		Rng rng((uint64_t) vj.value("seed", 5.0), 3);
+		float radius	= (float) vj.value("radius", 2.0);
+		float center[3] = {0.5f, 0.5f, 0.5f};
+		vj.vec<3>("center", center);
+		std::vector<int> sizes = {128, 256, 512, 1024};
+		if (vj.contains("sizes")) {
+			sizes.clear();
+			for (auto &x : vj.at("sizes").items()) sizes.push_back((int) x.asNumber());
+		}
+		for (int v = 0; v < count; v++) {
+			// uniform direction on the sphere, upper hemisphere biased away from the ground
+			float z	  = 1.f - 2.f * rng.uniform();
+			float phi = 6.2831853f * rng.uniform();
+			float rr  = std::sqrt(std::max(0.f, 1.f - z * z));
+			V dir	  = {rr * std::cos(phi), std::fabs(z) * 0.8f + 0.1f, rr * std::sin(phi)};
+			dir		  = nrmz(dir);
+			int side  = sizes[rng.next() % sizes.size()];
+			V c		  = {center[0], center[1], center[2]};
+			s.cameras.push_back(lookAtCamera(add(c, mul(dir, radius)), c, {0, 1, 0}, side, side, cj));
+			s.viewSizes.push_back({side, side});
+		}
+	} else {
+		s.cameras.push_back(lookAtCamera({eye[0], eye[1], eye[2]}, {target[0], target[1], target[2]}, {up[0], up[1], up[2]}, w, h, cj));
+		s.viewSizes.push_back({w, h});
+	}
+	for (const krr_camera &c : s.cameras)
+		if (c.medium >= (int) s.media.size()) throw std::runtime_error("camera medium index out of range");
+}
+
+} // namespace
+} // namespace krr
+
+extern "C" {
+
+int krr_scene_create(const char *json, krr_scene **out) {
+	krr::clearError();
+	if (!json || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_create: null argument");
+	*out = nullptr;
+	try {
+		krr::Json j = krr::Json::parse(json);
+		if (!j.isObject()) return krr::fail(KRR_ERR_BAD_CONFIG, "scene config must be a JSON object");
+		auto s		= std::make_unique<krr_scene>();
+		krr::buildScene(*s, j);
+		*out = s.release();
+		return KRR_OK;
+	} catch (const std::bad_alloc &) {
+		return krr::fail(KRR_ERR_OUT_OF_MEMORY, "krr_scene_create: out of host memory");
+	} catch (const std::exception &e) {
+		return krr::fail(KRR_ERR_BAD_CONFIG, "krr_scene_create: %s", e.what());
+	}
+}
+
+int krr_scene_get_desc(const krr_scene *s, krr_scene_desc *d) {
+	if (!s || !d) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_get_desc: null argument");
+	std::memset(d, 0, sizeof(*d));
+	d->n_vertices		  = (int64_t) (s->positions.size() / 3);
+	d->positions		  = s->positions.data();
+	d->normals			  = s->anyNormals ? s->normals.data() : nullptr;
+	d->n_triangles		  = (int64_t) (s->indices.size() / 3);
+	d->indices			  = s->indices.data();
+	d->n_meshes			  = (int32_t) s->meshes.size();
+	d->meshes			  = s->meshes.data();
+	d->n_materials		  = (int32_t) s->materials.size();
+	d->materials		  = s->materials.data();
+	d->n_media			  = (int32_t) s->media.size();
+	d->media			  = s->media.data();
+	d->n_infinite_lights  = (int32_t) s->infinite.size();
+	d->infinite_lights	  = s->infinite.data();
+	return KRR_OK;
+}
+
+int krr_scene_get_camera(const krr_scene *s, int view, krr_camera *out) {
+	if (!s || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_get_camera: null argument");
+	if (view < 0 || view >= (int) s->cameras.size()) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_get_camera: view %d out of range", view);
+	*out = s->cameras[view];
+	return KRR_OK;
+}
+
+int krr_scene_view_count(const krr_scene *s) { return s ? (int) s->cameras.size() : 0; }
+
+int krr_scene_view_size(const krr_scene *s, int view, int32_t *w, int32_t *h) {
+	if (!s || !w || !h) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_view_size: null argument");
+	if (view < 0 || view >= (int) s->viewSizes.size()) return krr::fail(KRR_ERR_INVALID_ARG, "krr_scene_view_size: view out of range");
+	*w = s->viewSizes[view].first;
+	*h = s->viewSizes[view].second;
+	return KRR_OK;
+}
+
+int krr_scene_destroy(krr_scene *s) {
+	delete s;
+	return KRR_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,191 @@
+// traverse.h - software BVH4 traversal and watertight ray/triangle test for
+// gfx950 (replaces optixTrace, src/render/wavefront/device.cu:12-29).
+//
+// Each lane traverses its own ray through the 4-wide BVH (bvh.h): one node
+// fetch = eight 16-byte loads of one 128-byte line, four slab tests, leaves
+// intersected on the spot, inner children sorted front to back with the far
+// ones pushed on a per-lane stack in LDS (stride = block size, so the 64
+// lanes of a wave hit 64 different banks).
+//
+// Hit rule (shared with the CPU checker): the Woop-Benthin-Wald watertight
+// test (JCGT 2(1), 2013), hit iff t > 0 and t < tbest, or t == tbest with a
+// lower primitive index; the closest hit is then independent of the tree.
+// The slab test picks near/far planes by direction sign and uses NaN-ignoring
+// min/max, so a zero direction component (0*inf) only ever drops that
+// axis; the far distance is widened by 1 + 2*gamma(3) (Ize, "Robust BVH Ray
+// Traversal", JCGT 2(2), 2013) and boxes are padded by 2^-20 relative in
+// the builder.
+#pragma once
+
+#include "bvh.h"
+#include "vecmath.h"
+
+namespace krr {
+
+struct TraceBvh {
+	const Bvh4Node *nodes;
+	const LeafTri *tris;
+};
+
+struct RayPre {
+	f3 o;
+	int kx, ky, kz;
+	float Sx, Sy, Sz;
+};
+
+KRR_DEV float sel3(const f3 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+KRR_DEV RayPre rayPre(const f3 &o, const f3 &d) {
+	RayPre r;
+	r.o	   = o;
+	int kz = 0;
+	if (fabsf(d.y) > fabsf(sel3(d, kz))) kz = 1;
+	if (fabsf(d.z) > fabsf(sel3(d, kz))) kz = 2;
+	int kx = kz + 1 == 3 ? 0 : kz + 1;
+	int ky = kx + 1 == 3 ? 0 : kx + 1;
+	if (sel3(d, kz) < 0.0f) {
+		int t = kx;
+		kx	  = ky;
+		ky	  = t;
+	}
+	r.kx	  = kx;
+	r.ky	  = ky;
+	r.kz	  = kz;
+	float dz  = sel3(d, kz);
+	r.Sx	  = sel3(d, kx) / dz;
+	r.Sy	  = sel3(d, ky) / dz;
+	r.Sz	  = 1.0f / dz;
+	return r;
+}
+
+// Watertight test; true with t > 0 and the barycentrics of v1, v2.
+KRR_DEV bool intersectTri(const RayPre &r, const f3 &v0, const f3 &v1, const f3 &v2, float &t, float &b1, float &b2) {
+	const f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+	const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
+	const float Ax = sel3(A, r.kx) - r.Sx * Akz, Ay = sel3(A, r.ky) - r.Sy * Akz;
+	const float Bx = sel3(B, r.kx) - r.Sx * Bkz, By = sel3(B, r.ky) - r.Sy * Bkz;
+	const float Cx = sel3(C, r.kx) - r.Sx * Ckz, Cy = sel3(C, r.ky) - r.Sy * Ckz;
+	float U = Cx * By - Cy * Bx;
+	float V = Ax * Cy - Ay * Cx;
+	float W = Bx * Ay - By * Ax;
+	if (U == 0.0f || V == 0.0f || W == 0.0f) {
+		U = (float) ((double) Cx * (double) By - (double) Cy * (double) Bx);
+		V = (float) ((double) Ax * (double) Cy - (double) Ay * (double) Cx);
+		W = (float) ((double) Bx * (double) Ay - (double) By * (double) Ax);
+	}
+	if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+	const float det = U + V + W;
+	if (det == 0.0f) return false;
+	const float T	= U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+	const float inv = 1.0f / det;
+	t				= T * inv;
+	if (!(t > 0.0f)) return false;
+	b1 = V * inv;
+	b2 = W * inv;
+	return true;
+}
+
+struct TraceResult {
+	float t;
+	int prim;
+	float b1, b2;
+};
+
+// Traverses one ray.  ANY: stop at the first hit of a non-null-material
+// triangle (shadow rays, device.cu:83-100).  stack: this lane's LDS slot 0,
+// entries `stride` ints apart.
+template <bool ANY, bool COUNT>
+KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, int *stack, int stride,
+							 uint32_t &nodeVisits, uint32_t &triTests) {
+	TraceResult res{tmax, -1, 0.f, 0.f};
+	const RayPre rp = rayPre(o, d);
+	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+	const bool sx = __builtin_signbit(d.x), sy = __builtin_signbit(d.y), sz = __builtin_signbit(d.z);
+	const float kWiden = 1.00000036f;					// 1 + 2*gamma(3)
+	const float kTie   = 1.00000095367431640625f;		// 1 + 2^-20
+	int sp			   = 0;
+	int node		   = 0;
+	while (true) {
+		if (COUNT) nodeVisits++;
+		const float4 *np = reinterpret_cast<const float4 *>(bvh.nodes + node);
+		const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
+		const int4 ch	 = reinterpret_cast<const int4 *>(np)[6];
+		const float4 nx = sx ? hix : lox, fx = sx ? lox : hix;
+		const float4 ny = sy ? hiy : loy, fy = sy ? loy : hiy;
+		const float4 nz = sz ? hiz : loz, fz = sz ? loz : hiz;
+		const float tlim = res.t * kTie;
+		float tn[4];
+		bool hit[4];
+		const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
+#define KRR_SLAB(c, comp)                                                                                            \
+	{                                                                                                                \
+		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
+		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
+		t1		 = fminf(t1, tlim);                                                                                  \
+		tn[c]	 = t0;                                                                                               \
+		hit[c]	 = t0 <= t1;                                                                                         \
+	}
+		KRR_SLAB(0, x)
+		KRR_SLAB(1, y)
+		KRR_SLAB(2, z)
+		KRR_SLAB(3, w)
+#undef KRR_SLAB
+		// Leaves first, on the spot.
+#pragma unroll
+		for (int c = 0; c < 4; c++) {
+			if (!hit[c] || cc[c] >= 0) continue;
+			const int code	= ~cc[c];
+			const int first = code >> 3, cnt = (code & 7) + 1;
+			for (int k = 0; k < cnt; k++) {
+				const float4 *tp = reinterpret_cast<const float4 *>(bvh.tris + first + k);
+				const float4 a = tp[0], b = tp[1], e = tp[2];
+				if (ANY && (__float_as_int(b.w) & 1)) continue;
+				if (COUNT) triTests++;
+				float t, b1, b2;
+				if (!intersectTri(rp, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(e.x, e.y, e.z), t, b1, b2)) continue;
+				const int prim = __float_as_int(a.w);
+				if (t < res.t || (t == res.t && res.prim >= 0 && prim < res.prim)) {
+					res.t	 = t;
+					res.prim = prim;
+					res.b1	 = b1;
+					res.b2	 = b2;
+					if (ANY) return res;
+				}
+			}
+		}
+		// Inner children, nearest first.
+		float k0 = (hit[0] && cc[0] >= 0) ? tn[0] : __builtin_inff();
+		float k1 = (hit[1] && cc[1] >= 0) ? tn[1] : __builtin_inff();
+		float k2 = (hit[2] && cc[2] >= 0) ? tn[2] : __builtin_inff();
+		float k3 = (hit[3] && cc[3] >= 0) ? tn[3] : __builtin_inff();
+		int c0 = cc[0], c1 = cc[1], c2 = cc[2], c3 = cc[3];
+		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
+#define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
+	if (kb < ka) {                                                                                                   \
+		float tk = ka;                                                                                               \
+		ka		 = kb;                                                                                               \
+		kb		 = tk;                                                                                               \
+		int tc	 = ca;                                                                                               \
+		ca		 = cb;                                                                                               \
+		cb		 = tc;                                                                                               \
+	}
+		KRR_CSWAP(k0, c0, k1, c1)
+		KRR_CSWAP(k2, c2, k3, c3)
+		KRR_CSWAP(k0, c0, k2, c2)
+		KRR_CSWAP(k1, c1, k3, c3)
+		KRR_CSWAP(k1, c1, k2, c2)
+#undef KRR_CSWAP
+		if (n == 0) {
+			if (sp == 0) break;
+			node = stack[--sp * stride];
+			continue;
+		}
+		if (n >= 4) stack[sp++ * stride] = c3;
+		if (n >= 3) stack[sp++ * stride] = c2;
+		if (n >= 2) stack[sp++ * stride] = c1;
+		node = c0;
+	}
+	return res;
+}
+
+} // namespace krr

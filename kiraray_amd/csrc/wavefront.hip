@@ -1,0 +1,1289 @@
+// wavefront.hip - gfx950 wavefront path tracer: kernels and the C ABI.
+//
+// Per sample and loop depth the reference launches camera generation,
+// traceClosest (OptiX raygen + closest-hit/miss programs), handleHit,
+// handleMiss, generateScatterRays and traceShadow as separate kernels that
+// hand work over through SoA queues (src/render/wavefront/integrator.cpp:
+// 328-372).  Here the closest-hit routing, medium sampling, handleHit,
+// handleMiss, medium scattering and generateScatterRays of one ray run in a
+// single "shade" kernel: a path owns at most one item per queue per stage, so
+// running those stages back to back in one thread keeps the reference's
+// per-pixel order of radiance additions and RNG draws while the hit-light,
+// miss, scatter and medium queues (464 B per path per bounce in the
+// reference layout) never touch HBM.  What stays in memory is what crosses a
+// traversal: the ray queues (double-buffered), the hit records and the shadow
+// queue.  Queue slots are reserved with one atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "errors.h"
+#include "json.h"
+#include "kernels.h"
+#include "krr_wavefront.h"
+
+namespace krr {
+
+std::string &lastErrorStorage() {
+	static thread_local std::string s;
+	return s;
+}
+
+// ===========================================================================
+// Kernels
+// ===========================================================================
+
+__global__ void k_reset_totals(Counters *cnt) {
+	cnt->closestTotal	 = 0;
+	cnt->shadowTotal	 = 0;
+	cnt->nodeVisits		 = 0;
+	cnt->triTests		 = 0;
+	cnt->closestLaunches = 0;
+	cnt->rayCount[0] = cnt->rayCount[1] = cnt->shadowCount = 0;
+}
+
+// beginFrame (integrator.cpp:310-326): reset L and pixel, seed the sampler,
+// draw the wavelength sample whose main index picks the hero channel.
+__global__ void k_begin_frame(WfParams P, PixelStateDev px) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.N) return;
+	const int x = i % P.W, y = i / P.W;
+	px.L[i]		= make_float4(0.f, 0.f, 0.f, 0.f);
+	px.pixel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	Pcg s;
+	s.setSeed(interleave32((uint32_t) x, (uint32_t) y), (uint64_t) P.sampleIndex);
+	s.advance((int64_t) (256 * i));
+	const float u		= s.get1D();
+	const float lambda0 = (1.f - u) * 360.f + u * 830.f;
+	int ch				= (int) ((lambda0 - 360.f) / 470.f * 3);
+	px.channel[i]		= ch < 0 ? 0 : (ch > 2 ? 2 : ch);
+	px.rng[i]			= s.state;
+}
+
+// generateCameraRays (integrator.cpp:271-284) + CameraData::getRay
+// (camera.h:36-58).  Camera rays go to slot = pixelId, so the first trace
+// sees image-coherent rays.
+__global__ void k_camera(WfParams P, CameraDev cam, PixelStateDev px, RayQueueDev q, Counters *cnt) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) cnt->rayCount[0] = P.N;
+	if (i >= P.N) return;
+	if (!P.legacySpp) px.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	Pcg s{px.rng[i], P.inc};
+	float cs[5];
+	for (int k = 0; k < 5; k++) cs[k] = s.get1D();
+	const int x = i % P.W, y = i / P.W;
+	const float p0	 = (float) x + 0.5f + cs[0];
+	const float p1	 = (float) y + 0.5f + cs[1];
+	const float ndc0 = (2.f * p0) / (float) P.W + (-1.f);
+	const float ndc1 = (2.f * p1) / (float) P.H + (-1.f);
+	const float fov	 = detmath::atan2f(cam.filmH * 0.5f, cam.focalLength);
+	const f3 fd		 = normalize(f3(detmath::tanf(fov) * cam.aspect * ndc0, detmath::tanf(fov) * ndc1, -1.f));
+	f3 o, d;
+	if (cam.lensRadius > 1e-5f) {
+		float ax, ay;
+		uniformSampleDisk(cs[2], cs[3], ax, ay);
+		o = f3(cam.lensRadius * ax, cam.lensRadius * ay, 0.f);
+		d = normalize(fd * cam.focalDistance - o);
+	} else {
+		o = f3();
+		d = fd;
+	}
+	const float *M = cam.M;
+	RayItem r;
+	r.o		   = f3(M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
+					M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]);
+	r.d		   = f3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z, M[8] * d.x + M[9] * d.y + M[10] * d.z);
+	r.thp	   = f3(1.f);
+	r.pu	   = f3(1.f);
+	r.pl	   = f3(1.f);
+	r.cp	   = f3();
+	r.cn	   = f3();
+	r.depth	   = 0;
+	r.bsdfType = BSDF_UNSET;
+	r.medium   = cam.medium;
+	r.pixel	   = i;
+	storeRayItem(q, i, r);
+	px.rng[i] = s.state;
+}
+
+__global__ void k_depth_begin(Counters *cnt, int cur) {
+	const int nxt = cur ^ 1;
+	cnt->closestTotal += (unsigned long long) cnt->rayCount[cur];
+	cnt->closestLaunches += 1;
+	cnt->rayCount[nxt] = 0;
+	cnt->shadowCount   = 0;
+}
+
+// Closest-hit trace over the first *count rays (count from device memory, so
+// the host never waits for queue sizes).
+template <bool COUNT>
+__global__ void k_trace_closest(TraceBvh bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, const int *countPtr,
+								int fixedCount, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
+	const int count = countPtr ? *countPtr : fixedCount;
+	uint32_t nv = 0, nt = 0;
+	if (i < count) {
+		const float4 a = rays[2 * i], b = rays[2 * i + 1];
+		TraceResult r  = traverse<false, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
+		hits[i]		   = make_float4(r.prim >= 0 ? r.t : __builtin_inff(), __int_as_float(r.prim), r.b1, r.b2);
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->nodeVisits, nv);
+		waveAddU64(&cnt->triTests, nt);
+	}
+}
+
+// Any-hit trace with a visibility result per ray (krr_accel_trace_any).
+template <bool COUNT>
+__global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *__restrict__ visible, int count, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t nv = 0, nt = 0;
+	if (i < count) {
+		const float4 a = rays[2 * i], b = rays[2 * i + 1];
+		TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
+		visible[i]	   = r.prim < 0 ? 1 : 0;
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->nodeVisits, nv);
+		waveAddU64(&cnt->triTests, nt);
+	}
+}
+
+// traceShadow without media (device.cu:83-100): any hit on a non-null
+// triangle in (0, 1) occludes; visible shadow rays add Ld / mean(pu + pl).
+template <bool COUNT>
+__global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
+	const int count = cnt->shadowCount;
+	if (i == 0) cnt->shadowTotal += (unsigned long long) count;
+	uint32_t nv = 0, nt = 0;
+	if (i < count) {
+		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1];
+		TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
+		if (r.prim < 0) {
+			const int pixel = __float_as_int(b.w);
+			const float4 c	= sq.c[i];
+			float4 L		= px.L[pixel];
+			L.x				= L.x + c.x;
+			L.y				= L.y + c.y;
+			L.z				= L.z + c.z;
+			px.L[pixel]		= L;
+		}
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->nodeVisits, nv);
+		waveAddU64(&cnt->triTests, nt);
+	}
+}
+
+// sampleT_maj (wavefront.h:163-207) on the medium of the ray.  callback(p,
+// mp, sigma_maj, T_maj) returns false to stop; the result is the remaining
+// majorant transmittance (or 1 when stopped by the callback).
+template <class F>
+KRR_DEV f3 sampleTmaj(const DevScene &sc, int medium, f3 o, f3 d, float tMax, Pcg &rng, int channel, F &&callback) {
+	tMax			   = tMax * length(d);
+	d				   = normalize(d);
+	f3 T_maj		   = f3(1.f);
+	const DevMedium &m = sc.media[medium];
+	MajorantIter it;
+	sampleRay(m, o, d, tMax, it);
+	MajorantSeg seg;
+	while (true) {
+		if (!it.next(seg)) return T_maj;
+		if (seg.sigma_maj[channel] == 0.f) {
+			float dt = seg.tMax - seg.tMin;
+			if (__builtin_isinf(dt)) dt = kFloatMax;
+			T_maj = T_maj * vexp(-dt * seg.sigma_maj);
+			continue;
+		}
+		float tMin = seg.tMin;
+		while (true) {
+			const float t = tMin + sampleExponential(rng.get1D(), seg.sigma_maj[channel]);
+			if (t < seg.tMax) {
+				T_maj			  = T_maj * vexp(-(t - tMin) * seg.sigma_maj);
+				const f3 p		  = o + d * t;
+				MediumProps mp	  = samplePoint(m, p);
+				if (!callback(p, mp, seg.sigma_maj, T_maj)) return f3(1.f);
+				T_maj = f3(1.f);
+				tMin  = t;
+			} else {
+				float dt = seg.tMax - tMin;
+				if (__builtin_isinf(dt)) dt = kFloatMax;
+				T_maj = T_maj * vexp(-dt * seg.sigma_maj);
+				break;
+			}
+		}
+	}
+}
+
+// The fused shading stage for the rays of the current queue.  Per ray, in the
+// reference's launch order: closest-hit / miss routing (device.cu:43-74),
+// sampleMediumInteraction (medium.cpp:13-103), handleHit
+// (integrator.cpp:183-195), handleMiss (:197-213), the depth == maxDepth
+// break, sampleMediumScattering (medium.cpp:105-153) and
+// generateScatterRays (integrator.cpp:215-269).
+template <bool MEDIA>
+__global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueueDev cur, const float4 *__restrict__ hits,
+											  RayQueueDev nxt, ShadowQueueDev sq, PixelStateDev px, Counters *cnt, int curIdx,
+											  int loopDepth) {
+	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
+	const int count = cnt->rayCount[curIdx];
+	bool wantNext = false, wantShadow = false;
+	RayItem nx;
+	f3 sO, sD, sLd, sPu, sPl;
+	int sMedium = -1;
+	int pixel	= 0;
+	if (i < count) {
+		const RayItem r = loadRayItem(cur, i);
+		pixel			= r.pixel;
+		const float4 h	= hits[i];
+		const int prim	= __float_as_int(h.y);
+		Pcg rng{px.rng[pixel], P.inc};
+		const int channel = MEDIA ? px.channel[pixel] : 0;
+		const float4 Lv	  = px.L[pixel];
+		f3 L(Lv.x, Lv.y, Lv.z);
+		f3 thp = r.thp, pu = r.pu, pl = r.pl;
+		bool hasHL = false, hasScatter = false, hasMiss = false, hasMS = false, hasMSc = false;
+		SurfaceInteraction intr;
+		float msTmax   = __builtin_inff();
+		bool msHasIntr = false;
+		f3 mscP, mscThp, mscPu, mscWo;
+		float mscG = 0.f;
+		if (prim >= 0) {
+			prepareSurfaceInteraction(sc, prim, h.z, h.w, r.d, r.medium, intr);
+			if (MEDIA && P.enableMedium && r.medium >= 0) {
+				hasMS	  = true;
+				msTmax	  = h.x;
+				msHasIntr = true;
+			} else if (intr.material < 0) {
+				wantNext  = true;
+				nx		  = r;
+				nx.o	  = offsetRayOrigin(intr.p, intr.n, r.d);
+				nx.medium = intr.getMedium(r.d);
+			} else {
+				if (intr.light >= 0) hasHL = true;
+				if (thp.any()) hasScatter = true;
+			}
+		} else {
+			if (MEDIA && P.enableMedium && r.medium >= 0) hasMS = true;
+			else hasMiss = true;
+		}
+
+		if (MEDIA && hasMS) {
+			f3 Lm;
+			bool scattered = false;
+			const f3 T_maj = sampleTmaj(sc, r.medium, r.o, r.d, msTmax, rng, channel,
+				[&](const f3 &p, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
+					if (r.depth < P.maxDepth && mp.Le.any()) {
+						const float pr = sigma_maj[channel] * Tm[channel];
+						const f3 pe	   = pu * sigma_maj * Tm / pr;
+						if (pe.any()) Lm = Lm + thp * mp.sigma_a * Tm * mp.Le / (pr * pe.mean());
+					}
+					const float pAbsorb	 = mp.sigma_a[channel] / sigma_maj[channel];
+					const float pScatter = mp.sigma_s[channel] / sigma_maj[channel];
+					const float pNull	 = maxf(0.f, 1.f - pAbsorb - pScatter);
+					const float w[3]	 = {pAbsorb, pScatter, pNull};
+					const int mode		 = sampleDiscrete3(w, rng.get1D());
+					if (mode == 0) {
+						thp = f3();
+						return false;
+					} else if (mode == 1) {
+						const float pr = Tm[channel] * mp.sigma_s[channel];
+						thp			   = thp * (Tm * mp.sigma_s / pr);
+						pu			   = pu * (Tm * mp.sigma_s / pr);
+						if (thp.any() && pu.any()) {
+							hasMSc = true;
+							mscP   = p;
+							mscThp = thp;
+							mscPu  = pu;
+							mscWo  = -r.d;
+							mscG   = mp.g;
+						}
+						scattered = true;
+						return false;
+					} else {
+						const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
+						const float pr	 = Tm[channel] * sigma_n[channel];
+						thp				 = thp * (Tm * sigma_n / pr);
+						if (pr == 0.f) thp = f3();
+						pu = pu * (Tm * sigma_n / pr);
+						pl = pl * (Tm * sigma_maj / pr);
+						return thp.any() && pu.any();
+					}
+				});
+			if (Lm.any()) L = L + Lm;
+			if (!scattered && thp.any()) {
+				thp = thp * (T_maj / T_maj[channel]);
+				pu	= pu * (T_maj / T_maj[channel]);
+				pl	= pl * (T_maj / T_maj[channel]);
+			}
+			if (!(scattered || !thp.any() || !pu.any() || r.depth == P.maxDepth)) {
+				if (!msHasIntr) hasMiss = true;
+				else if (intr.material < 0) {
+					wantNext  = true;
+					nx		  = r;
+					nx.o	  = offsetRayOrigin(intr.p, intr.n, r.d);
+					nx.medium = intr.getMedium(r.d);
+					nx.thp	  = thp;
+					nx.pu	  = pu;
+					nx.pl	  = pl;
+				} else {
+					if (intr.light >= 0) hasHL = true;
+					hasScatter = true;
+				}
+			}
+		}
+
+		const float lightSelectPdf = 1.f / (float) sc.nLights;
+		if (hasHL) { // handleHit
+			const DevLight Lr = sc.lights[intr.light];
+			f3 Le			  = areaLightL(Lr, intr.n, intr.wo) * thp;
+			if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
+				const float lightPdf = lightPdfLi(sc, Lr, intr.p, intr.n, r.cp) * lightSelectPdf;
+				Le					 = Le / (pl * lightPdf + pu).mean();
+			} else Le = Le / pu.mean();
+			L = L + Le;
+		}
+		if (hasMiss) { // handleMiss
+			f3 Lm;
+			for (int li = 0; li < sc.nLights; li++) {
+				const DevLight Lr = sc.lights[li];
+				if (Lr.type != 1) continue;
+				const f3 Li = Lr.scale * f3(Lr.Le[0], Lr.Le[1], Lr.Le[2]);
+				if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
+					const float lp = kInv4Pi * lightSelectPdf;
+					Lm			   = Lm + Li / (pu + pl * lp).mean();
+				} else Lm = Lm + Li / pu.mean();
+			}
+			L = L + thp * Lm;
+		}
+
+		if (loopDepth != P.maxDepth) {
+			if (MEDIA && hasMSc) { // sampleMediumScattering
+				if (P.nee && sc.nLights > 0) {
+					const float ul	 = rng.get1D();
+					const int li	 = (int) (uint32_t) (ul * (float) sc.nLights);
+					const float u0	 = rng.get1D(), u1 = rng.get1D();
+					const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, mscP);
+					const f3 to		 = offsetRayOrigin(ls.p, ls.n, mscP - ls.p);
+					const f3 p_o	 = offsetRayOrigin(mscP, f3(), to - mscP);
+					const f3 sdir	 = to - p_o;
+					const f3 wi		 = normalize(sdir);
+					const f3 thpS	 = mscThp * hgP(mscG, mscWo, wi);
+					const float lightPdf = lightSelectPdf * ls.pdf;
+					const float phasePdf = hgP(mscG, mscWo, wi);
+					const f3 Ld			 = thpS * ls.L;
+					if (Ld.any() && lightPdf > 0.f) {
+						wantShadow = true;
+						sO		   = p_o;
+						sD		   = sdir;
+						sLd		   = Ld;
+						sPu		   = mscPu * phasePdf;
+						sPl		   = mscPu * lightPdf;
+						sMedium	   = r.medium;
+					}
+				}
+				const float u0 = rng.get1D(), u1 = rng.get1D();
+				float ppdf;
+				const f3 wi	   = hgSample(mscG, mscWo, u0, u1, &ppdf);
+				f3 thpN		   = mscThp * ppdf / ppdf;
+				const float rrProb = (thpN / mscPu.mean()).maxCoeff();
+				bool alive		   = true;
+				if (r.depth >= 1 && rrProb < 1.f) {
+					if (rng.get1D() >= rrProb) alive = false;
+					else thpN = thpN / rrProb;
+				}
+				if (alive && !thpN.isZero() && !thpN.hasNaN()) {
+					wantNext	= true;
+					nx.o		= mscP;
+					nx.d		= wi;
+					nx.medium	= r.medium;
+					nx.cp		= mscP;
+					nx.cn		= f3();
+					nx.thp		= thpN;
+					nx.pu		= mscPu;
+					nx.pl		= mscPu / ppdf;
+					nx.depth	= r.depth + 1;
+					nx.bsdfType = BSDF_SMOOTH;
+					nx.pixel	= pixel;
+				}
+			}
+			if (hasScatter) { // generateScatterRays
+				bool alive = true;
+				if (r.depth >= P.rrMinDepth) {
+					if (rng.get1D() >= P.rr) alive = false;
+					else thp = thp / P.rr;
+				}
+				if (alive) {
+					const f3 woL		  = intr.toLocal(intr.wo);
+					const DevMaterial mat = sc.materials[intr.material];
+					const int bt		  = bsdfTypeOf(mat.type, mat.roughness);
+					BSDF bsdf;
+					bsdf.setup(mat);
+					if (P.nee && (bt & BSDF_SMOOTH) && sc.nLights > 0) {
+						const float ul		 = rng.get1D();
+						const int li		 = (int) (uint32_t) (ul * (float) sc.nLights);
+						const float u0		 = rng.get1D(), u1 = rng.get1D();
+						const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
+						const f3 to			 = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
+						const f3 p_o		 = offsetRayOrigin(intr.p, intr.n, to - intr.p);
+						const f3 sdir		 = to - p_o;
+						const f3 wiL		 = intr.toLocal(normalize(sdir));
+						const float lightPdf = lightSelectPdf * ls.pdf;
+						const f3 bv			 = bsdf.f(woL, wiL);
+						const float bp		 = bsdf.pdf(woL, wiL);
+						if (lightPdf > 0.f && bv.any()) {
+							const f3 Ld = ls.L * thp * bv * fabsf(wiL.z);
+							if (Ld.any()) {
+								wantShadow = true;
+								sO		   = p_o;
+								sD		   = sdir;
+								sLd		   = Ld;
+								sPu		   = pu * bp;
+								sPl		   = pu * lightPdf;
+								sMedium	   = intr.getMedium(sdir);
+							}
+						}
+					}
+					const BSDFSample s = bsdf.sample(woL, rng);
+					if (s.pdf != 0.f && s.f.any()) {
+						const f3 wiW  = intr.toWorld(s.wi);
+						const f3 nthp = thp * s.f * fabsf(s.wi.z) / s.pdf;
+						if (nthp.any()) {
+							wantNext	= true;
+							nx.bsdfType = s.flags;
+							nx.pu		= pu;
+							nx.pl		= pu / s.pdf;
+							nx.o		= offsetRayOrigin(intr.p, intr.n, wiW);
+							nx.d		= wiW;
+							nx.medium	= intr.getMedium(wiW);
+							nx.cp		= intr.p;
+							nx.cn		= intr.n;
+							nx.depth	= r.depth + 1;
+							nx.thp		= nthp;
+							nx.pixel	= pixel;
+						}
+					}
+				}
+			}
+		}
+		px.L[pixel]	  = make_float4(L.x, L.y, L.z, 0.f);
+		px.rng[pixel] = rng.state;
+	}
+	const int slotN = wavePush(&cnt->rayCount[curIdx ^ 1], wantNext);
+	const int slotS = wavePush(&cnt->shadowCount, wantShadow);
+	if (wantNext) storeRayItem(nxt, slotN, nx);
+	if (wantShadow) {
+		sq.ray[2 * slotS]	  = make_float4(sO.x, sO.y, sO.z, 1.f);
+		sq.ray[2 * slotS + 1] = make_float4(sD.x, sD.y, sD.z, __int_as_float(pixel));
+		if (MEDIA && P.enableMedium) {
+			sq.c[slotS]			  = make_float4(0.f, 0.f, 0.f, __int_as_float(sMedium + 1));
+			sq.ext[3 * slotS]	  = make_float4(sLd.x, sLd.y, sLd.z, 0.f);
+			sq.ext[3 * slotS + 1] = make_float4(sPu.x, sPu.y, sPu.z, 0.f);
+			sq.ext[3 * slotS + 2] = make_float4(sPl.x, sPl.y, sPl.z, 0.f);
+		} else {
+			const f3 c	= sLd / (sPl + sPu).mean();
+			sq.c[slotS] = make_float4(c.x, c.y, c.z, __int_as_float(sMedium + 1));
+		}
+	}
+}
+
+// traceShadow with media: ShadowTr raygen + traceTransmittance
+// (device.cu:102-128, wavefront.h:209-268).  Ratio tracking through media,
+// stepping across null-material surfaces.
+template <bool COUNT>
+__global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
+	const int count = cnt->shadowCount;
+	if (i == 0) cnt->shadowTotal += (unsigned long long) count;
+	uint32_t nv = 0, nt = 0;
+	if (i < count) {
+		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1];
+		const int pixel		 = __float_as_int(b.w);
+		const int origMedium = __float_as_int(sq.c[i].w) - 1;
+		const float4 e0 = sq.ext[3 * i], e1 = sq.ext[3 * i + 1], e2 = sq.ext[3 * i + 2];
+		const f3 Ld(e0.x, e0.y, e0.z), spu(e1.x, e1.y, e1.z), spl(e2.x, e2.y, e2.z);
+		Pcg rng{px.rng[pixel], P.inc};
+		const int channel = px.channel[pixel];
+		const float tMax  = a.w;
+		f3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
+		int medium		= origMedium;
+		const f3 pLight = o + d * tMax;
+		f3 T_ray(1.f), pu(1.f), pl(1.f);
+		SurfaceInteraction intr;
+		const float zeroEps = kRayEps * 2.f;
+		while (!(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps)) {
+			const TraceResult h = traverse<false, COUNT>(bvh, o, d, tMax, ldsStack + threadIdx.x, blockDim.x, nv, nt);
+			const bool visible	= h.prim < 0;
+			if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
+			if (!visible && intr.material >= 0) {
+				T_ray = f3();
+				break;
+			}
+			if (medium >= 0) {
+				const float tEnd = visible ? tMax : length(intr.p - o) / length(d);
+				const f3 T_maj	 = sampleTmaj(sc, medium, o, d, tEnd, rng, channel,
+					  [&](const f3 &, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
+						  const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
+						  const float pr   = Tm[channel] * sigma_maj[channel];
+						  T_ray			   = T_ray * (Tm * sigma_n / pr);
+						  pl			   = pl * (Tm * sigma_maj / pr);
+						  pu			   = pu * (Tm * sigma_n / pr);
+						  const f3 Tr	   = T_ray / (pu + pl).mean();
+						  if (Tr.maxCoeff() < 0.05f) {
+							  if (rng.get1D() < 0.75f) T_ray = f3();
+							  else T_ray = T_ray / 0.25f;
+						  }
+						  return T_ray.any();
+					  });
+				T_ray = T_ray * (T_maj / T_maj[channel]);
+				pu	  = pu * (T_maj / T_maj[channel]);
+				pl	  = pl * (T_maj / T_maj[channel]);
+			}
+			if (visible || T_ray.isZero()) break;
+			const f3 p_o = offsetRayOrigin(intr.p, intr.n, pLight - intr.p);
+			d			 = pLight - p_o;
+			o			 = p_o;
+			medium		 = intr.getMedium(d);
+		}
+		if (T_ray.any()) {
+			const f3 c = Ld * T_ray / (spu * pu + spl * pl).mean();
+			float4 L   = px.L[pixel];
+			L.x		   = L.x + c.x;
+			L.y		   = L.y + c.y;
+			L.z		   = L.z + c.z;
+			px.L[pixel] = L;
+		}
+		px.rng[pixel] = rng.state;
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->nodeVisits, nv);
+		waveAddU64(&cnt->triTests, nt);
+	}
+}
+
+// pixel += L (integrator.cpp:362-365).
+__global__ void k_accumulate(int N, PixelStateDev px) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= N) return;
+	const float4 L = px.L[i];
+	float4 p	   = px.pixel[i];
+	p.x			   = p.x + L.x;
+	p.y			   = p.y + L.y;
+	p.z			   = p.z + L.z;
+	px.pixel[i]	   = p;
+}
+
+// Film write (integrator.cpp:367-371, CudaRenderTarget::write,
+// src/core/device/cuda.h:33-45): RGBA(pixel / spp, 1) at row H-1-y.
+__global__ void k_film(WfParams P, PixelStateDev px, float4 *film) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.N) return;
+	const int x = i % P.W, y = i / P.W;
+	const float4 p = px.pixel[i];
+	float r = p.x / (float) P.spp, g = p.y / (float) P.spp, b = p.z / (float) P.spp;
+	if (P.enableClamp) {
+		r = clampf(r, 0.f, P.clampMax);
+		g = clampf(g, 0.f, P.clampMax);
+		b = clampf(b, 0.f, P.clampMax);
+	}
+	film[(size_t) (P.H - 1 - y) * P.W + x] = make_float4(r, g, b, 1.f);
+}
+
+// Error metrics (metrics.cu:63-128): per-pixel value clamped at 100, FP64
+// sum per block, then one atomic per block.
+__global__ void k_metric(int metric, const float4 *img, const float4 *ref, int64_t n, double *out) {
+	__shared__ double part[256];
+	double acc = 0.0;
+	for (int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x; i < n; i += (int64_t) gridDim.x * blockDim.x) {
+		const float4 y = img[i], r = ref[i];
+		float e		   = 0.f;
+		const bool bad = __builtin_isinf(r.x) || __builtin_isinf(r.y) || __builtin_isinf(r.z) || r.x != r.x || r.y != r.y || r.z != r.z;
+		if (!bad) {
+			float yy[3] = {y.x, y.y, y.z}, rr[3] = {r.x, r.y, r.z}, a[3];
+			for (int c = 0; c < 3; c++) {
+				const float dd = fabsf(yy[c] - rr[c]);
+				switch (metric) {
+					case KRR_METRIC_MSE: a[c] = dd * dd; break;
+					case KRR_METRIC_MAPE: a[c] = dd / (rr[c] + 0.f); break;
+					case KRR_METRIC_SMAPE: a[c] = dd / (rr[c] + yy[c] + 0.f); break;
+					default: a[c] = rr[c] == 0.f ? 0.f : (dd / rr[c]) * (dd / rr[c]); break;
+				}
+			}
+			e = (a[0] + a[1] + a[2]) / 3.f;
+		}
+		acc += (double) fminf(e, 100.f);
+	}
+	part[threadIdx.x] = acc;
+	__syncthreads();
+	for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+		if ((int) threadIdx.x < s) part[threadIdx.x] += part[threadIdx.x + s];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+__global__ void k_zero_double(double *p) { *p = 0.0; }
+
+} // namespace krr
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+
+using namespace krr;
+
+#define KRR_HIP(call)                                                                                               \
+	do {                                                                                                            \
+		hipError_t e_ = (call);                                                                                     \
+		if (e_ != hipSuccess) return krr::fail(e_ == hipErrorOutOfMemory ? KRR_ERR_OUT_OF_MEMORY : KRR_ERR_HIP,      \
+											   "%s failed: %s", #call, hipGetErrorString(e_));                      \
+	} while (0)
+
+namespace {
+
+bool deviceAvailable() {
+	int n			 = 0;
+	hipError_t e	 = hipGetDeviceCount(&n);
+	return e == hipSuccess && n > 0;
+}
+
+// Device allocation owned by the library.
+struct DevBuf {
+	void *p	  = nullptr;
+	size_t sz = 0;
+	DevBuf() = default;
+	DevBuf(const DevBuf &) = delete;
+	DevBuf &operator=(const DevBuf &) = delete;
+	~DevBuf() { release(); }
+	void release() {
+		if (p) (void) hipFree(p);
+		p  = nullptr;
+		sz = 0;
+	}
+	hipError_t alloc(size_t bytes) {
+		release();
+		if (bytes == 0) bytes = 16;
+		hipError_t e = hipMalloc(&p, bytes);
+		if (e == hipSuccess) sz = bytes;
+		else p = nullptr;
+		return e;
+	}
+	hipError_t upload(const void *src, size_t bytes) {
+		hipError_t e = alloc(bytes);
+		if (e != hipSuccess) return e;
+		return src && bytes ? hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+	}
+	template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct AccelData {
+	DevBuf nodes, tris;
+	int stackSize = 0;
+	Bvh4Host host; // kept for statistics (nodes/tris arrays cleared after upload)
+	int64_t nNodes = 0, nLeafTris = 0;
+	int maxDepth	= 0;
+	double buildSeconds = 0;
+	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>()}; }
+};
+
+int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize) {
+	if (!s || s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
+		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing triangle arrays");
+	if (s->n_triangles > (int64_t) 0x0fffffff) return krr::fail(KRR_ERR_CAPACITY, "scene: at most 2^28 triangles");
+	for (int64_t i = 0; i < 3 * s->n_triangles; i++)
+		if (s->indices[i] < 0 || s->indices[i] >= s->n_vertices) return krr::fail(KRR_ERR_INVALID_ARG, "scene: vertex index %lld out of range", (long long) i);
+	std::vector<uint8_t> nullFlag((size_t) s->n_triangles, 0);
+	for (int m = 0; m < s->n_meshes; m++) {
+		const krr_mesh &mm = s->meshes[m];
+		if (mm.first_triangle < 0 || mm.n_triangles < 0 || (int64_t) mm.first_triangle + mm.n_triangles > s->n_triangles)
+			return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d triangle range out of bounds", m);
+		if (mm.material < 0)
+			for (int t = 0; t < mm.n_triangles; t++) nullFlag[mm.first_triangle + t] = 1;
+	}
+	BvhBuildOptions opt;
+	opt.maxLeafSize = leafSize;
+	buildBvh4(s->positions, s->indices, s->n_triangles, nullFlag.data(), opt, a.host);
+	KRR_HIP(a.nodes.upload(a.host.nodes.data(), a.host.nodes.size() * sizeof(Bvh4Node)));
+	KRR_HIP(a.tris.upload(a.host.tris.data(), a.host.tris.size() * sizeof(LeafTri)));
+	a.stackSize	   = bvhStackSize(a.host);
+	a.nNodes	   = (int64_t) a.host.nodes.size();
+	a.nLeafTris	   = (int64_t) a.host.tris.size();
+	a.maxDepth	   = a.host.maxDepth;
+	a.buildSeconds = a.host.buildSeconds;
+	a.host.nodes.clear();
+	a.host.nodes.shrink_to_fit();
+	a.host.tris.clear();
+	a.host.tris.shrink_to_fit();
+	if (a.stackSize > 96) return krr::fail(KRR_ERR_CAPACITY, "BVH depth %d needs a %d-entry stack (max 96)", a.maxDepth, a.stackSize);
+	return KRR_OK;
+}
+
+int traceBlock(int stackSize) {
+	// keep dynamic LDS per block <= 48 KB
+	int tb = 256;
+	while (tb > 64 && (size_t) stackSize * tb * 4 > 48 * 1024) tb >>= 1;
+	return tb;
+}
+
+} // namespace
+
+struct krr_accel {
+	AccelData a;
+	DevBuf counters;
+};
+
+struct krr_wf {
+	// params (integrator.h:96-103 + additions)
+	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false;
+	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4;
+	float rr = 0.8f, clampMax = 1e3f;
+	// scene
+	bool haveScene = false;
+	AccelData accel;
+	DevBuf pos, nrm, idx, triMesh, meshes, materials, lights, media;
+	std::vector<std::unique_ptr<DevBuf>> mediumBufs;
+	DevScene ds{};
+	bool sceneHasMedia = false;
+	// frame
+	int W = 0, H = 0;
+	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters;
+	krr_camera cam{};
+	uint64_t frameIndex = 0;
+	bool beginPending	= false;
+	hipStream_t lastStream = nullptr;
+	krr_stats stats{};
+};
+
+namespace {
+
+int parseParams(krr_wf *wf, const char *json) {
+	if (!json || !*json) return KRR_OK;
+	try {
+		Json j = Json::parse(json);
+		if (!j.isObject()) return krr::fail(KRR_ERR_BAD_CONFIG, "pass params must be a JSON object");
+		wf->nee			   = j.value("nee", true);
+		wf->enableMedium   = j.value("enable_medium", true);
+		wf->maxDepth	   = (int) j.value("max_depth", 10.0);
+		wf->rr			   = (float) j.value("rr", 0.8);
+		wf->enableClamp	   = j.value("enable_clamp", false);
+		wf->clampMax	   = (float) j.value("clamp_max", 1e3);
+		wf->spp			   = (int) j.value("spp", 1.0);
+		wf->rrMinDepth	   = (int) j.value("rr_min_depth", 0.0);
+		wf->legacySpp	   = j.value("legacy_spp_accumulate", false);
+		wf->countTraversal = j.value("count_traversal", false);
+		wf->blockSize	   = (int) j.value("block_size", 256.0);
+		wf->traceBlockSize = (int) j.value("trace_block_size", 128.0);
+		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
+	} catch (const std::exception &e) {
+		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
+	}
+	if (wf->maxDepth < 0 || wf->maxDepth > 250) return krr::fail(KRR_ERR_BAD_CONFIG, "max_depth must be in [0, 250]");
+	if (wf->spp < 1 || wf->spp > 1 << 20) return krr::fail(KRR_ERR_BAD_CONFIG, "spp must be >= 1");
+	if (wf->blockSize < 64 || wf->blockSize > 256 || wf->blockSize % 64) return krr::fail(KRR_ERR_BAD_CONFIG, "block_size must be 64, 128, 192 or 256");
+	if (wf->traceBlockSize < 64 || wf->traceBlockSize > 256 || wf->traceBlockSize % 64)
+		return krr::fail(KRR_ERR_BAD_CONFIG, "trace_block_size must be 64, 128, 192 or 256");
+	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
+	return KRR_OK;
+}
+
+// Majorant grid build of src/render/media.cpp:18-75 on the dense grid (max
+// density over the voxels each 1/64 cell touches, one voxel of slop).
+void buildMajorant(const krr_medium &m, const float invVoxel[3], std::vector<float> &g) {
+	g.assign((size_t) 64 * 64 * 64, 0.f);
+	for (int z = 0; z < 64; z++)
+		for (int y = 0; y < 64; y++)
+			for (int x = 0; x < 64; x++) {
+				const int c[3] = {x, y, z};
+				int n0[3], n1[3];
+				for (int a = 0; a < 3; a++) {
+					const float diag = m.bounds_max[a] - m.bounds_min[a];
+					const float lo	 = m.bounds_min[a] + (float(c[a]) / 64.f) * diag;
+					const float hi	 = m.bounds_min[a] + (float(c[a] + 1) / 64.f) * diag;
+					const float ilo	 = (lo - m.bounds_min[a]) * invVoxel[a];
+					const float ihi	 = (hi - m.bounds_min[a]) * invVoxel[a];
+					n0[a]			 = std::max((int) (ilo - 1.f), 0);
+					n1[a]			 = std::min((int) (ihi + 1.f), m.res[a] - 1);
+				}
+				float mx = 0.f;
+				for (int nz = n0[2]; nz <= n1[2]; ++nz)
+					for (int ny = n0[1]; ny <= n1[1]; ++ny)
+						for (int nx = n0[0]; nx <= n1[0]; ++nx)
+							mx = std::max(mx, m.density[(size_t) nx + (size_t) m.res[0] * ((size_t) ny + (size_t) m.res[1] * nz)]);
+				g[x + 64 * (y + 64 * z)] = mx;
+			}
+}
+
+int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
+	if (s->n_vertices < 0 || (s->n_vertices > 0 && !s->positions)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing positions");
+	if (s->n_meshes < 0 || (s->n_meshes > 0 && !s->meshes)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing meshes");
+	if (s->n_materials < 0 || (s->n_materials > 0 && !s->materials)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing materials");
+	if (s->n_media < 0 || (s->n_media > 0 && !s->media)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing media");
+	if (s->n_infinite_lights < 0 || (s->n_infinite_lights > 0 && !s->infinite_lights))
+		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing infinite lights");
+	// triangle -> mesh, mesh table with light bases (mesh lights in mesh order,
+	// then infinite lights: RTScene::uploadSceneLightData, device/scene.cpp:80-135)
+	std::vector<int> triMesh((size_t) s->n_triangles, -1);
+	std::vector<DevMesh> meshes((size_t) s->n_meshes);
+	std::vector<DevLight> lights;
+	for (int mi = 0; mi < s->n_meshes; mi++) {
+		const krr_mesh &m = s->meshes[mi];
+		if (m.material >= s->n_materials) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d material out of range", mi);
+		if (m.medium_inside >= s->n_media || m.medium_outside >= s->n_media)
+			return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d medium out of range", mi);
+		DevMesh &d	 = meshes[mi];
+		d.material	 = m.material < 0 ? -1 : m.material;
+		d.firstTri	 = m.first_triangle;
+		d.mediumIn	 = m.medium_inside < 0 ? -1 : m.medium_inside;
+		d.mediumOut	 = m.medium_outside < 0 ? -1 : m.medium_outside;
+		d.hasNormals = m.has_normals ? 1 : 0;
+		if (m.has_normals && !s->normals) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_normals but normals is NULL", mi);
+		const bool emissive = m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0;
+		d.lightBase			= -1;
+		if (emissive) {
+			if (!m.has_normals) return krr::fail(KRR_ERR_INVALID_ARG, "scene: emissive mesh %d needs vertex normals", mi);
+			d.lightBase		  = (int) lights.size();
+			const float scale = std::max(std::max(m.emission[0], m.emission[1]), m.emission[2]);
+			for (int t = 0; t < m.n_triangles; t++) {
+				DevLight L;
+				L.type	= 0;
+				L.prim	= m.first_triangle + t;
+				L.Le[0] = m.emission[0] / scale;
+				L.Le[1] = m.emission[1] / scale;
+				L.Le[2] = m.emission[2] / scale;
+				L.scale = scale;
+				lights.push_back(L);
+			}
+		}
+		for (int t = 0; t < m.n_triangles; t++) triMesh[m.first_triangle + t] = mi;
+	}
+	for (int64_t t = 0; t < s->n_triangles; t++)
+		if (triMesh[t] < 0) return krr::fail(KRR_ERR_INVALID_ARG, "scene: triangle %lld belongs to no mesh", (long long) t);
+	for (int i = 0; i < s->n_infinite_lights; i++) {
+		DevLight L;
+		L.type = 1;
+		L.prim = -1;
+		for (int a = 0; a < 3; a++) L.Le[a] = s->infinite_lights[i].tint[a];
+		L.scale = s->infinite_lights[i].scale;
+		lights.push_back(L);
+	}
+	std::vector<DevMaterial> mats((size_t) s->n_materials);
+	for (int i = 0; i < s->n_materials; i++) {
+		const krr_material &m = s->materials[i];
+		if (m.type < KRR_MAT_NULL || m.type > KRR_MAT_CONDUCTOR) return krr::fail(KRR_ERR_UNSUPPORTED, "material %d: unsupported type %d", i, m.type);
+		mats[i].type		= m.type;
+		mats[i].roughness	= 1.f - m.specular[3];
+		mats[i].ior			= m.ior;
+		mats[i].specTrans	= m.specular_transmission;
+		mats[i].anisotropic = m.anisotropic;
+		for (int a = 0; a < 3; a++) mats[i].diffuse[a] = m.diffuse[a];
+	}
+	// media
+	wf->mediumBufs.clear();
+	std::vector<DevMedium> media((size_t) s->n_media);
+	for (int i = 0; i < s->n_media; i++) {
+		const krr_medium &m = s->media[i];
+		DevMedium &d		= media[i];
+		std::memset(&d, 0, sizeof(d));
+		d.type = m.type;
+		for (int a = 0; a < 3; a++) {
+			d.sigma_t[a] = m.sigma_t[a];
+			d.albedo[a]	 = m.albedo[a];
+			d.Le[a]		 = m.Le[a];
+		}
+		d.g		= m.g;
+		d.scale = m.scale;
+		if (m.type == KRR_MEDIUM_GRID) {
+			for (int a = 0; a < 3; a++) {
+				if (m.res[a] < 1 || !(m.bounds_max[a] > m.bounds_min[a])) return krr::fail(KRR_ERR_INVALID_ARG, "medium %d: bad grid", i);
+				d.res[a]	  = m.res[a];
+				d.bmin[a]	  = m.bounds_min[a];
+				d.bmax[a]	  = m.bounds_max[a];
+				d.invVoxel[a] = (float) m.res[a] / (m.bounds_max[a] - m.bounds_min[a]);
+			}
+			if (!m.density) return krr::fail(KRR_ERR_INVALID_ARG, "medium %d: null density", i);
+			const size_t nvox = (size_t) m.res[0] * m.res[1] * m.res[2];
+			auto db			  = std::make_unique<DevBuf>();
+			KRR_HIP(db->upload(m.density, nvox * sizeof(float)));
+			std::vector<float> maj;
+			buildMajorant(m, d.invVoxel, maj);
+			auto mb = std::make_unique<DevBuf>();
+			KRR_HIP(mb->upload(maj.data(), maj.size() * sizeof(float)));
+			d.density  = db->as<float>();
+			d.majorant = mb->as<float>();
+			wf->mediumBufs.push_back(std::move(db));
+			wf->mediumBufs.push_back(std::move(mb));
+		} else if (m.type != KRR_MEDIUM_HOMOGENEOUS) return krr::fail(KRR_ERR_UNSUPPORTED, "medium %d: unsupported type", i);
+	}
+	float sceneRadius = 1e5f;
+	if (s->n_vertices > 0) {
+		float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+		for (int64_t v = 0; v < s->n_vertices; v++)
+			for (int a = 0; a < 3; a++) {
+				lo[a] = std::min(lo[a], s->positions[3 * v + a]);
+				hi[a] = std::max(hi[a], s->positions[3 * v + a]);
+			}
+		float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+		sceneRadius = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+	}
+	KRR_HIP(wf->pos.upload(s->positions, (size_t) s->n_vertices * 3 * sizeof(float)));
+	if (s->normals) KRR_HIP(wf->nrm.upload(s->normals, (size_t) s->n_vertices * 3 * sizeof(float)));
+	else wf->nrm.release();
+	KRR_HIP(wf->idx.upload(s->indices, (size_t) s->n_triangles * 3 * sizeof(int32_t)));
+	KRR_HIP(wf->triMesh.upload(triMesh.data(), triMesh.size() * sizeof(int)));
+	KRR_HIP(wf->meshes.upload(meshes.data(), meshes.size() * sizeof(DevMesh)));
+	KRR_HIP(wf->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial)));
+	KRR_HIP(wf->lights.upload(lights.data(), lights.size() * sizeof(DevLight)));
+	KRR_HIP(wf->media.upload(media.data(), media.size() * sizeof(DevMedium)));
+	int rc = buildAccel(s, wf->accel, wf->leafSize);
+	if (rc) return rc;
+	DevScene &ds	= wf->ds;
+	ds.pos			= wf->pos.as<float>();
+	ds.nrm			= s->normals ? wf->nrm.as<float>() : nullptr;
+	ds.idx			= wf->idx.as<int>();
+	ds.triMesh		= wf->triMesh.as<int>();
+	ds.meshes		= wf->meshes.as<DevMesh>();
+	ds.materials	= wf->materials.as<DevMaterial>();
+	ds.lights		= wf->lights.as<DevLight>();
+	ds.nLights		= (int) lights.size();
+	ds.media		= wf->media.as<DevMedium>();
+	ds.nMedia		= s->n_media;
+	ds.sceneRadius	= sceneRadius;
+	wf->sceneHasMedia = s->n_media > 0;
+	wf->haveScene	  = true;
+	return KRR_OK;
+}
+
+int allocFrame(krr_wf *wf) {
+	const size_t N = (size_t) wf->W * wf->H;
+	for (int q = 0; q < 2; q++) {
+		KRR_HIP(wf->rayQ[q][0].alloc(N * 2 * sizeof(float4)));
+		KRR_HIP(wf->rayQ[q][1].alloc(N * 4 * sizeof(float4)));
+	}
+	KRR_HIP(wf->hits.alloc(N * sizeof(float4)));
+	KRR_HIP(wf->shadowRay.alloc(N * 2 * sizeof(float4)));
+	KRR_HIP(wf->shadowC.alloc(N * sizeof(float4)));
+	KRR_HIP(wf->shadowExt.alloc(N * 3 * sizeof(float4)));
+	KRR_HIP(wf->L.alloc(N * sizeof(float4)));
+	KRR_HIP(wf->pixel.alloc(N * sizeof(float4)));
+	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
+	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
+	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+	return KRR_OK;
+}
+
+RayQueueDev rayQueue(krr_wf *wf, int q) { return RayQueueDev{wf->rayQ[q][0].as<float4>(), wf->rayQ[q][1].as<float4>()}; }
+
+} // namespace
+
+extern "C" {
+
+const char *krr_last_error(void) { return krr::lastErrorStorage().c_str(); }
+
+int krr_abi_version(void) { return KRR_ABI_VERSION; }
+
+int krr_device_count(void) {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+	return n;
+}
+
+// ---------------------------------------------------------------------------
+// Acceleration structure
+// ---------------------------------------------------------------------------
+
+int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_accel **out) {
+	krr::clearError();
+	if (!scene || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_build: null argument");
+	*out = nullptr;
+	int leafSize = 4;
+	if (json_opts && *json_opts) {
+		try {
+			Json j	 = Json::parse(json_opts);
+			leafSize = (int) j.value("leaf_size", 4.0);
+		} catch (const std::exception &e) {
+			return krr::fail(KRR_ERR_BAD_CONFIG, "krr_accel_build: %s", e.what());
+		}
+		if (leafSize < 1 || leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
+	}
+	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_accel_build: no HIP device");
+	for (int m = 0; m < scene->n_meshes; m++)
+		if (scene->meshes[m].material >= scene->n_materials) return krr::fail(KRR_ERR_INVALID_ARG, "mesh %d material out of range", m);
+	auto a = std::make_unique<krr_accel>();
+	int rc = buildAccel(scene, a->a, leafSize);
+	if (rc) return rc;
+	KRR_HIP(a->counters.alloc(sizeof(Counters)));
+	KRR_HIP(hipMemset(a->counters.p, 0, sizeof(Counters)));
+	*out = a.release();
+	return KRR_OK;
+}
+
+int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev, void *hip_stream) {
+	krr::clearError();
+	if (!accel || n < 0 || (n > 0 && (!rays_dev || !hits_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_closest: bad argument");
+	if (n > INT32_MAX) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_closest: at most 2^31-1 rays per call");
+	if (n == 0) return KRR_OK;
+	const int tb		= traceBlock(accel->a.stackSize);
+	const size_t lds	= (size_t) accel->a.stackSize * tb * sizeof(int);
+	const unsigned grid = (unsigned) ((n + tb - 1) / tb);
+	hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(tb), lds, (hipStream_t) hip_stream, accel->a.dev(),
+					   (const float4 *) rays_dev, (float4 *) hits_dev, (const int *) nullptr, (int) n, accel->counters.as<Counters>());
+	KRR_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+int krr_accel_trace_any(krr_accel *accel, const float *rays_dev, int64_t n, int32_t *visible_dev, void *hip_stream) {
+	krr::clearError();
+	if (!accel || n < 0 || (n > 0 && (!rays_dev || !visible_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_any: bad argument");
+	if (n > INT32_MAX) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_any: at most 2^31-1 rays per call");
+	if (n == 0) return KRR_OK;
+	const int tb		= traceBlock(accel->a.stackSize);
+	const size_t lds	= (size_t) accel->a.stackSize * tb * sizeof(int);
+	const unsigned grid = (unsigned) ((n + tb - 1) / tb);
+	hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(tb), lds, (hipStream_t) hip_stream, accel->a.dev(), (const float4 *) rays_dev,
+					   (int *) visible_dev, (int) n, accel->counters.as<Counters>());
+	KRR_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+int krr_accel_info(const krr_accel *accel, int64_t *n_nodes, int32_t *max_depth, int64_t *n_leaf_tris, int64_t *bytes) {
+	if (!accel) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_info: null accel");
+	if (n_nodes) *n_nodes = accel->a.nNodes;
+	if (max_depth) *max_depth = accel->a.maxDepth;
+	if (n_leaf_tris) *n_leaf_tris = accel->a.nLeafTris;
+	if (bytes) *bytes = (int64_t) (accel->a.nodes.sz + accel->a.tris.sz);
+	return KRR_OK;
+}
+
+int krr_accel_destroy(krr_accel *accel) {
+	delete accel;
+	return KRR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// WavefrontPathTracer
+// ---------------------------------------------------------------------------
+
+int krr_wf_create(const char *json_params, krr_wf **out) {
+	krr::clearError();
+	if (!out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_create: null out");
+	*out	= nullptr;
+	auto wf = std::make_unique<krr_wf>();
+	int rc	= parseParams(wf.get(), json_params);
+	if (rc) return rc;
+	*out = wf.release();
+	return KRR_OK;
+}
+
+int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
+	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_get_params: null handle");
+	Json j = Json::object();
+	j.set("nee", Json(wf->nee));
+	j.set("enable_medium", Json(wf->enableMedium));
+	j.set("max_depth", Json((double) wf->maxDepth));
+	j.set("rr", Json((double) wf->rr));
+	j.set("enable_clamp", Json(wf->enableClamp));
+	j.set("clamp_max", Json((double) wf->clampMax));
+	j.set("spp", Json((double) wf->spp));
+	j.set("rr_min_depth", Json((double) wf->rrMinDepth));
+	j.set("legacy_spp_accumulate", Json(wf->legacySpp));
+	j.set("count_traversal", Json(wf->countTraversal));
+	j.set("block_size", Json((double) wf->blockSize));
+	j.set("trace_block_size", Json((double) wf->traceBlockSize));
+	j.set("leaf_size", Json((double) wf->leafSize));
+	std::string s = j.dump();
+	if (buf && cap > 0) {
+		size_t n = std::min((size_t) cap - 1, s.size());
+		std::memcpy(buf, s.data(), n);
+		buf[n] = 0;
+	}
+	return (int) s.size();
+}
+
+int krr_wf_set_scene(krr_wf *wf, const krr_scene_desc *scene) {
+	krr::clearError();
+	if (!wf || !scene) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_scene: null argument");
+	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_set_scene: no HIP device");
+	KRR_HIP(hipDeviceSynchronize());
+	wf->haveScene = false;
+	int rc		  = uploadScene(wf, scene);
+	if (rc) return rc;
+	if (wf->W > 0 && wf->H > 0) return allocFrame(wf);
+	return KRR_OK;
+}
+
+int krr_wf_resize(krr_wf *wf, int32_t width, int32_t height) {
+	krr::clearError();
+	if (!wf || width < 0 || height < 0) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_resize: bad argument");
+	if ((int64_t) width * height > (int64_t) 1 << 26) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_resize: at most 2^26 pixels per frame");
+	if (width > 65536 || height > 65536) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_resize: width and height must be <= 65536");
+	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_resize: no HIP device");
+	KRR_HIP(hipDeviceSynchronize());
+	wf->W = width;
+	wf->H = height;
+	if (width == 0 || height == 0) return KRR_OK;
+	return allocFrame(wf);
+}
+
+int krr_wf_begin_frame(krr_wf *wf, const krr_camera *camera, uint64_t frame_index) {
+	krr::clearError();
+	if (!wf || !camera) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_begin_frame: null argument");
+	if (!wf->haveScene || wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:311 early return
+	if (camera->medium >= wf->ds.nMedia) return krr::fail(KRR_ERR_INVALID_ARG, "camera medium out of range");
+	wf->cam			 = *camera;
+	wf->frameIndex	 = frame_index;
+	wf->beginPending = true;
+	return KRR_OK;
+}
+
+int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
+	krr::clearError();
+	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: null handle");
+	if (!wf->haveScene) return krr::fail(KRR_ERR_NO_SCENE, "krr_wf_render: no scene set");
+	if (wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:329 early return
+	if (!film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: null film");
+	hipStream_t st = (hipStream_t) hip_stream;
+	wf->lastStream = st;
+	const int N	   = wf->W * wf->H;
+	WfParams P;
+	P.W			  = wf->W;
+	P.H			  = wf->H;
+	P.N			  = N;
+	P.spp		  = wf->spp;
+	P.maxDepth	  = wf->maxDepth;
+	P.rr		  = wf->rr;
+	P.rrMinDepth  = wf->rrMinDepth;
+	P.nee		  = wf->nee ? 1 : 0;
+	const bool media = wf->enableMedium && wf->sceneHasMedia;
+	P.enableMedium	 = media ? 1 : 0;
+	P.enableClamp	 = wf->enableClamp ? 1 : 0;
+	P.clampMax		 = wf->clampMax;
+	P.legacySpp		 = wf->legacySpp ? 1 : 0;
+	P.sampleIndex	 = (uint32_t) (wf->frameIndex * (uint64_t) wf->spp);
+	P.inc			 = ((uint64_t) P.sampleIndex << 1u) | 1u;
+	CameraDev cam;
+	cam.filmH		  = wf->cam.film_size[1];
+	cam.focalLength	  = wf->cam.focal_length;
+	cam.focalDistance = wf->cam.focal_distance;
+	cam.lensRadius	  = wf->cam.lens_radius;
+	cam.aspect		  = wf->cam.aspect_ratio;
+	std::memcpy(cam.M, wf->cam.transform, sizeof(cam.M));
+	cam.medium = wf->cam.medium;
+
+	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
+	ShadowQueueDev sq{wf->shadowRay.as<float4>(), wf->shadowC.as<float4>(), wf->shadowExt.as<float4>()};
+	Counters *cnt = wf->counters.as<Counters>();
+	const TraceBvh bvh = wf->accel.dev();
+	const int bs	   = wf->blockSize;
+	const unsigned gN  = (unsigned) ((N + bs - 1) / bs);
+	const int tb	   = std::min(wf->traceBlockSize, traceBlock(wf->accel.stackSize));
+	const unsigned gT  = (unsigned) ((N + tb - 1) / tb);
+	const size_t lds   = (size_t) wf->accel.stackSize * tb * sizeof(int);
+	const bool count   = wf->countTraversal;
+
+	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
+	if (wf->beginPending) {
+		hipLaunchKernelGGL(k_begin_frame, dim3(gN), dim3(bs), 0, st, P, px);
+		wf->beginPending = false;
+	}
+	for (int s = 0; s < wf->spp; s++) {
+		hipLaunchKernelGGL(k_camera, dim3(gN), dim3(bs), 0, st, P, cam, px, rayQueue(wf, 0), cnt);
+		for (int depth = 0; depth <= wf->maxDepth; depth++) {
+			const int cur = depth & 1;
+			hipLaunchKernelGGL(k_depth_begin, dim3(1), dim3(1), 0, st, cnt, cur);
+			const RayQueueDev cq = rayQueue(wf, cur);
+			if (count)
+				hipLaunchKernelGGL(k_trace_closest<true>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray, wf->hits.as<float4>(),
+								   (const int *) &cnt->rayCount[cur], 0, cnt);
+			else
+				hipLaunchKernelGGL(k_trace_closest<false>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray, wf->hits.as<float4>(),
+								   (const int *) &cnt->rayCount[cur], 0, cnt);
+			if (media)
+				hipLaunchKernelGGL(k_shade<true>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+								   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
+			else
+				hipLaunchKernelGGL(k_shade<false>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+								   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
+			if (wf->nee && depth != wf->maxDepth) {
+				if (media) {
+					if (count) hipLaunchKernelGGL(k_shadow_tr<true>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
+					else hipLaunchKernelGGL(k_shadow_tr<false>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
+				} else {
+					if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
+					else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
+				}
+			}
+		}
+		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
+	}
+	hipLaunchKernelGGL(k_film, dim3(gN), dim3(bs), 0, st, P, px, (float4 *) film_dev);
+	KRR_HIP(hipGetLastError());
+	wf->stats.paths = (uint64_t) N * wf->spp;
+	return KRR_OK;
+}
+
+int krr_wf_end_frame(krr_wf *wf) {
+	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_end_frame: null handle");
+	return KRR_OK;
+}
+
+int krr_wf_stats(krr_wf *wf, krr_stats *out) {
+	krr::clearError();
+	if (!wf || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_stats: null argument");
+	std::memset(out, 0, sizeof(*out));
+	if (!wf->counters.p) return KRR_OK;
+	KRR_HIP(hipStreamSynchronize(wf->lastStream));
+	Counters c;
+	KRR_HIP(hipMemcpy(&c, wf->counters.p, sizeof(c), hipMemcpyDeviceToHost));
+	out->closest_rays	  = c.closestTotal;
+	out->shadow_rays	  = c.shadowTotal;
+	out->paths			  = wf->stats.paths;
+	out->node_visits	  = c.nodeVisits;
+	out->tri_tests		  = c.triTests;
+	out->closest_launches = c.closestLaunches;
+	return KRR_OK;
+}
+
+int krr_wf_destroy(krr_wf *wf) {
+	if (wf) {
+		if (deviceAvailable()) (void) hipDeviceSynchronize();
+		delete wf;
+	}
+	return KRR_OK;
+}
+
+int krr_error_metric(int32_t metric, const float *img_dev, const float *ref_dev, int64_t n_pixels, double *out, void *hip_stream) {
+	krr::clearError();
+	if (!out || n_pixels < 0 || (n_pixels > 0 && (!img_dev || !ref_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_error_metric: bad argument");
+	if (metric < KRR_METRIC_MSE || metric > KRR_METRIC_RELMSE) return krr::fail(KRR_ERR_INVALID_ARG, "krr_error_metric: unknown metric %d", metric);
+	*out = 0.0;
+	if (n_pixels == 0) return KRR_OK;
+	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_error_metric: no HIP device");
+	hipStream_t st = (hipStream_t) hip_stream;
+	double *acc	   = nullptr;
+	KRR_HIP(hipMalloc(&acc, sizeof(double)));
+	hipLaunchKernelGGL(k_zero_double, dim3(1), dim3(1), 0, st, acc);
+	const unsigned grid = (unsigned) std::min<int64_t>(1024, (n_pixels + 255) / 256);
+	hipLaunchKernelGGL(k_metric, dim3(grid), dim3(256), 0, st, metric, (const float4 *) img_dev, (const float4 *) ref_dev, n_pixels, acc);
+	hipError_t e = hipGetLastError();
+	double sum	 = 0;
+	if (e == hipSuccess) e = hipStreamSynchronize(st);
+	if (e == hipSuccess) e = hipMemcpy(&sum, acc, sizeof(double), hipMemcpyDeviceToHost);
+	(void) hipFree(acc);
+	if (e != hipSuccess) return krr::fail(KRR_ERR_HIP, "krr_error_metric: %s", hipGetErrorString(e));
+	*out = sum / (double) n_pixels;
+	return KRR_OK;
+}
+
+} // extern "C"

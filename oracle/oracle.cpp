@@ -1,0 +1,682 @@
+// oracle.cpp - CPU restatement of KiRaRay's WavefrontPathTracer (integrator
+// and C API).
+//
+// TEST INFRASTRUCTURE: the checker for kiraray_amd's HIP tracer.  Only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark load it;
+// the product never does.
+//
+// One pixel's path runs through the reference's per-depth stage sequence
+// (src/render/wavefront/integrator.cpp:328-372).  Each stage sees at most one
+// work item of the pixel, as in the queues, so a sequential per-pixel loop
+// reproduces the wavefront's RNG consumption and radiance accumulation order.
+#include "stages.h"
+
+namespace oracle {
+
+template <class R> struct Integrator {
+	const SceneView &sv;
+	const Tracer &tr;
+	const oracle_params &P;
+	bool enableMedium;
+	int nLights;
+	uint64_t closestRays = 0, shadowRays = 0;
+
+	Integrator(const SceneView &s, const Tracer &t, const oracle_params &p)
+		: sv(s), tr(t), P(p), enableMedium(p.enable_medium && s.d->n_media > 0), nLights((int) s.lights.size()) {}
+
+	Hit closest(const Ray<R> &ray, float tmax) const {
+		float o[3] = {(float) ray.o.x, (float) ray.o.y, (float) ray.o.z};
+		float d[3] = {(float) ray.d.x, (float) ray.d.y, (float) ray.d.z};
+		return tr.trace<false>(o, d, tmax);
+	}
+	bool unoccluded(const Ray<R> &ray, float tmax) const {
+		float o[3] = {(float) ray.o.x, (float) ray.o.y, (float) ray.o.z};
+		float d[3] = {(float) ray.d.x, (float) ray.d.y, (float) ray.d.z};
+		return tr.trace<true>(o, d, tmax).prim < 0;
+	}
+	R lightSelectPdf() const { return R(1.f / (float) nLights); }
+
+	// CameraData::getRay (src/core/camera.h:36-58).
+	Ray<R> cameraRay(const krr_camera &c, int px, int py, int W, int H, const float s[5]) const {
+		R p0 = R((float) px) + R(0.5f) + R(s[0]);
+		R p1 = R((float) py) + R(0.5f) + R(s[1]);
+		R ndc0 = (R(2) * p0) / R((float) W) + R(-1.f);
+		R ndc1 = (R(2) * p1) / R((float) H) + R(-1.f);
+		R fov  = m_atan2(R(c.film_size[1]) * R(0.5f), R(c.focal_length));
+		V3<R> focalDirection = normalize(V3<R>(m_tan(fov) * R(c.aspect_ratio) * ndc0, m_tan(fov) * ndc1, R(-1)));
+		V3<R> o, d;
+		if (c.lens_radius > 1e-5f) {
+			R ax, ay;
+			uniformSampleDisk(R(s[2]), R(s[3]), ax, ay);
+			o = V3<R>(R(c.lens_radius) * ax, R(c.lens_radius) * ay, R(0));
+			d = normalize(focalDirection * R(c.focal_distance) - o);
+		} else {
+			o = V3<R>();
+			d = focalDirection;
+		}
+		const float *M = c.transform;
+		Ray<R> r;
+		for (int i = 0; i < 3; i++) {
+			r.o[i] = R(M[4 * i]) * o.x + R(M[4 * i + 1]) * o.y + R(M[4 * i + 2]) * o.z + R(M[4 * i + 3]);
+			r.d[i] = R(M[4 * i]) * d.x + R(M[4 * i + 1]) * d.y + R(M[4 * i + 2]) * d.z;
+		}
+		r.medium = c.medium;
+		return r;
+	}
+
+	// handleHit (integrator.cpp:183-195).
+	void handleHit(const HitLightItem<R> &w, V3<R> &L) const {
+		const LightRec &Lr = sv.lights[w.light];
+		V3<R> Le		   = areaL<R>(Lr, w.n, w.wo) * w.thp;
+		if (P.nee && w.depth && !(w.bsdfType & BSDF_DELTA)) {
+			R lightPdf = areaPdfLi<R>(sv, Lr, w.p, w.n, w.ctx) * lightSelectPdf();
+			Le /= (w.pl * lightPdf + w.pu).mean();
+		} else Le /= w.pu.mean();
+		L += Le;
+	}
+
+	// handleMiss (integrator.cpp:197-213).
+	void handleMiss(const RayItem<R> &w, V3<R> &L) const {
+		V3<R> Lm;
+		for (const LightRec &Lr : sv.lights) {
+			if (Lr.type != 1) continue;
+			V3<R> Li = R(Lr.scale) * V3<R>(R(Lr.Le[0]), R(Lr.Le[1]), R(Lr.Le[2]));
+			if (P.nee && w.depth && !(w.bsdfType & BSDF_DELTA)) {
+				R lightPdf = kInv4Pi<R>() * lightSelectPdf();
+				Lm += Li / (w.pu + w.pl * lightPdf).mean();
+			} else Lm += Li / w.pu.mean();
+		}
+		L += w.thp * Lm;
+	}
+
+	// One pixel: beginFrame (integrator.cpp:310-326) + render (328-372).
+	void renderPixel(int px, int py, int W, int H, uint32_t sampleIndex, const krr_camera &cam, V3<R> &pixel) {
+		const int pixelId = py * W + px;
+		Pcg sampler;
+		sampler.setSeed(interleave32((uint32_t) px, (uint32_t) py), (uint64_t) sampleIndex);
+		sampler.advance((int64_t) (256 * pixelId));
+		const float ul		= sampler.get1D();
+		const float lambda0 = (1.f - ul) * 360.f + ul * 830.f;		  // SampledWavelengths::sampleUniform
+		const int channel	= clampv((int) ((lambda0 - 360.f) / 470.f * 3), 0, 2); // mainIndex()
+		V3<R> L;
+		pixel = V3<R>();
+		for (int s = 0; s < P.spp; s++) {
+			if (!P.legacy_spp_accumulate) L = V3<R>();
+			// generateCameraRays (integrator.cpp:271-284): {get2D(), get2D(), get1D()}
+			float cs[5];
+			for (int i = 0; i < 5; i++) cs[i] = sampler.get1D();
+			RayItem<R> cur;
+			cur.ray = cameraRay(cam, px, py, W, H, cs);
+			cur.thp = cur.pu = cur.pl = V3<R>(R(1));
+			bool have				  = true;
+			for (int depth = 0; have; depth++) {
+				bool hasNext = false, hasShadow = false, hasHitLight = false, hasMiss = false, hasScatter = false;
+				bool hasMediumSample = false, hasMediumScatter = false;
+				RayItem<R> next, miss;
+				ShadowItem<R> shadow;
+				HitLightItem<R> hitLight;
+				ScatterItem<R> scatter;
+				MediumSampleItem<R> ms;
+				MediumScatterItem<R> msc;
+
+				// traceClosest: raygen + closest-hit / miss (device.cu:43-81).
+				closestRays++;
+				Hit h = closest(cur.ray, INFINITY);
+				if (h.prim >= 0) {
+					SurfInt<R> intr = prepareSurfaceInteraction<R>(sv, h, cur.ray);
+					if (enableMedium && cur.ray.medium >= 0) {
+						hasMediumSample = true;
+						ms				= {cur.ray, cur.ctx, cur.thp, cur.pu, cur.pl, R(h.t), true, intr, cur.bsdfType, cur.depth};
+					} else if (intr.material < 0) {
+						hasNext = true;
+						next	= cur;
+						next.ray = intr.spawnRayTowards(cur.ray.d);
+					} else {
+						if (intr.light >= 0) {
+							hasHitLight = true;
+							hitLight	= {intr.light, cur.ctx, intr.p, intr.wo, intr.n, cur.thp, cur.pu, cur.pl, cur.bsdfType, cur.depth};
+						}
+						if (cur.thp.any()) {
+							hasScatter = true;
+							scatter	   = {cur.thp, cur.pu, intr, cur.depth};
+						}
+					}
+				} else {
+					if (enableMedium && cur.ray.medium >= 0) {
+						hasMediumSample = true;
+						ms				= {cur.ray, cur.ctx, cur.thp, cur.pu, cur.pl, R(INFINITY), false, SurfInt<R>(), cur.bsdfType, cur.depth};
+					} else {
+						hasMiss = true;
+						miss	= cur;
+					}
+				}
+
+				// sampleMediumInteraction (medium.cpp:13-103).
+				if (hasMediumSample) {
+					const MediumSampleItem<R> &w = ms;
+					V3<R> Lm;
+					Ray<R> ray	   = w.ray;
+					V3<R> thp	   = w.thp, pu = w.pu, pl = w.pl;
+					bool scattered = false;
+					V3<R> T_maj = sampleT_maj<R>(sv, ray, w.tMax, sampler, channel,
+						[&](const V3<R> &p, const MediumProps<R> &mp, const V3<R> &sigma_maj, const V3<R> &Tm) -> bool {
+							if (w.depth < P.max_depth && mp.Le.any()) {
+								R pr	 = sigma_maj[channel] * Tm[channel];
+								V3<R> pe = pu * sigma_maj * Tm / pr;
+								if (pe.any()) Lm += thp * mp.sigma_a * Tm * mp.Le / (pr * pe.mean());
+							}
+							R pAbsorb  = mp.sigma_a[channel] / sigma_maj[channel];
+							R pScatter = mp.sigma_s[channel] / sigma_maj[channel];
+							R pNull	   = m_max(R(0), R(1) - pAbsorb - pScatter);
+							R wts[3]   = {pAbsorb, pScatter, pNull};
+							int mode   = sampleDiscrete3<R>(wts, R(sampler.get1D()), nullptr);
+							if (mode == 0) {
+								thp = V3<R>();
+								return false;
+							} else if (mode == 1) {
+								R pr = Tm[channel] * mp.sigma_s[channel];
+								thp *= Tm * mp.sigma_s / pr;
+								pu *= Tm * mp.sigma_s / pr;
+								if (thp.any() && pu.any()) {
+									hasMediumScatter = true;
+									msc				 = {p, thp, pu, -ray.d, ray.medium, mp.g, w.depth};
+								}
+								scattered = true;
+								return false;
+							} else {
+								V3<R> sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, R(0));
+								R pr		  = Tm[channel] * sigma_n[channel];
+								thp *= Tm * sigma_n / pr;
+								if (pr == 0) thp = V3<R>();
+								pu *= Tm * sigma_n / pr;
+								pl *= Tm * sigma_maj / pr;
+								return thp.any() && pu.any();
+							}
+						});
+					if (Lm.any()) L += Lm;
+					if (!scattered && thp.any()) {
+						thp *= T_maj / T_maj[channel];
+						pu *= T_maj / T_maj[channel];
+						pl *= T_maj / T_maj[channel];
+					}
+					if (!(scattered || !thp.any() || !pu.any() || w.depth == P.max_depth)) {
+						if (!w.hasIntr) {
+							hasMiss = true;
+							miss	= {ray, w.ctx, thp, pu, pl, w.bsdfType, w.depth};
+						} else if (w.intr.material < 0) {
+							hasNext = true;
+							next	= {w.intr.spawnRayTowards(ray.d), w.ctx, thp, pu, pl, w.bsdfType, w.depth};
+						} else {
+							if (w.intr.light >= 0) {
+								hasHitLight = true;
+								hitLight	= {w.intr.light, w.ctx, w.intr.p, w.intr.wo, w.intr.n, thp, pu, pl, w.bsdfType, w.depth};
+							}
+							hasScatter = true;
+							scatter	   = {thp, pu, w.intr, w.depth};
+						}
+					}
+				}
+
+				if (hasHitLight) handleHit(hitLight, L);
+				if (hasMiss) handleMiss(miss, L);
+				if (depth == P.max_depth) break;
+
+				// sampleMediumScattering (medium.cpp:105-153).
+				if (hasMediumScatter) {
+					const MediumScatterItem<R> &w = msc;
+					Ctx<R> ctx{w.p, V3<R>()};
+					if (P.nee && nLights > 0) {
+						float ulight = sampler.get1D();
+						int li		 = (int) (uint32_t) (ulight * (float) nLights);
+						R u0 = R(sampler.get1D()), u1 = R(sampler.get1D());
+						LightSample<R> ls = sampleLi<R>(sv, sv.lights[li], u0, u1, ctx);
+						// Interaction(p, time, medium).spawnRayTo(ls.intr): n = 0, no offset.
+						V3<R> to  = offsetRayOrigin(ls.intr.p, ls.intr.n, w.p - ls.intr.p);
+						V3<R> p_o = offsetRayOrigin(w.p, V3<R>(), to - w.p);
+						Ray<R> shadowRay{p_o, to - p_o, w.medium};
+						V3<R> wi	  = normalize(shadowRay.d);
+						V3<R> thp	  = w.thp * hgP<R>(w.g, w.wo, wi);
+						R lightPdf	  = lightSelectPdf() * ls.pdf;
+						R phasePdf	  = hgP<R>(w.g, w.wo, wi);
+						V3<R> Ld	  = thp * ls.L;
+						if (Ld.any() && lightPdf > 0) {
+							hasShadow = true;
+							shadow	  = {shadowRay, R(1), Ld, w.pu * phasePdf, w.pu * lightPdf};
+						}
+					}
+					R u0 = R(sampler.get1D()), u1 = R(sampler.get1D());
+					R ppdf;
+					V3<R> wi  = hgSample<R>(w.g, w.wo, u0, u1, &ppdf);
+					R pval	  = ppdf;
+					V3<R> thp = w.thp * pval / ppdf;
+					R rrProb  = (thp / w.pu.mean()).maxCoeff();
+					bool alive = true;
+					if (w.depth >= 1 && rrProb < 1) {
+						if (sampler.get1D() >= rrProb) alive = false;
+						else thp /= rrProb;
+					}
+					if (alive && !thp.isZero() && !thp.hasNaN()) {
+						hasNext = true;
+						next	= {Ray<R>{w.p, wi, w.medium}, ctx, thp, w.pu, w.pu / ppdf, BSDF_SMOOTH, w.depth + 1};
+					}
+				}
+
+				// generateScatterRays (integrator.cpp:215-269).
+				if (hasScatter) {
+					ScatterItem<R> &w = scatter;
+					bool alive		  = true;
+					if (w.depth >= P.rr_min_depth) {
+						if (sampler.get1D() >= P.rr) alive = false;
+						else w.thp /= R(P.rr);
+					}
+					if (alive) {
+						const SurfInt<R> &intr = w.intr;
+						V3<R> woLocal		   = intr.toLocal(intr.wo);
+						int bsdfType		   = intr.sd.bsdfType();
+						BSDF<R> bsdf(intr.sd);
+						if (P.nee && (bsdfType & BSDF_SMOOTH) && nLights > 0) {
+							float ulight = sampler.get1D();
+							int li		 = (int) (uint32_t) (ulight * (float) nLights);
+							R u0 = R(sampler.get1D()), u1 = R(sampler.get1D());
+							LightSample<R> ls = sampleLi<R>(sv, sv.lights[li], u0, u1, Ctx<R>{intr.p, intr.n});
+							Ray<R> shadowRay  = intr.spawnRayTo(ls.intr);
+							V3<R> wiWorld	  = normalize(shadowRay.d);
+							V3<R> wiLocal	  = intr.toLocal(wiWorld);
+							R lightPdf		  = lightSelectPdf() * ls.pdf;
+							V3<R> bsdfVal	  = bsdf.f(woLocal, wiLocal);
+							R bsdfPdf		  = bsdf.pdf(woLocal, wiLocal);
+							if (lightPdf > 0 && bsdfVal.any()) {
+								V3<R> Ld = ls.L * w.thp * bsdfVal * m_abs(wiLocal.z);
+								if (Ld.any()) {
+									hasShadow = true;
+									shadow	  = {shadowRay, R(1), Ld, w.pu * bsdfPdf, w.pu * lightPdf};
+								}
+							}
+						}
+						BSDFSample<R> sample = bsdf.sample(woLocal, sampler);
+						if (sample.pdf != 0 && sample.f.any()) {
+							V3<R> wiWorld = intr.toWorld(sample.wi);
+							RayItem<R> r;
+							r.bsdfType = sample.flags;
+							r.pu	   = w.pu;
+							r.pl	   = w.pu / sample.pdf;
+							r.ray	   = intr.spawnRayTowards(wiWorld);
+							r.ctx	   = {intr.p, intr.n};
+							r.depth	   = w.depth + 1;
+							r.thp	   = w.thp * sample.f * m_abs(sample.wi.z) / sample.pdf;
+							if (r.thp.any()) {
+								hasNext = true;
+								next	= r;
+							}
+						}
+					}
+				}
+
+				// traceShadow (device.cu:83-128).
+				if (P.nee && hasShadow) {
+					shadowRays++;
+					if (!enableMedium) {
+						if (unoccluded(shadow.ray, float(shadow.tMax))) L += shadow.Ld / (shadow.pl + shadow.pu).mean();
+					} else {
+						traceTransmittance(shadow, sampler, channel, L);
+					}
+				}
+				have = hasNext;
+				if (have) cur = next;
+			}
+			pixel += L;
+		}
+	}
+
+	// traceTransmittance (wavefront.h:209-268) with the ShadowTr closest-hit
+	// program (device.cu:102-128): the hit interaction takes the medium of the
+	// original shadow ray.
+	void traceTransmittance(const ShadowItem<R> &sr, Pcg &sampler, int channel, V3<R> &L) {
+		Ray<R> ray	 = sr.ray;
+		R tMax		 = sr.tMax;
+		V3<R> pLight = ray.at(tMax);
+		V3<R> T_ray(R(1)), pu(R(1)), pl(R(1));
+		SurfInt<R> intr;
+		const R zeroEps = R(kRayEps * 2);
+		while (!(m_abs(ray.d.x) <= zeroEps && m_abs(ray.d.y) <= zeroEps && m_abs(ray.d.z) <= zeroEps)) {
+			Hit h		 = closest(ray, float(tMax));
+			bool visible = h.prim < 0;
+			if (!visible) {
+				Ray<R> orig = ray;
+				orig.medium = sr.ray.medium;
+				intr		= prepareSurfaceInteraction<R>(sv, h, orig);
+			}
+			if (!visible && intr.material >= 0) {
+				T_ray = V3<R>();
+				break;
+			}
+			if (ray.medium >= 0) {
+				R tEnd		= visible ? tMax : norm(intr.p - ray.o) / norm(ray.d);
+				V3<R> T_maj = sampleT_maj<R>(sv, ray, tEnd, sampler, channel,
+					[&](const V3<R> &, const MediumProps<R> &mp, const V3<R> &sigma_maj, const V3<R> &Tm) -> bool {
+						V3<R> sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, R(0));
+						R pr		  = Tm[channel] * sigma_maj[channel];
+						T_ray *= Tm * sigma_n / pr;
+						pl *= Tm * sigma_maj / pr;
+						pu *= Tm * sigma_n / pr;
+						V3<R> Tr = T_ray / (pu + pl).mean();
+						if (Tr.maxCoeff() < R(0.05f)) {
+							if (sampler.get1D() < 0.75f) T_ray = V3<R>();
+							else T_ray /= R(0.25f);
+						}
+						return T_ray.any();
+					});
+				T_ray *= T_maj / T_maj[channel];
+				pu *= T_maj / T_maj[channel];
+				pl *= T_maj / T_maj[channel];
+			}
+			if (visible || T_ray.isZero()) break;
+			ray = intr.spawnRayTo(pLight);
+		}
+		if (T_ray.any()) L += sr.Ld * T_ray / (sr.pu * pu + sr.pl * pl).mean();
+	}
+};
+
+static void sceneRadius(const krr_scene_desc *d, SceneView &sv) {
+	if (d->n_vertices <= 0) return;
+	float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+	for (int64_t i = 0; i < d->n_vertices; i++)
+		for (int a = 0; a < 3; a++) {
+			lo[a] = std::min(lo[a], d->positions[3 * i + a]);
+			hi[a] = std::max(hi[a], d->positions[3 * i + a]);
+		}
+	float e0 = hi[0] - lo[0], e1 = hi[1] - lo[1], e2 = hi[2] - lo[2];
+	sv.sceneRadius = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+}
+
+template <class R>
+static void renderT(const SceneView &sv, const Tracer &tr, const krr_camera &cam, const oracle_params &p, uint64_t frameIndex,
+					int W, int H, float *film, oracle_stats *st) {
+	const uint32_t sampleIndex = (uint32_t) (frameIndex * (uint64_t) p.spp);
+	int rowEnd				   = p.row_end > 0 ? std::min(p.row_end, H) : H;
+	int rowStep				   = p.row_step > 0 ? p.row_step : 1;
+	int pixStep				   = p.pixel_step > 0 ? p.pixel_step : 1;
+	std::vector<int> rows;
+	for (int y = std::max(0, p.row_begin); y < rowEnd; y += rowStep) rows.push_back(y);
+	uint64_t cr = 0, sr = 0, np = 0;
+	int threads = 1;
+#pragma omp parallel reduction(+ : cr, sr, np)
+	{
+#ifdef _OPENMP
+#pragma omp single
+		threads = omp_get_num_threads();
+#endif
+		Integrator<R> integ(sv, tr, p);
+#pragma omp for schedule(dynamic, 1)
+		for (int ri = 0; ri < (int) rows.size(); ri++) {
+			int y = rows[ri];
+			for (int x = 0; x < W; x += pixStep) {
+				V3<R> pixel;
+				integ.renderPixel(x, y, W, H, sampleIndex, cam, pixel);
+				V3<R> v = pixel / R((float) p.spp);
+				if (p.enable_clamp) {
+					for (int c = 0; c < 3; c++) v[c] = clampv(v[c], R(0), R(p.clamp_max));
+				}
+				float *o = film + 4 * ((size_t) (H - 1 - y) * W + x);
+				o[0]	 = (float) v.x;
+				o[1]	 = (float) v.y;
+				o[2]	 = (float) v.z;
+				o[3]	 = 1.f;
+				np += (uint64_t) p.spp;
+			}
+		}
+		cr += integ.closestRays;
+		sr += integ.shadowRays;
+	}
+	if (st) {
+		st->closest_rays = cr;
+		st->shadow_rays	 = sr;
+		st->paths		 = np;
+		st->threads		 = threads;
+	}
+}
+
+} // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+void oracle_default_params(oracle_params *p) {
+	std::memset(p, 0, sizeof(*p));
+	p->nee			 = 1;
+	p->enable_medium = 1;
+	p->max_depth	 = 10;
+	p->rr			 = 0.8f;
+	p->clamp_max	 = 1e3f;
+	p->spp			 = 1;
+	p->precision	 = 32;
+	p->row_step		 = 1;
+	p->pixel_step	 = 1;
+}
+
+int oracle_render(const krr_scene_desc *scene, const krr_camera *camera, const oracle_params *p, uint64_t frame_index,
+				  int32_t width, int32_t height, float *film, oracle_stats *stats) {
+	if (!scene || !camera || !p || !film || width <= 0 || height <= 0) return KRR_ERR_INVALID_ARG;
+#ifdef _OPENMP
+	int prevThreads = omp_get_max_threads();
+	if (p->n_threads > 0) omp_set_num_threads(p->n_threads);
+#endif
+	auto t0 = std::chrono::steady_clock::now();
+	SceneView sv;
+	buildSceneView(scene, sv);
+	sceneRadius(scene, sv);
+	Bvh bvh;
+	Tracer tr;
+	tr.sv = &sv;
+	if (scene->n_triangles > 64) {
+		buildBvh(scene, bvh);
+		tr.bvh = &bvh;
+	}
+	auto t1 = std::chrono::steady_clock::now();
+	if (p->precision == 64) renderT<double>(sv, tr, *camera, *p, frame_index, width, height, film, stats);
+	else renderT<float>(sv, tr, *camera, *p, frame_index, width, height, film, stats);
+	auto t2 = std::chrono::steady_clock::now();
+	if (stats) stats->seconds = std::chrono::duration<double>(t2 - t1).count();
+	(void) t0;
+#ifdef _OPENMP
+	omp_set_num_threads(prevThreads);
+#endif
+	return KRR_OK;
+}
+
+void oracle_pcg32_sequence(uint64_t initstate, uint64_t initseq, int32_t n, uint32_t *out) {
+	Pcg g;
+	g.setSeed(initstate, initseq);
+	for (int i = 0; i < n; i++) out[i] = g.nextUint();
+}
+
+void oracle_pcg32_advance(uint64_t initstate, uint64_t initseq, int64_t delta, uint64_t *state_out, uint64_t *inc_out) {
+	Pcg g;
+	g.setSeed(initstate, initseq);
+	g.advance(delta);
+	*state_out = g.state;
+	*inc_out   = g.inc;
+}
+
+void oracle_pixel_samples(int32_t x, int32_t y, uint32_t sample_index, int64_t adv, int32_t n, float *out) {
+	Pcg g;
+	g.setSeed(interleave32((uint32_t) x, (uint32_t) y), (uint64_t) sample_index);
+	g.advance(adv);
+	for (int i = 0; i < n; i++) out[i] = g.get1D();
+}
+
+uint32_t oracle_interleave_32bit(uint32_t x, uint32_t y) { return interleave32(x, y); }
+
+static int traceImpl(const krr_scene_desc *scene, const float *rays, int64_t n, krr_hit *hits, int32_t *vis, int32_t use_bvh) {
+	if (!scene || !rays || n < 0) return KRR_ERR_INVALID_ARG;
+	SceneView sv;
+	buildSceneView(scene, sv);
+	Bvh bvh;
+	Tracer tr;
+	tr.sv = &sv;
+	if (use_bvh) {
+		buildBvh(scene, bvh);
+		tr.bvh = &bvh;
+	}
+#pragma omp parallel for schedule(dynamic, 256)
+	for (int64_t i = 0; i < n; i++) {
+		const float *r = rays + 8 * i;
+		float o[3] = {r[0], r[1], r[2]}, d[3] = {r[4], r[5], r[6]};
+		if (hits) {
+			Hit h	   = tr.trace<false>(o, d, r[3]);
+			hits[i].t	 = h.prim >= 0 ? h.t : INFINITY;
+			hits[i].prim = h.prim;
+			hits[i].b1	 = h.b1;
+			hits[i].b2	 = h.b2;
+		} else {
+			vis[i] = tr.trace<true>(o, d, r[3]).prim < 0 ? 1 : 0;
+		}
+	}
+	return KRR_OK;
+}
+
+int oracle_trace_closest(const krr_scene_desc *scene, const float *rays, int64_t n, krr_hit *hits, int32_t use_bvh) {
+	if (!hits) return KRR_ERR_INVALID_ARG;
+	return traceImpl(scene, rays, n, hits, nullptr, use_bvh);
+}
+
+int oracle_trace_any(const krr_scene_desc *scene, const float *rays, int64_t n, int32_t *visible, int32_t use_bvh) {
+	if (!visible) return KRR_ERR_INVALID_ARG;
+	return traceImpl(scene, rays, n, nullptr, visible, use_bvh);
+}
+
+void oracle_camera_ray(const krr_camera *camera, int32_t x, int32_t y, int32_t width, int32_t height, const float sample[5],
+					   float origin[3], float dir[3]) {
+	krr_scene_desc empty;
+	std::memset(&empty, 0, sizeof(empty));
+	SceneView sv;
+	sv.d = &empty;
+	Tracer tr;
+	tr.sv = &sv;
+	oracle_params p;
+	oracle_default_params(&p);
+	Integrator<float> integ(sv, tr, p);
+	Ray<float> r = integ.cameraRay(*camera, x, y, width, height, sample);
+	for (int i = 0; i < 3; i++) {
+		origin[i] = r.o[i];
+		dir[i]	  = r.d[i];
+	}
+}
+
+static ShadingData<float> shadingOf(const krr_material *m) {
+	ShadingData<float> sd;
+	sd.type					= m->type;
+	sd.IoR					= m->ior;
+	sd.diffuse				= V3f(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+	sd.specular				= V3f(m->specular[0], m->specular[1], m->specular[2]);
+	sd.specularTransmission = m->specular_transmission;
+	sd.roughness			= 1.f - m->specular[3];
+	sd.anisotropic			= m->anisotropic;
+	return sd;
+}
+
+void oracle_bsdf_eval(const krr_material *m, const float wo[3], const float wi[3], float f[3], float *pdf) {
+	BSDF<float> b(shadingOf(m));
+	V3f o(wo[0], wo[1], wo[2]), i(wi[0], wi[1], wi[2]);
+	V3f fv = b.f(o, i);
+	f[0] = fv.x; f[1] = fv.y; f[2] = fv.z;
+	*pdf = b.pdf(o, i);
+}
+
+// BSDF sampling driven by given uniforms (in the order the BSDF draws them).
+int32_t oracle_bsdf_sample(const krr_material *m, const float wo[3], const float u[3], float f[3], float wi[3], float *pdf) {
+	struct FixedU {
+		const float *u;
+		int i;
+		float get1D() { return u[i < 3 ? i++ : 2]; }
+	} us{u, 0};
+	BSDF<float> b(shadingOf(m));
+	BSDFSample<float> s = b.sample(V3f(wo[0], wo[1], wo[2]), us);
+	f[0]  = s.f.x; f[1] = s.f.y; f[2] = s.f.z;
+	wi[0] = s.wi.x; wi[1] = s.wi.y; wi[2] = s.wi.z;
+	*pdf  = s.pdf;
+	return s.flags;
+}
+
+int32_t oracle_bsdf_type(const krr_material *m) { return shadingOf(m).bsdfType(); }
+
+float oracle_fresnel_dielectric(float cos_theta_i, float eta) { return FrDielectric<float>(cos_theta_i, eta); }
+
+void oracle_fresnel_complex(float cos_theta_i, const float eta[3], const float k[3], float out[3]) {
+	V3f r = FrComplex<float>(cos_theta_i, V3f(eta[0], eta[1], eta[2]), V3f(k[0], k[1], k[2]));
+	out[0] = r.x; out[1] = r.y; out[2] = r.z;
+}
+
+float oracle_ggx_D(float ax, float ay, const float wh[3]) { return GGX<float>(ax, ay).D(V3f(wh[0], wh[1], wh[2])); }
+float oracle_ggx_G1(float ax, float ay, const float w[3]) { return GGX<float>(ax, ay).G1(V3f(w[0], w[1], w[2])); }
+float oracle_ggx_pdf(float ax, float ay, const float wo[3], const float wh[3]) {
+	return GGX<float>(ax, ay).Pdf(V3f(wo[0], wo[1], wo[2]), V3f(wh[0], wh[1], wh[2]));
+}
+void oracle_ggx_sample(float ax, float ay, const float wo[3], const float u[2], float wh[3]) {
+	V3f h = GGX<float>(ax, ay).Sample(V3f(wo[0], wo[1], wo[2]), u[0], u[1]);
+	wh[0] = h.x; wh[1] = h.y; wh[2] = h.z;
+}
+
+float oracle_hg_p(float g, const float wo[3], const float wi[3]) {
+	return hgP<float>(g, V3f(wo[0], wo[1], wo[2]), V3f(wi[0], wi[1], wi[2]));
+}
+void oracle_hg_sample(float g, const float wo[3], const float u[2], float wi[3], float *pdf) {
+	V3f w = hgSample<float>(g, V3f(wo[0], wo[1], wo[2]), u[0], u[1], pdf);
+	wi[0] = w.x; wi[1] = w.y; wi[2] = w.z;
+}
+
+int32_t oracle_sample_discrete3(const float w[3], float u, float *pmf) { return sampleDiscrete3<float>(w, u, pmf); }
+
+int oracle_majorant_grid(const krr_medium *m, float *out) {
+	if (!m || m->type != KRR_MEDIUM_GRID || !out) return KRR_ERR_INVALID_ARG;
+	Medium md;
+	md.m = m;
+	buildMajorant(md);
+	std::memcpy(out, md.majorant.voxels.data(), sizeof(float) * 64 * 64 * 64);
+	return KRR_OK;
+}
+
+float oracle_density(const krr_medium *m, const float p[3]) {
+	Medium md;
+	md.m = m;
+	for (int a = 0; a < 3; a++) md.inv_voxel[a] = (float) m->res[a] / (m->bounds_max[a] - m->bounds_min[a]);
+	return gridDensity(md, p);
+}
+
+float oracle_dm_sin(float x) { return detmath::sinf(x); }
+float oracle_dm_cos(float x) { return detmath::cosf(x); }
+float oracle_dm_log(float x) { return detmath::logf(x); }
+float oracle_dm_exp(float x) { return detmath::expf(x); }
+float oracle_dm_tan(float x) { return detmath::tanf(x); }
+float oracle_dm_atan2(float y, float x) { return detmath::atan2f(y, x); }
+
+// metrics.cu:63-90 per pixel, clamp at 100 (:117-119), FP64 mean.
+double oracle_error_metric(int32_t metric, const float *img, const float *ref, int64_t n) {
+	double sum = 0;
+	for (int64_t i = 0; i < n; i++) {
+		const float *y = img + 4 * i, *r = ref + 4 * i;
+		bool bad = false;
+		for (int c = 0; c < 3; c++)
+			if (std::isinf(r[c]) || std::isnan(r[c])) bad = true;
+		float e = 0;
+		if (!bad) {
+			float acc[3];
+			for (int c = 0; c < 3; c++) {
+				float d = std::fabs(y[c] - r[c]);
+				switch (metric) {
+					case KRR_METRIC_MSE: acc[c] = d * d; break;
+					case KRR_METRIC_MAPE: acc[c] = d / (r[c] + 0.f); break;
+					case KRR_METRIC_SMAPE: acc[c] = d / (r[c] + y[c] + 0.f); break;
+					default: acc[c] = r[c] == 0.f ? 0.f : (d / r[c]) * (d / r[c]); break;
+				}
+			}
+			e = (acc[0] + acc[1] + acc[2]) / 3.f;
+		}
+		e = std::fmin(e, 100.f);
+		sum += (double) e;
+	}
+	return n > 0 ? sum / (double) n : 0.0;
+}
+
+} // extern "C"

@@ -1,0 +1,505 @@
+// stages.h - rays, interactions, lights, media and work items of the CPU checker.
+//
+// TEST INFRASTRUCTURE: the checker for kiraray_amd's HIP tracer.  Only the
+// tests, the smoke check and the CPU-baseline leg of the benchmark load it;
+// the product never does.
+//
+// One pixel's path runs through the reference's per-depth stage sequence
+// (src/render/wavefront/integrator.cpp:328-372): closest trace with the
+// closest-hit/miss routing of device.cu:43-81, medium sampling
+// (medium.cpp:13-103), handleHit (integrator.cpp:183-195), handleMiss
+// (:197-213), the maxDepth break, medium scattering (medium.cpp:105-153),
+// generateScatterRays (integrator.cpp:215-269) and the shadow trace
+// (device.cu:83-128, wavefront.h:209-268).  Each stage sees at most one work
+// item of the pixel, exactly as in the queues.
+#pragma once
+#include "core.h"
+
+namespace oracle {
+
+template <class R> struct Ray {
+	V3<R> o, d;
+	int medium = -1;
+	V3<R> at(R t) const { return o + d * t; }
+};
+
+template <class R> struct Ctx { V3<R> p, n; }; // LightSampleContext
+
+struct Hit {
+	float t	 = INFINITY;
+	int prim = -1;
+	float b1 = 0, b2 = 0;
+};
+
+// ----------------------------------------------------------------------------
+// Ray casting over the scene (closest: device.cu:76-81 with tmin 0; any:
+// device.cu:83-100 where the any-hit program skips null-material triangles).
+// ----------------------------------------------------------------------------
+struct Tracer {
+	const SceneView *sv = nullptr;
+	const Bvh *bvh		= nullptr; // null: brute force
+
+	void tri(int prim, const float *&v0, const float *&v1, const float *&v2) const {
+		const krr_scene_desc *d = sv->d;
+		v0 = d->positions + 3 * (size_t) d->indices[3 * (size_t) prim];
+		v1 = d->positions + 3 * (size_t) d->indices[3 * (size_t) prim + 1];
+		v2 = d->positions + 3 * (size_t) d->indices[3 * (size_t) prim + 2];
+	}
+	bool isNull(int prim) const { return sv->d->meshes[sv->triMesh[prim]].material < 0; }
+
+	template <bool ANY> Hit trace(const float o[3], const float dir[3], float tmax) const {
+		Hit h;
+		h.t			= tmax;
+		RayPre rp	= preprocess(o, dir);
+		auto consider = [&](int prim) -> bool {
+			if (ANY && isNull(prim)) return false;
+			const float *v0, *v1, *v2;
+			tri(prim, v0, v1, v2);
+			float t, b1, b2;
+			if (!intersectTri(rp, v0, v1, v2, t, b1, b2)) return false;
+			if (t < h.t || (t == h.t && h.prim >= 0 && prim < h.prim)) {
+				h.t	   = t;
+				h.prim = prim;
+				h.b1   = b1;
+				h.b2   = b2;
+				return true;
+			}
+			return false;
+		};
+		if (!bvh) {
+			for (int64_t p = 0; p < sv->d->n_triangles; p++)
+				if (consider((int) p) && ANY) return h;
+			return h;
+		}
+		double od[3] = {o[0], o[1], o[2]}, inv[3];
+		int sgn[3];
+		for (int a = 0; a < 3; a++) {
+			inv[a] = 1.0 / (double) dir[a];
+			sgn[a] = std::signbit(dir[a]) ? 1 : 0;
+		}
+		int stack[128];
+		int sp		= 0;
+		stack[sp++] = 0;
+		while (sp) {
+			const BvhNode &n = bvh->nodes[stack[--sp]];
+			double tn;
+			if (!boxHit(n, od, inv, sgn, (double) h.t, tn)) continue;
+			if (n.left < 0) {
+				for (int i = n.first; i < n.first + n.count; i++)
+					if (consider(bvh->order[i]) && ANY) return h;
+				continue;
+			}
+			double tl, tr;
+			bool hl = boxHit(bvh->nodes[n.left], od, inv, sgn, (double) h.t, tl);
+			bool hr = boxHit(bvh->nodes[n.right], od, inv, sgn, (double) h.t, tr);
+			if (hl && hr) {
+				if (tl <= tr) { stack[sp++] = n.right; stack[sp++] = n.left; }
+				else { stack[sp++] = n.left; stack[sp++] = n.right; }
+			} else if (hl) stack[sp++] = n.left;
+			else if (hr) stack[sp++] = n.right;
+		}
+		return h;
+	}
+};
+
+// ----------------------------------------------------------------------------
+// Interactions (src/core/raytracing.h:106-180, src/render/shared.h:303-324).
+// ----------------------------------------------------------------------------
+template <class R> V3<R> offsetRayOrigin(const V3<R> &p, const V3<R> &n, const V3<R> &w) {
+	V3<R> offset = n * R(kRayEps);
+	if (dot(n, w) < R(0)) offset = -offset;
+	return p + offset;
+}
+
+template <class R> struct Interaction { // light-sample / medium interaction
+	V3<R> p, n, wo;
+	int medium = -1;
+};
+
+template <class R> struct SurfInt {
+	V3<R> p, wo, n, tangent, bitangent;
+	int material = -1, light = -1, medium = -1;
+	bool hasInterface = false;
+	int inside = -1, outside = -1;
+	ShadingData<R> sd;
+
+	V3<R> toLocal(const V3<R> &v) const { return {dot(tangent, v), dot(bitangent, v), dot(n, v)}; }
+	V3<R> toWorld(const V3<R> &v) const { return tangent * v.x + bitangent * v.y + n * v.z; }
+	int getMedium(const V3<R> &w) const {
+		if (hasInterface) return dot(w, n) > R(0) ? outside : inside;
+		return medium;
+	}
+	Ray<R> spawnRayTowards(const V3<R> &dir) const { return {offsetRayOrigin(p, n, dir), dir, getMedium(dir)}; }
+	Ray<R> spawnRayTo(const V3<R> &to) const {
+		V3<R> p_o = offsetRayOrigin(p, n, to - p);
+		V3<R> d	  = to - p_o;
+		return {p_o, d, getMedium(d)};
+	}
+	Ray<R> spawnRayTo(const Interaction<R> &it) const {
+		V3<R> to = offsetRayOrigin(it.p, it.n, p - it.p);
+		return spawnRayTo(to);
+	}
+};
+
+// getHitInfo + prepareSurfaceInteraction (src/render/shading.h:78-87,115-226)
+// for an identity instance transform (p unchanged, frame vectors normalised
+// once more as transposedInverse()*v).normalized() does).
+template <class R> SurfInt<R> prepareSurfaceInteraction(const SceneView &sv, const Hit &h, const Ray<R> &ray) {
+	const krr_scene_desc *d = sv.d;
+	SurfInt<R> it;
+	const int prim	  = h.prim;
+	const krr_mesh &m = d->meshes[sv.triMesh[prim]];
+	R bb1 = R(h.b1), bb2 = R(h.b2);
+	R b0 = R(1) - bb1 - bb2;
+	int v0 = d->indices[3 * (size_t) prim], v1 = d->indices[3 * (size_t) prim + 1], v2 = d->indices[3 * (size_t) prim + 2];
+	it.medium = ray.medium;
+	it.wo	  = normalize(-normalize(ray.d));
+	V3<R> p0 = cvt<R>(vtx(d, v0)), p1 = cvt<R>(vtx(d, v1)), p2 = cvt<R>(vtx(d, v2));
+	it.p	 = b0 * p0 + bb1 * p1 + bb2 * p2;
+	if (m.has_normals)
+		it.n = normalize(b0 * cvt<R>(nrm(d, v0)) + bb1 * cvt<R>(nrm(d, v1)) + bb2 * cvt<R>(nrm(d, v2)));
+	else
+		it.n = normalize(cross(p1 - p0, p2 - p0));
+	it.tangent	 = getPerpendicular(it.n);
+	it.bitangent = normalize(cross(it.n, it.tangent));
+	it.light	 = sv.triLight[prim];
+	it.material	 = m.material;
+	if (m.medium_inside != m.medium_outside) {
+		it.hasInterface = true;
+		it.inside		= m.medium_inside;
+		it.outside		= m.medium_outside;
+	}
+	it.n		 = normalize(it.n);
+	it.tangent	 = normalize(it.tangent);
+	it.bitangent = normalize(it.bitangent);
+	if (it.material < 0) return it;
+	const krr_material &mat	 = d->materials[it.material];
+	it.sd.type				 = mat.type;
+	it.sd.specularTransmission = R(mat.specular_transmission);
+	it.sd.IoR				 = R(mat.ior);
+	it.sd.diffuse			 = V3<R>(R(mat.diffuse[0]), R(mat.diffuse[1]), R(mat.diffuse[2]));
+	it.sd.specular			 = V3<R>(R(mat.specular[0]), R(mat.specular[1]), R(mat.specular[2]));
+	it.sd.roughness			 = R(1.f - mat.specular[3]);
+	it.sd.anisotropic		 = R(mat.anisotropic);
+	return it;
+}
+
+// ----------------------------------------------------------------------------
+// Lights (src/core/light.h:152-259, src/core/shape.h:23-123,
+// src/render/lightsampler.h:18-40).
+// ----------------------------------------------------------------------------
+template <class R> struct LightSample {
+	Interaction<R> intr;
+	V3<R> L;
+	R pdf = 0;
+};
+
+template <class R> R triArea(const krr_scene_desc *d, int prim) {
+	V3<R> p0 = cvt<R>(vtx(d, d->indices[3 * (size_t) prim])), p1 = cvt<R>(vtx(d, d->indices[3 * (size_t) prim + 1])),
+		  p2 = cvt<R>(vtx(d, d->indices[3 * (size_t) prim + 2]));
+	return R(0.5f) * norm(cross(p1 - p0, p2 - p0));
+}
+
+template <class R> V3<R> areaL(const LightRec &L, const V3<R> &n, const V3<R> &w) {
+	if (dot(n, w) < R(0)) return V3<R>(); // one-sided (twoSided = false for mesh lights)
+	return R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
+}
+
+template <class R> LightSample<R> sampleLi(const SceneView &sv, const LightRec &L, R u0, R u1, const Ctx<R> &ctx) {
+	LightSample<R> ls;
+	const krr_scene_desc *d = sv.d;
+	if (L.type == 1) { // InfiniteLight::sampleLi
+		V3<R> wi	 = uniformSampleSphere(u0, u1);
+		ls.intr.p	 = ctx.p + wi * R(2) * R(sv.sceneRadius);
+		ls.L		 = R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
+		ls.pdf		 = kInv4Pi<R>();
+		return ls;
+	}
+	const int prim = L.prim;
+	const int i0 = d->indices[3 * (size_t) prim], i1 = d->indices[3 * (size_t) prim + 1], i2 = d->indices[3 * (size_t) prim + 2];
+	V3<R> p0 = cvt<R>(vtx(d, i0)), p1 = cvt<R>(vtx(d, i1)), p2 = cvt<R>(vtx(d, i2));
+	V3<R> n0 = cvt<R>(nrm(d, i0)), n1 = cvt<R>(nrm(d, i1)), n2 = cvt<R>(nrm(d, i2));
+	R b0, b1, b2;
+	uniformSampleTriangle(u0, u1, b0, b1, b2);
+	V3<R> p	 = b0 * p0 + b1 * p1 + b2 * p2;
+	V3<R> n	 = normalize(cross(p1 - p0, p2 - p0));
+	V3<R> ns = normalize(b0 * n0 + b1 * n1 + b2 * n2);
+	if (dot(n, ns) < R(0)) n *= R(-1);
+	n		= normalize(n);
+	R pdf	= R(1) / triArea<R>(d, prim);
+	V3<R> wi = normalize(p - ctx.p);
+	pdf		 = pdf / (m_abs(dot(n, wi)) / squaredNorm(ctx.p - p));
+	if (norm(wi) == R(0) || is_inf(pdf)) pdf = 0;
+	ls.intr.p  = p;
+	ls.intr.n  = n;
+	ls.intr.wo = normalize(ctx.p - p);
+	ls.pdf	   = pdf;
+	ls.L	   = areaL<R>(L, n, ls.intr.wo);
+	return ls;
+}
+
+// Triangle::pdf(sample, ctx) (shape.h:114-123).
+template <class R> R areaPdfLi(const SceneView &sv, const LightRec &L, const V3<R> &p, const V3<R> &n, const Ctx<R> &ctx) {
+	if (L.type == 1) return kInv4Pi<R>();
+	V3<R> wi = normalize(p - ctx.p);
+	R pdf	 = (R(1) / triArea<R>(sv.d, L.prim)) / (m_abs(dot(n, -wi)) / squaredNorm(ctx.p - p));
+	if (norm(wi) == R(0) || is_inf(pdf)) pdf = 0;
+	return pdf;
+}
+
+// ----------------------------------------------------------------------------
+// Media (src/render/media.h:41-209, src/render/media.cpp:83-111,
+// src/render/wavefront/wavefront.h:163-207).
+// ----------------------------------------------------------------------------
+template <class R> struct MediumProps {
+	V3<R> sigma_a, sigma_s, Le;
+	float g;
+};
+
+template <class R> struct MajorantSeg {
+	R tMin, tMax;
+	V3<R> sigma_maj;
+};
+
+template <class R> struct MajorantIter {
+	R tMin = R(std::numeric_limits<float>::max()), tMax = R(std::numeric_limits<float>::min());
+	V3<R> sigma_t;
+	const MajorantGrid *grid = nullptr;
+	R nextCrossingT[3], deltaT[3];
+	int step[3], voxelLimit[3], voxel[3];
+
+	MajorantIter() {}
+	MajorantIter(const Ray<R> &ray, R tmin, R tmax, const V3<R> &st, const MajorantGrid *g)
+		: tMin(tmin), tMax(tmax), sigma_t(st), grid(g) {
+		if (!grid) return;
+		V3<R> oo, dd;
+		for (int a = 0; a < 3; a++) {
+			R diag = R(grid->bmax[a]) - R(grid->bmin[a]);
+			if (diag == 0) diag = 1;
+			oo[a] = (ray.o[a] - R(grid->bmin[a])) / diag;
+			dd[a] = ray.d[a] / (R(grid->bmax[a]) - R(grid->bmin[a]));
+		}
+		V3<R> gridIntersect = oo + dd * tMin;
+		for (int axis = 0; axis < 3; ++axis) {
+			voxel[axis]	 = (int) clampv(gridIntersect[axis] * R(grid->res[axis]), R(0), R(grid->res[axis]) - R(1));
+			deltaT[axis] = R(1) / (m_abs(dd[axis]) * R(grid->res[axis]));
+			if (dd[axis] == R(-0.f)) dd[axis] = R(0);
+			if (dd[axis] >= 0) {
+				R nextVoxelPos		= R(voxel[axis] + 1) / R(grid->res[axis]);
+				nextCrossingT[axis] = tMin + (nextVoxelPos - gridIntersect[axis]) / dd[axis];
+				step[axis]			= 1;
+				voxelLimit[axis]	= grid->res[axis];
+			} else {
+				R nextVoxelPos		= R(voxel[axis]) / R(grid->res[axis]);
+				nextCrossingT[axis] = tMin + (nextVoxelPos - gridIntersect[axis]) / dd[axis];
+				step[axis]			= -1;
+				voxelLimit[axis]	= -1;
+			}
+		}
+	}
+
+	bool next(MajorantSeg<R> &seg) {
+		if (tMin >= tMax) return false;
+		if (!grid) {
+			seg	 = {tMin, tMax, sigma_t};
+			tMin = tMax;
+			return true;
+		}
+		int bits = ((nextCrossingT[0] < nextCrossingT[1]) << 2) + ((nextCrossingT[0] < nextCrossingT[2]) << 1) +
+				   ((nextCrossingT[1] < nextCrossingT[2]));
+		const int cmpToAxis[8] = {2, 1, 2, 1, 2, 2, 0, 0};
+		int stepAxis		   = cmpToAxis[bits];
+		R tVoxelExit		   = std::fmin(tMax, nextCrossingT[stepAxis]);
+		V3<R> sigma_maj		   = sigma_t * R(grid->lookup(voxel[0], voxel[1], voxel[2]));
+		seg					   = {tMin, tVoxelExit, sigma_maj};
+		tMin				   = tVoxelExit;
+		if (nextCrossingT[stepAxis] > tMax) tMin = tMax;
+		voxel[stepAxis] += step[stepAxis];
+		if (voxel[stepAxis] == voxelLimit[stepAxis]) tMin = tMax;
+		nextCrossingT[stepAxis] += deltaT[stepAxis];
+		return true;
+	}
+};
+
+// Trilinear density at world p, zero outside the grid.
+inline float gridDensity(const Medium &md, const float p[3]) {
+	const krr_medium &m = *md.m;
+	float idx[3];
+	for (int a = 0; a < 3; a++) idx[a] = (p[a] - m.bounds_min[a]) * md.inv_voxel[a];
+	int i0[3];
+	float w[3];
+	for (int a = 0; a < 3; a++) {
+		float f = std::floor(idx[a]);
+		i0[a]	= (int) f;
+		w[a]	= idx[a] - f;
+	}
+	auto at = [&](int x, int y, int z) -> float {
+		if (x < 0 || y < 0 || z < 0 || x >= m.res[0] || y >= m.res[1] || z >= m.res[2]) return 0.f;
+		return m.density[(size_t) x + (size_t) m.res[0] * ((size_t) y + (size_t) m.res[1] * z)];
+	};
+	auto lerp = [](float a, float b, float t) { return a + t * (b - a); };
+	int x = i0[0], y = i0[1], z = i0[2];
+	float c00 = lerp(at(x, y, z), at(x, y, z + 1), w[2]);
+	float c01 = lerp(at(x, y + 1, z), at(x, y + 1, z + 1), w[2]);
+	float c10 = lerp(at(x + 1, y, z), at(x + 1, y, z + 1), w[2]);
+	float c11 = lerp(at(x + 1, y + 1, z), at(x + 1, y + 1, z + 1), w[2]);
+	return lerp(lerp(c00, c01, w[1]), lerp(c10, c11, w[1]), w[0]);
+}
+
+template <class R> MediumProps<R> samplePoint(const Medium &md, const V3<R> &p) {
+	const krr_medium &m = *md.m;
+	MediumProps<R> mp;
+	V3<R> st(R(m.sigma_t[0]), R(m.sigma_t[1]), R(m.sigma_t[2]));
+	V3<R> al(R(m.albedo[0]), R(m.albedo[1]), R(m.albedo[2]));
+	mp.g = m.g;
+	if (m.type == KRR_MEDIUM_HOMOGENEOUS) {
+		V3<R> sigma_s = st * al;
+		mp.sigma_a	  = st - sigma_s;
+		mp.sigma_s	  = sigma_s;
+		mp.Le		  = V3<R>(R(m.Le[0]), R(m.Le[1]), R(m.Le[2]));
+		return mp;
+	}
+	float pf[3]		= {(float) p.x, (float) p.y, (float) p.z};
+	R dens			= R(gridDensity(md, pf));
+	V3<R> sigma_t_s = dens * R(m.scale) * st;
+	V3<R> sigma_s	= sigma_t_s * al;
+	mp.sigma_a		= sigma_t_s - sigma_s;
+	mp.sigma_s		= sigma_s;
+	mp.Le			= V3<R>();
+	return mp;
+}
+
+// AABB3f::intersect (src/core/math/include/krrmath/aabb.h:58-76).
+template <class R> bool aabbIntersect(const float bmin[3], const float bmax[3], const V3<R> &o, const V3<R> &d, R tMax, R *tHit0, R *tHit1) {
+	R t0 = 0, t1 = tMax;
+	for (int i = 0; i < 3; ++i) {
+		R invRayDir = R(1) / d[i];
+		R tNear		= (R(bmin[i]) - o[i]) * invRayDir;
+		R tFar		= (R(bmax[i]) - o[i]) * invRayDir;
+		if (tNear > tFar) std::swap(tNear, tFar);
+		t0 = tNear > t0 ? tNear : t0;
+		t1 = tFar < t1 ? tFar : t1;
+		if (t0 > t1) return false;
+	}
+	*tHit0 = t0;
+	*tHit1 = t1;
+	return true;
+}
+
+template <class R> MajorantIter<R> sampleRay(const Medium &md, const Ray<R> &ray, R raytMax) {
+	const krr_medium &m = *md.m;
+	V3<R> st(R(m.sigma_t[0]), R(m.sigma_t[1]), R(m.sigma_t[2]));
+	if (m.type == KRR_MEDIUM_HOMOGENEOUS) return MajorantIter<R>(ray, R(0), raytMax, st, nullptr);
+	R tMin = 0, tMax = raytMax;
+	if (!aabbIntersect(m.bounds_min, m.bounds_max, ray.o, ray.d, raytMax, &tMin, &tMax)) return MajorantIter<R>();
+	return MajorantIter<R>(ray, tMin, tMax, R(m.scale) * st, &md.majorant);
+}
+
+// HGPhaseFunction (src/render/media.cpp:83-106).
+template <class R> R hgP(float gv, const V3<R> &wo, const V3<R> &wi) {
+	R g		= clampv(R(gv), R(-.99f), R(.99f));
+	R denom = R(1) + pow2(g) + R(2) * g * dot(wo, wi);
+	return kInv4Pi<R>() * (R(1) - pow2(g)) / (denom * safe_sqrt(denom));
+}
+
+template <class R> V3<R> hgSample(float gv, const V3<R> &wo, R u0, R u1, R *pdf) {
+	R g = clampv(R(gv), R(-.99f), R(.99f));
+	R cosTheta;
+	if (m_abs(g) < R(1e-3f)) cosTheta = R(1) - R(2) * u0;
+	else cosTheta = R(-1) / (R(2) * g) * (R(1) + pow2(g) - pow2((R(1) - pow2(g)) / (R(1) + g - R(2) * g * u0)));
+	R sinTheta	 = safe_sqrt(R(1) - pow2(cosTheta));
+	R phi		 = k2Pi<R>() * u1;
+	V3<R> won	 = normalize(wo);
+	V3<R> T		 = getPerpendicular(won);
+	V3<R> B		 = normalize(cross(won, T));
+	V3<R> local(sinTheta * m_cos(phi), sinTheta * m_sin(phi), cosTheta);
+	V3<R> wi = T * local.x + B * local.y + won * local.z;
+	*pdf	 = hgP(gv, normalize(wo), wi);
+	return wi;
+}
+
+// sampleT_maj (wavefront.h:163-207).  callback(p, mp, sigma_maj, T_maj) -> bool.
+template <class R, class F> V3<R> sampleT_maj(const SceneView &sv, Ray<R> ray, R tMax, Pcg &sampler, int channel, F callback) {
+	tMax *= norm(ray.d);
+	ray.d			 = normalize(ray.d);
+	V3<R> T_maj		 = V3<R>(R(1));
+	const Medium &md = sv.media[ray.medium];
+	MajorantIter<R> iter = sampleRay(md, ray, tMax);
+	MajorantSeg<R> seg;
+	while (true) {
+		if (!iter.next(seg)) return T_maj;
+		if (seg.sigma_maj[channel] == 0) {
+			R dt = seg.tMax - seg.tMin;
+			if (is_inf(dt)) dt = R(std::numeric_limits<float>::max());
+			T_maj *= vexp(-dt * seg.sigma_maj);
+			continue;
+		}
+		R tMin = seg.tMin;
+		while (true) {
+			R t = tMin + sampleExponential(R(sampler.get1D()), seg.sigma_maj[channel]);
+			if (t < seg.tMax) {
+				T_maj *= vexp(-(t - tMin) * seg.sigma_maj);
+				MediumProps<R> mp = samplePoint(md, ray.at(t));
+				if (!callback(ray.at(t), mp, seg.sigma_maj, T_maj)) return V3<R>(R(1));
+				T_maj = V3<R>(R(1));
+				tMin  = t;
+			} else {
+				R dt = seg.tMax - tMin;
+				if (is_inf(dt)) dt = R(std::numeric_limits<float>::max());
+				T_maj *= vexp(-dt * seg.sigma_maj);
+				break;
+			}
+		}
+	}
+	return V3<R>(R(1));
+}
+
+// ----------------------------------------------------------------------------
+// Work items (src/render/wavefront/workitem.h:14-95).
+// ----------------------------------------------------------------------------
+template <class R> struct RayItem {
+	Ray<R> ray;
+	Ctx<R> ctx;
+	V3<R> thp, pu, pl;
+	int bsdfType = BSDF_UNSET;
+	int depth	 = 0;
+};
+
+template <class R> struct ShadowItem {
+	Ray<R> ray;
+	R tMax;
+	V3<R> Ld, pu, pl;
+};
+
+template <class R> struct ScatterItem {
+	V3<R> thp, pu;
+	SurfInt<R> intr;
+	int depth;
+};
+
+template <class R> struct MediumSampleItem {
+	Ray<R> ray;
+	Ctx<R> ctx;
+	V3<R> thp, pu, pl;
+	R tMax;
+	bool hasIntr = false;
+	SurfInt<R> intr;
+	int bsdfType, depth;
+};
+
+template <class R> struct MediumScatterItem {
+	V3<R> p, thp, pu, wo;
+	int medium;
+	float g;
+	int depth;
+};
+
+template <class R> struct HitLightItem {
+	int light;
+	Ctx<R> ctx;
+	V3<R> p, wo, n;
+	V3<R> thp, pu, pl;
+	int bsdfType, depth;
+};
+
+} // namespace oracle
