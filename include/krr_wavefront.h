@@ -135,9 +135,15 @@ typedef struct krr_stats {
 	uint64_t closest_rays;	 /* rays traced for closest hits, summed over depth */
 	uint64_t shadow_rays;	 /* shadow rays traced */
 	uint64_t paths;			 /* camera samples started: W*H*spp */
-	uint64_t node_visits;	 /* BVH nodes visited (only when "count_traversal" is set) */
-	uint64_t tri_tests;		 /* ray-triangle tests (only when "count_traversal" is set) */
+	uint64_t node_visits;	 /* BVH nodes visited by closest-hit rays ("count_traversal") */
+	uint64_t tri_tests;		 /* ray-triangle tests of closest-hit rays ("count_traversal") */
+	uint64_t shadow_node_visits; /* same for shadow rays */
+	uint64_t shadow_tri_tests;
 	uint64_t closest_launches; /* number of closest-hit trace kernels launched */
+	double closest_trace_ms; /* summed HIP-event time of the closest-hit trace kernels ("time_kernels") */
+	double shade_ms;		 /* summed time of the shade kernels ("time_kernels") */
+	double shadow_trace_ms;	 /* summed time of the shadow trace kernels ("time_kernels") */
+	double render_ms;		 /* HIP-event time of the whole render call ("time_kernels") */
 } krr_stats;
 
 /* Closest hit record: t along the (unnormalised) ray direction, primitive
@@ -213,6 +219,7 @@ typedef struct krr_wf krr_wf;
  *   rr_min_depth (0)     first path depth at which Russian roulette applies
  *   legacy_spp_accumulate (false)  keep the reference's running-sum spp loop
  *   count_traversal (false)  count node visits / triangle tests
+ *   time_kernels (false)     time each trace / shade launch with HIP events
  *   block_size (256), trace_block_size (128)  launch shapes
  * NULL or "" means all defaults. */
 int krr_wf_create(const char *json_params, krr_wf **out);

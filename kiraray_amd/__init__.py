@@ -176,7 +176,7 @@ class WavefrontPathTracer:
     def stats(self):
         s = krr_stats()
         _check(self._lib.krr_wf_stats(self._h, C.byref(s)))
-        return {f: int(getattr(s, f)) for f, _ in krr_stats._fields_}
+        return {f: (float(getattr(s, f)) if t is C.c_double else int(getattr(s, f))) for f, t in krr_stats._fields_}
 
     def close(self):
         if getattr(self, "_h", None):
@@ -194,37 +194,42 @@ class RenderApp:
     """Headless counterpart of the reference's frame loop
     (src/core/window.cpp:450-485, src/main/renderer.cpp:84-122): the frame
     index starts at 1 and each frame calls beginFrame, render and endFrame of
-    every pass.  The film is a device float4 buffer (a torch tensor)."""
+    every pass.  The film is a device float4 buffer owned by the app."""
 
-    def __init__(self, scene, passes, width, height, device="cuda"):
-        import torch
+    def __init__(self, scene, passes, width, height, stream=0):
+        from . import hip
         self.scene = scene
         self.passes = passes
         self.frame_index = 0
-        self.film = torch.zeros((height, width, 4), dtype=torch.float32, device=device)
+        self.width, self.height = int(width), int(height)
+        self.stream = stream
+        self.film = hip.DeviceBuffer(self.width * self.height * 16)
         for p in passes:
             p.set_scene(scene)
             p.resize(width, height)
 
     def frame(self, view=0):
-        import torch
         self.frame_index += 1
-        stream = torch.cuda.current_stream().cuda_stream
         cam = self.scene.camera(view)
         for p in self.passes:
             p.begin_frame(cam, self.frame_index)
         for p in self.passes:
-            p.render(self.film.data_ptr(), stream)
+            p.render(self.film.ptr, self.stream)
         for p in self.passes:
             p.end_frame()
         return self.film
 
+    def film_numpy(self):
+        import numpy as np
+        from . import hip
+        hip.synchronize()
+        return self.film.to_numpy(np.float32, (self.height, self.width, 4))
 
-def error_metric(metric, img, ref, stream=0):
-    """relMSE / MSE / MAPE / SMAPE of two device float4 images (torch)."""
+
+def error_metric(metric, img_ptr, ref_ptr, n_pixels, stream=0):
+    """relMSE / MSE / MAPE / SMAPE of two device float4 images."""
     lib = load_library()
     out = C.c_double()
-    n = img.numel() // 4
-    _check(lib.krr_error_metric(int(metric), C.c_void_p(img.data_ptr()), C.c_void_p(ref.data_ptr()), n,
-                                C.byref(out), C.c_void_p(stream)))
+    _check(lib.krr_error_metric(int(metric), C.c_void_p(img_ptr), C.c_void_p(ref_ptr), int(n_pixels), C.byref(out),
+                                C.c_void_p(stream)))
     return out.value

@@ -63,7 +63,10 @@ class krr_camera(C.Structure):
 
 class krr_stats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
-                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("closest_launches", C.c_uint64)]
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("closest_launches", C.c_uint64),
+                ("closest_trace_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_trace_ms", C.c_double),
+                ("render_ms", C.c_double)]
 
 
 class krr_hit(C.Structure):

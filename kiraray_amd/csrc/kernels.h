@@ -47,10 +47,11 @@ struct Counters {
 	int pad0;
 	unsigned long long closestTotal;
 	unsigned long long shadowTotal;
-	unsigned long long nodeVisits;
+	unsigned long long nodeVisits;	   // closest-hit traversal
 	unsigned long long triTests;
+	unsigned long long shadowNodeVisits; // shadow traversal
+	unsigned long long shadowTriTests;
 	unsigned long long closestLaunches;
-	unsigned long long pad1[2];
 };
 
 struct CameraDev {

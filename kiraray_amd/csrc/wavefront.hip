@@ -43,9 +43,11 @@ std::string &lastErrorStorage() {
 __global__ void k_reset_totals(Counters *cnt) {
 	cnt->closestTotal	 = 0;
 	cnt->shadowTotal	 = 0;
-	cnt->nodeVisits		 = 0;
-	cnt->triTests		 = 0;
-	cnt->closestLaunches = 0;
+	cnt->nodeVisits		  = 0;
+	cnt->triTests		  = 0;
+	cnt->shadowNodeVisits = 0;
+	cnt->shadowTriTests	  = 0;
+	cnt->closestLaunches  = 0;
 	cnt->rayCount[0] = cnt->rayCount[1] = cnt->shadowCount = 0;
 }
 
@@ -181,8 +183,8 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 		}
 	}
 	if (COUNT) {
-		waveAddU64(&cnt->nodeVisits, nv);
-		waveAddU64(&cnt->triTests, nt);
+		waveAddU64(&cnt->shadowNodeVisits, nv);
+		waveAddU64(&cnt->shadowTriTests, nt);
 	}
 }
 
@@ -372,7 +374,7 @@ __global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueue
 			if (MEDIA && hasMSc) { // sampleMediumScattering
 				if (P.nee && sc.nLights > 0) {
 					const float ul	 = rng.get1D();
-					const int li	 = (int) (uint32_t) (ul * (float) sc.nLights);
+					const int li	 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
 					const float u0	 = rng.get1D(), u1 = rng.get1D();
 					const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, mscP);
 					const f3 to		 = offsetRayOrigin(ls.p, ls.n, mscP - ls.p);
@@ -432,7 +434,7 @@ __global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueue
 					bsdf.setup(mat);
 					if (P.nee && (bt & BSDF_SMOOTH) && sc.nLights > 0) {
 						const float ul		 = rng.get1D();
-						const int li		 = (int) (uint32_t) (ul * (float) sc.nLights);
+						const int li		 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
 						const float u0		 = rng.get1D(), u1 = rng.get1D();
 						const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
 						const f3 to			 = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
@@ -498,6 +500,8 @@ __global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueue
 	}
 }
 
+constexpr int kMaxNullCrossings = 1024;
+
 // traceShadow with media: ShadowTr raygen + traceTransmittance
 // (device.cu:102-128, wavefront.h:209-268).  Ratio tracking through media,
 // stepping across null-material surfaces.
@@ -523,7 +527,10 @@ __global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDe
 		f3 T_ray(1.f), pu(1.f), pl(1.f);
 		SurfaceInteraction intr;
 		const float zeroEps = kRayEps * 2.f;
-		while (!(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps)) {
+		// The reference loops until the segment is shorter than 2 KRR_RAY_EPS;
+		// the cap only bounds pathological stacks of coplanar null surfaces.
+		for (int crossing = 0; crossing < kMaxNullCrossings && !(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps);
+			 crossing++) {
 			const TraceResult h = traverse<false, COUNT>(bvh, o, d, tMax, ldsStack + threadIdx.x, blockDim.x, nv, nt);
 			const bool visible	= h.prim < 0;
 			if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
@@ -568,8 +575,8 @@ __global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDe
 		px.rng[pixel] = rng.state;
 	}
 	if (COUNT) {
-		waveAddU64(&cnt->nodeVisits, nv);
-		waveAddU64(&cnt->triTests, nt);
+		waveAddU64(&cnt->shadowNodeVisits, nv);
+		waveAddU64(&cnt->shadowTriTests, nt);
 	}
 }
 
@@ -746,7 +753,7 @@ struct krr_accel {
 
 struct krr_wf {
 	// params (integrator.h:96-103 + additions)
-	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false;
+	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false, timeKernels = false;
 	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4;
 	float rr = 0.8f, clampMax = 1e3f;
 	// scene
@@ -764,6 +771,32 @@ struct krr_wf {
 	bool beginPending	= false;
 	hipStream_t lastStream = nullptr;
 	krr_stats stats{};
+	// HIP-event timing of kernel launches ("time_kernels"): kind 0 closest
+	// trace, 1 shade, 2 shadow trace, 3 whole render.
+	struct Span {
+		int kind, begin, end;
+	};
+	std::vector<hipEvent_t> events;
+	std::vector<Span> spans;
+	int eventsUsed = 0;
+
+	hipEvent_t event(int k) {
+		while ((int) events.size() <= k) {
+			hipEvent_t e = nullptr;
+			if (hipEventCreate(&e) != hipSuccess) return nullptr;
+			events.push_back(e);
+		}
+		return events[k];
+	}
+	int mark(hipStream_t st) {
+		int k = eventsUsed++;
+		hipEvent_t e = event(k);
+		if (e) (void) hipEventRecord(e, st);
+		return k;
+	}
+	~krr_wf() {
+		for (hipEvent_t e : events) (void) hipEventDestroy(e);
+	}
 };
 
 namespace {
@@ -783,6 +816,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->rrMinDepth	   = (int) j.value("rr_min_depth", 0.0);
 		wf->legacySpp	   = j.value("legacy_spp_accumulate", false);
 		wf->countTraversal = j.value("count_traversal", false);
+		wf->timeKernels	   = j.value("time_kernels", false);
 		wf->blockSize	   = (int) j.value("block_size", 256.0);
 		wf->traceBlockSize = (int) j.value("trace_block_size", 128.0);
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
@@ -1099,6 +1133,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("rr_min_depth", Json((double) wf->rrMinDepth));
 	j.set("legacy_spp_accumulate", Json(wf->legacySpp));
 	j.set("count_traversal", Json(wf->countTraversal));
+	j.set("time_kernels", Json(wf->timeKernels));
 	j.set("block_size", Json((double) wf->blockSize));
 	j.set("trace_block_size", Json((double) wf->traceBlockSize));
 	j.set("leaf_size", Json((double) wf->leafSize));
@@ -1192,6 +1227,19 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const size_t lds   = (size_t) wf->accel.stackSize * tb * sizeof(int);
 	const bool count   = wf->countTraversal;
 
+	const bool timing = wf->timeKernels;
+	wf->spans.clear();
+	wf->eventsUsed = 0;
+	const int renderBegin = timing ? wf->mark(st) : -1;
+	auto timed = [&](int kind, auto &&launch) {
+		if (!timing) {
+			launch();
+			return;
+		}
+		const int b = wf->mark(st);
+		launch();
+		wf->spans.push_back({kind, b, wf->mark(st)});
+	};
 	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
 	if (wf->beginPending) {
 		hipLaunchKernelGGL(k_begin_frame, dim3(gN), dim3(bs), 0, st, P, px);
@@ -1203,31 +1251,38 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 			const int cur = depth & 1;
 			hipLaunchKernelGGL(k_depth_begin, dim3(1), dim3(1), 0, st, cnt, cur);
 			const RayQueueDev cq = rayQueue(wf, cur);
-			if (count)
-				hipLaunchKernelGGL(k_trace_closest<true>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray, wf->hits.as<float4>(),
-								   (const int *) &cnt->rayCount[cur], 0, cnt);
-			else
-				hipLaunchKernelGGL(k_trace_closest<false>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray, wf->hits.as<float4>(),
-								   (const int *) &cnt->rayCount[cur], 0, cnt);
-			if (media)
-				hipLaunchKernelGGL(k_shade<true>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
-								   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
-			else
-				hipLaunchKernelGGL(k_shade<false>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
-								   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
+			timed(0, [&] {
+				if (count)
+					hipLaunchKernelGGL(k_trace_closest<true>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
+									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], 0, cnt);
+				else
+					hipLaunchKernelGGL(k_trace_closest<false>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
+									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], 0, cnt);
+			});
+			timed(1, [&] {
+				if (media)
+					hipLaunchKernelGGL(k_shade<true>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+									   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
+				else
+					hipLaunchKernelGGL(k_shade<false>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+									   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
+			});
 			if (wf->nee && depth != wf->maxDepth) {
-				if (media) {
-					if (count) hipLaunchKernelGGL(k_shadow_tr<true>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
-					else hipLaunchKernelGGL(k_shadow_tr<false>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
-				} else {
-					if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
-					else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
-				}
+				timed(2, [&] {
+					if (media) {
+						if (count) hipLaunchKernelGGL(k_shadow_tr<true>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
+						else hipLaunchKernelGGL(k_shadow_tr<false>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
+					} else {
+						if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
+						else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
+					}
+				});
 			}
 		}
 		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
 	}
 	hipLaunchKernelGGL(k_film, dim3(gN), dim3(bs), 0, st, P, px, (float4 *) film_dev);
+	if (timing) wf->spans.push_back({3, renderBegin, wf->mark(st)});
 	KRR_HIP(hipGetLastError());
 	wf->stats.paths = (uint64_t) N * wf->spp;
 	return KRR_OK;
@@ -1251,7 +1306,24 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 	out->paths			  = wf->stats.paths;
 	out->node_visits	  = c.nodeVisits;
 	out->tri_tests		  = c.triTests;
-	out->closest_launches = c.closestLaunches;
+	out->closest_launches	= c.closestLaunches;
+	out->shadow_node_visits = c.shadowNodeVisits;
+	out->shadow_tri_tests	= c.shadowTriTests;
+	if (wf->timeKernels) {
+		double ms[4] = {0, 0, 0, 0};
+		for (const auto &sp : wf->spans) {
+			float t = 0.f;
+			hipEvent_t a = wf->event(sp.begin), b = wf->event(sp.end);
+			if (!a || !b) continue;
+			KRR_HIP(hipEventSynchronize(b));
+			KRR_HIP(hipEventElapsedTime(&t, a, b));
+			ms[sp.kind] += t;
+		}
+		out->closest_trace_ms = ms[0];
+		out->shade_ms		  = ms[1];
+		out->shadow_trace_ms  = ms[2];
+		out->render_ms		  = ms[3];
+	}
 	return KRR_OK;
 }
 

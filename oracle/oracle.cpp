@@ -338,7 +338,8 @@ template <class R> struct Integrator {
 		V3<R> T_ray(R(1)), pu(R(1)), pl(R(1));
 		SurfInt<R> intr;
 		const R zeroEps = R(kRayEps * 2);
-		while (!(m_abs(ray.d.x) <= zeroEps && m_abs(ray.d.y) <= zeroEps && m_abs(ray.d.z) <= zeroEps)) {
+		for (int crossing = 0; crossing < 1024 && !(m_abs(ray.d.x) <= zeroEps && m_abs(ray.d.y) <= zeroEps && m_abs(ray.d.z) <= zeroEps);
+			 crossing++) {
 			Hit h		 = closest(ray, float(tMax));
 			bool visible = h.prim < 0;
 			if (!visible) {

@@ -28,11 +28,13 @@ def krr():
 
 
 @pytest.fixture(scope="session")
-def torch_cuda():
-    import torch
-    if not torch.cuda.is_available():
+def gpu():
+    """HIP runtime helpers of the tracer library (one runtime per process)."""
+    from kiraray_amd import hip
+    if hip.device_count() == 0:
         pytest.fail("GPU test collected but no HIP device is visible")
-    return torch
+    hip.check(hip.runtime().hipSetDevice(0))
+    return hip
 
 
 CORNELL = {"model": [{"type": "cornell_box"}], "resolution": [256, 256],

@@ -15,18 +15,17 @@ from conftest import CORNELL, icosphere_scene
 pytestmark = pytest.mark.gpu
 
 
-def gpu_render(krr, torch, scene, w, h, params, frame_index=1, frames=1):
+def gpu_render(krr, gpu, scene, w, h, params, frame_index=1, frames=1):
     wf = krr.WavefrontPathTracer(params)
     wf.set_scene(scene)
     wf.resize(w, h)
-    film = torch.zeros((h, w, 4), dtype=torch.float32, device="cuda")
-    stream = torch.cuda.current_stream().cuda_stream
+    film = gpu.DeviceBuffer(w * h * 16)
     for f in range(frames):
         wf.begin_frame(scene.camera(0), frame_index + f)
-        wf.render(film.data_ptr(), stream)
+        wf.render(film.ptr, 0)
         wf.end_frame()
     st = wf.stats()
-    out = film.cpu().numpy()
+    out = film.to_numpy(np.float32, (h, w, 4))
     wf.close()
     return out, st
 
@@ -41,12 +40,12 @@ def assert_images_match(g, o, rtol=1e-4, atol=1e-6):
     assert np.all(g[..., 3] == 1.0)
 
 
-def test_cornell_cfg1_parity(krr, oracle, torch_cuda):
+def test_cornell_cfg1_parity(krr, oracle, gpu):
     """Cornell box 256x256, 16 spp, max depth 8, NEE, rr 0.8 (frame 1)."""
     print("case: cornell 256x256 16spp depth 8", flush=True)
     s = krr.Scene(CORNELL)
     params = {"spp": 16, "max_depth": 8, "nee": True, "rr": 0.8}
-    g, gs = gpu_render(krr, torch_cuda, s, 256, 256, params)
+    g, gs = gpu_render(krr, gpu, s, 256, 256, params)
     o, os_ = oracle.render(s.desc, s.camera(0), 256, 256, 1, spp=16, max_depth=8, nee=1, rr=0.8)
     print("gpu", gs, "cpu", os_, flush=True)
     assert gs["closest_rays"] == os_["closest_rays"]
@@ -56,12 +55,12 @@ def test_cornell_cfg1_parity(krr, oracle, torch_cuda):
 
 
 @pytest.mark.parametrize("nee,depth,clamp,legacy", [(False, 5, False, False), (True, 0, True, False), (True, 3, False, True)])
-def test_cornell_param_variants(krr, oracle, torch_cuda, nee, depth, clamp, legacy):
+def test_cornell_param_variants(krr, oracle, gpu, nee, depth, clamp, legacy):
     print(f"case: cornell variants nee={nee} depth={depth} clamp={clamp} legacy={legacy}", flush=True)
     s = krr.Scene(CORNELL)
     params = {"spp": 2, "max_depth": depth, "nee": nee, "enable_clamp": clamp, "clamp_max": 0.5,
               "legacy_spp_accumulate": legacy}
-    g, gs = gpu_render(krr, torch_cuda, s, 96, 80, params, frame_index=7)
+    g, gs = gpu_render(krr, gpu, s, 96, 80, params, frame_index=7)
     o, os_ = oracle.render(s.desc, s.camera(0), 96, 80, 7, spp=2, max_depth=depth, nee=int(nee),
                            enable_clamp=int(clamp), clamp_max=0.5, legacy_spp_accumulate=int(legacy))
     assert gs["closest_rays"] == os_["closest_rays"]
@@ -69,13 +68,13 @@ def test_cornell_param_variants(krr, oracle, torch_cuda, nee, depth, clamp, lega
     assert_images_match(g, o)
 
 
-def test_icosphere_cfg2_parity_subsample(krr, oracle, torch_cuda):
+def test_icosphere_cfg2_parity_subsample(krr, oracle, gpu):
     """The 1.31 M-triangle scene (diffuse / GGX conductor / glass) at
     240x135, 1 spp, max depth 16, RR from depth 3."""
     print("case: icosphere field 240x135 depth 16", flush=True)
     s = krr.Scene(icosphere_scene(res=(240, 135)))
     params = {"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3}
-    g, gs = gpu_render(krr, torch_cuda, s, 240, 135, params)
+    g, gs = gpu_render(krr, gpu, s, 240, 135, params)
     o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
     print("gpu", gs, "cpu", os_, flush=True)
     assert gs["closest_rays"] == os_["closest_rays"]
@@ -83,13 +82,13 @@ def test_icosphere_cfg2_parity_subsample(krr, oracle, torch_cuda):
     assert_images_match(g, o)
 
 
-def test_volume_cfg3_parity_subsample(krr, oracle, torch_cuda):
+def test_volume_cfg3_parity_subsample(krr, oracle, gpu):
     """Icosphere field inside a 64^3 value-noise medium (sigma_t max 20,
     albedo 0.8, HG g 0.3): delta tracking, medium NEE with ratio tracking."""
     print("case: volume 128x72 spp 2 depth 32", flush=True)
     s = krr.Scene(icosphere_scene(count=256, level=2, res=(128, 72), media=True, grid_res=64))
     params = {"spp": 2, "max_depth": 32}
-    g, gs = gpu_render(krr, torch_cuda, s, 128, 72, params)
+    g, gs = gpu_render(krr, gpu, s, 128, 72, params)
     o, os_ = oracle.render(s.desc, s.camera(0), 128, 72, 1, spp=2, max_depth=32)
     print("gpu", gs, "cpu", os_, flush=True)
     assert gs["closest_rays"] == os_["closest_rays"]
@@ -97,19 +96,20 @@ def test_volume_cfg3_parity_subsample(krr, oracle, torch_cuda):
     assert_images_match(g, o)
 
 
-def test_multi_frame_seeds_differ_and_match(krr, oracle, torch_cuda):
+def test_multi_frame_seeds_differ_and_match(krr, oracle, gpu):
     """Frame index feeds the seeds: frames 1..3 each match the checker."""
     print("case: frames 1..3", flush=True)
     s = krr.Scene(CORNELL)
     wf = krr.WavefrontPathTracer({"spp": 1, "max_depth": 4})
     wf.set_scene(s)
     wf.resize(64, 64)
-    film = torch_cuda.zeros((64, 64, 4), dtype=torch_cuda.float32, device="cuda")
+    film = gpu.DeviceBuffer(64 * 64 * 16)
     prev = None
     for f in (1, 2, 3):
         wf.begin_frame(s.camera(0), f)
-        wf.render(film.data_ptr(), torch_cuda.cuda.current_stream().cuda_stream)
-        g = film.cpu().numpy()
+        wf.render(film.ptr, 0)
+        gpu.synchronize()
+        g = film.to_numpy(np.float32, (64, 64, 4))
         o, _ = oracle.render(s.desc, s.camera(0), 64, 64, f, spp=1, max_depth=4)
         assert_images_match(g, o)
         if prev is not None:

@@ -27,24 +27,24 @@ def _rays_towards_triangles(pos, idx, n, rng, lo, hi, tmax=np.inf, jitter=0.2):
     return rays
 
 
-def _gpu_closest(krr, torch, scene, rays):
+def _gpu_closest(krr, gpu, scene, rays):
     acc = krr.Accel(scene.desc)
-    r = torch.from_numpy(rays).cuda()
-    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
-    acc.trace_closest(r.data_ptr(), len(rays), hits.data_ptr(), torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
-    h = hits.cpu().numpy()
+    r = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(rays, np.float32))
+    hits = gpu.DeviceBuffer(len(rays) * 16)
+    acc.trace_closest(r.ptr, len(rays), hits.ptr, 0)
+    gpu.synchronize()
+    h = hits.to_numpy(np.float32, (len(rays), 4))
     out = np.zeros(len(rays), dtype=[("t", "<f4"), ("prim", "<i4"), ("b1", "<f4"), ("b2", "<f4")])
     out["t"] = h[:, 0]
     out["prim"] = h[:, 1].view(np.int32)
     out["b1"] = h[:, 2]
     out["b2"] = h[:, 3]
-    vis = torch.zeros(len(rays), dtype=torch.int32, device="cuda")
-    acc.trace_any(r.data_ptr(), len(rays), vis.data_ptr(), torch.cuda.current_stream().cuda_stream)
-    torch.cuda.synchronize()
+    vis = gpu.DeviceBuffer(len(rays) * 4)
+    acc.trace_any(r.ptr, len(rays), vis.ptr, 0)
+    gpu.synchronize()
     info = acc.info()
     acc.close()
-    return out, vis.cpu().numpy(), info
+    return out, vis.to_numpy(np.int32, (len(rays),)), info
 
 
 def _assert_same_hits(g, o):
@@ -55,7 +55,7 @@ def _assert_same_hits(g, o):
     assert np.array_equal(g["b2"][hit], o["b2"][hit])
 
 
-def test_cornell_closest_bit_exact(krr, oracle, torch_cuda):
+def test_cornell_closest_bit_exact(krr, oracle, gpu):
     print("case: cornell closest/any", flush=True)
     s = krr.Scene(CORNELL)
     pos, idx, _ = s.arrays()
@@ -63,7 +63,7 @@ def test_cornell_closest_bit_exact(krr, oracle, torch_cuda):
     rays = _rays_towards_triangles(pos, idx, 20000, rng, np.array([-0.9, 0.1, -0.9]), np.array([0.9, 1.9, 2.5]))
     # a quarter with finite tmax, as shadow rays have
     rays[::4, 3] = rng.random(len(rays[::4])).astype(np.float32) * 2
-    g, vis, info = _gpu_closest(krr, torch_cuda, s, rays)
+    g, vis, info = _gpu_closest(krr, gpu, s, rays)
     o = oracle.trace_closest(s.desc, rays)
     _assert_same_hits(g, o)
     assert (o["prim"] >= 0).mean() > 0.5
@@ -71,20 +71,20 @@ def test_cornell_closest_bit_exact(krr, oracle, torch_cuda):
     assert info["leaf_tris"] == s.n_triangles
 
 
-def test_icosphere_closest_bit_exact(krr, oracle, torch_cuda):
+def test_icosphere_closest_bit_exact(krr, oracle, gpu):
     print("case: icosphere field 256 x level 2 closest/any", flush=True)
     s = krr.Scene(icosphere_scene(count=256, level=2, res=(64, 64)))
     pos, idx, _ = s.arrays()
     rng = np.random.default_rng(2)
     rays = _rays_towards_triangles(pos, idx, 30000, rng, np.array([-0.5, 0.0, -0.5]), np.array([1.5, 1.7, 1.5]), jitter=0.01)
-    g, vis, info = _gpu_closest(krr, torch_cuda, s, rays)
+    g, vis, info = _gpu_closest(krr, gpu, s, rays)
     o = oracle.trace_closest(s.desc, rays)
     _assert_same_hits(g, o)
     assert np.array_equal(vis, oracle.trace_any(s.desc, rays))
     assert info["max_depth"] >= 2
 
 
-def test_watertight_vertices_and_edges(krr, oracle, torch_cuda):
+def test_watertight_vertices_and_edges(krr, oracle, gpu):
     """Rays aimed exactly at shared vertices and edge midpoints of closed
     icospheres from outside must never slip through."""
     print("case: watertight vertex/edge rays", flush=True)
@@ -99,13 +99,13 @@ def test_watertight_vertices_and_edges(krr, oracle, torch_cuda):
     rays[:, 0:3] = org
     rays[:, 3] = np.inf
     rays[:, 4:7] = tg - org
-    g, _, _ = _gpu_closest(krr, torch_cuda, s, rays)
+    g, _, _ = _gpu_closest(krr, gpu, s, rays)
     o = oracle.trace_closest(s.desc, rays)
     _assert_same_hits(g, o)
     assert (o["prim"] >= 0).all(), "a ray through a shared vertex/edge missed every triangle"
 
 
-def test_full_scene_bvh_matches_checker_bvh(krr, oracle, torch_cuda):
+def test_full_scene_bvh_matches_checker_bvh(krr, oracle, gpu):
     """1024 icospheres (1.31 M triangles): GPU hits equal the checker's own
     (independently built) BVH, itself pinned to brute force above."""
     print("case: 1.31M-triangle closest", flush=True)
@@ -114,7 +114,7 @@ def test_full_scene_bvh_matches_checker_bvh(krr, oracle, torch_cuda):
     pos, idx, _ = s.arrays()
     rng = np.random.default_rng(4)
     rays = _rays_towards_triangles(pos, idx, 20000, rng, np.array([-0.5, 0.05, -0.5]), np.array([1.5, 1.7, 2.5]), jitter=0.02)
-    g, vis, _ = _gpu_closest(krr, torch_cuda, s, rays)
+    g, vis, _ = _gpu_closest(krr, gpu, s, rays)
     o = oracle.trace_closest(s.desc, rays, use_bvh=True)
     _assert_same_hits(g, o)
     assert np.array_equal(vis, oracle.trace_any(s.desc, rays, use_bvh=True))
