@@ -1,0 +1,277 @@
+#!/usr/bin/env python3
+"""Benchmark of the wavefront path tracer on one MI355X.
+
+Workload (default): 1024 icospheres (subdivision level 3, 1,310,720 triangles)
+in the unit cube over a ground quad with 16 quad lights (1,310,754 triangles
+in all), 1920x1080, 1 sample per pixel per frame, max depth 16, Russian
+roulette from depth 3.  A step is one frame (beginFrame + render).
+
+value = (closest-hit rays + shadow rays) / wall time of the timed frames, in
+Mrays/s.  Rays are counted on the device (queue sizes summed over depth).
+The roofline entry covers the kernel with the most HIP-event time per frame
+(see kernel_rooflines for the byte model of each); its measured HBM traffic
+comes from profiles/traffic_<config>.json when that file exists (rocprofv3
+FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy-kernel calibration).
+cpu_baseline = the CPU checker (OpenMP) on full frames of the same scene.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s + Mpaths/s on 1 MI355X; equal-time relMSE: NRC vs plain PT"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    "cfg2": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
+                        "resolution": [1920, 1080],
+                        "camera": {"eye": [0.5, 0.75, 2.3], "target": [0.5, 0.3, 0.45]}},
+                 params={"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3},
+                 desc="icosphere_field_1.31Mtri_1920x1080_1spp_depth16_rr3"),
+    "cfg1": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
+                        "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
+                 params={"spp": 16, "max_depth": 8, "rr": 0.8},
+                 desc="cornell_box_36tri_256x256_16spp_depth8"),
+    "cfg3": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
+                        "media": [{"type": "value_noise_grid", "res": 256, "octaves": 5, "seed": 11,
+                                   "sigma_t_max": 20.0, "albedo": 0.8, "g": 0.3}],
+                        "resolution": [1280, 720],
+                        "camera": {"eye": [0.5, 0.75, 2.3], "target": [0.5, 0.3, 0.45], "medium": 0}},
+                 params={"spp": 4, "max_depth": 32, "rr": 0.8, "rr_min_depth": 3},
+                 desc="icosphere_field_1.31Mtri_volume256_1280x720_4spp_depth32"),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gpu_bench(cfg, steps, warmup, traffic):
+    import numpy as np
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    if hip.device_count() < 1:
+        raise SystemExit("bench.py: no HIP device visible")
+    hip.check(hip.runtime().hipSetDevice(0))
+    w, h = cfg["scene"]["resolution"]
+    t0 = time.time()
+    scene = krr.Scene(cfg["scene"])
+    log(f"scene: {scene.n_triangles} triangles ({time.time() - t0:.1f}s)")
+    params = dict(cfg["params"], time_kernels=True)
+    wf = krr.WavefrontPathTracer(params)
+    t0 = time.time()
+    wf.set_scene(scene)
+    log(f"set_scene (BVH build + upload): {time.time() - t0:.1f}s")
+    wf.resize(w, h)
+    film = hip.DeviceBuffer(w * h * 16)
+    stream = hip.Stream()
+    cam = scene.camera(0)
+
+    def frame(idx):
+        wf.begin_frame(cam, idx)
+        wf.render(film.ptr, stream.handle)
+        wf.end_frame()
+        return wf.stats()
+
+    for i in range(warmup):
+        frame(1000 + i)
+    stream.synchronize()
+    hip.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    tot = dict(closest=0, shadow=0, paths=0, trace_ms=0.0, shade_ms=0.0, shadow_ms=0.0, mega_ms=0.0, launches=0,
+               mega_closest=0, mega_shadow=0)
+    wall0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(steps):
+        st = frame(1 + i)
+        tot["closest"] += st["closest_rays"]
+        tot["shadow"] += st["shadow_rays"]
+        tot["paths"] += st["paths"]
+        tot["trace_ms"] += st["closest_trace_ms"]
+        tot["shade_ms"] += st["shade_ms"]
+        tot["shadow_ms"] += st["shadow_trace_ms"]
+        tot["mega_ms"] += st["megakernel_ms"]
+        tot["launches"] += st["closest_launches"]
+        tot["mega_closest"] += st["megakernel_closest_rays"]
+        tot["mega_shadow"] += st["megakernel_shadow_rays"]
+    ev1.record(stream)
+    ev1.synchronize()
+    stream.synchronize()
+    wall = time.perf_counter() - wall0
+    gpu_ms = ev0.elapsed_ms(ev1)
+    log(f"timed {steps} frames: wall {wall * 1e3:.1f} ms, events {gpu_ms:.1f} ms")
+    # Node / triangle counts of exactly these rays: same frame indices again,
+    # with the device counters on (outside the timed region).
+    wfc = krr.WavefrontPathTracer(dict(cfg["params"], count_traversal=True))
+    wfc.set_scene(scene)
+    wfc.resize(w, h)
+    keys = ("closest_rays", "shadow_rays", "node_visits", "tri_tests", "shadow_node_visits", "shadow_tri_tests",
+            "megakernel_closest_rays", "megakernel_shadow_rays", "megakernel_node_visits", "megakernel_tri_tests")
+    cnt = dict.fromkeys(keys, 0)
+    for i in range(steps):
+        wfc.begin_frame(cam, 1 + i)
+        wfc.render(film.ptr, stream.handle)
+        st = wfc.stats()
+        for key in keys:
+            cnt[key] += st[key]
+    assert cnt["closest_rays"] == tot["closest"] and cnt["shadow_rays"] == tot["shadow"], \
+        "counting pass traced different rays"
+    img = film.to_numpy(np.float32, (h, w, 4))
+    assert np.isfinite(img).all()
+    wfc.close()
+    wf.close()
+
+    rays = tot["closest"] + tot["shadow"]
+    secs = wall
+    kernels = kernel_rooflines(cfg, steps, tot, cnt)
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    k = kernels[dom]
+    tr = (traffic or {}).get(k["kernel"])
+    roofline = {"bound": "hbm", "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                "traffic": tr.get("bytes_per_launch") if tr else None,
+                "kernel": k["kernel"], "avg_launch_ms": k["avg_launch_ms"],
+                "algorithmic_bytes_per_launch": k["bytes_per_launch"],
+                "bytes_model": k["bytes_model"], "per_ray": k["per_ray"]}
+    if tr:
+        roofline["traffic_source"] = tr.get("source")
+    out = {
+        "value": round(rays / secs / 1e6, 2),
+        "ms_per_step": round(secs * 1e3 / steps, 3),
+        "mpaths_per_s": round(tot["paths"] / secs / 1e6, 2),
+        "closest_mrays_per_s": round(tot["closest"] / secs / 1e6, 2),
+        "rays_per_frame": rays // steps,
+        "kernel_ms_per_step": {"closest_trace": round(tot["trace_ms"] / steps, 3),
+                               "shade": round(tot["shade_ms"] / steps, 3),
+                               "shadow_trace": round(tot["shadow_ms"] / steps, 3),
+                               "megakernel": round(tot["mega_ms"] / steps, 3),
+                               "gpu_events_total": round(gpu_ms / steps, 3)},
+        "kernels": kernels,
+        "roofline": roofline,
+        "_scene": scene,
+    }
+    return out
+
+
+# Algorithmic bytes: what each kernel must move at least once per item, with
+# no re-reads - a BVH4 node is one 128-byte line, a leaf triangle 48 bytes,
+# a queued ray 32 bytes {o, tmax, d, meta} and its hit record 16 bytes.
+NODE_B, TRI_B, RAY_B, HIT_B = 128, 48, 32, 16
+
+
+def kernel_rooflines(cfg, steps, tot, cnt):
+    """Achieved algorithmic GB/s of each traversal kernel over the timed frames.
+
+    HIP-event times of the launches (time_kernels) and device counts of the
+    same frames (count_traversal, a second untimed pass)."""
+    spp = cfg["params"].get("spp", 1)
+    max_depth = cfg["params"].get("max_depth", 10)
+    wdepths = cfg["params"].get("wavefront_depths", 3)
+    mega_launches = steps * spp if wdepths <= max_depth else 0
+    wf_closest = cnt["closest_rays"] - cnt["megakernel_closest_rays"]
+    wf_shadow = cnt["shadow_rays"] - cnt["megakernel_shadow_rays"]
+    out = {}
+
+    def entry(name, kernel, ms, launches, byts, model, rays, nodes, tris):
+        out[name] = {"kernel": kernel, "ms_per_step": round(ms / steps, 4),
+                     "avg_launch_ms": round(ms / max(1, launches), 4), "launches_per_step": launches / steps,
+                     "bytes_per_launch": int(byts / max(1, launches)), "bytes_model": model,
+                     "achieved_gbs": round(byts / (ms * 1e-3) / 1e9, 2) if ms > 0 else 0.0,
+                     "per_ray": {"node_visits": round(nodes / max(1, rays), 2),
+                                 "tri_tests": round(tris / max(1, rays), 2)}}
+
+    wf_launches = tot["launches"] - mega_launches
+    entry("closest_trace", "k_trace_closest_queue", tot["trace_ms"], wf_launches,
+          wf_closest * (RAY_B * 2 + HIT_B) + cnt["node_visits"] * NODE_B + cnt["tri_tests"] * TRI_B,
+          "64 B ray + 16 B hit per ray + 128 B per node visit + 48 B per triangle test",
+          wf_closest, cnt["node_visits"], cnt["tri_tests"])
+    shadow_launches = 0
+    if cfg["params"].get("nee", True):  # no shadow rays are traced at the last depth
+        shadow_launches = wf_launches - (steps * spp if max_depth < wdepths else 0)
+    media = bool(cfg["scene"].get("media")) and cfg["params"].get("enable_medium", True)
+    entry("shadow_trace", "k_shadow_tr" if media else "k_trace_shadow", tot["shadow_ms"], shadow_launches,
+          wf_shadow * (RAY_B * 2 + 16) + cnt["shadow_node_visits"] * NODE_B + cnt["shadow_tri_tests"] * TRI_B,
+          "64 B ray + 16 B contribution per ray + 128 B per node visit + 48 B per triangle test",
+          wf_shadow, cnt["shadow_node_visits"], cnt["shadow_tri_tests"])
+    mrays = cnt["megakernel_closest_rays"] + cnt["megakernel_shadow_rays"]
+    entry("megakernel", "k_megakernel", tot["mega_ms"], mega_launches,
+          cnt["megakernel_node_visits"] * NODE_B + cnt["megakernel_tri_tests"] * TRI_B,
+          "traversal only (lower bound): 128 B per node visit + 48 B per triangle test, "
+          "closest and shadow rays",
+          mrays, cnt["megakernel_node_visits"], cnt["megakernel_tri_tests"])
+    return out
+
+
+def cpu_baseline(cfg, scene, budget_s=12.0, max_frames=20):
+    import oracle.binding as ob
+    w, h = cfg["scene"]["resolution"]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        cores = os.cpu_count() or 1
+    cores = min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)))
+    rays = paths = 0
+    secs = 0.0
+    frames = 0
+    cam = scene.camera(0)
+    prm = dict(cfg["params"])
+    kw = {k: int(v) if isinstance(v, bool) else v for k, v in prm.items()
+          if k in ("spp", "max_depth", "rr", "rr_min_depth")}
+    while frames < max_frames and secs < budget_s:
+        _, st = ob.render(scene.desc, cam, w, h, 1 + frames, n_threads=cores, **kw)
+        rays += st["closest_rays"] + st["shadow_rays"]
+        paths += st["paths"]
+        secs += st["seconds"]
+        frames += 1
+        cores_used = st["threads"]
+    return {"value": round(rays / secs / 1e6, 3), "unit": "Mrays/s", "cores": cores_used, "kind": "port",
+            "sample": f"{frames} full frame(s) {w}x{h} of the same scene and params (frames 1..{frames}), "
+                      f"{paths} paths, {secs:.1f}s of render time, BVH build excluded",
+            "mpaths_per_s": round(paths / secs / 1e6, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is described in DESIGN.md "
+              "and not launched (use --gpus 1)", file=sys.stderr)
+        raise SystemExit(2)
+    cfg = CONFIGS[args.config]
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)
+    r = gpu_bench(cfg, args.steps, args.warmup, traffic)
+    scene = r.pop("_scene")
+    line = {"metric": METRIC, "value": r["value"], "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic (procedural seeded scene)",
+            "config": {"workload": cfg["desc"], "triangles": scene.n_triangles,
+                       "resolution": cfg["scene"]["resolution"], **cfg["params"]},
+            "mpaths_per_s": r["mpaths_per_s"], "closest_mrays_per_s": r["closest_mrays_per_s"],
+            "rays_per_frame": r["rays_per_frame"], "kernel_ms_per_step": r["kernel_ms_per_step"],
+            "kernels": r["kernels"],
+            "roofline": r["roofline"]}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(cfg, scene, budget_s=args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -135,15 +135,20 @@ typedef struct krr_stats {
 	uint64_t closest_rays;	 /* rays traced for closest hits, summed over depth */
 	uint64_t shadow_rays;	 /* shadow rays traced */
 	uint64_t paths;			 /* camera samples started: W*H*spp */
-	uint64_t node_visits;	 /* BVH nodes visited by closest-hit rays ("count_traversal") */
-	uint64_t tri_tests;		 /* ray-triangle tests of closest-hit rays ("count_traversal") */
-	uint64_t shadow_node_visits; /* same for shadow rays */
+	uint64_t node_visits;	 /* BVH nodes visited by closest-hit rays of the per-depth kernels ("count_traversal") */
+	uint64_t tri_tests;		 /* their ray-triangle tests ("count_traversal") */
+	uint64_t shadow_node_visits; /* same for shadow rays of the per-depth kernels */
 	uint64_t shadow_tri_tests;
 	uint64_t closest_launches; /* number of closest-hit trace kernels launched */
 	double closest_trace_ms; /* summed HIP-event time of the closest-hit trace kernels ("time_kernels") */
 	double shade_ms;		 /* summed time of the shade kernels ("time_kernels") */
 	double shadow_trace_ms;	 /* summed time of the shadow trace kernels ("time_kernels") */
 	double render_ms;		 /* HIP-event time of the whole render call ("time_kernels") */
+	double megakernel_ms;	 /* summed time of the tail megakernel ("time_kernels") */
+	uint64_t megakernel_closest_rays; /* closest-hit rays traced inside the tail megakernel (part of closest_rays) */
+	uint64_t megakernel_shadow_rays;  /* shadow rays traced inside it (part of shadow_rays) */
+	uint64_t megakernel_node_visits; /* BVH node visits of both ("count_traversal") */
+	uint64_t megakernel_tri_tests;	 /* their ray-triangle tests ("count_traversal") */
 } krr_stats;
 
 /* Closest hit record: t along the (unnormalised) ray direction, primitive
@@ -192,9 +197,14 @@ int krr_scene_destroy(krr_scene *scene);
 
 typedef struct krr_accel krr_accel;
 
+/* json_opts (NULL or "" = defaults): leaf_size (4) max triangles per leaf,
+ * stack_lds_entries (16) traversal stack entries per lane kept in LDS, the
+ * deeper ones in a global overflow area. */
 int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_accel **out);
 /* rays_dev: n records of 8 floats {ox,oy,oz,tmax,dx,dy,dz,0} in device memory.
- * Closest hit in (0, tmax); ties in t go to the lowest primitive index. */
+ * Closest hit in (0, tmax); ties in t go to the lowest primitive index.
+ * n <= 2^30.  Trace calls on one accel share its work counter and stack
+ * overflow area: they must not overlap (same stream, or synchronized). */
 int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev,
 							void *hip_stream);
 /* visible_dev[i] = 1 if no triangle of a non-null material is hit in (0, tmax). */
@@ -221,6 +231,10 @@ typedef struct krr_wf krr_wf;
  *   count_traversal (false)  count node visits / triangle tests
  *   time_kernels (false)     time each trace / shade launch with HIP events
  *   block_size (256), trace_block_size (128)  launch shapes
+ *   wavefront_depths (3)     depths run as separate wavefront stages; the
+ *                            remaining depths of every path run in one
+ *                            persistent launch (same per-path stage order)
+ *   leaf_size (4), stack_lds_entries (16)  as for krr_accel_build
  * NULL or "" means all defaults. */
 int krr_wf_create(const char *json_params, krr_wf **out);
 /* Uploads the scene and builds the acceleration structure (setScene). */

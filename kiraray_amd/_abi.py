@@ -66,7 +66,10 @@ class krr_stats(C.Structure):
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("closest_launches", C.c_uint64),
                 ("closest_trace_ms", C.c_double), ("shade_ms", C.c_double), ("shadow_trace_ms", C.c_double),
-                ("render_ms", C.c_double)]
+                ("render_ms", C.c_double), ("megakernel_ms", C.c_double),
+                ("megakernel_closest_rays", C.c_uint64), ("megakernel_shadow_rays", C.c_uint64),
+                ("megakernel_node_visits", C.c_uint64),
+                ("megakernel_tri_tests", C.c_uint64)]
 
 
 class krr_hit(C.Structure):

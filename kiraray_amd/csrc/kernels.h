@@ -47,11 +47,16 @@ struct Counters {
 	int pad0;
 	unsigned long long closestTotal;
 	unsigned long long shadowTotal;
-	unsigned long long nodeVisits;	   // closest-hit traversal
+	unsigned long long nodeVisits;	   // closest-hit traversal of the wavefront kernels
 	unsigned long long triTests;
-	unsigned long long shadowNodeVisits; // shadow traversal
+	unsigned long long shadowNodeVisits; // shadow traversal of the wavefront kernels
 	unsigned long long shadowTriTests;
 	unsigned long long closestLaunches;
+	unsigned long long megaClosest;	   // closest-hit rays traced in the tail megakernel
+	unsigned long long megaShadow;	   // shadow rays traced in the tail megakernel
+	unsigned long long megaNodeVisits; // closest + shadow traversal in the megakernel
+	unsigned long long megaTriTests;
+	int work[4]; // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
 };
 
 struct CameraDev {
@@ -89,6 +94,14 @@ KRR_DEV int wavePush(int *counter, bool want) {
 	if (!want) return -1;
 	const uint64_t below = laneId() == 0 ? 0ull : (mask & (~0ull >> (64 - laneId())));
 	return base + __popcll(below);
+}
+
+// Persistent-kernel work distribution: each wave takes the next 64 items of
+// the queue with one atomic (wave-uniform result).
+KRR_DEV int waveFetch(int *work) {
+	int base = 0;
+	if (laneId() == 0) base = atomicAdd(work, 64);
+	return __shfl(base, 0);
 }
 
 // Sum of a per-lane counter over the wave, added to a 64-bit global counter

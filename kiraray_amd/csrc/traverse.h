@@ -25,6 +25,8 @@ namespace krr {
 struct TraceBvh {
 	const Bvh4Node *nodes;
 	const LeafTri *tris;
+	int stackLds; // traversal stack entries held in LDS per lane
+	int nNodes;	  // node count: a traversal visits each node at most once
 };
 
 struct RayPre {
@@ -91,11 +93,31 @@ struct TraceResult {
 	float b1, b2;
 };
 
+// Per-lane traversal stack: the first nLds entries in LDS (entries `stride`
+// ints apart, so a wave's 64 lanes hit 64 banks), deeper entries in a global
+// overflow area laid out lane-fastest (entries `ovStride` ints apart:
+// coalesced when the lanes of a wave spill together).  A small LDS footprint
+// keeps the traversal kernels at high occupancy; deep stacks are rare but
+// stay exact.  nLds is uniform over the launch (kShortStack by default).
+constexpr int kShortStack = 16;
+
+struct LaneStack {
+	int *lds;
+	int stride;
+	int *ovf;
+	int ovStride;
+	int nLds;
+	KRR_DEV void put(int sp, int v) const {
+		if (sp < nLds) lds[sp * stride] = v;
+		else ovf[(size_t) (sp - nLds) * ovStride] = v;
+	}
+	KRR_DEV int get(int sp) const { return sp < nLds ? lds[sp * stride] : ovf[(size_t) (sp - nLds) * ovStride]; }
+};
+
 // Traverses one ray.  ANY: stop at the first hit of a non-null-material
-// triangle (shadow rays, device.cu:83-100).  stack: this lane's LDS slot 0,
-// entries `stride` ints apart.
+// triangle (shadow rays, device.cu:83-100).
 template <bool ANY, bool COUNT>
-KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, int *stack, int stride,
+KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, const LaneStack &stack,
 							 uint32_t &nodeVisits, uint32_t &triTests) {
 	TraceResult res{tmax, -1, 0.f, 0.f};
 	const RayPre rp = rayPre(o, d);
@@ -105,7 +127,11 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 	const float kTie   = 1.00000095367431640625f;		// 1 + 2^-20
 	int sp			   = 0;
 	int node		   = 0;
-	while (true) {
+	// Visits beyond the node count can only come from a corrupt stack or
+	// tree; the budget turns that into a finished (wrong) traversal instead
+	// of a wave that never ends.
+	int budget = bvh.nNodes;
+	while (--budget >= 0) {
 		if (COUNT) nodeVisits++;
 		const float4 *np = reinterpret_cast<const float4 *>(bvh.nodes + node);
 		const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
@@ -177,12 +203,12 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 #undef KRR_CSWAP
 		if (n == 0) {
 			if (sp == 0) break;
-			node = stack[--sp * stride];
+			node = stack.get(--sp);
 			continue;
 		}
-		if (n >= 4) stack[sp++ * stride] = c3;
-		if (n >= 3) stack[sp++ * stride] = c2;
-		if (n >= 2) stack[sp++ * stride] = c1;
+		if (n >= 4) stack.put(sp++, c3);
+		if (n >= 3) stack.put(sp++, c2);
+		if (n >= 2) stack.put(sp++, c1);
 		node = c0;
 	}
 	return res;

@@ -48,6 +48,10 @@ __global__ void k_reset_totals(Counters *cnt) {
 	cnt->shadowNodeVisits = 0;
 	cnt->shadowTriTests	  = 0;
 	cnt->closestLaunches  = 0;
+	cnt->megaClosest	  = 0;
+	cnt->megaShadow		  = 0;
+	cnt->megaNodeVisits	  = 0;
+	cnt->megaTriTests	  = 0;
 	cnt->rayCount[0] = cnt->rayCount[1] = cnt->shadowCount = 0;
 }
 
@@ -121,21 +125,63 @@ __global__ void k_depth_begin(Counters *cnt, int cur) {
 	cnt->closestLaunches += 1;
 	cnt->rayCount[nxt] = 0;
 	cnt->shadowCount   = 0;
+	cnt->work[0] = cnt->work[1] = 0;
 }
 
-// Closest-hit trace over the first *count rays (count from device memory, so
-// the host never waits for queue sizes).
+// This lane's traversal stack: LDS slots of the block, overflow slots of the
+// grid (ovf holds (stack size - kShortStack) * grid threads ints).
+KRR_DEV LaneStack laneStack(const TraceBvh &bvh, int *lds, int *ovf) {
+	return LaneStack{lds + threadIdx.x, (int) blockDim.x, ovf + (size_t) blockIdx.x * blockDim.x + threadIdx.x, (int) (gridDim.x * blockDim.x),
+					 bvh.stackLds};
+}
+
+// Closest-hit trace of one ray queue entry into its hit record.
 template <bool COUNT>
-__global__ void k_trace_closest(TraceBvh bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, const int *countPtr,
-								int fixedCount, Counters *cnt) {
+KRR_DEV void traceClosestItem(const TraceBvh &bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, int i,
+							  const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
+	const float4 a = rays[2 * i], b = rays[2 * i + 1];
+	TraceResult r  = traverse<false, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
+	hits[i]		   = make_float4(r.prim >= 0 ? r.t : __builtin_inff(), __int_as_float(r.prim), r.b1, r.b2);
+}
+
+// Persistent closest-hit trace of the current ray queue: a grid sized to the
+// resident capacity of the GPU, waves taking 64 rays at a time until the
+// count in device memory is drained (no host round trip, no launch of empty
+// workgroups when the queue is short, per-wave load balance).
+template <bool COUNT>
+__global__ void k_trace_closest_queue(TraceBvh bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, const int *countPtr,
+									  int *ovf, Counters *cnt) {
 	extern __shared__ int ldsStack[];
-	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
-	const int count = countPtr ? *countPtr : fixedCount;
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
+	const int count		  = *countPtr;
 	uint32_t nv = 0, nt = 0;
-	if (i < count) {
-		const float4 a = rays[2 * i], b = rays[2 * i + 1];
-		TraceResult r  = traverse<false, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
-		hits[i]		   = make_float4(r.prim >= 0 ? r.t : __builtin_inff(), __int_as_float(r.prim), r.b1, r.b2);
+	while (true) {
+		const int base = waveFetch(&cnt->work[0]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i < count) traceClosestItem<COUNT>(bvh, rays, hits, i, stack, nv, nt);
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->nodeVisits, nv);
+		waveAddU64(&cnt->triTests, nt);
+	}
+}
+
+__global__ void k_accel_begin(Counters *cnt) { cnt->work[0] = 0; }
+
+// Closest-hit trace of a caller's ray array (krr_accel_trace_closest),
+// persistent like the queue version.
+template <bool COUNT>
+__global__ void k_trace_closest(TraceBvh bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, int count, int *ovf,
+								Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
+	uint32_t nv = 0, nt = 0;
+	while (true) {
+		const int base = waveFetch(&cnt->work[0]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i < count) traceClosestItem<COUNT>(bvh, rays, hits, i, stack, nv, nt);
 	}
 	if (COUNT) {
 		waveAddU64(&cnt->nodeVisits, nv);
@@ -145,14 +191,20 @@ __global__ void k_trace_closest(TraceBvh bvh, const float4 *__restrict__ rays, f
 
 // Any-hit trace with a visibility result per ray (krr_accel_trace_any).
 template <bool COUNT>
-__global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *__restrict__ visible, int count, Counters *cnt) {
+__global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *__restrict__ visible, int count, int *ovf,
+							Counters *cnt) {
 	extern __shared__ int ldsStack[];
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	uint32_t nv = 0, nt = 0;
-	if (i < count) {
-		const float4 a = rays[2 * i], b = rays[2 * i + 1];
-		TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
-		visible[i]	   = r.prim < 0 ? 1 : 0;
+	while (true) {
+		const int base = waveFetch(&cnt->work[0]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i < count) {
+			const float4 a = rays[2 * i], b = rays[2 * i + 1];
+			TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
+			visible[i]	   = r.prim < 0 ? 1 : 0;
+		}
 	}
 	if (COUNT) {
 		waveAddU64(&cnt->nodeVisits, nv);
@@ -163,30 +215,7 @@ __global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *
 // traceShadow without media (device.cu:83-100): any hit on a non-null
 // triangle in (0, 1) occludes; visible shadow rays add Ld / mean(pu + pl).
 template <bool COUNT>
-__global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, Counters *cnt) {
-	extern __shared__ int ldsStack[];
-	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
-	const int count = cnt->shadowCount;
-	if (i == 0) cnt->shadowTotal += (unsigned long long) count;
-	uint32_t nv = 0, nt = 0;
-	if (i < count) {
-		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1];
-		TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, ldsStack + threadIdx.x, blockDim.x, nv, nt);
-		if (r.prim < 0) {
-			const int pixel = __float_as_int(b.w);
-			const float4 c	= sq.c[i];
-			float4 L		= px.L[pixel];
-			L.x				= L.x + c.x;
-			L.y				= L.y + c.y;
-			L.z				= L.z + c.z;
-			px.L[pixel]		= L;
-		}
-	}
-	if (COUNT) {
-		waveAddU64(&cnt->shadowNodeVisits, nv);
-		waveAddU64(&cnt->shadowTriTests, nt);
-	}
-}
+__global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, int *ovf, Counters *cnt);
 
 // sampleT_maj (wavefront.h:163-207) on the medium of the ray.  callback(p,
 // mp, sigma_maj, T_maj) returns false to stop; the result is the remaining
@@ -234,254 +263,269 @@ KRR_DEV f3 sampleTmaj(const DevScene &sc, int medium, f3 o, f3 d, float tMax, Pc
 // (integrator.cpp:183-195), handleMiss (:197-213), the depth == maxDepth
 // break, sampleMediumScattering (medium.cpp:105-153) and
 // generateScatterRays (integrator.cpp:215-269).
+struct ShadowOut {
+	f3 o, d, Ld, pu, pl;
+	int medium;
+};
+
+// One ray of the queue through the stages above; new work is returned in
+// (wantNext, nx) and (wantShadow, so), radiance and RNG state go to memory.
 template <bool MEDIA>
-__global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueueDev cur, const float4 *__restrict__ hits,
-											  RayQueueDev nxt, ShadowQueueDev sq, PixelStateDev px, Counters *cnt, int curIdx,
-											  int loopDepth) {
-	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
-	const int count = cnt->rayCount[curIdx];
-	bool wantNext = false, wantShadow = false;
-	RayItem nx;
-	f3 sO, sD, sLd, sPu, sPl;
-	int sMedium = -1;
-	int pixel	= 0;
-	if (i < count) {
-		const RayItem r = loadRayItem(cur, i);
-		pixel			= r.pixel;
-		const float4 h	= hits[i];
-		const int prim	= __float_as_int(h.y);
-		Pcg rng{px.rng[pixel], P.inc};
-		const int channel = MEDIA ? px.channel[pixel] : 0;
-		const float4 Lv	  = px.L[pixel];
-		f3 L(Lv.x, Lv.y, Lv.z);
-		f3 thp = r.thp, pu = r.pu, pl = r.pl;
-		bool hasHL = false, hasScatter = false, hasMiss = false, hasMS = false, hasMSc = false;
-		SurfaceInteraction intr;
-		float msTmax   = __builtin_inff();
-		bool msHasIntr = false;
-		f3 mscP, mscThp, mscPu, mscWo;
-		float mscG = 0.f;
-		if (prim >= 0) {
-			prepareSurfaceInteraction(sc, prim, h.z, h.w, r.d, r.medium, intr);
-			if (MEDIA && P.enableMedium && r.medium >= 0) {
-				hasMS	  = true;
-				msTmax	  = h.x;
-				msHasIntr = true;
-			} else if (intr.material < 0) {
+KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, const PixelStateDev &px, int loopDepth,
+					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so) {
+	const int pixel = r.pixel;
+	f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
+	int &sMedium = so.medium;
+	const int prim	= __float_as_int(h.y);
+	Pcg rng{px.rng[pixel], P.inc};
+	const int channel = MEDIA ? px.channel[pixel] : 0;
+	const float4 Lv	  = px.L[pixel];
+	f3 L(Lv.x, Lv.y, Lv.z);
+	f3 thp = r.thp, pu = r.pu, pl = r.pl;
+	bool hasHL = false, hasScatter = false, hasMiss = false, hasMS = false, hasMSc = false;
+	SurfaceInteraction intr;
+	float msTmax   = __builtin_inff();
+	bool msHasIntr = false;
+	f3 mscP, mscThp, mscPu, mscWo;
+	float mscG = 0.f;
+	if (prim >= 0) {
+		prepareSurfaceInteraction(sc, prim, h.z, h.w, r.d, r.medium, intr);
+		if (MEDIA && P.enableMedium && r.medium >= 0) {
+			hasMS	  = true;
+			msTmax	  = h.x;
+			msHasIntr = true;
+		} else if (intr.material < 0) {
+			wantNext  = true;
+			nx		  = r;
+			nx.o	  = offsetRayOrigin(intr.p, intr.n, r.d);
+			nx.medium = intr.getMedium(r.d);
+		} else {
+			if (intr.light >= 0) hasHL = true;
+			if (thp.any()) hasScatter = true;
+		}
+	} else {
+		if (MEDIA && P.enableMedium && r.medium >= 0) hasMS = true;
+		else hasMiss = true;
+	}
+
+	if (MEDIA && hasMS) {
+		f3 Lm;
+		bool scattered = false;
+		const f3 T_maj = sampleTmaj(sc, r.medium, r.o, r.d, msTmax, rng, channel,
+			[&](const f3 &p, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
+				if (r.depth < P.maxDepth && mp.Le.any()) {
+					const float pr = sigma_maj[channel] * Tm[channel];
+					const f3 pe	   = pu * sigma_maj * Tm / pr;
+					if (pe.any()) Lm = Lm + thp * mp.sigma_a * Tm * mp.Le / (pr * pe.mean());
+				}
+				const float pAbsorb	 = mp.sigma_a[channel] / sigma_maj[channel];
+				const float pScatter = mp.sigma_s[channel] / sigma_maj[channel];
+				const float pNull	 = maxf(0.f, 1.f - pAbsorb - pScatter);
+				const float w[3]	 = {pAbsorb, pScatter, pNull};
+				const int mode		 = sampleDiscrete3(w, rng.get1D());
+				if (mode == 0) {
+					thp = f3();
+					return false;
+				} else if (mode == 1) {
+					const float pr = Tm[channel] * mp.sigma_s[channel];
+					thp			   = thp * (Tm * mp.sigma_s / pr);
+					pu			   = pu * (Tm * mp.sigma_s / pr);
+					if (thp.any() && pu.any()) {
+						hasMSc = true;
+						mscP   = p;
+						mscThp = thp;
+						mscPu  = pu;
+						mscWo  = -r.d;
+						mscG   = mp.g;
+					}
+					scattered = true;
+					return false;
+				} else {
+					const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
+					const float pr	 = Tm[channel] * sigma_n[channel];
+					thp				 = thp * (Tm * sigma_n / pr);
+					if (pr == 0.f) thp = f3();
+					pu = pu * (Tm * sigma_n / pr);
+					pl = pl * (Tm * sigma_maj / pr);
+					return thp.any() && pu.any();
+				}
+			});
+		if (Lm.any()) L = L + Lm;
+		if (!scattered && thp.any()) {
+			thp = thp * (T_maj / T_maj[channel]);
+			pu	= pu * (T_maj / T_maj[channel]);
+			pl	= pl * (T_maj / T_maj[channel]);
+		}
+		if (!(scattered || !thp.any() || !pu.any() || r.depth == P.maxDepth)) {
+			if (!msHasIntr) hasMiss = true;
+			else if (intr.material < 0) {
 				wantNext  = true;
 				nx		  = r;
 				nx.o	  = offsetRayOrigin(intr.p, intr.n, r.d);
 				nx.medium = intr.getMedium(r.d);
+				nx.thp	  = thp;
+				nx.pu	  = pu;
+				nx.pl	  = pl;
 			} else {
 				if (intr.light >= 0) hasHL = true;
-				if (thp.any()) hasScatter = true;
-			}
-		} else {
-			if (MEDIA && P.enableMedium && r.medium >= 0) hasMS = true;
-			else hasMiss = true;
-		}
-
-		if (MEDIA && hasMS) {
-			f3 Lm;
-			bool scattered = false;
-			const f3 T_maj = sampleTmaj(sc, r.medium, r.o, r.d, msTmax, rng, channel,
-				[&](const f3 &p, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
-					if (r.depth < P.maxDepth && mp.Le.any()) {
-						const float pr = sigma_maj[channel] * Tm[channel];
-						const f3 pe	   = pu * sigma_maj * Tm / pr;
-						if (pe.any()) Lm = Lm + thp * mp.sigma_a * Tm * mp.Le / (pr * pe.mean());
-					}
-					const float pAbsorb	 = mp.sigma_a[channel] / sigma_maj[channel];
-					const float pScatter = mp.sigma_s[channel] / sigma_maj[channel];
-					const float pNull	 = maxf(0.f, 1.f - pAbsorb - pScatter);
-					const float w[3]	 = {pAbsorb, pScatter, pNull};
-					const int mode		 = sampleDiscrete3(w, rng.get1D());
-					if (mode == 0) {
-						thp = f3();
-						return false;
-					} else if (mode == 1) {
-						const float pr = Tm[channel] * mp.sigma_s[channel];
-						thp			   = thp * (Tm * mp.sigma_s / pr);
-						pu			   = pu * (Tm * mp.sigma_s / pr);
-						if (thp.any() && pu.any()) {
-							hasMSc = true;
-							mscP   = p;
-							mscThp = thp;
-							mscPu  = pu;
-							mscWo  = -r.d;
-							mscG   = mp.g;
-						}
-						scattered = true;
-						return false;
-					} else {
-						const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
-						const float pr	 = Tm[channel] * sigma_n[channel];
-						thp				 = thp * (Tm * sigma_n / pr);
-						if (pr == 0.f) thp = f3();
-						pu = pu * (Tm * sigma_n / pr);
-						pl = pl * (Tm * sigma_maj / pr);
-						return thp.any() && pu.any();
-					}
-				});
-			if (Lm.any()) L = L + Lm;
-			if (!scattered && thp.any()) {
-				thp = thp * (T_maj / T_maj[channel]);
-				pu	= pu * (T_maj / T_maj[channel]);
-				pl	= pl * (T_maj / T_maj[channel]);
-			}
-			if (!(scattered || !thp.any() || !pu.any() || r.depth == P.maxDepth)) {
-				if (!msHasIntr) hasMiss = true;
-				else if (intr.material < 0) {
-					wantNext  = true;
-					nx		  = r;
-					nx.o	  = offsetRayOrigin(intr.p, intr.n, r.d);
-					nx.medium = intr.getMedium(r.d);
-					nx.thp	  = thp;
-					nx.pu	  = pu;
-					nx.pl	  = pl;
-				} else {
-					if (intr.light >= 0) hasHL = true;
-					hasScatter = true;
-				}
+				hasScatter = true;
 			}
 		}
-
-		const float lightSelectPdf = 1.f / (float) sc.nLights;
-		if (hasHL) { // handleHit
-			const DevLight Lr = sc.lights[intr.light];
-			f3 Le			  = areaLightL(Lr, intr.n, intr.wo) * thp;
-			if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
-				const float lightPdf = lightPdfLi(sc, Lr, intr.p, intr.n, r.cp) * lightSelectPdf;
-				Le					 = Le / (pl * lightPdf + pu).mean();
-			} else Le = Le / pu.mean();
-			L = L + Le;
-		}
-		if (hasMiss) { // handleMiss
-			f3 Lm;
-			for (int li = 0; li < sc.nLights; li++) {
-				const DevLight Lr = sc.lights[li];
-				if (Lr.type != 1) continue;
-				const f3 Li = Lr.scale * f3(Lr.Le[0], Lr.Le[1], Lr.Le[2]);
-				if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
-					const float lp = kInv4Pi * lightSelectPdf;
-					Lm			   = Lm + Li / (pu + pl * lp).mean();
-				} else Lm = Lm + Li / pu.mean();
-			}
-			L = L + thp * Lm;
-		}
-
-		if (loopDepth != P.maxDepth) {
-			if (MEDIA && hasMSc) { // sampleMediumScattering
-				if (P.nee && sc.nLights > 0) {
-					const float ul	 = rng.get1D();
-					const int li	 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
-					const float u0	 = rng.get1D(), u1 = rng.get1D();
-					const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, mscP);
-					const f3 to		 = offsetRayOrigin(ls.p, ls.n, mscP - ls.p);
-					const f3 p_o	 = offsetRayOrigin(mscP, f3(), to - mscP);
-					const f3 sdir	 = to - p_o;
-					const f3 wi		 = normalize(sdir);
-					const f3 thpS	 = mscThp * hgP(mscG, mscWo, wi);
-					const float lightPdf = lightSelectPdf * ls.pdf;
-					const float phasePdf = hgP(mscG, mscWo, wi);
-					const f3 Ld			 = thpS * ls.L;
-					if (Ld.any() && lightPdf > 0.f) {
-						wantShadow = true;
-						sO		   = p_o;
-						sD		   = sdir;
-						sLd		   = Ld;
-						sPu		   = mscPu * phasePdf;
-						sPl		   = mscPu * lightPdf;
-						sMedium	   = r.medium;
-					}
-				}
-				const float u0 = rng.get1D(), u1 = rng.get1D();
-				float ppdf;
-				const f3 wi	   = hgSample(mscG, mscWo, u0, u1, &ppdf);
-				f3 thpN		   = mscThp * ppdf / ppdf;
-				const float rrProb = (thpN / mscPu.mean()).maxCoeff();
-				bool alive		   = true;
-				if (r.depth >= 1 && rrProb < 1.f) {
-					if (rng.get1D() >= rrProb) alive = false;
-					else thpN = thpN / rrProb;
-				}
-				if (alive && !thpN.isZero() && !thpN.hasNaN()) {
-					wantNext	= true;
-					nx.o		= mscP;
-					nx.d		= wi;
-					nx.medium	= r.medium;
-					nx.cp		= mscP;
-					nx.cn		= f3();
-					nx.thp		= thpN;
-					nx.pu		= mscPu;
-					nx.pl		= mscPu / ppdf;
-					nx.depth	= r.depth + 1;
-					nx.bsdfType = BSDF_SMOOTH;
-					nx.pixel	= pixel;
-				}
-			}
-			if (hasScatter) { // generateScatterRays
-				bool alive = true;
-				if (r.depth >= P.rrMinDepth) {
-					if (rng.get1D() >= P.rr) alive = false;
-					else thp = thp / P.rr;
-				}
-				if (alive) {
-					const f3 woL		  = intr.toLocal(intr.wo);
-					const DevMaterial mat = sc.materials[intr.material];
-					const int bt		  = bsdfTypeOf(mat.type, mat.roughness);
-					BSDF bsdf;
-					bsdf.setup(mat);
-					if (P.nee && (bt & BSDF_SMOOTH) && sc.nLights > 0) {
-						const float ul		 = rng.get1D();
-						const int li		 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
-						const float u0		 = rng.get1D(), u1 = rng.get1D();
-						const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
-						const f3 to			 = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
-						const f3 p_o		 = offsetRayOrigin(intr.p, intr.n, to - intr.p);
-						const f3 sdir		 = to - p_o;
-						const f3 wiL		 = intr.toLocal(normalize(sdir));
-						const float lightPdf = lightSelectPdf * ls.pdf;
-						const f3 bv			 = bsdf.f(woL, wiL);
-						const float bp		 = bsdf.pdf(woL, wiL);
-						if (lightPdf > 0.f && bv.any()) {
-							const f3 Ld = ls.L * thp * bv * fabsf(wiL.z);
-							if (Ld.any()) {
-								wantShadow = true;
-								sO		   = p_o;
-								sD		   = sdir;
-								sLd		   = Ld;
-								sPu		   = pu * bp;
-								sPl		   = pu * lightPdf;
-								sMedium	   = intr.getMedium(sdir);
-							}
-						}
-					}
-					const BSDFSample s = bsdf.sample(woL, rng);
-					if (s.pdf != 0.f && s.f.any()) {
-						const f3 wiW  = intr.toWorld(s.wi);
-						const f3 nthp = thp * s.f * fabsf(s.wi.z) / s.pdf;
-						if (nthp.any()) {
-							wantNext	= true;
-							nx.bsdfType = s.flags;
-							nx.pu		= pu;
-							nx.pl		= pu / s.pdf;
-							nx.o		= offsetRayOrigin(intr.p, intr.n, wiW);
-							nx.d		= wiW;
-							nx.medium	= intr.getMedium(wiW);
-							nx.cp		= intr.p;
-							nx.cn		= intr.n;
-							nx.depth	= r.depth + 1;
-							nx.thp		= nthp;
-							nx.pixel	= pixel;
-						}
-					}
-				}
-			}
-		}
-		px.L[pixel]	  = make_float4(L.x, L.y, L.z, 0.f);
-		px.rng[pixel] = rng.state;
 	}
+
+	const float lightSelectPdf = 1.f / (float) sc.nLights;
+	if (hasHL) { // handleHit
+		const DevLight Lr = sc.lights[intr.light];
+		f3 Le			  = areaLightL(Lr, intr.n, intr.wo) * thp;
+		if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
+			const float lightPdf = lightPdfLi(sc, Lr, intr.p, intr.n, r.cp) * lightSelectPdf;
+			Le					 = Le / (pl * lightPdf + pu).mean();
+		} else Le = Le / pu.mean();
+		L = L + Le;
+	}
+	if (hasMiss) { // handleMiss
+		f3 Lm;
+		for (int li = 0; li < sc.nLights; li++) {
+			const DevLight Lr = sc.lights[li];
+			if (Lr.type != 1) continue;
+			const f3 Li = Lr.scale * f3(Lr.Le[0], Lr.Le[1], Lr.Le[2]);
+			if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
+				const float lp = kInv4Pi * lightSelectPdf;
+				Lm			   = Lm + Li / (pu + pl * lp).mean();
+			} else Lm = Lm + Li / pu.mean();
+		}
+		L = L + thp * Lm;
+	}
+
+	if (loopDepth != P.maxDepth) {
+		if (MEDIA && hasMSc) { // sampleMediumScattering
+			if (P.nee && sc.nLights > 0) {
+				const float ul	 = rng.get1D();
+				const int li	 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
+				const float u0	 = rng.get1D(), u1 = rng.get1D();
+				const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, mscP);
+				const f3 to		 = offsetRayOrigin(ls.p, ls.n, mscP - ls.p);
+				const f3 p_o	 = offsetRayOrigin(mscP, f3(), to - mscP);
+				const f3 sdir	 = to - p_o;
+				const f3 wi		 = normalize(sdir);
+				const f3 thpS	 = mscThp * hgP(mscG, mscWo, wi);
+				const float lightPdf = lightSelectPdf * ls.pdf;
+				const float phasePdf = hgP(mscG, mscWo, wi);
+				const f3 Ld			 = thpS * ls.L;
+				if (Ld.any() && lightPdf > 0.f) {
+					wantShadow = true;
+					sO		   = p_o;
+					sD		   = sdir;
+					sLd		   = Ld;
+					sPu		   = mscPu * phasePdf;
+					sPl		   = mscPu * lightPdf;
+					sMedium	   = r.medium;
+				}
+			}
+			const float u0 = rng.get1D(), u1 = rng.get1D();
+			float ppdf;
+			const f3 wi	   = hgSample(mscG, mscWo, u0, u1, &ppdf);
+			f3 thpN		   = mscThp * ppdf / ppdf;
+			const float rrProb = (thpN / mscPu.mean()).maxCoeff();
+			bool alive		   = true;
+			if (r.depth >= 1 && rrProb < 1.f) {
+				if (rng.get1D() >= rrProb) alive = false;
+				else thpN = thpN / rrProb;
+			}
+			if (alive && !thpN.isZero() && !thpN.hasNaN()) {
+				wantNext	= true;
+				nx.o		= mscP;
+				nx.d		= wi;
+				nx.medium	= r.medium;
+				nx.cp		= mscP;
+				nx.cn		= f3();
+				nx.thp		= thpN;
+				nx.pu		= mscPu;
+				nx.pl		= mscPu / ppdf;
+				nx.depth	= r.depth + 1;
+				nx.bsdfType = BSDF_SMOOTH;
+				nx.pixel	= pixel;
+			}
+		}
+		if (hasScatter) { // generateScatterRays
+			bool alive = true;
+			if (r.depth >= P.rrMinDepth) {
+				if (rng.get1D() >= P.rr) alive = false;
+				else thp = thp / P.rr;
+			}
+			if (alive) {
+				const f3 woL		  = intr.toLocal(intr.wo);
+				const DevMaterial mat = sc.materials[intr.material];
+				const int bt		  = bsdfTypeOf(mat.type, mat.roughness);
+				BSDF bsdf;
+				bsdf.setup(mat);
+				if (P.nee && (bt & BSDF_SMOOTH) && sc.nLights > 0) {
+					const float ul		 = rng.get1D();
+					const int li		 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
+					const float u0		 = rng.get1D(), u1 = rng.get1D();
+					const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
+					const f3 to			 = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
+					const f3 p_o		 = offsetRayOrigin(intr.p, intr.n, to - intr.p);
+					const f3 sdir		 = to - p_o;
+					const f3 wiL		 = intr.toLocal(normalize(sdir));
+					const float lightPdf = lightSelectPdf * ls.pdf;
+					const f3 bv			 = bsdf.f(woL, wiL);
+					const float bp		 = bsdf.pdf(woL, wiL);
+					if (lightPdf > 0.f && bv.any()) {
+						const f3 Ld = ls.L * thp * bv * fabsf(wiL.z);
+						if (Ld.any()) {
+							wantShadow = true;
+							sO		   = p_o;
+							sD		   = sdir;
+							sLd		   = Ld;
+							sPu		   = pu * bp;
+							sPl		   = pu * lightPdf;
+							sMedium	   = intr.getMedium(sdir);
+						}
+					}
+				}
+				const BSDFSample s = bsdf.sample(woL, rng);
+				if (s.pdf != 0.f && s.f.any()) {
+					const f3 wiW  = intr.toWorld(s.wi);
+					const f3 nthp = thp * s.f * fabsf(s.wi.z) / s.pdf;
+					if (nthp.any()) {
+						wantNext	= true;
+						nx.bsdfType = s.flags;
+						nx.pu		= pu;
+						nx.pl		= pu / s.pdf;
+						nx.o		= offsetRayOrigin(intr.p, intr.n, wiW);
+						nx.d		= wiW;
+						nx.medium	= intr.getMedium(wiW);
+						nx.cp		= intr.p;
+						nx.cn		= intr.n;
+						nx.depth	= r.depth + 1;
+						nx.thp		= nthp;
+						nx.pixel	= pixel;
+					}
+				}
+			}
+		}
+	}
+	px.L[pixel]	  = make_float4(L.x, L.y, L.z, 0.f);
+	px.rng[pixel] = rng.state;
+}
+
+template <bool MEDIA>
+KRR_DEV void shadeItem(const WfParams &P, const DevScene &sc, const RayQueueDev &cur, const float4 *__restrict__ hits,
+					   const RayQueueDev &nxt, const ShadowQueueDev &sq, const PixelStateDev &px, Counters *cnt, int curIdx,
+					   int loopDepth, const int i, const int count) {
+	bool wantNext = false, wantShadow = false;
+	RayItem nx;
+	ShadowOut so;
+	so.medium = -1;
+	int pixel = 0;
+	if (i < count) {
+		const RayItem r = loadRayItem(cur, i);
+		pixel			= r.pixel;
+		shadeCore<MEDIA>(P, sc, r, hits[i], px, loopDepth, wantNext, nx, wantShadow, so);
+	}
+	const f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
+	const int sMedium = so.medium;
 	const int slotN = wavePush(&cnt->rayCount[curIdx ^ 1], wantNext);
 	const int slotS = wavePush(&cnt->shadowCount, wantShadow);
 	if (wantNext) storeRayItem(nxt, slotN, nx);
@@ -500,83 +544,196 @@ __global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueue
 	}
 }
 
-constexpr int kMaxNullCrossings = 1024;
+// Grid-stride loop over the current queue (count read on the device); the
+// loop bound is block-uniform, so every lane of a wave reaches the
+// wave-aggregated pushes together.
+template <bool MEDIA>
+__global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueueDev cur, const float4 *__restrict__ hits,
+											  RayQueueDev nxt, ShadowQueueDev sq, PixelStateDev px, Counters *cnt, int curIdx,
+											  int loopDepth) {
+	const int count = cnt->rayCount[curIdx];
+	for (int base = blockIdx.x * blockDim.x; base < count; base += gridDim.x * blockDim.x)
+		shadeItem<MEDIA>(P, sc, cur, hits, nxt, sq, px, cnt, curIdx, loopDepth, base + threadIdx.x, count);
+}
 
 // traceShadow with media: ShadowTr raygen + traceTransmittance
 // (device.cu:102-128, wavefront.h:209-268).  Ratio tracking through media,
 // stepping across null-material surfaces.
+constexpr int kMaxNullCrossings = 1024;
+
+// traceTransmittance of one shadow ray (tMax = 1 to the offset light point):
+// ratio tracking through media, stepping across null-material surfaces.
 template <bool COUNT>
-__global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, Counters *cnt) {
+KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const TraceBvh &bvh, f3 o, f3 d, const float tMax, const int pixel,
+							   const int origMedium, const f3 &Ld, const f3 &spu, const f3 &spl, const PixelStateDev &px,
+							   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
+	Pcg rng{px.rng[pixel], P.inc};
+	const int channel = px.channel[pixel];
+	int medium		  = origMedium;
+	const f3 pLight	  = o + d * tMax;
+	f3 T_ray(1.f), pu(1.f), pl(1.f);
+	SurfaceInteraction intr;
+	const float zeroEps = kRayEps * 2.f;
+	// The reference loops until the segment is shorter than 2 KRR_RAY_EPS;
+	// the cap only bounds pathological stacks of coplanar null surfaces.
+	for (int crossing = 0; crossing < kMaxNullCrossings && !(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps);
+		 crossing++) {
+		const TraceResult h = traverse<false, COUNT>(bvh, o, d, tMax, stack, nv, nt);
+		const bool visible	= h.prim < 0;
+		if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
+		if (!visible && intr.material >= 0) {
+			T_ray = f3();
+			break;
+		}
+		if (medium >= 0) {
+			const float tEnd = visible ? tMax : length(intr.p - o) / length(d);
+			const f3 T_maj	 = sampleTmaj(sc, medium, o, d, tEnd, rng, channel,
+				  [&](const f3 &, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
+					  const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
+					  const float pr   = Tm[channel] * sigma_maj[channel];
+					  T_ray			   = T_ray * (Tm * sigma_n / pr);
+					  pl			   = pl * (Tm * sigma_maj / pr);
+					  pu			   = pu * (Tm * sigma_n / pr);
+					  const f3 Tr	   = T_ray / (pu + pl).mean();
+					  if (Tr.maxCoeff() < 0.05f) {
+						  if (rng.get1D() < 0.75f) T_ray = f3();
+						  else T_ray = T_ray / 0.25f;
+					  }
+					  return T_ray.any();
+				  });
+			T_ray = T_ray * (T_maj / T_maj[channel]);
+			pu	  = pu * (T_maj / T_maj[channel]);
+			pl	  = pl * (T_maj / T_maj[channel]);
+		}
+		if (visible || T_ray.isZero()) break;
+		const f3 p_o = offsetRayOrigin(intr.p, intr.n, pLight - intr.p);
+		d			 = pLight - p_o;
+		o			 = p_o;
+		medium		 = intr.getMedium(d);
+	}
+	if (T_ray.any()) {
+		const f3 c	= Ld * T_ray / (spu * pu + spl * pl).mean();
+		float4 L	= px.L[pixel];
+		L.x			= L.x + c.x;
+		L.y			= L.y + c.y;
+		L.z			= L.z + c.z;
+		px.L[pixel] = L;
+	}
+	px.rng[pixel] = rng.state;
+}
+
+// Non-media shadow test of one shadow ray: add Ld / mean(pu + pl) if visible.
+template <bool COUNT>
+KRR_DEV void occlusionItem(const TraceBvh &bvh, const f3 &o, const f3 &d, float tMax, int pixel, const f3 &c, const PixelStateDev &px,
+						   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
+	const TraceResult r = traverse<true, COUNT>(bvh, o, d, tMax, stack, nv, nt);
+	if (r.prim < 0) {
+		float4 L	= px.L[pixel];
+		L.x			= L.x + c.x;
+		L.y			= L.y + c.y;
+		L.z			= L.z + c.z;
+		px.L[pixel] = L;
+	}
+}
+
+// traceShadow with media: ShadowTr raygen + traceTransmittance
+// (device.cu:102-128, wavefront.h:209-268).
+template <bool COUNT>
+__global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, int *ovf, Counters *cnt) {
 	extern __shared__ int ldsStack[];
-	const int i		= blockIdx.x * blockDim.x + threadIdx.x;
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	const int count = cnt->shadowCount;
-	if (i == 0) cnt->shadowTotal += (unsigned long long) count;
+	if (blockIdx.x == 0 && threadIdx.x == 0) cnt->shadowTotal += (unsigned long long) count;
 	uint32_t nv = 0, nt = 0;
-	if (i < count) {
+	while (true) {
+		const int base = waveFetch(&cnt->work[1]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i >= count) continue;
 		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1];
-		const int pixel		 = __float_as_int(b.w);
-		const int origMedium = __float_as_int(sq.c[i].w) - 1;
 		const float4 e0 = sq.ext[3 * i], e1 = sq.ext[3 * i + 1], e2 = sq.ext[3 * i + 2];
-		const f3 Ld(e0.x, e0.y, e0.z), spu(e1.x, e1.y, e1.z), spl(e2.x, e2.y, e2.z);
-		Pcg rng{px.rng[pixel], P.inc};
-		const int channel = px.channel[pixel];
-		const float tMax  = a.w;
-		f3 o(a.x, a.y, a.z), d(b.x, b.y, b.z);
-		int medium		= origMedium;
-		const f3 pLight = o + d * tMax;
-		f3 T_ray(1.f), pu(1.f), pl(1.f);
-		SurfaceInteraction intr;
-		const float zeroEps = kRayEps * 2.f;
-		// The reference loops until the segment is shorter than 2 KRR_RAY_EPS;
-		// the cap only bounds pathological stacks of coplanar null surfaces.
-		for (int crossing = 0; crossing < kMaxNullCrossings && !(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps);
-			 crossing++) {
-			const TraceResult h = traverse<false, COUNT>(bvh, o, d, tMax, ldsStack + threadIdx.x, blockDim.x, nv, nt);
-			const bool visible	= h.prim < 0;
-			if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
-			if (!visible && intr.material >= 0) {
-				T_ray = f3();
-				break;
-			}
-			if (medium >= 0) {
-				const float tEnd = visible ? tMax : length(intr.p - o) / length(d);
-				const f3 T_maj	 = sampleTmaj(sc, medium, o, d, tEnd, rng, channel,
-					  [&](const f3 &, const MediumProps &mp, const f3 &sigma_maj, const f3 &Tm) -> bool {
-						  const f3 sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, 0.f);
-						  const float pr   = Tm[channel] * sigma_maj[channel];
-						  T_ray			   = T_ray * (Tm * sigma_n / pr);
-						  pl			   = pl * (Tm * sigma_maj / pr);
-						  pu			   = pu * (Tm * sigma_n / pr);
-						  const f3 Tr	   = T_ray / (pu + pl).mean();
-						  if (Tr.maxCoeff() < 0.05f) {
-							  if (rng.get1D() < 0.75f) T_ray = f3();
-							  else T_ray = T_ray / 0.25f;
-						  }
-						  return T_ray.any();
-					  });
-				T_ray = T_ray * (T_maj / T_maj[channel]);
-				pu	  = pu * (T_maj / T_maj[channel]);
-				pl	  = pl * (T_maj / T_maj[channel]);
-			}
-			if (visible || T_ray.isZero()) break;
-			const f3 p_o = offsetRayOrigin(intr.p, intr.n, pLight - intr.p);
-			d			 = pLight - p_o;
-			o			 = p_o;
-			medium		 = intr.getMedium(d);
-		}
-		if (T_ray.any()) {
-			const f3 c = Ld * T_ray / (spu * pu + spl * pl).mean();
-			float4 L   = px.L[pixel];
-			L.x		   = L.x + c.x;
-			L.y		   = L.y + c.y;
-			L.z		   = L.z + c.z;
-			px.L[pixel] = L;
-		}
-		px.rng[pixel] = rng.state;
+		transmittanceItem<COUNT>(P, sc, bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, __float_as_int(b.w), __float_as_int(sq.c[i].w) - 1,
+								 f3(e0.x, e0.y, e0.z), f3(e1.x, e1.y, e1.z), f3(e2.x, e2.y, e2.z), px, stack, nv, nt);
 	}
 	if (COUNT) {
 		waveAddU64(&cnt->shadowNodeVisits, nv);
 		waveAddU64(&cnt->shadowTriTests, nt);
+	}
+}
+
+template <bool COUNT>
+__global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, int *ovf, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
+	const int count		  = cnt->shadowCount;
+	if (blockIdx.x == 0 && threadIdx.x == 0) cnt->shadowTotal += (unsigned long long) count;
+	uint32_t nv = 0, nt = 0;
+	while (true) {
+		const int base = waveFetch(&cnt->work[1]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i >= count) continue;
+		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1], c = sq.c[i];
+		occlusionItem<COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, __float_as_int(b.w), f3(c.x, c.y, c.z), px, stack, nv,
+							 nt);
+	}
+	if (COUNT) {
+		waveAddU64(&cnt->shadowNodeVisits, nv);
+		waveAddU64(&cnt->shadowTriTests, nt);
+	}
+}
+
+__global__ void k_mega_begin(Counters *cnt) {
+	cnt->work[2] = 0;
+	cnt->closestLaunches += 1;
+}
+
+// Tail of the sample: every path left in the queue after the wavefront depths
+// runs to completion in one thread - closest trace, shade, shadow trace,
+// next depth - in the same per-path stage order as the wavefront, so the
+// radiance additions and RNG draws of each pixel are unchanged.  Short queues
+// at deep bounces would otherwise cost one latency-bound launch per depth,
+// each as long as its slowest ray.
+template <bool MEDIA, bool COUNT>
+__global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
+							 int *ovf, Counters *cnt) {
+	extern __shared__ int ldsStack[];
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
+	const int count = *countPtr;
+	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
+	while (true) {
+		const int base = waveFetch(&cnt->work[2]);
+		if (base >= count) break;
+		const int i = base + laneId();
+		if (i >= count) continue;
+		RayItem r = loadRayItem(q, i);
+		for (int depth = startDepth; depth <= P.maxDepth; depth++) {
+			nClosest++;
+			const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
+			const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
+			bool wantNext = false, wantShadow = false;
+			RayItem nx;
+			ShadowOut so;
+			so.medium = -1;
+			shadeCore<MEDIA>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so);
+			if (wantShadow) {
+				nShadow++;
+				if (MEDIA && P.enableMedium)
+					transmittanceItem<COUNT>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
+				else
+					occlusionItem<COUNT>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+			}
+			if (!wantNext) break;
+			r = nx;
+		}
+	}
+	waveAddU64(&cnt->closestTotal, nClosest);
+	waveAddU64(&cnt->shadowTotal, nShadow);
+	waveAddU64(&cnt->megaClosest, nClosest);
+	waveAddU64(&cnt->megaShadow, nShadow);
+	if (COUNT) {
+		waveAddU64(&cnt->megaNodeVisits, nv + snv);
+		waveAddU64(&cnt->megaTriTests, nt + snt);
 	}
 }
 
@@ -698,11 +855,13 @@ struct DevBuf {
 struct AccelData {
 	DevBuf nodes, tris;
 	int stackSize = 0;
+	int stackLds  = kShortStack; // stack entries per lane kept in LDS (the rest spill to global memory)
 	Bvh4Host host; // kept for statistics (nodes/tris arrays cleared after upload)
 	int64_t nNodes = 0, nLeafTris = 0;
 	int maxDepth	= 0;
 	double buildSeconds = 0;
-	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>()}; }
+	int ldsEntries() const { return std::min(stackSize, stackLds); }
+	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes}; }
 };
 
 int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize) {
@@ -737,24 +896,41 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize) {
 	return KRR_OK;
 }
 
-int traceBlock(int stackSize) {
-	// keep dynamic LDS per block <= 48 KB
-	int tb = 256;
-	while (tb > 64 && (size_t) stackSize * tb * 4 > 48 * 1024) tb >>= 1;
-	return tb;
+// Blocks that fit on the GPU at once for this kernel and launch shape
+// (persistent kernels launch exactly that many), capped at maxBlocks.
+template <class K> int persistentGrid(K kernel, int block, size_t lds, int maxBlocks) {
+	int dev = 0, cus = 0, nb = 0;
+	if (hipGetDevice(&dev) != hipSuccess) return maxBlocks;
+	if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) return maxBlocks;
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb < 1) nb = 1;
+	return std::max(1, std::min(maxBlocks, nb * cus));
+}
+
+// LDS bytes per block of the short traversal stack, and ints per grid thread
+// of its global overflow.
+size_t stackLds(const AccelData &a, int block) { return (size_t) a.ldsEntries() * block * sizeof(int); }
+size_t stackOverflowInts(const AccelData &a) { return (size_t) (a.stackSize - a.ldsEntries()); }
+
+// Grows buf to hold the traversal-stack overflow of `threads` grid threads.
+int reserveOverflow(DevBuf &buf, const AccelData &a, size_t threads) {
+	const size_t bytes = stackOverflowInts(a) * threads * sizeof(int);
+	if (bytes > buf.sz) KRR_HIP(buf.alloc(bytes));
+	return KRR_OK;
 }
 
 } // namespace
 
 struct krr_accel {
 	AccelData a;
-	DevBuf counters;
+	DevBuf counters, ovf;
+	int gClosest = 0, gAny = 0;
 };
 
 struct krr_wf {
 	// params (integrator.h:96-103 + additions)
 	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false, timeKernels = false;
-	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4;
+	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 3;
+	int stackLdsEntries = kShortStack;
 	float rr = 0.8f, clampMax = 1e3f;
 	// scene
 	bool haveScene = false;
@@ -765,14 +941,16 @@ struct krr_wf {
 	bool sceneHasMedia = false;
 	// frame
 	int W = 0, H = 0;
-	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters;
+	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters, ovf;
 	krr_camera cam{};
 	uint64_t frameIndex = 0;
+	uint64_t gridKey	= ~0ull;
+	int gClosest = 1, gShadow = 1, gShade = 1, gMega = 1;
 	bool beginPending	= false;
 	hipStream_t lastStream = nullptr;
 	krr_stats stats{};
 	// HIP-event timing of kernel launches ("time_kernels"): kind 0 closest
-	// trace, 1 shade, 2 shadow trace, 3 whole render.
+	// trace, 1 shade, 2 shadow trace, 3 whole render, 4 tail megakernel.
 	struct Span {
 		int kind, begin, end;
 	};
@@ -820,6 +998,8 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->blockSize	   = (int) j.value("block_size", 256.0);
 		wf->traceBlockSize = (int) j.value("trace_block_size", 128.0);
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
+		wf->wavefrontDepths = (int) j.value("wavefront_depths", 3.0);
+		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
 	}
@@ -829,6 +1009,8 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->traceBlockSize < 64 || wf->traceBlockSize > 256 || wf->traceBlockSize % 64)
 		return krr::fail(KRR_ERR_BAD_CONFIG, "trace_block_size must be 64, 128, 192 or 256");
 	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
+	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
+	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 	return KRR_OK;
 }
 
@@ -1041,15 +1223,17 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	krr::clearError();
 	if (!scene || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_build: null argument");
 	*out = nullptr;
-	int leafSize = 4;
+	int leafSize = 4, stackLdsEntries = kShortStack;
 	if (json_opts && *json_opts) {
 		try {
-			Json j	 = Json::parse(json_opts);
-			leafSize = (int) j.value("leaf_size", 4.0);
+			Json j			= Json::parse(json_opts);
+			leafSize		= (int) j.value("leaf_size", 4.0);
+			stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 		} catch (const std::exception &e) {
 			return krr::fail(KRR_ERR_BAD_CONFIG, "krr_accel_build: %s", e.what());
 		}
 		if (leafSize < 1 || leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
+		if (stackLdsEntries < 0 || stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 	}
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_accel_build: no HIP device");
 	for (int m = 0; m < scene->n_meshes; m++)
@@ -1057,22 +1241,32 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	auto a = std::make_unique<krr_accel>();
 	int rc = buildAccel(scene, a->a, leafSize);
 	if (rc) return rc;
+	a->a.stackLds = stackLdsEntries;
 	KRR_HIP(a->counters.alloc(sizeof(Counters)));
 	KRR_HIP(hipMemset(a->counters.p, 0, sizeof(Counters)));
 	*out = a.release();
 	return KRR_OK;
 }
 
+// Both accel trace calls run a persistent grid that shares the accel's work
+// counter and stack overflow area: calls on one accel must not overlap.
+static constexpr int kAccelBlock = 128;
+
 int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev, void *hip_stream) {
 	krr::clearError();
 	if (!accel || n < 0 || (n > 0 && (!rays_dev || !hits_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_closest: bad argument");
-	if (n > INT32_MAX) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_closest: at most 2^31-1 rays per call");
+	if (n > (1 << 30)) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_closest: at most 2^30 rays per call");
 	if (n == 0) return KRR_OK;
-	const int tb		= traceBlock(accel->a.stackSize);
-	const size_t lds	= (size_t) accel->a.stackSize * tb * sizeof(int);
-	const unsigned grid = (unsigned) ((n + tb - 1) / tb);
-	hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(tb), lds, (hipStream_t) hip_stream, accel->a.dev(),
-					   (const float4 *) rays_dev, (float4 *) hits_dev, (const int *) nullptr, (int) n, accel->counters.as<Counters>());
+	const int tb	 = kAccelBlock;
+	const size_t lds = stackLds(accel->a, tb);
+	if (!accel->gClosest) accel->gClosest = persistentGrid(k_trace_closest<false>, tb, lds, 1 << 20);
+	const int grid = std::min(accel->gClosest, (int) ((n + tb - 1) / tb));
+	if (int rc = reserveOverflow(accel->ovf, accel->a, (size_t) grid * tb)) return rc;
+	hipStream_t st = (hipStream_t) hip_stream;
+	Counters *cnt  = accel->counters.as<Counters>();
+	hipLaunchKernelGGL(k_accel_begin, dim3(1), dim3(1), 0, st, cnt);
+	hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(tb), lds, st, accel->a.dev(), (const float4 *) rays_dev, (float4 *) hits_dev,
+					   (int) n, accel->ovf.as<int>(), cnt);
 	KRR_HIP(hipGetLastError());
 	return KRR_OK;
 }
@@ -1080,13 +1274,18 @@ int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, 
 int krr_accel_trace_any(krr_accel *accel, const float *rays_dev, int64_t n, int32_t *visible_dev, void *hip_stream) {
 	krr::clearError();
 	if (!accel || n < 0 || (n > 0 && (!rays_dev || !visible_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_any: bad argument");
-	if (n > INT32_MAX) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_any: at most 2^31-1 rays per call");
+	if (n > (1 << 30)) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_any: at most 2^30 rays per call");
 	if (n == 0) return KRR_OK;
-	const int tb		= traceBlock(accel->a.stackSize);
-	const size_t lds	= (size_t) accel->a.stackSize * tb * sizeof(int);
-	const unsigned grid = (unsigned) ((n + tb - 1) / tb);
-	hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(tb), lds, (hipStream_t) hip_stream, accel->a.dev(), (const float4 *) rays_dev,
-					   (int *) visible_dev, (int) n, accel->counters.as<Counters>());
+	const int tb	 = kAccelBlock;
+	const size_t lds = stackLds(accel->a, tb);
+	if (!accel->gAny) accel->gAny = persistentGrid(k_trace_any<false>, tb, lds, 1 << 20);
+	const int grid = std::min(accel->gAny, (int) ((n + tb - 1) / tb));
+	if (int rc = reserveOverflow(accel->ovf, accel->a, (size_t) grid * tb)) return rc;
+	hipStream_t st = (hipStream_t) hip_stream;
+	Counters *cnt  = accel->counters.as<Counters>();
+	hipLaunchKernelGGL(k_accel_begin, dim3(1), dim3(1), 0, st, cnt);
+	hipLaunchKernelGGL(k_trace_any<false>, dim3(grid), dim3(tb), lds, st, accel->a.dev(), (const float4 *) rays_dev, (int *) visible_dev,
+					   (int) n, accel->ovf.as<int>(), cnt);
 	KRR_HIP(hipGetLastError());
 	return KRR_OK;
 }
@@ -1137,6 +1336,8 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("block_size", Json((double) wf->blockSize));
 	j.set("trace_block_size", Json((double) wf->traceBlockSize));
 	j.set("leaf_size", Json((double) wf->leafSize));
+	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
+	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	std::string s = j.dump();
 	if (buf && cap > 0) {
 		size_t n = std::min((size_t) cap - 1, s.size());
@@ -1200,8 +1401,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	P.rr		  = wf->rr;
 	P.rrMinDepth  = wf->rrMinDepth;
 	P.nee		  = wf->nee ? 1 : 0;
-	const bool media = wf->enableMedium && wf->sceneHasMedia;
-	P.enableMedium	 = media ? 1 : 0;
+	P.enableMedium	 = (wf->enableMedium && wf->sceneHasMedia) ? 1 : 0;
 	P.enableClamp	 = wf->enableClamp ? 1 : 0;
 	P.clampMax		 = wf->clampMax;
 	P.legacySpp		 = wf->legacySpp ? 1 : 0;
@@ -1218,14 +1418,34 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 
 	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
 	ShadowQueueDev sq{wf->shadowRay.as<float4>(), wf->shadowC.as<float4>(), wf->shadowExt.as<float4>()};
-	Counters *cnt = wf->counters.as<Counters>();
+	Counters *cnt	   = wf->counters.as<Counters>();
+	wf->accel.stackLds = wf->stackLdsEntries; // before dev(): the kernels' LDS stack depth must match lds below
 	const TraceBvh bvh = wf->accel.dev();
 	const int bs	   = wf->blockSize;
 	const unsigned gN  = (unsigned) ((N + bs - 1) / bs);
-	const int tb	   = std::min(wf->traceBlockSize, traceBlock(wf->accel.stackSize));
-	const unsigned gT  = (unsigned) ((N + tb - 1) / tb);
-	const size_t lds   = (size_t) wf->accel.stackSize * tb * sizeof(int);
+	const int tb	   = wf->traceBlockSize;
+	const int gTmax	   = (N + tb - 1) / tb;
+	const size_t lds   = stackLds(wf->accel, tb);
+	if ((size_t) bvh.stackLds * tb * sizeof(int) != lds) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: stack layout mismatch");
 	const bool count   = wf->countTraversal;
+	const bool media   = wf->enableMedium && wf->sceneHasMedia;
+	// persistent grid sizes (resident blocks), computed once per shape
+	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
+							  (count ? 2 : 0) ^ (media ? 1 : 0);
+	if (shapeKey != wf->gridKey) {
+		// Counting variants use more registers: size each grid for the variant launched.
+		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
+		if (media) wf->gShadow = count ? persistentGrid(k_shadow_tr<true>, tb, lds, gTmax) : persistentGrid(k_shadow_tr<false>, tb, lds, gTmax);
+		else wf->gShadow = count ? persistentGrid(k_trace_shadow<true>, tb, lds, gTmax) : persistentGrid(k_trace_shadow<false>, tb, lds, gTmax);
+		wf->gShade = media ? persistentGrid(k_shade<true>, bs, 0, (int) gN) : persistentGrid(k_shade<false>, bs, 0, (int) gN);
+		if (media) wf->gMega = count ? persistentGrid(k_megakernel<true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<true, false>, tb, lds, gTmax);
+		else wf->gMega = count ? persistentGrid(k_megakernel<false, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false>, tb, lds, gTmax);
+		const int gMax = std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega);
+		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) gMax * tb)) return rc;
+		wf->gridKey = shapeKey;
+	}
+	int *ovf = wf->ovf.as<int>();
+	const unsigned gTc = (unsigned) wf->gClosest, gTs = (unsigned) wf->gShadow, gS = (unsigned) wf->gShade;
 
 	const bool timing = wf->timeKernels;
 	wf->spans.clear();
@@ -1248,33 +1468,48 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	for (int s = 0; s < wf->spp; s++) {
 		hipLaunchKernelGGL(k_camera, dim3(gN), dim3(bs), 0, st, P, cam, px, rayQueue(wf, 0), cnt);
 		for (int depth = 0; depth <= wf->maxDepth; depth++) {
-			const int cur = depth & 1;
-			hipLaunchKernelGGL(k_depth_begin, dim3(1), dim3(1), 0, st, cnt, cur);
+			const int cur		 = depth & 1;
 			const RayQueueDev cq = rayQueue(wf, cur);
+			if (depth >= wf->wavefrontDepths) { // the rest of every path in one launch
+				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
+				timed(4, [&] {
+					const unsigned g = (unsigned) wf->gMega;
+					const int *cp	 = (const int *) &cnt->rayCount[cur];
+					if (media) {
+						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+					} else {
+						if (count) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+					}
+				});
+				break;
+			}
+			hipLaunchKernelGGL(k_depth_begin, dim3(1), dim3(1), 0, st, cnt, cur);
 			timed(0, [&] {
 				if (count)
-					hipLaunchKernelGGL(k_trace_closest<true>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
-									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], 0, cnt);
+					hipLaunchKernelGGL(k_trace_closest_queue<true>, dim3(gTc), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
+									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], ovf, cnt);
 				else
-					hipLaunchKernelGGL(k_trace_closest<false>, dim3(gT), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
-									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], 0, cnt);
+					hipLaunchKernelGGL(k_trace_closest_queue<false>, dim3(gTc), dim3(tb), lds, st, bvh, (const float4 *) cq.ray,
+									   wf->hits.as<float4>(), (const int *) &cnt->rayCount[cur], ovf, cnt);
 			});
 			timed(1, [&] {
 				if (media)
-					hipLaunchKernelGGL(k_shade<true>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+					hipLaunchKernelGGL(k_shade<true>, dim3(gS), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
 									   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
 				else
-					hipLaunchKernelGGL(k_shade<false>, dim3(gN), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
+					hipLaunchKernelGGL(k_shade<false>, dim3(gS), dim3(bs), 0, st, P, wf->ds, cq, (const float4 *) wf->hits.as<float4>(),
 									   rayQueue(wf, cur ^ 1), sq, px, cnt, cur, depth);
 			});
 			if (wf->nee && depth != wf->maxDepth) {
 				timed(2, [&] {
 					if (media) {
-						if (count) hipLaunchKernelGGL(k_shadow_tr<true>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
-						else hipLaunchKernelGGL(k_shadow_tr<false>, dim3(gT), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, cnt);
+						if (count) hipLaunchKernelGGL(k_shadow_tr<true>, dim3(gTs), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, ovf, cnt);
+						else hipLaunchKernelGGL(k_shadow_tr<false>, dim3(gTs), dim3(tb), lds, st, P, wf->ds, bvh, sq, px, ovf, cnt);
 					} else {
-						if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
-						else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(gT), dim3(tb), lds, st, bvh, sq, px, cnt);
+						if (count) hipLaunchKernelGGL(k_trace_shadow<true>, dim3(gTs), dim3(tb), lds, st, bvh, sq, px, ovf, cnt);
+						else hipLaunchKernelGGL(k_trace_shadow<false>, dim3(gTs), dim3(tb), lds, st, bvh, sq, px, ovf, cnt);
 					}
 				});
 			}
@@ -1309,8 +1544,12 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 	out->closest_launches	= c.closestLaunches;
 	out->shadow_node_visits = c.shadowNodeVisits;
 	out->shadow_tri_tests	= c.shadowTriTests;
+	out->megakernel_closest_rays = c.megaClosest;
+	out->megakernel_shadow_rays	 = c.megaShadow;
+	out->megakernel_node_visits = c.megaNodeVisits;
+	out->megakernel_tri_tests	 = c.megaTriTests;
 	if (wf->timeKernels) {
-		double ms[4] = {0, 0, 0, 0};
+		double ms[5] = {0, 0, 0, 0, 0};
 		for (const auto &sp : wf->spans) {
 			float t = 0.f;
 			hipEvent_t a = wf->event(sp.begin), b = wf->event(sp.end);
@@ -1323,6 +1562,7 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 		out->shade_ms		  = ms[1];
 		out->shadow_trace_ms  = ms[2];
 		out->render_ms		  = ms[3];
+		out->megakernel_ms	  = ms[4];
 	}
 	return KRR_OK;
 }

@@ -116,3 +116,34 @@ def test_multi_frame_seeds_differ_and_match(krr, oracle, gpu):
             assert not np.array_equal(prev, g)
         prev = g
     wf.close()
+
+
+@pytest.mark.parametrize("wd", [0, 1, 2, 100])
+def test_megakernel_switch_depth_parity(krr, oracle, gpu, wd):
+    """Depths >= wavefront_depths run per path in one persistent launch; the
+    per-pixel stage order is the same, so images and ray counts must not
+    depend on the switch point (0 = all megakernel, 100 = all wavefront)."""
+    print(f"case: wavefront_depths={wd} cornell + volume", flush=True)
+    s = krr.Scene(CORNELL)
+    g, gs = gpu_render(krr, gpu, s, 128, 128, {"spp": 3, "max_depth": 8, "wavefront_depths": wd})
+    o, os_ = oracle.render(s.desc, s.camera(0), 128, 128, 1, spp=3, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+    v = krr.Scene(icosphere_scene(count=128, level=2, res=(64, 48), media=True, grid_res=32))
+    g, gs = gpu_render(krr, gpu, v, 64, 48, {"spp": 2, "max_depth": 12, "wavefront_depths": wd, "rr_min_depth": 2})
+    o, os_ = oracle.render(v.desc, v.camera(0), 64, 48, 1, spp=2, max_depth=12, rr_min_depth=2)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+
+
+@pytest.mark.parametrize("lds_entries", [0, 4])
+def test_stack_overflow_render_parity(krr, oracle, gpu, lds_entries):
+    """Closest, shadow and megakernel traversal with (nearly) the whole stack
+    in the global overflow area: images and ray counts are unchanged."""
+    print(f"case: icosphere 96x54 stack_lds_entries={lds_entries}", flush=True)
+    s = krr.Scene(icosphere_scene(count=256, level=2, res=(96, 54)))
+    params = {"spp": 2, "max_depth": 10, "rr_min_depth": 3, "stack_lds_entries": lds_entries, "wavefront_depths": 2}
+    g, gs = gpu_render(krr, gpu, s, 96, 54, params)
+    o, os_ = oracle.render(s.desc, s.camera(0), 96, 54, 1, spp=2, max_depth=10, rr_min_depth=3)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)

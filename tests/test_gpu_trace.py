@@ -27,8 +27,8 @@ def _rays_towards_triangles(pos, idx, n, rng, lo, hi, tmax=np.inf, jitter=0.2):
     return rays
 
 
-def _gpu_closest(krr, gpu, scene, rays):
-    acc = krr.Accel(scene.desc)
+def _gpu_closest(krr, gpu, scene, rays, opts=None):
+    acc = krr.Accel(scene.desc, opts)
     r = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(rays, np.float32))
     hits = gpu.DeviceBuffer(len(rays) * 16)
     acc.trace_closest(r.ptr, len(rays), hits.ptr, 0)
@@ -118,3 +118,46 @@ def test_full_scene_bvh_matches_checker_bvh(krr, oracle, gpu):
     o = oracle.trace_closest(s.desc, rays, use_bvh=True)
     _assert_same_hits(g, o)
     assert np.array_equal(vis, oracle.trace_any(s.desc, rays, use_bvh=True))
+
+
+@pytest.mark.parametrize("lds_entries", [0, 4, 96])
+def test_stack_overflow_area_is_exact(krr, oracle, gpu, lds_entries):
+    """The traversal stack keeps its first stack_lds_entries entries per lane
+    in LDS and the rest in a global overflow area: with 0 every push goes to
+    global memory, with 4 most deep pushes do, with 96 none do.  Hits must
+    not change."""
+    print(f"case: 1.31M-triangle closest/any, stack_lds_entries={lds_entries}", flush=True)
+    s = krr.Scene(icosphere_scene())
+    pos, idx, _ = s.arrays()
+    rng = np.random.default_rng(5)
+    rays = _rays_towards_triangles(pos, idx, 8000, rng, np.array([-0.5, 0.05, -0.5]), np.array([1.5, 1.7, 2.5]), jitter=0.05)
+    g, vis, info = _gpu_closest(krr, gpu, s, rays, {"stack_lds_entries": lds_entries})
+    assert 3 * info["max_depth"] + 4 > 16  # the default 16 LDS entries do not cover the whole stack
+    o = oracle.trace_closest(s.desc, rays, use_bvh=True)
+    _assert_same_hits(g, o)
+    assert np.array_equal(vis, oracle.trace_any(s.desc, rays, use_bvh=True))
+
+
+def test_repeated_calls_of_varying_size(krr, oracle, gpu):
+    """The accel trace kernels are persistent (a work counter reset before
+    each launch): back-to-back calls of 1, 63, 64, 65 and 5000 rays on one
+    accel each trace exactly their own rays."""
+    print("case: repeated accel calls n=1,63,64,65,5000", flush=True)
+    s = krr.Scene(icosphere_scene(count=64, level=2, res=(16, 16)))
+    pos, idx, _ = s.arrays()
+    rng = np.random.default_rng(6)
+    rays = _rays_towards_triangles(pos, idx, 5000, rng, np.array([-0.5, 0.05, -0.5]), np.array([1.5, 1.7, 2.5]))
+    o = oracle.trace_closest(s.desc, rays)
+    acc = krr.Accel(s.desc)
+    r = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(rays, np.float32))
+    hits = gpu.DeviceBuffer(len(rays) * 16)
+    for n in (1, 63, 64, 65, 5000):
+        hits.zero()
+        acc.trace_closest(r.ptr, n, hits.ptr, 0)
+        gpu.synchronize()
+        h = hits.to_numpy(np.float32, (len(rays), 4))
+        prim = h[:, 1].view(np.int32)
+        assert np.array_equal(prim[:n], o["prim"][:n]), n
+        assert np.array_equal(h[:n, 0][o["prim"][:n] >= 0], o["t"][:n][o["prim"][:n] >= 0]), n
+        assert not h[n:].any(), f"call with n={n} wrote past its rays"
+    acc.close()
