@@ -10,7 +10,7 @@ value = (closest-hit rays + shadow rays) / wall time of the timed frames, in
 Mrays/s.  Rays are counted on the device (queue sizes summed over depth).
 The roofline entry covers the kernel with the most HIP-event time per frame
 (see kernel_rooflines for the byte model of each); its measured HBM traffic
-comes from profiles/traffic_<config>.json when that file exists (rocprofv3
+comes from profiles/hbm_traffic_<config>.json when that file exists (rocprofv3
 FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy-kernel calibration).
 cpu_baseline = the CPU checker (OpenMP) on full frames of the same scene.
 """
@@ -27,16 +27,16 @@ METRIC = "Mrays/s + Mpaths/s on 1 MI355X; equal-time relMSE: NRC vs plain PT"
 HBM_PEAK_GBS = 8000.0
 
 CONFIGS = {
-    "cfg2": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
+    "icosphere_field_1080p": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
                         "resolution": [1920, 1080],
                         "camera": {"eye": [0.5, 0.75, 2.3], "target": [0.5, 0.3, 0.45]}},
                  params={"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3},
                  desc="icosphere_field_1.31Mtri_1920x1080_1spp_depth16_rr3"),
-    "cfg1": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
+    "cornell_box_256": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
                         "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
                  params={"spp": 16, "max_depth": 8, "rr": 0.8},
                  desc="cornell_box_36tri_256x256_16spp_depth8"),
-    "cfg3": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
+    "volume_720p": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
                         "media": [{"type": "value_noise_grid", "res": 256, "octaves": 5, "seed": 11,
                                    "sigma_t_max": 20.0, "albedo": 0.8, "g": 0.3}],
                         "resolution": [1280, 720],
@@ -235,15 +235,36 @@ def cpu_baseline(cfg, scene, budget_s=12.0, max_frames=20):
             "mpaths_per_s": round(paths / secs / 1e6, 3)}
 
 
+def copy_calibration(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (torch's elementwise copy
+    kernel), run under rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: the counters
+    should read nbytes each; the measured ratio scales the tracer kernels'
+    counter readings into HBM bytes."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    assert bool((b == 1.0).all())
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="cfg2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="icosphere_field_1080p", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is described in DESIGN.md "
@@ -251,7 +272,7 @@ def main():
         raise SystemExit(2)
     cfg = CONFIGS[args.config]
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.config}.json")
+    tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.config}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f)

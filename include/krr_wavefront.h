@@ -236,22 +236,37 @@ typedef struct krr_wf krr_wf;
  *                            persistent launch (same per-path stage order)
  *   leaf_size (4), stack_lds_entries (16)  as for krr_accel_build
  * NULL or "" means all defaults. */
+/* Replaces RenderPassFactory::deserialize<WavefrontPathTracer> + from_json
+ * (src/core/renderpass.h:202-273, src/render/wavefront/integrator.h:96-103)
+ * and WavefrontPathTracer::initialize (integrator.cpp:22-48). */
 int krr_wf_create(const char *json_params, krr_wf **out);
-/* Uploads the scene and builds the acceleration structure (setScene). */
+/* Uploads the scene and builds the acceleration structure: replaces
+ * WavefrontPathTracer::setScene (integrator.cpp:186-203) with the
+ * RTScene upload (src/core/device/scene.cpp:28-135). */
 int krr_wf_set_scene(krr_wf *wf, const krr_scene_desc *scene);
-/* Reallocates the per-pixel queues for a width x height frame (resize). */
+/* Reallocates the per-pixel queues for a width x height frame: replaces
+ * WavefrontPathTracer::resize (integrator.cpp:181-184). */
 int krr_wf_resize(krr_wf *wf, int32_t width, int32_t height);
 /* Copies the camera and resets per-pixel state; frame_index is the 1-based
- * frame counter of the host loop (beginFrame). */
+ * frame counter of the host loop (src/core/window.cpp:457): replaces
+ * WavefrontPathTracer::beginFrame (integrator.cpp:205-221). */
 int krr_wf_begin_frame(krr_wf *wf, const krr_camera *camera, uint64_t frame_index);
 /* Traces spp samples per pixel and writes RGBA32F into film_dev, W x H
- * float4, row (H-1-y) for pixel row y (render). */
+ * float4, row (H-1-y) for pixel row y (src/core/device/cuda.h:42-45):
+ * replaces WavefrontPathTracer::render (integrator.cpp:223-267).  Work is
+ * queued on hip_stream (0 = null stream) and the call returns without
+ * waiting for it. */
 int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream);
+/* RenderPass::endFrame (src/core/renderpass.h:162); nothing to do. */
 int krr_wf_end_frame(krr_wf *wf);
-/* Counters of the last render; synchronises the stream of that render. */
+/* Counters of the last render; synchronises the stream of that render.  No
+ * reference counterpart (the reference shows none of these figures). */
 int krr_wf_stats(krr_wf *wf, krr_stats *out);
-/* Current params as JSON (to_json); buffer of size cap, returns length. */
+/* Current params as JSON: replaces to_json (integrator.h:85-94); writes at
+ * most cap-1 bytes plus a terminating 0 into buf, returns the full length. */
 int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap);
+/* Frees every device allocation of the pass (~RenderPass,
+ * src/core/renderpass.h:146; the reference never frees its queues). */
 int krr_wf_destroy(krr_wf *wf);
 
 /* ---------------------------------------------------------------------- */

@@ -192,9 +192,11 @@ class WavefrontPathTracer:
 
 class RenderApp:
     """Headless counterpart of the reference's frame loop
-    (src/core/window.cpp:450-485, src/main/renderer.cpp:84-122): the frame
+    (src/core/window.cpp:412-434,457, src/main/renderer.cpp:84-97): the frame
     index starts at 1 and each frame calls beginFrame, render and endFrame of
-    every pass.  The film is a device float4 buffer owned by the app."""
+    every pass.  The film is a device float4 buffer owned by the app.  As
+    RenderApp::backBufferResized (renderer.cpp:25-31), the camera's aspect
+    ratio is width / height."""
 
     def __init__(self, scene, passes, width, height, stream=0):
         from . import hip
@@ -211,6 +213,7 @@ class RenderApp:
     def frame(self, view=0):
         self.frame_index += 1
         cam = self.scene.camera(view)
+        cam.aspect_ratio = self.width / self.height
         for p in self.passes:
             p.begin_frame(cam, self.frame_index)
         for p in self.passes:

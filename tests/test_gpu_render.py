@@ -40,7 +40,7 @@ def assert_images_match(g, o, rtol=1e-4, atol=1e-6):
     assert np.all(g[..., 3] == 1.0)
 
 
-def test_cornell_cfg1_parity(krr, oracle, gpu):
+def test_cornell_256_16spp_parity(krr, oracle, gpu):
     """Cornell box 256x256, 16 spp, max depth 8, NEE, rr 0.8 (frame 1)."""
     print("case: cornell 256x256 16spp depth 8", flush=True)
     s = krr.Scene(CORNELL)
@@ -68,7 +68,7 @@ def test_cornell_param_variants(krr, oracle, gpu, nee, depth, clamp, legacy):
     assert_images_match(g, o)
 
 
-def test_icosphere_cfg2_parity_subsample(krr, oracle, gpu):
+def test_icosphere_field_parity_subsample(krr, oracle, gpu):
     """The 1.31 M-triangle scene (diffuse / GGX conductor / glass) at
     240x135, 1 spp, max depth 16, RR from depth 3."""
     print("case: icosphere field 240x135 depth 16", flush=True)
@@ -82,7 +82,7 @@ def test_icosphere_cfg2_parity_subsample(krr, oracle, gpu):
     assert_images_match(g, o)
 
 
-def test_volume_cfg3_parity_subsample(krr, oracle, gpu):
+def test_volume_parity_subsample(krr, oracle, gpu):
     """Icosphere field inside a 64^3 value-noise medium (sigma_t max 20,
     albedo 0.8, HG g 0.3): delta tracking, medium NEE with ratio tracking."""
     print("case: volume 128x72 spp 2 depth 32", flush=True)
