@@ -694,37 +694,52 @@ __global__ void k_mega_begin(Counters *cnt) {
 // radiance additions and RNG draws of each pixel are unchanged.  Short queues
 // at deep bounces would otherwise cost one latency-bound launch per depth,
 // each as long as its slowest ray.
+//
+// Each loop iteration advances every lane's path by one depth; a lane whose
+// path has ended takes the next queued path at once (one atomic per wave for
+// all such lanes), so no lane idles while the queue still holds work.
 template <bool MEDIA, bool COUNT>
 __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
 							 int *ovf, Counters *cnt) {
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
-	const int count = *countPtr;
+	const int count		  = *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
+	bool active = false, exhausted = false;
+	int depth	= startDepth;
+	RayItem r;
 	while (true) {
-		const int base = waveFetch(&cnt->work[2]);
-		if (base >= count) break;
-		const int i = base + laneId();
-		if (i >= count) continue;
-		RayItem r = loadRayItem(q, i);
-		for (int depth = startDepth; depth <= P.maxDepth; depth++) {
-			nClosest++;
-			const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
-			const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
-			bool wantNext = false, wantShadow = false;
-			RayItem nx;
-			ShadowOut so;
-			so.medium = -1;
-			shadeCore<MEDIA>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so);
-			if (wantShadow) {
-				nShadow++;
-				if (MEDIA && P.enableMedium)
-					transmittanceItem<COUNT>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
-				else
-					occlusionItem<COUNT>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+		if (!exhausted) {
+			const int i = wavePush(&cnt->work[2], !active);
+			if (__ballot(!active && i >= count)) exhausted = true;
+			if (!active && i < count) {
+				r	   = loadRayItem(q, i);
+				depth  = startDepth;
+				active = true;
 			}
-			if (!wantNext) break;
+		}
+		if (__ballot(active) == 0) break;
+		if (!active) continue;
+		nClosest++;
+		const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
+		const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
+		bool wantNext = false, wantShadow = false;
+		RayItem nx;
+		ShadowOut so;
+		so.medium = -1;
+		shadeCore<MEDIA>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so);
+		if (wantShadow) {
+			nShadow++;
+			if (MEDIA && P.enableMedium)
+				transmittanceItem<COUNT>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
+			else
+				occlusionItem<COUNT>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+		}
+		if (wantNext && depth < P.maxDepth) {
 			r = nx;
+			depth++;
+		} else {
+			active = false;
 		}
 	}
 	waveAddU64(&cnt->closestTotal, nClosest);

@@ -147,3 +147,39 @@ def test_stack_overflow_render_parity(krr, oracle, gpu, lds_entries):
     o, os_ = oracle.render(s.desc, s.camera(0), 96, 54, 1, spp=2, max_depth=10, rr_min_depth=3)
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
+
+
+@pytest.mark.parametrize("nee", [True, False])
+def test_emissive_furnace_parity(krr, oracle, gpu, nee):
+    """Closed cube of 12 emissive diffuse triangles (every wall is a light):
+    light selection over many emitters, MIS at every bounce.  The checker's
+    image is pinned to the closed form in test_oracle_integrator.py."""
+    from test_oracle_integrator import _furnace_scene
+    print(f"case: emissive furnace 48x48 nee={nee}", flush=True)
+    s = _furnace_scene(krr, 0.5, 1.0)
+    params = {"spp": 4, "max_depth": 6, "nee": nee, "rr_min_depth": 1}
+    g, gs = gpu_render(krr, gpu, s, 48, 48, params)
+    o, os_ = oracle.render(s.desc, s.camera(0), 48, 48, 1, spp=4, max_depth=6, nee=int(nee), rr_min_depth=1)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+
+
+@pytest.mark.parametrize("albedo", [0.0, 0.9])
+def test_homogeneous_medium_parity(krr, oracle, gpu, albedo):
+    """Camera inside a homogeneous medium (pure absorber, and a scattering one
+    with HG g = 0.5) lit by a quad emitter and looking at a diffuse floor."""
+    print(f"case: homogeneous medium albedo={albedo}", flush=True)
+    s = krr.Scene({"materials": [{"bsdf": "diffuse", "diffuse": [0.6, 0.5, 0.4]}],
+                   "media": [{"type": "homogeneous", "sigma_t": [0.7, 0.5, 0.3], "albedo": [albedo] * 3, "g": 0.5}],
+                   "model": [{"type": "quad", "corners": [[-1, 1.5, -3], [1, 1.5, -3], [1, 1.5, -1], [-1, 1.5, -1]],
+                              "material": 0, "emission": [4, 4, 4]},
+                             {"type": "quad", "corners": [[-4, -1, 1], [4, -1, 1], [4, -1, -6], [-4, -1, -6]],
+                              "material": 0}],
+                   "resolution": [64, 48],
+                   "camera": {"eye": [0, 0, 0], "target": [0, -0.3, -1], "medium": 0}})
+    params = {"spp": 4, "max_depth": 8}
+    g, gs = gpu_render(krr, gpu, s, 64, 48, params)
+    o, os_ = oracle.render(s.desc, s.camera(0), 64, 48, 1, spp=4, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+    assert o[..., :3].max() > 0
