@@ -64,6 +64,7 @@ def gpu_bench(cfg, steps, warmup, traffic):
     log(f"scene: {scene.n_triangles} triangles ({time.time() - t0:.1f}s)")
     params = dict(cfg["params"], time_kernels=True)
     wf = krr.WavefrontPathTracer(params)
+    prm = wf.to_json()
     t0 = time.time()
     wf.set_scene(scene)
     log(f"set_scene (BVH build + upload): {time.time() - t0:.1f}s")
@@ -128,7 +129,7 @@ def gpu_bench(cfg, steps, warmup, traffic):
 
     rays = tot["closest"] + tot["shadow"]
     secs = wall
-    kernels = kernel_rooflines(cfg, steps, tot, cnt)
+    kernels = kernel_rooflines(cfg, prm, steps, tot, cnt)
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     k = kernels[dom]
     tr = (traffic or {}).get(k["kernel"])
@@ -164,14 +165,15 @@ def gpu_bench(cfg, steps, warmup, traffic):
 NODE_B, TRI_B, RAY_B, HIT_B = 128, 48, 32, 16
 
 
-def kernel_rooflines(cfg, steps, tot, cnt):
+def kernel_rooflines(cfg, prm, steps, tot, cnt):
     """Achieved algorithmic GB/s of each traversal kernel over the timed frames.
 
-    HIP-event times of the launches (time_kernels) and device counts of the
-    same frames (count_traversal, a second untimed pass)."""
-    spp = cfg["params"].get("spp", 1)
-    max_depth = cfg["params"].get("max_depth", 10)
-    wdepths = cfg["params"].get("wavefront_depths", 3)
+    prm: the pass's effective params (to_json).  HIP-event times of the
+    launches (time_kernels) and device counts of the same frames
+    (count_traversal, a second untimed pass)."""
+    spp = prm["spp"]
+    max_depth = prm["max_depth"]
+    wdepths = prm["wavefront_depths"]
     mega_launches = steps * spp if wdepths <= max_depth else 0
     wf_closest = cnt["closest_rays"] - cnt["megakernel_closest_rays"]
     wf_shadow = cnt["shadow_rays"] - cnt["megakernel_shadow_rays"]
@@ -191,9 +193,9 @@ def kernel_rooflines(cfg, steps, tot, cnt):
           "64 B ray + 16 B hit per ray + 128 B per node visit + 48 B per triangle test",
           wf_closest, cnt["node_visits"], cnt["tri_tests"])
     shadow_launches = 0
-    if cfg["params"].get("nee", True):  # no shadow rays are traced at the last depth
+    if prm["nee"]:  # no shadow rays are traced at the last depth
         shadow_launches = wf_launches - (steps * spp if max_depth < wdepths else 0)
-    media = bool(cfg["scene"].get("media")) and cfg["params"].get("enable_medium", True)
+    media = bool(cfg["scene"].get("media")) and prm["enable_medium"]
     entry("shadow_trace", "k_shadow_tr" if media else "k_trace_shadow", tot["shadow_ms"], shadow_launches,
           wf_shadow * (RAY_B * 2 + 16) + cnt["shadow_node_visits"] * NODE_B + cnt["shadow_tri_tests"] * TRI_B,
           "64 B ray + 16 B contribution per ray + 128 B per node visit + 48 B per triangle test",

@@ -117,7 +117,7 @@ struct Builder2 {
 				acc.grow(bb[k]);
 				c += cnt[k];
 				if (c == 0 || rightCnt[k + 1] == 0) continue;
-				float cost = 1.f + (acc.area() * c + rightArea[k + 1] * rightCnt[k + 1]) / std::max(pA, 1e-30f);
+				float cost = opt.nodeCost + (acc.area() * c + rightArea[k + 1] * rightCnt[k + 1]) / std::max(pA, 1e-30f);
 				if (cost < bestCost) {
 					bestCost = cost;
 					bestAxis = a;
@@ -267,6 +267,7 @@ void buildBvh4(const float *pos, const int32_t *idx, int64_t nTris, const uint8_
 	b.n	  = nTris;
 	b.opt = opt;
 	b.opt.maxLeafSize = std::max(1, std::min(8, opt.maxLeafSize));
+	b.opt.nodeCost	  = std::max(0.f, opt.nodeCost);
 	unsigned hw		  = std::thread::hardware_concurrency();
 	int threads		  = opt.threads > 0 ? opt.threads : (int) std::min(16u, std::max(1u, hw));
 	b.threadsLeft	  = threads - 1;

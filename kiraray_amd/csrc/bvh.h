@@ -43,6 +43,7 @@ struct Bvh4Host {
 struct BvhBuildOptions {
 	int maxLeafSize = 4;  // triangles per leaf (<= 8)
 	int bins		= 16; // SAH bins per axis
+	float nodeCost	= 1.f; // SAH cost of a node visit, in triangle tests
 	int threads		= 0;  // 0 = hardware concurrency (capped at 16)
 };
 

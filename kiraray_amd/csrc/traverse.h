@@ -29,44 +29,64 @@ struct TraceBvh {
 	int nNodes;	  // node count: a traversal visits each node at most once
 };
 
+// Per-ray constants of the watertight test: origin, the axis permutation
+// (kx, ky, kz) as the two bits of each index, and the shear.
 struct RayPre {
 	f3 o;
-	int kx, ky, kz;
+	bool x1, x2, y1, y2, z1, z2;
 	float Sx, Sy, Sz;
 };
 
-KRR_DEV float sel3(const f3 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+// (x, y, z)[k] for k = b1 + 2*b2 in {0, 1, 2}: two selects on per-ray
+// booleans, over values (a select chain on an integer index, or a select of
+// two fields of a referenced struct, compiles to a divergent branch or to a
+// dynamically indexed private array in scratch memory on gfx950).
+KRR_DEV float pick3(float x, float y, float z, bool b1, bool b2) {
+	const float xy = b1 ? y : x;
+	return b2 ? z : xy;
+}
 
 KRR_DEV RayPre rayPre(const f3 &o, const f3 &d) {
 	RayPre r;
-	r.o	   = o;
+	r.o		 = o;
+	const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+	// kz = largest |d| component, first index on ties
 	int kz = 0;
-	if (fabsf(d.y) > fabsf(sel3(d, kz))) kz = 1;
-	if (fabsf(d.z) > fabsf(sel3(d, kz))) kz = 2;
+	float m = ax;
+	if (ay > m) {
+		kz = 1;
+		m  = ay;
+	}
+	if (az > m) kz = 2;
 	int kx = kz + 1 == 3 ? 0 : kz + 1;
 	int ky = kx + 1 == 3 ? 0 : kx + 1;
-	if (sel3(d, kz) < 0.0f) {
+	const float dz = kz == 0 ? d.x : (kz == 1 ? d.y : d.z);
+	if (dz < 0.0f) {
 		int t = kx;
 		kx	  = ky;
 		ky	  = t;
 	}
-	r.kx	  = kx;
-	r.ky	  = ky;
-	r.kz	  = kz;
-	float dz  = sel3(d, kz);
-	r.Sx	  = sel3(d, kx) / dz;
-	r.Sy	  = sel3(d, ky) / dz;
-	r.Sz	  = 1.0f / dz;
+	r.x1 = kx & 1;
+	r.x2 = kx & 2;
+	r.y1 = ky & 1;
+	r.y2 = ky & 2;
+	r.z1 = kz & 1;
+	r.z2 = kz & 2;
+	r.Sx = pick3(d.x, d.y, d.z, r.x1, r.x2) / dz;
+	r.Sy = pick3(d.x, d.y, d.z, r.y1, r.y2) / dz;
+	r.Sz = 1.0f / dz;
 	return r;
 }
 
 // Watertight test; true with t > 0 and the barycentrics of v1, v2.
 KRR_DEV bool intersectTri(const RayPre &r, const f3 &v0, const f3 &v1, const f3 &v2, float &t, float &b1, float &b2) {
-	const f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-	const float Akz = sel3(A, r.kz), Bkz = sel3(B, r.kz), Ckz = sel3(C, r.kz);
-	const float Ax = sel3(A, r.kx) - r.Sx * Akz, Ay = sel3(A, r.ky) - r.Sy * Akz;
-	const float Bx = sel3(B, r.kx) - r.Sx * Bkz, By = sel3(B, r.ky) - r.Sy * Bkz;
-	const float Cx = sel3(C, r.kx) - r.Sx * Ckz, Cy = sel3(C, r.ky) - r.Sy * Ckz;
+	const float a0 = v0.x - r.o.x, a1 = v0.y - r.o.y, a2 = v0.z - r.o.z;
+	const float b0 = v1.x - r.o.x, bb1 = v1.y - r.o.y, bb2 = v1.z - r.o.z;
+	const float c0 = v2.x - r.o.x, c1 = v2.y - r.o.y, c2 = v2.z - r.o.z;
+	const float Akz = pick3(a0, a1, a2, r.z1, r.z2), Bkz = pick3(b0, bb1, bb2, r.z1, r.z2), Ckz = pick3(c0, c1, c2, r.z1, r.z2);
+	const float Ax = pick3(a0, a1, a2, r.x1, r.x2) - r.Sx * Akz, Ay = pick3(a0, a1, a2, r.y1, r.y2) - r.Sy * Akz;
+	const float Bx = pick3(b0, bb1, bb2, r.x1, r.x2) - r.Sx * Bkz, By = pick3(b0, bb1, bb2, r.y1, r.y2) - r.Sy * Bkz;
+	const float Cx = pick3(c0, c1, c2, r.x1, r.x2) - r.Sx * Ckz, Cy = pick3(c0, c1, c2, r.y1, r.y2) - r.Sy * Ckz;
 	float U = Cx * By - Cy * Bx;
 	float V = Ax * Cy - Ay * Cx;
 	float W = Bx * Ay - By * Ax;

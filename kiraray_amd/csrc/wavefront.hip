@@ -879,7 +879,7 @@ struct AccelData {
 	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes}; }
 };
 
-int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize) {
+int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCost) {
 	if (!s || s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
 		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing triangle arrays");
 	if (s->n_triangles > (int64_t) 0x0fffffff) return krr::fail(KRR_ERR_CAPACITY, "scene: at most 2^28 triangles");
@@ -895,6 +895,7 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize) {
 	}
 	BvhBuildOptions opt;
 	opt.maxLeafSize = leafSize;
+	opt.nodeCost	= nodeCost;
 	buildBvh4(s->positions, s->indices, s->n_triangles, nullFlag.data(), opt, a.host);
 	KRR_HIP(a.nodes.upload(a.host.nodes.data(), a.host.nodes.size() * sizeof(Bvh4Node)));
 	KRR_HIP(a.tris.upload(a.host.tris.data(), a.host.tris.size() * sizeof(LeafTri)));
@@ -944,9 +945,9 @@ struct krr_accel {
 struct krr_wf {
 	// params (integrator.h:96-103 + additions)
 	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false, timeKernels = false;
-	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 3;
+	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 0;
 	int stackLdsEntries = kShortStack;
-	float rr = 0.8f, clampMax = 1e3f;
+	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = 1.f;
 	// scene
 	bool haveScene = false;
 	AccelData accel;
@@ -1013,8 +1014,9 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->blockSize	   = (int) j.value("block_size", 256.0);
 		wf->traceBlockSize = (int) j.value("trace_block_size", 128.0);
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
-		wf->wavefrontDepths = (int) j.value("wavefront_depths", 3.0);
+		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
+		wf->sahNodeCost		= (float) j.value("sah_node_cost", 1.0);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
 	}
@@ -1026,6 +1028,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
 	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
+	if (!(wf->sahNodeCost >= 0.f && wf->sahNodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	return KRR_OK;
 }
 
@@ -1177,7 +1180,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	KRR_HIP(wf->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial)));
 	KRR_HIP(wf->lights.upload(lights.data(), lights.size() * sizeof(DevLight)));
 	KRR_HIP(wf->media.upload(media.data(), media.size() * sizeof(DevMedium)));
-	int rc = buildAccel(s, wf->accel, wf->leafSize);
+	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost);
 	if (rc) return rc;
 	DevScene &ds	= wf->ds;
 	ds.pos			= wf->pos.as<float>();
@@ -1239,22 +1242,25 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	if (!scene || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_build: null argument");
 	*out = nullptr;
 	int leafSize = 4, stackLdsEntries = kShortStack;
+	float nodeCost = 1.f;
 	if (json_opts && *json_opts) {
 		try {
 			Json j			= Json::parse(json_opts);
 			leafSize		= (int) j.value("leaf_size", 4.0);
 			stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
+			nodeCost		= (float) j.value("sah_node_cost", 1.0);
 		} catch (const std::exception &e) {
 			return krr::fail(KRR_ERR_BAD_CONFIG, "krr_accel_build: %s", e.what());
 		}
 		if (leafSize < 1 || leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
 		if (stackLdsEntries < 0 || stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
+		if (!(nodeCost >= 0.f && nodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	}
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_accel_build: no HIP device");
 	for (int m = 0; m < scene->n_meshes; m++)
 		if (scene->meshes[m].material >= scene->n_materials) return krr::fail(KRR_ERR_INVALID_ARG, "mesh %d material out of range", m);
 	auto a = std::make_unique<krr_accel>();
-	int rc = buildAccel(scene, a->a, leafSize);
+	int rc = buildAccel(scene, a->a, leafSize, nodeCost);
 	if (rc) return rc;
 	a->a.stackLds = stackLdsEntries;
 	KRR_HIP(a->counters.alloc(sizeof(Counters)));
@@ -1353,6 +1359,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("leaf_size", Json((double) wf->leafSize));
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
+	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	std::string s = j.dump();
 	if (buf && cap > 0) {
 		size_t n = std::min((size_t) cap - 1, s.size());

@@ -48,7 +48,7 @@ def test_pass_params_roundtrip(krr):
     # reference defaults (integrator.h:96-103)
     assert d["nee"] is True and d["enable_medium"] is True and d["max_depth"] == 10
     assert abs(d["rr"] - 0.8) < 1e-7 and d["enable_clamp"] is False and d["clamp_max"] == 1000
-    assert d["wavefront_depths"] == 3 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
+    assert d["wavefront_depths"] == 0 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
 
 
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',

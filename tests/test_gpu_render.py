@@ -68,12 +68,14 @@ def test_cornell_param_variants(krr, oracle, gpu, nee, depth, clamp, legacy):
     assert_images_match(g, o)
 
 
-def test_icosphere_field_parity_subsample(krr, oracle, gpu):
+@pytest.mark.parametrize("wd", [0, 100])
+def test_icosphere_field_parity_subsample(krr, oracle, gpu, wd):
     """The 1.31 M-triangle scene (diffuse / GGX conductor / glass) at
-    240x135, 1 spp, max depth 16, RR from depth 3."""
-    print("case: icosphere field 240x135 depth 16", flush=True)
+    240x135, 1 spp, max depth 16, RR from depth 3; all depths in the
+    persistent per-path launch (0) or all as wavefront stages (100)."""
+    print(f"case: icosphere field 240x135 depth 16 wavefront_depths={wd}", flush=True)
     s = krr.Scene(icosphere_scene(res=(240, 135)))
-    params = {"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3}
+    params = {"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3, "wavefront_depths": wd}
     g, gs = gpu_render(krr, gpu, s, 240, 135, params)
     o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
     print("gpu", gs, "cpu", os_, flush=True)
@@ -82,12 +84,13 @@ def test_icosphere_field_parity_subsample(krr, oracle, gpu):
     assert_images_match(g, o)
 
 
-def test_volume_parity_subsample(krr, oracle, gpu):
+@pytest.mark.parametrize("wd", [0, 100])
+def test_volume_parity_subsample(krr, oracle, gpu, wd):
     """Icosphere field inside a 64^3 value-noise medium (sigma_t max 20,
     albedo 0.8, HG g 0.3): delta tracking, medium NEE with ratio tracking."""
-    print("case: volume 128x72 spp 2 depth 32", flush=True)
+    print(f"case: volume 128x72 spp 2 depth 32 wavefront_depths={wd}", flush=True)
     s = krr.Scene(icosphere_scene(count=256, level=2, res=(128, 72), media=True, grid_res=64))
-    params = {"spp": 2, "max_depth": 32}
+    params = {"spp": 2, "max_depth": 32, "wavefront_depths": wd}
     g, gs = gpu_render(krr, gpu, s, 128, 72, params)
     o, os_ = oracle.render(s.desc, s.camera(0), 128, 72, 1, spp=2, max_depth=32)
     print("gpu", gs, "cpu", os_, flush=True)
