@@ -198,7 +198,7 @@ int krr_scene_destroy(krr_scene *scene);
 typedef struct krr_accel krr_accel;
 
 /* json_opts (NULL or "" = defaults): leaf_size (4) max triangles per leaf,
- * sah_node_cost (1) SAH cost of a node visit in triangle tests,
+ * sah_node_cost (0.25) SAH cost of a node visit in triangle tests,
  * stack_lds_entries (16) traversal stack entries per lane kept in LDS, the
  * deeper ones in a global overflow area. */
 int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_accel **out);
@@ -237,7 +237,7 @@ typedef struct krr_wf krr_wf;
  *                            depths of every path run in one persistent
  *                            launch with the same per-path stage order, so
  *                            the image does not depend on this value
- *   leaf_size (4), sah_node_cost (1), stack_lds_entries (16)  as for krr_accel_build
+ *   leaf_size (4), sah_node_cost (0.25), stack_lds_entries (16)  as for krr_accel_build
  * NULL or "" means all defaults. */
 /* Replaces RenderPassFactory::deserialize<WavefrontPathTracer> + from_json
  * (src/core/renderpass.h:202-273, src/render/wavefront/integrator.h:96-103)

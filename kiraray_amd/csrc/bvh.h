@@ -40,11 +40,17 @@ struct Bvh4Host {
 	double buildSeconds = 0;
 };
 
+// Default SAH cost of a binary-tree node, in triangle tests: the fastest of
+// 0.15 .. 3 for the gfx950 traversal on the 1.31 M-triangle icosphere field
+// (low values give small leaves; half the binary nodes vanish in the 4-wide
+// collapse, and a watertight triangle test costs more than a 4-wide box test).
+constexpr float kSahNodeCost = 0.25f;
+
 struct BvhBuildOptions {
-	int maxLeafSize = 4;  // triangles per leaf (<= 8)
-	int bins		= 16; // SAH bins per axis
-	float nodeCost	= 1.f; // SAH cost of a node visit, in triangle tests
-	int threads		= 0;  // 0 = hardware concurrency (capped at 16)
+	int maxLeafSize = 4;			// triangles per leaf (<= 8)
+	int bins		= 16;			// SAH bins per axis
+	float nodeCost	= kSahNodeCost; // SAH cost of a node visit, in triangle tests
+	int threads		= 0;			// 0 = hardware concurrency (capped at 16)
 };
 
 // pos: 3 floats per vertex; idx: 3 ints per triangle; nullFlag may be null.

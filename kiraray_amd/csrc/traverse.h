@@ -119,6 +119,13 @@ struct TraceResult {
 // coalesced when the lanes of a wave spill together).  A small LDS footprint
 // keeps the traversal kernels at high occupancy; deep stacks are rare but
 // stay exact.  nLds is uniform over the launch (kShortStack by default).
+//
+// Each lane owns nLds + 1 LDS slots.  Every push writes LDS slot
+// min(sp, nLds) - the extra slot absorbs the pushes that belong to the
+// overflow area - and every pop reads that slot, so the common case is one
+// ds_write / ds_read; only entries at sp >= nLds also take a (rarely
+// entered) branch to global memory.  A pointer chosen between LDS and
+// global memory would compile to flat loads and stores on every access.
 constexpr int kShortStack = 16;
 
 struct LaneStack {
@@ -128,10 +135,17 @@ struct LaneStack {
 	int ovStride;
 	int nLds;
 	KRR_DEV void put(int sp, int v) const {
-		if (sp < nLds) lds[sp * stride] = v;
-		else ovf[(size_t) (sp - nLds) * ovStride] = v;
+		lds[min(sp, nLds) * stride] = v;
+		if (sp >= nLds) ovf[(size_t) (sp - nLds) * ovStride] = v;
 	}
-	KRR_DEV int get(int sp) const { return sp < nLds ? lds[sp * stride] : ovf[(size_t) (sp - nLds) * ovStride]; }
+	KRR_DEV int get(int sp) const {
+		int v = lds[min(sp, nLds) * stride];
+		// keeps the LDS read a ds_read: without the barrier the compiler folds
+		// both reads into one flat load of a selected address
+		asm volatile("" : "+v"(v));
+		if (sp >= nLds) v = ovf[(size_t) (sp - nLds) * ovStride];
+		return v;
+	}
 };
 
 // Traverses one ray.  ANY: stop at the first hit of a non-null-material
@@ -142,7 +156,11 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 	TraceResult res{tmax, -1, 0.f, 0.f};
 	const RayPre rp = rayPre(o, d);
 	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-	const bool sx = __builtin_signbit(d.x), sy = __builtin_signbit(d.y), sz = __builtin_signbit(d.z);
+	// Byte offsets of the near/far planes inside a node, by direction sign
+	// (lo of axis a at 32a, hi at 32a + 16): the loads fetch the planes the
+	// ray needs directly, with 32-bit offsets from the node array base.
+	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
+	const char *nodeBase = reinterpret_cast<const char *>(bvh.nodes);
 	const float kWiden = 1.00000036f;					// 1 + 2*gamma(3)
 	const float kTie   = 1.00000095367431640625f;		// 1 + 2^-20
 	int sp			   = 0;
@@ -153,12 +171,12 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 	int budget = bvh.nNodes;
 	while (--budget >= 0) {
 		if (COUNT) nodeVisits++;
-		const float4 *np = reinterpret_cast<const float4 *>(bvh.nodes + node);
-		const float4 lox = np[0], hix = np[1], loy = np[2], hiy = np[3], loz = np[4], hiz = np[5];
-		const int4 ch	 = reinterpret_cast<const int4 *>(np)[6];
-		const float4 nx = sx ? hix : lox, fx = sx ? lox : hix;
-		const float4 ny = sy ? hiy : loy, fy = sy ? loy : hiy;
-		const float4 nz = sz ? hiz : loz, fz = sz ? loz : hiz;
+		const uint32_t nb = (uint32_t) node * 128u; // node array < 4 GiB (checked at build)
+#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
+		const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
+		const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
+#undef KRR_PLANE
+		const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
 		const float tlim = res.t * kTie;
 		float tn[4];
 		bool hit[4];

@@ -897,6 +897,9 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCo
 	opt.maxLeafSize = leafSize;
 	opt.nodeCost	= nodeCost;
 	buildBvh4(s->positions, s->indices, s->n_triangles, nullFlag.data(), opt, a.host);
+	// traversal addresses nodes with 32-bit byte offsets
+	if (a.host.nodes.size() * sizeof(Bvh4Node) >= ((size_t) 1 << 32))
+		return krr::fail(KRR_ERR_CAPACITY, "BVH of %zu nodes exceeds the 4 GiB node array limit", a.host.nodes.size());
 	KRR_HIP(a.nodes.upload(a.host.nodes.data(), a.host.nodes.size() * sizeof(Bvh4Node)));
 	KRR_HIP(a.tris.upload(a.host.tris.data(), a.host.tris.size() * sizeof(LeafTri)));
 	a.stackSize	   = bvhStackSize(a.host);
@@ -924,7 +927,8 @@ template <class K> int persistentGrid(K kernel, int block, size_t lds, int maxBl
 
 // LDS bytes per block of the short traversal stack, and ints per grid thread
 // of its global overflow.
-size_t stackLds(const AccelData &a, int block) { return (size_t) a.ldsEntries() * block * sizeof(int); }
+// nLds entries + one slot that absorbs the pushes bound for the overflow area
+size_t stackLds(const AccelData &a, int block) { return (size_t) (a.ldsEntries() + 1) * block * sizeof(int); }
 size_t stackOverflowInts(const AccelData &a) { return (size_t) (a.stackSize - a.ldsEntries()); }
 
 // Grows buf to hold the traversal-stack overflow of `threads` grid threads.
@@ -947,7 +951,7 @@ struct krr_wf {
 	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false, timeKernels = false;
 	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 0;
 	int stackLdsEntries = kShortStack;
-	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = 1.f;
+	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	// scene
 	bool haveScene = false;
 	AccelData accel;
@@ -1016,7 +1020,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
-		wf->sahNodeCost		= (float) j.value("sah_node_cost", 1.0);
+		wf->sahNodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
 	}
@@ -1242,13 +1246,13 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	if (!scene || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_build: null argument");
 	*out = nullptr;
 	int leafSize = 4, stackLdsEntries = kShortStack;
-	float nodeCost = 1.f;
+	float nodeCost = kSahNodeCost;
 	if (json_opts && *json_opts) {
 		try {
 			Json j			= Json::parse(json_opts);
 			leafSize		= (int) j.value("leaf_size", 4.0);
 			stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
-			nodeCost		= (float) j.value("sah_node_cost", 1.0);
+			nodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 		} catch (const std::exception &e) {
 			return krr::fail(KRR_ERR_BAD_CONFIG, "krr_accel_build: %s", e.what());
 		}
@@ -1448,7 +1452,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const int tb	   = wf->traceBlockSize;
 	const int gTmax	   = (N + tb - 1) / tb;
 	const size_t lds   = stackLds(wf->accel, tb);
-	if ((size_t) bvh.stackLds * tb * sizeof(int) != lds) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: stack layout mismatch");
+	if ((size_t) (bvh.stackLds + 1) * tb * sizeof(int) != lds) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: stack layout mismatch");
 	const bool count   = wf->countTraversal;
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
 	// persistent grid sizes (resident blocks), computed once per shape
