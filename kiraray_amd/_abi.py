@@ -9,6 +9,7 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libkrr_hip.so")
+NRC_PARAMS = 5 * 64 * 64 + 64 * 3  # KRR_NRC_PARAMS
 
 KRR_OK = 0
 KRR_ERR_INVALID_ARG = -1
@@ -103,6 +104,17 @@ PROTOTYPES = {
     "krr_wf_get_params": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "krr_wf_destroy": (C.c_int, [C.c_void_p]),
     "krr_error_metric": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_void_p]),
+    # neural radiance cache (include/krr_nrc.h)
+    "krr_nrc_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "krr_nrc_destroy": (C.c_int, [C.c_void_p]),
+    "krr_nrc_get_weights": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "krr_nrc_set_weights": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "krr_nrc_encode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "krr_nrc_infer": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "krr_nrc_gradient": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.POINTER(C.c_float),
+                                   C.c_void_p]),
+    "krr_nrc_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_float), C.c_void_p]),
+    "krr_nrc_steps": (C.c_int64, [C.c_void_p]),
 }
 
 _lib = None

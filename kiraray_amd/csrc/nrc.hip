@@ -1,0 +1,637 @@
+// nrc.hip - neural radiance cache on gfx950 matrix cores (krr_nrc.h).
+//
+// Network: x0 = enc(vertex) (64), z_l = W_l^T x_l, x_{l+1} = relu(z_l) for
+// l = 0..4, y = W_5^T x_5 (3 outputs, no activation, no biases; the encoding
+// carries two constant 1 inputs).  W_l is stored [in][out].
+//
+// MFMA formulation.  Each wave owns a tile of 32 samples and computes the
+// transposed products Z^T = W^T X^T with v_mfma_f32_32x32x8_bf16: the A
+// operand is a 32x8 slice of W^T (LDS), the B operand an 8x32 slice of X^T
+// held by the lanes (lane m + 32g holds features 8s + 4g + j, j < 4, of
+// sample m for k-step s), and the 32x32 accumulator tile leaves feature
+// 8(r/4) + 4g + r%4 of sample m in register r of lane m + 32g - exactly the
+// B-operand layout of the next layer.  Activations therefore never leave the
+// registers between layers.  Backward uses the same layout with A = W
+// (dX^T = W dZ^T).  Weight gradients dW_l = X_l dZ_l^T are a split-K GEMM over
+// the batch on X^T and dZ^T stored feature-major in HBM, reduced in a fixed
+// order (deterministic), then Adam updates the FP32 master weights and
+// rewrites both BF16 copies.
+//
+// Encoding (64 features, lane half g = 0/1 computes slots q = 4s + j of
+// feature 8s + 4g + j):
+//   half 0: q = 10c + 2f + t: sin (t = 0) / cos (t = 1) of 2^f pi pos_c,
+//           c < 3, f < 5; q = 30, 31: 1.
+//   half 1: q 0..2 pos; 3..6 / 7..10 one-blob(4) of dir theta / phi;
+//           11..14 / 15..18 of normal theta / phi; 19..22 of roughness;
+//           23..25 diffuse; 26..28 specular; 29..31: 0.
+//   one-blob(4) of x in [0,1]: exp(-0.5 (4 (x - (i + 0.5) / 4))^2), i < 4.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "detmath.h"
+#include "errors.h"
+#include "json.h"
+#include "krr_nrc.h"
+#include "nrc_dev.h"
+
+namespace krr {
+namespace nrc {
+
+// ---------------------------------------------------------------- device
+
+__device__ __forceinline__ uint16_t toBf16(float f) {
+	uint32_t u = __float_as_uint(f);
+	if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t) ((u >> 16) | ((u & 0xffff) ? 0x40 : 0)); // inf / nan
+	u += 0x7fffu + ((u >> 16) & 1u);																// round to nearest even
+	return (uint16_t) (u >> 16);
+}
+__device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float((uint32_t) h << 16); }
+
+__device__ __forceinline__ short4v pack4(float a, float b, float c, float d) {
+	short4v v;
+	v[0] = (short) toBf16(a);
+	v[1] = (short) toBf16(b);
+	v[2] = (short) toBf16(c);
+	v[3] = (short) toBf16(d);
+	return v;
+}
+
+__device__ __forceinline__ float oneBlob(float x, int i) {
+	const float d = 4.0f * (x - ((float) i + 0.5f) * 0.25f);
+	return detmath::expf(-0.5f * d * d);
+}
+
+// Encoding slot q (0..31) of lane half g for vertex record v (16 floats).
+__device__ float encodeSlot(const float *v, int g, int q) {
+	if (g == 0) {
+		if (q >= 30) return 1.0f;
+		const int c = q / 10, f = (q % 10) / 2, t = q & 1;
+		const float a = (float) (1 << f) * 3.14159265358979323846f * v[c];
+		return t ? detmath::cosf(a) : detmath::sinf(a);
+	}
+	if (q < 3) return v[q];						  // pos
+	if (q < 7) return oneBlob(v[3], q - 3);		  // dir theta
+	if (q < 11) return oneBlob(v[4], q - 7);	  // dir phi
+	if (q < 15) return oneBlob(v[5], q - 11);	  // normal theta
+	if (q < 19) return oneBlob(v[6], q - 15);	  // normal phi
+	if (q < 23) return oneBlob(v[7], q - 19);	  // roughness
+	if (q < 29) return v[8 + (q - 23)];			  // diffuse, specular
+	return 0.0f;
+}
+
+// B operands of layer 0 for sample record v: b[s] = features 8s + 4g + j.
+__device__ void encodeB(const float *v, int g, short4v b[8]) {
+#pragma unroll
+	for (int s = 0; s < 8; s++) b[s] = pack4(encodeSlot(v, g, 4 * s), encodeSlot(v, g, 4 * s + 1), encodeSlot(v, g, 4 * s + 2), encodeSlot(v, g, 4 * s + 3));
+}
+
+__device__ __forceinline__ float16v zero16() {
+	float16v z;
+#pragma unroll
+	for (int i = 0; i < 16; i++) z[i] = 0.f;
+	return z;
+}
+
+__device__ __forceinline__ float16v mfma(short4v a, short4v b, float16v c) { return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0); }
+
+// A operand: 4 consecutive bf16 at row `row` (64 bf16 per row), column 8s + 4g.
+__device__ __forceinline__ short4v ldsA(const uint16_t *m, int row, int s, int g, int rowLen = 64) {
+	return *reinterpret_cast<const short4v *>(m + row * rowLen + 8 * s + 4 * g);
+}
+
+// One hidden layer, forward: out (B layout, relu, bf16) = W^T x.
+__device__ __forceinline__ void hiddenForward(const uint16_t *wt, int lane, int g, const short4v x[8], short4v out[8], float16v &acc0,
+											  float16v &acc1) {
+	acc0 = zero16();
+	acc1 = zero16();
+	const int r0 = lane & 31;
+#pragma unroll
+	for (int s = 0; s < 8; s++) {
+		acc0 = mfma(ldsA(wt, r0, s, g), x[s], acc0);
+		acc1 = mfma(ldsA(wt, 32 + r0, s, g), x[s], acc1);
+	}
+#pragma unroll
+	for (int s = 0; s < 4; s++) {
+		out[s]	   = pack4(fmaxf(acc0[4 * s], 0.f), fmaxf(acc0[4 * s + 1], 0.f), fmaxf(acc0[4 * s + 2], 0.f), fmaxf(acc0[4 * s + 3], 0.f));
+		out[4 + s] = pack4(fmaxf(acc1[4 * s], 0.f), fmaxf(acc1[4 * s + 1], 0.f), fmaxf(acc1[4 * s + 2], 0.f), fmaxf(acc1[4 * s + 3], 0.f));
+	}
+}
+
+// Output layer forward: y_c (c < 3) lands in acc[c] of lanes with g == 0.
+__device__ __forceinline__ float16v outputForward(const uint16_t *wt5, int lane, int g, const short4v x[8]) {
+	float16v acc = zero16();
+	const int r0 = lane & 31;
+#pragma unroll
+	for (int s = 0; s < 8; s++) acc = mfma(ldsA(wt5, r0, s, g), x[s], acc);
+	return acc;
+}
+
+// Copies the bf16 weights (n elements, multiple of 8) into LDS.
+__device__ void loadWeights(uint16_t *lds, const uint16_t *src, int n) {
+	const uint4 *s = reinterpret_cast<const uint4 *>(src);
+	uint4 *d	   = reinterpret_cast<uint4 *>(lds);
+	for (int i = threadIdx.x; i < n / 8; i += blockDim.x) d[i] = s[i];
+}
+
+// ----------------------------------------------------------------- kernels
+
+__global__ void __launch_bounds__(256) k_encode(const float *vertices, int64_t n, float *out) {
+	const int64_t i = (int64_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n * 64) return;
+	const int64_t row = i / 64;
+	const int k		  = (int) (i % 64);
+	const int s = k / 8, g = (k / 4) & 1, j = k & 3;
+	out[i] = encodeSlot(vertices + row * 16, g, 4 * s + j);
+}
+
+// Inference: each wave evaluates tiles of 32 samples.
+__global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const float *vertices, int64_t n, float *radiance) {
+	extern __shared__ uint16_t ldsW[];
+	loadWeights(ldsW, wtAll, kFwdElems);
+	__syncthreads();
+	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
+	const int wavesPerBlock = blockDim.x >> 6;
+	const int64_t tiles		= (n + 31) / 32;
+	for (int64_t t = (int64_t) blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); t < tiles; t += (int64_t) gridDim.x * wavesPerBlock) {
+		const int64_t i = t * 32 + m;
+		float v[16];
+		if (i < n) {
+			const float4 *p = reinterpret_cast<const float4 *>(vertices + i * 16);
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const float4 f = p[q];
+				v[4 * q]	   = f.x;
+				v[4 * q + 1]   = f.y;
+				v[4 * q + 2]   = f.z;
+				v[4 * q + 3]   = f.w;
+			}
+		} else {
+#pragma unroll
+			for (int q = 0; q < 16; q++) v[q] = 0.f;
+		}
+		short4v x[8], y[8];
+		encodeB(v, g, x);
+		float16v a0, a1;
+#pragma unroll
+		for (int l = 0; l < kHidden; l++) {
+			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
+#pragma unroll
+			for (int s = 0; s < 8; s++) x[s] = y[s];
+		}
+		const float16v o = outputForward(ldsW + kHidden * 4096, lane, g, x);
+		if (g == 0 && i < n) {
+			radiance[3 * i]		= o[0];
+			radiance[3 * i + 1] = o[1];
+			radiance[3 * i + 2] = o[2];
+		}
+	}
+}
+
+// Writes the B-layout bf16 operand (features 8s + 4g + j of sample m) to a
+// feature-major [feature][ld] array at column `col`.
+__device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int64_t col, int g, const short4v x[8], int features = 64) {
+#pragma unroll
+	for (int s = 0; s < 8; s++)
+#pragma unroll
+		for (int j = 0; j < 4; j++) {
+			const int f = 8 * s + 4 * g + j;
+			if (f < features) dst[(int64_t) f * ld + col] = (uint16_t) x[s][j];
+		}
+}
+
+// Forward + backward of one tile of 32 samples per wave.  Writes X_l^T and
+// dZ_l^T (bf16, feature-major, leading dimension ld = padded batch) and the
+// tile's loss sum.  Samples i >= n contribute nothing.
+__global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const uint16_t *wAll, const float *vertices, const float *targets,
+													int64_t n, int64_t nTiles, int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT,
+													float *tileLoss) {
+	// forward A operands from LDS; backward ones straight from global memory
+	// (45 KB, cache resident), which keeps the block within 64 KB of LDS
+	extern __shared__ uint16_t lds[];
+	uint16_t *ldsWt		= lds;
+	const uint16_t *ldsW = wAll;
+	loadWeights(ldsWt, wtAll, kFwdElems);
+	__syncthreads();
+	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
+	const int wavesPerBlock = blockDim.x >> 6;
+	const float invCount	= 1.0f / (3.0f * (float) n);
+	for (int64_t t = (int64_t) blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); t < nTiles; t += (int64_t) gridDim.x * wavesPerBlock) {
+		const int64_t i	  = t * 32 + m;
+		const bool valid = i < n;
+		float v[16];
+		if (valid) {
+			const float4 *p = reinterpret_cast<const float4 *>(vertices + i * 16);
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const float4 f = p[q];
+				v[4 * q]	   = f.x;
+				v[4 * q + 1]   = f.y;
+				v[4 * q + 2]   = f.z;
+				v[4 * q + 3]   = f.w;
+			}
+		} else {
+#pragma unroll
+			for (int q = 0; q < 16; q++) v[q] = 0.f;
+		}
+		// forward, keeping every layer input
+		short4v x[kHidden + 1][8];
+		encodeB(v, g, x[0]);
+		float16v a0, a1;
+#pragma unroll
+		for (int l = 0; l < kHidden; l++) hiddenForward(ldsWt + l * 4096, lane, g, x[l], x[l + 1], a0, a1);
+		const float16v o = outputForward(ldsWt + kHidden * 4096, lane, g, x[kHidden]);
+		// relative L2 loss and its gradient (lanes g == 0 hold the 3 outputs)
+		float d[3]	 = {0.f, 0.f, 0.f};
+		float lsum	 = 0.f;
+		if (g == 0 && valid) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				const float y = o[c], r = y - targets[3 * i + c];
+				const float den = y * y + lossEps;
+				lsum += r * r / den;
+				d[c] = 2.0f * r / den * invCount;
+			}
+		}
+		// wave sum of the loss
+		for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+		if (lane == 0) tileLoss[t] = lsum * invCount;
+		// dZ_5 as a B operand: outputs n = 8s + 4g + j, only n < 3 nonzero
+		short4v dz[8];
+#pragma unroll
+		for (int s = 0; s < 8; s++) dz[s] = pack4(0.f, 0.f, 0.f, 0.f);
+		if (g == 0) dz[0] = pack4(d[0], d[1], d[2], 0.f);
+		// store X_5^T, dZ_5^T (32 padded output rows)
+		storeFeatureMajor(xT + (int64_t) kHidden * 64 * ld, ld, i, g, x[kHidden]);
+		storeFeatureMajor(dzT + (int64_t) kHidden * 64 * ld, ld, i, g, dz, 32);
+		// backward through the output layer: dX_5^T = W_5 dZ_5^T (K = 32 padded outputs)
+		const int r0 = lane & 31;
+		{
+			const uint16_t *w5 = ldsW + kHidden * 4096;
+			a0 = zero16();
+			a1 = zero16();
+#pragma unroll
+			for (int s = 0; s < 4; s++) {
+				a0 = mfma(ldsA(w5, r0, s, g, 32), dz[s], a0);
+				a1 = mfma(ldsA(w5, 32 + r0, s, g, 32), dz[s], a1);
+			}
+		}
+#pragma unroll
+		for (int l = kHidden - 1; l >= 0; l--) {
+			// dZ_l = dX_{l+1} * (x_{l+1} > 0), as the next B operand
+#pragma unroll
+			for (int s = 0; s < 4; s++)
+#pragma unroll
+				for (int j = 0; j < 4; j++) {
+					const float m0 = fromBf16((uint16_t) x[l + 1][s][j]) > 0.f ? a0[4 * s + j] : 0.f;
+					const float m1 = fromBf16((uint16_t) x[l + 1][4 + s][j]) > 0.f ? a1[4 * s + j] : 0.f;
+					dz[s][j]	   = (short) toBf16(m0);
+					dz[4 + s][j]   = (short) toBf16(m1);
+				}
+			storeFeatureMajor(xT + (int64_t) l * 64 * ld, ld, i, g, x[l]);
+			storeFeatureMajor(dzT + (int64_t) l * 64 * ld, ld, i, g, dz);
+			if (l > 0) {
+				const uint16_t *w = ldsW + l * 4096;
+				a0				  = zero16();
+				a1				  = zero16();
+#pragma unroll
+				for (int s = 0; s < 8; s++) {
+					a0 = mfma(ldsA(w, r0, s, g), dz[s], a0);
+					a1 = mfma(ldsA(w, 32 + r0, s, g), dz[s], a1);
+				}
+			}
+		}
+	}
+}
+
+// Split-K weight gradient of one layer: partial[chunk][k][nOut] =
+// sum over the chunk's samples of X^T[k][m] dZ^T[n][m].  One wave per 32x32
+// output tile; blockIdx.y = chunk.
+__global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xT, const uint16_t *dzT, int64_t ld, int chunk, int nOutTiles,
+													 int nOut, float *partial) {
+	const int lane = threadIdx.x & 63, g = lane >> 5, r0 = lane & 31;
+	const int tile = threadIdx.x >> 6; // 0..(2 * nOutTiles - 1)
+	if (tile >= 2 * nOutTiles) return;
+	const int kt = tile / nOutTiles, nt = tile % nOutTiles;
+	const int64_t m0 = (int64_t) blockIdx.y * chunk;
+	float16v acc	 = zero16();
+	const uint16_t *arow = xT + (int64_t) (32 * kt + r0) * ld + m0;
+	const uint16_t *brow = dzT + (int64_t) (32 * nt + r0) * ld + m0;
+	for (int s = 0; s < chunk / 8; s++) {
+		const short4v a = *reinterpret_cast<const short4v *>(arow + 8 * s + 4 * g);
+		const short4v b = *reinterpret_cast<const short4v *>(brow + 8 * s + 4 * g);
+		acc				= mfma(a, b, acc);
+	}
+	// D[i = k][j = n]: lane holds column n = r0, rows k = 8(r/4) + 4g + r%4
+	float *dst = partial + (int64_t) blockIdx.y * 64 * nOut;
+#pragma unroll
+	for (int r = 0; r < 16; r++) {
+		const int k = 32 * kt + 8 * (r / 4) + 4 * g + (r % 4);
+		const int n = 32 * nt + r0;
+		if (n < nOut) dst[k * nOut + n] = acc[r];
+	}
+}
+
+// grad[p] = sum over chunks (fixed order) of partial[chunk][p].
+__global__ void k_reduce_chunks(const float *partial, int chunks, int count, float *grad) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= count) return;
+	float s = 0.f;
+	for (int c = 0; c < chunks; c++) s += partial[(int64_t) c * count + p];
+	grad[p] = s;
+}
+
+// Deterministic loss total over tiles (one block).
+__global__ void k_reduce_loss(const float *tileLoss, int64_t nTiles, float *out) {
+	__shared__ float part[256];
+	float s = 0.f;
+	for (int64_t t = threadIdx.x; t < nTiles; t += blockDim.x) s += tileLoss[t];
+	part[threadIdx.x] = s;
+	__syncthreads();
+	for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+		if ((int) threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+		__syncthreads();
+	}
+	if (threadIdx.x == 0) *out = part[0];
+}
+
+// Rewrites both bf16 copies from the fp32 master weights.
+__device__ void writeCopies(const float *w, uint16_t *wt, uint16_t *wb, int p) {
+	// p indexes the master array: layer l < 5 at 4096 l + 64 k + n; layer 5 at 20480 + 3 k + n.
+	if (p < kHidden * 4096) {
+		const int l = p / 4096, k = (p / 64) % 64, n = p % 64;
+		wt[l * 4096 + n * 64 + k] = toBf16(w[p]);
+		wb[l * 4096 + k * 64 + n] = toBf16(w[p]);
+	} else {
+		const int q = p - kHidden * 4096, k = q / 3, n = q % 3;
+		wt[kHidden * 4096 + n * 64 + k] = toBf16(w[p]);
+		wb[kHidden * 4096 + k * 32 + n] = toBf16(w[p]);
+	}
+}
+
+__global__ void k_write_copies(const float *w, uint16_t *wt, uint16_t *wb) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p < KRR_NRC_PARAMS) writeCopies(w, wt, wb, p);
+}
+
+// Adam (Kingma & Ba 2015) with bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
+__global__ void k_adam(float *w, float *mo, float *ve, const float *grad, float lr, float b1, float b2, float eps, float c1, float c2,
+					   uint16_t *wt, uint16_t *wb) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= KRR_NRC_PARAMS) return;
+	const float gr = grad[p];
+	const float m  = b1 * mo[p] + (1.f - b1) * gr;
+	const float v  = b2 * ve[p] + (1.f - b2) * gr * gr;
+	mo[p]		   = m;
+	ve[p]		   = v;
+	w[p] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
+	writeCopies(w, wt, wb, p);
+}
+
+} // namespace nrc
+} // namespace krr
+
+// ------------------------------------------------------------------- host
+
+using namespace krr::nrc;
+
+namespace {
+
+struct Buf {
+	void *p	  = nullptr;
+	size_t sz = 0;
+	~Buf() {
+		if (p) (void) hipFree(p);
+	}
+	hipError_t alloc(size_t bytes) {
+		if (p) (void) hipFree(p);
+		p  = nullptr;
+		sz = 0;
+		hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+		if (e == hipSuccess) sz = bytes;
+		return e;
+	}
+	template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+#define NRC_HIP(expr)                                                                                                  \
+	do {                                                                                                               \
+		hipError_t e_ = (expr);                                                                                        \
+		if (e_ != hipSuccess) return krr::fail(KRR_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));                  \
+	} while (0)
+
+constexpr int kChunk = 1024; // samples per split-K chunk of the weight gradient
+
+} // namespace
+
+struct krr_nrc {
+	float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f, lossEps = 0.01f;
+	uint64_t seed	= 1;
+	int64_t maxBatch = 65536, ld = 0;
+	int64_t steps	= 0;
+	Buf w, m, v, grad, wt, wb, xT, dzT, partial, tileLoss, loss;
+};
+
+namespace {
+
+int uploadWeights(krr_nrc *h, const std::vector<float> &host) {
+	NRC_HIP(hipMemcpy(h->w.p, host.data(), sizeof(float) * KRR_NRC_PARAMS, hipMemcpyHostToDevice));
+	NRC_HIP(hipMemset(h->m.p, 0, sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(hipMemset(h->v.p, 0, sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(hipMemset(h->wt.p, 0, h->wt.sz));
+	NRC_HIP(hipMemset(h->wb.p, 0, h->wb.sz));
+	hipLaunchKernelGGL(k_write_copies, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, 0, h->w.as<float>(), h->wt.as<uint16_t>(),
+					   h->wb.as<uint16_t>());
+	NRC_HIP(hipGetLastError());
+	NRC_HIP(hipDeviceSynchronize());
+	h->steps = 0;
+	return KRR_OK;
+}
+
+// Seeded Xavier-uniform initial weights (PCG32 stream of the seed).
+std::vector<float> initialWeights(uint64_t seed) {
+	std::vector<float> w(KRR_NRC_PARAMS);
+	uint64_t state = 0, inc = (seed << 1u) | 1u;
+	auto next = [&]() {
+		uint64_t old = state;
+		state		 = old * 6364136223846793005ULL + inc;
+		uint32_t xs	 = (uint32_t) (((old >> 18u) ^ old) >> 27u);
+		uint32_t rot = (uint32_t) (old >> 59u);
+		return (xs >> rot) | (xs << ((-rot) & 31));
+	};
+	next();
+	state += 0x853c49e6748fea9bULL;
+	next();
+	for (int p = 0; p < KRR_NRC_PARAMS; p++) {
+		const int fanOut = p < kHidden * 4096 ? 64 : 3;
+		const float a	 = std::sqrt(6.0f / (64.0f + (float) fanOut));
+		const float u	 = (float) (next() >> 8) * (1.0f / 16777216.0f);
+		w[p]			 = (2.0f * u - 1.0f) * a;
+	}
+	return w;
+}
+
+bool validN(int64_t n) { return n >= 0 && n <= ((int64_t) 1 << 31); }
+
+int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st) {
+	const int64_t chunks = (n + kChunk - 1) / kChunk;
+	const int64_t nTiles = chunks * (kChunk / 32);
+	const int lds		 = kFwdElems * (int) sizeof(uint16_t);
+	const int blocks	 = (int) std::min<int64_t>(1024, (nTiles + 3) / 4);
+	hipLaunchKernelGGL(k_train_tile, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), (const float *) vert,
+					   targets, n, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(), h->tileLoss.as<float>());
+	NRC_HIP(hipGetLastError());
+	for (int l = 0; l <= kHidden; l++) {
+		const int nOut = l < kHidden ? 64 : 3, nOutTiles = l < kHidden ? 2 : 1;
+		hipLaunchKernelGGL(k_weight_grad, dim3(1, (unsigned) chunks), dim3(64 * 2 * nOutTiles), 0, st,
+						   h->xT.as<uint16_t>() + (int64_t) l * 64 * h->ld, h->dzT.as<uint16_t>() + (int64_t) l * 64 * h->ld, h->ld, kChunk,
+						   nOutTiles, nOut, h->partial.as<float>());
+		NRC_HIP(hipGetLastError());
+		const int count = 64 * nOut;
+		hipLaunchKernelGGL(k_reduce_chunks, dim3((count + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), (int) chunks, count,
+						   h->grad.as<float>() + l * 4096);
+		NRC_HIP(hipGetLastError());
+	}
+	hipLaunchKernelGGL(k_reduce_loss, dim3(1), dim3(256), 0, st, h->tileLoss.as<float>(), nTiles, h->loss.as<float>());
+	NRC_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int krr_nrc_create(const char *json, krr_nrc **out) {
+	krr::clearError();
+	if (!out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_create: null out");
+	*out   = nullptr;
+	auto h = std::make_unique<krr_nrc>();
+	if (json && *json) {
+		try {
+			krr::Json j = krr::Json::parse(json);
+			if (!j.isObject()) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc config must be a JSON object");
+			h->lr		= (float) j.value("lr", 1e-3);
+			h->beta1	= (float) j.value("beta1", 0.9);
+			h->beta2	= (float) j.value("beta2", 0.99);
+			h->eps		= (float) j.value("eps", 1e-8);
+			h->lossEps	= (float) j.value("loss_eps", 0.01);
+			h->seed		= (uint64_t) j.value("seed", 1.0);
+			h->maxBatch = (int64_t) j.value("max_batch", 65536.0);
+		} catch (const std::exception &e) {
+			return krr::fail(KRR_ERR_BAD_CONFIG, "nrc config: %s", e.what());
+		}
+	}
+	if (!(h->lr > 0) || !(h->beta1 >= 0 && h->beta1 < 1) || !(h->beta2 >= 0 && h->beta2 < 1) || !(h->eps > 0) || !(h->lossEps > 0))
+		return krr::fail(KRR_ERR_BAD_CONFIG, "nrc config: lr, eps and loss_eps must be > 0 and betas in [0, 1)");
+	if (h->maxBatch < 32 || h->maxBatch > (1 << 24)) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc config: max_batch must be in [32, 2^24]");
+	int nDev = 0;
+	if (hipGetDeviceCount(&nDev) != hipSuccess || nDev < 1) return krr::fail(KRR_ERR_NO_DEVICE, "krr_nrc_create: no HIP device");
+	h->ld = ((h->maxBatch + kChunk - 1) / kChunk) * kChunk;
+	const int64_t chunks = h->ld / kChunk;
+	NRC_HIP(h->w.alloc(sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(h->m.alloc(sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(h->v.alloc(sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(h->grad.alloc(sizeof(float) * KRR_NRC_PARAMS));
+	NRC_HIP(h->wt.alloc(sizeof(uint16_t) * kFwdElems));
+	NRC_HIP(h->wb.alloc(sizeof(uint16_t) * kBwdElems));
+	NRC_HIP(h->xT.alloc(sizeof(uint16_t) * 64 * (kHidden + 1) * h->ld));
+	NRC_HIP(h->dzT.alloc(sizeof(uint16_t) * 64 * (kHidden + 1) * h->ld));
+	NRC_HIP(h->partial.alloc(sizeof(float) * 4096 * chunks));
+	NRC_HIP(h->tileLoss.alloc(sizeof(float) * (h->ld / 32)));
+	NRC_HIP(h->loss.alloc(sizeof(float)));
+	// padded rows of dZ_5^T (outputs 3..31) and of X^T are written per tile;
+	// clear once so that unused feature rows are finite
+	NRC_HIP(hipMemset(h->xT.p, 0, h->xT.sz));
+	NRC_HIP(hipMemset(h->dzT.p, 0, h->dzT.sz));
+	if (int rc = uploadWeights(h.get(), initialWeights(h->seed))) return rc;
+	*out = h.release();
+	return KRR_OK;
+}
+
+int krr_nrc_destroy(krr_nrc *nrc) {
+	delete nrc;
+	return KRR_OK;
+}
+
+int krr_nrc_get_weights(const krr_nrc *nrc, float *host_out) {
+	krr::clearError();
+	if (!nrc || !host_out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_get_weights: null argument");
+	NRC_HIP(hipDeviceSynchronize());
+	NRC_HIP(hipMemcpy(host_out, nrc->w.p, sizeof(float) * KRR_NRC_PARAMS, hipMemcpyDeviceToHost));
+	return KRR_OK;
+}
+
+int krr_nrc_set_weights(krr_nrc *nrc, const float *host_in) {
+	krr::clearError();
+	if (!nrc || !host_in) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_set_weights: null argument");
+	std::vector<float> w(host_in, host_in + KRR_NRC_PARAMS);
+	for (float x : w)
+		if (!std::isfinite(x)) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_set_weights: non-finite weight");
+	return uploadWeights(nrc, w);
+}
+
+int krr_nrc_encode(const krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, int64_t n, float *encoded_dev, void *hip_stream) {
+	krr::clearError();
+	if (!nrc || !validN(n) || (n > 0 && (!vertices_dev || !encoded_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_encode: bad argument");
+	if (n == 0) return KRR_OK;
+	const int64_t total = n * 64;
+	hipLaunchKernelGGL(k_encode, dim3((unsigned) ((total + 255) / 256)), dim3(256), 0, (hipStream_t) hip_stream, (const float *) vertices_dev, n,
+					   encoded_dev);
+	NRC_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+int krr_nrc_infer(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, int64_t n, float *radiance_dev, void *hip_stream) {
+	krr::clearError();
+	if (!nrc || !validN(n) || (n > 0 && (!vertices_dev || !radiance_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_infer: bad argument");
+	if (n == 0) return KRR_OK;
+	const int64_t tiles = (n + 31) / 32;
+	const int blocks	= (int) std::min<int64_t>(2048, (tiles + 3) / 4);
+	hipLaunchKernelGGL(k_infer, dim3(blocks), dim3(256), kFwdElems * (int) sizeof(uint16_t), (hipStream_t) hip_stream, nrc->wt.as<uint16_t>(),
+					   (const float *) vertices_dev, n, radiance_dev);
+	NRC_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const float *targets_dev, int64_t n, float *grad_dev,
+					 float *loss_out, void *hip_stream) {
+	krr::clearError();
+	if (!nrc || n < 1 || !vertices_dev || !targets_dev || !grad_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_gradient: bad argument");
+	if (n > nrc->maxBatch) return krr::fail(KRR_ERR_CAPACITY, "krr_nrc_gradient: batch %lld exceeds max_batch %lld", (long long) n, (long long) nrc->maxBatch);
+	hipStream_t st = (hipStream_t) hip_stream;
+	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st)) return rc;
+	NRC_HIP(hipMemcpyAsync(grad_dev, nrc->grad.p, sizeof(float) * KRR_NRC_PARAMS, hipMemcpyDeviceToDevice, st));
+	if (loss_out) {
+		NRC_HIP(hipMemcpyAsync(loss_out, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
+		NRC_HIP(hipStreamSynchronize(st));
+	}
+	return KRR_OK;
+}
+
+int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const float *targets_dev, int64_t n, float *loss_out,
+					   void *hip_stream) {
+	krr::clearError();
+	if (!nrc || n < 1 || !vertices_dev || !targets_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_train_step: bad argument");
+	if (n > nrc->maxBatch) return krr::fail(KRR_ERR_CAPACITY, "krr_nrc_train_step: batch %lld exceeds max_batch %lld", (long long) n, (long long) nrc->maxBatch);
+	hipStream_t st = (hipStream_t) hip_stream;
+	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st)) return rc;
+	nrc->steps++;
+	const float c1 = 1.f - std::pow(nrc->beta1, (float) nrc->steps), c2 = 1.f - std::pow(nrc->beta2, (float) nrc->steps);
+	hipLaunchKernelGGL(k_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, nrc->w.as<float>(), nrc->m.as<float>(), nrc->v.as<float>(),
+					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, c1, c2, nrc->wt.as<uint16_t>(),
+					   nrc->wb.as<uint16_t>());
+	NRC_HIP(hipGetLastError());
+	if (loss_out) {
+		NRC_HIP(hipMemcpyAsync(loss_out, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
+		NRC_HIP(hipStreamSynchronize(st));
+	}
+	return KRR_OK;
+}
+
+int64_t krr_nrc_steps(const krr_nrc *nrc) { return nrc ? nrc->steps : -1; }
+
+} // extern "C"

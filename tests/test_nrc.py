@@ -1,0 +1,213 @@
+"""Neural radiance cache: the FP64 CPU reference (oracle/nrc.py) pinned by
+finite differences and the closed form of Adam, then the gfx950 BF16
+matrix-core network (include/krr_nrc.h) checked against it."""
+import numpy as np
+import pytest
+
+import oracle.nrc as ref
+
+
+@pytest.fixture(scope="module")
+def rng():
+    return np.random.default_rng(12)
+
+
+def _xavier(rng):
+    w = np.zeros(ref.PARAMS)
+    a = np.sqrt(6.0 / 128.0)
+    w[:ref.HIDDEN * 4096] = rng.uniform(-a, a, ref.HIDDEN * 4096)
+    a5 = np.sqrt(6.0 / 67.0)
+    w[ref.HIDDEN * 4096:] = rng.uniform(-a5, a5, 64 * 3)
+    return w
+
+
+# ----------------------------------------------------------------- CPU pins
+
+def test_encoding_definition(rng):
+    v = ref.random_vertices(50, rng)
+    e = ref.encode(v)
+    assert e.shape == (50, 64)
+    # half 0, slot 30/31 (features 8*7 + 2, 8*7 + 3) are the constant inputs
+    assert np.all(e[:, 58] == 1.0) and np.all(e[:, 59] == 1.0)
+    # sin/cos pairs: slot 2f, 2f+1 of coordinate c -> sin^2 + cos^2 = 1
+    def feat(g, q):
+        return 8 * (q // 4) + 4 * g + q % 4
+    for c in range(3):
+        for f in range(5):
+            s, co = e[:, feat(0, 10 * c + 2 * f)], e[:, feat(0, 10 * c + 2 * f + 1)]
+            np.testing.assert_allclose(s * s + co * co, 1.0, atol=1e-12)
+            np.testing.assert_allclose(s, np.sin(2 ** f * np.pi * v[:, c].astype(np.float64)), atol=1e-12)
+    # one-blob peaks at the bin centre
+    x = np.full((1, 16), 0.0, np.float32)
+    x[0, 3] = 0.375  # centre of bin 1
+    eb = ref.encode(x)
+    ob = [eb[0, feat(1, q)] for q in range(3, 7)]
+    assert np.argmax(ob) == 1 and abs(ob[1] - 1.0) < 1e-12
+
+
+def test_reference_gradient_matches_finite_differences(rng):
+    """Analytic backprop (denominator held constant, as in the loss) against
+    central differences of the same surrogate, FP64."""
+    w = _xavier(rng)
+    v = ref.random_vertices(48, rng)
+    x0 = ref.encode(v)
+    t = rng.random((48, 3)) * 2
+    y0, _ = ref.forward(w, x0)
+    den = y0 * y0 + 0.01
+
+    def surrogate(wp):
+        y, _ = ref.forward(wp, x0)
+        return np.sum((y - t) ** 2 / den) / y.size
+
+    loss, g = ref.loss_and_grad(w, x0, t)
+    assert abs(loss - surrogate(w)) < 1e-12
+    idx = rng.choice(ref.PARAMS, 300, replace=False)
+    h = 1e-6
+    fd = np.array([(surrogate(w + h * np.eye(1, ref.PARAMS, i)[0]) - surrogate(w - h * np.eye(1, ref.PARAMS, i)[0])) / (2 * h)
+                   for i in idx])
+    err = np.abs(fd - g[idx]) / (np.abs(g[idx]) + 1e-7)
+    assert np.median(err) < 1e-6 and np.mean(err < 1e-4) > 0.97, (np.median(err), np.max(err))
+
+
+def test_reference_adam_closed_form(rng):
+    """From zero moments the first bias-corrected step is lr * g / (|g| + eps)."""
+    w = rng.standard_normal(100)
+    g = rng.standard_normal(100)
+    w1, m, v = ref.adam_step(w, np.zeros(100), np.zeros(100), g, 1, lr=1e-3, eps=1e-8)
+    np.testing.assert_allclose(w - w1, 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+    np.testing.assert_allclose(m, 0.1 * g) and np.testing.assert_allclose(v, 0.01 * g * g)
+
+
+def test_bf16_rounding(rng):
+    assert ref.bf16(1.0) == 1.0
+    assert ref.bf16(1.0 + 2 ** -9) == 1.0            # halfway -> even
+    assert ref.bf16(1.0 + 3 * 2 ** -9) == 1.0 + 2 ** -7
+    x = rng.standard_normal(1000).astype(np.float32)
+    assert np.all(np.abs(ref.bf16(x) - x) <= np.abs(x) * 2 ** -8)
+
+
+# ------------------------------------------------------------------ GPU
+
+def _dev(gpu, a):
+    return gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(a))
+
+
+@pytest.mark.gpu
+def test_gpu_encoding_matches_reference(krr, gpu, rng):
+    from kiraray_amd.nrc import NeuralRadianceCache
+    print("case: nrc encode 1000", flush=True)
+    nrc = NeuralRadianceCache({"seed": 3})
+    v = ref.random_vertices(1000, rng)
+    out = gpu.DeviceBuffer(1000 * 64 * 4)
+    nrc.encode(_dev(gpu, v).ptr, 1000, out.ptr)
+    gpu.synchronize()
+    e = out.to_numpy(np.float32, (1000, 64))
+    np.testing.assert_allclose(e, ref.encode(v), atol=2e-6)
+    nrc.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [1, 31, 1000, 4096 + 17])
+def test_gpu_inference_matches_reference(krr, gpu, rng, n):
+    from kiraray_amd.nrc import NeuralRadianceCache
+    print(f"case: nrc infer n={n}", flush=True)
+    nrc = NeuralRadianceCache({"seed": 5})
+    w = nrc.get_weights().astype(np.float64)
+    v = ref.random_vertices(n, rng)
+    out = gpu.DeviceBuffer(n * 12)
+    nrc.infer(_dev(gpu, v).ptr, n, out.ptr)
+    gpu.synchronize()
+    y = out.to_numpy(np.float32, (n, 3)).astype(np.float64)
+    x0 = ref.encode(v)
+    ye, _ = ref.forward(w, x0, emulate_bf16=True)
+    yx, _ = ref.forward(w, x0)
+    scale = np.abs(yx).mean() + 1e-3
+    print(f"max |gpu - bf16 emulated| {np.abs(y - ye).max():.3g}, vs fp64 {np.abs(y - yx).max():.3g}, scale {scale:.3g}")
+    assert np.abs(y - ye).max() <= 1e-3 * scale + 1e-5   # same bf16 inputs; fp32 vs fp64 accumulation
+    assert np.abs(y - yx).max() <= 5e-2 * scale           # bf16 against exact
+    nrc.close()
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_reference(krr, gpu, rng):
+    """Weight gradient and loss of the fused forward/backward + split-K
+    kernels against the CPU FP64 backprop, per layer."""
+    from kiraray_amd.nrc import NeuralRadianceCache
+    print("case: nrc gradient n=3000", flush=True)
+    n = 3000
+    nrc = NeuralRadianceCache({"seed": 7})
+    w = nrc.get_weights().astype(np.float64)
+    v = ref.random_vertices(n, rng)
+    t = (rng.random((n, 3)) * 1.5).astype(np.float32)
+    grad = gpu.DeviceBuffer(ref.PARAMS * 4)
+    loss = nrc.gradient(_dev(gpu, v).ptr, _dev(gpu, t).ptr, n, grad.ptr)
+    gpu.synchronize()
+    g = grad.to_numpy(np.float32, (ref.PARAMS,)).astype(np.float64)
+    x0 = ref.encode(v)
+    le, ge = ref.loss_and_grad(w, x0, t, emulate_bf16=True)
+    lx, gx = ref.loss_and_grad(w, x0, t)
+    assert abs(loss - le) <= 1e-3 * le and abs(loss - lx) <= 2e-2 * lx, (loss, le, lx)
+    for l in range(ref.HIDDEN + 1):
+        sl = slice(l * 4096, (l + 1) * 4096) if l < ref.HIDDEN else slice(ref.HIDDEN * 4096, None)
+        re = np.linalg.norm(g[sl] - ge[sl]) / np.linalg.norm(ge[sl])
+        rx = np.linalg.norm(g[sl] - gx[sl]) / np.linalg.norm(gx[sl])
+        print(f"layer {l}: rel err vs bf16-emulated {re:.2e}, vs fp64 {rx:.2e}")
+        assert re < 1e-2 and rx < 5e-2
+    nrc.close()
+
+
+@pytest.mark.gpu
+def test_gpu_adam_step(krr, gpu, rng):
+    """One training step = the GPU gradient followed by the Adam update of
+    the reference, on the FP32 master weights."""
+    from kiraray_amd.nrc import NeuralRadianceCache
+    print("case: nrc adam step", flush=True)
+    n = 2048
+    nrc = NeuralRadianceCache({"seed": 9, "lr": 1e-3})
+    w0 = nrc.get_weights().astype(np.float64)
+    v = _dev(gpu, ref.random_vertices(n, rng))
+    t = _dev(gpu, (rng.random((n, 3))).astype(np.float32))
+    grad = gpu.DeviceBuffer(ref.PARAMS * 4)
+    nrc.gradient(v.ptr, t.ptr, n, grad.ptr)
+    gpu.synchronize()
+    g = grad.to_numpy(np.float32, (ref.PARAMS,)).astype(np.float64)
+    nrc.train_step(v.ptr, t.ptr, n)
+    w1 = nrc.get_weights().astype(np.float64)
+    we, _, _ = ref.adam_step(w0, np.zeros_like(w0), np.zeros_like(w0), g, 1)
+    np.testing.assert_allclose(w1, we, atol=1e-6)
+    assert nrc.steps == 1
+    nrc.close()
+
+
+@pytest.mark.gpu
+def test_gpu_training_fits_a_radiance_function(krr, gpu, rng):
+    """Online training on a smooth synthetic radiance field reduces the
+    relative L2 loss by more than 10x within 300 steps."""
+    from kiraray_amd.nrc import NeuralRadianceCache
+    print("case: nrc fit 300 steps", flush=True)
+    nrc = NeuralRadianceCache({"seed": 11, "lr": 3e-3})
+    n = 16384
+
+    def field(v):
+        p = v[:, :3].astype(np.float64)
+        base = 0.5 + 0.4 * np.sin(2 * np.pi * p[:, 0]) * np.cos(np.pi * p[:, 1]) + 0.3 * p[:, 2]
+        return np.stack([base * v[:, 8], base * v[:, 9], base * v[:, 10]], 1).astype(np.float32) + 0.05
+
+    losses = []
+    for step in range(300):
+        v = ref.random_vertices(n, rng)
+        l = nrc.train_step(_dev(gpu, v).ptr, _dev(gpu, field(v)).ptr, n)
+        losses.append(l)
+    print(f"loss {losses[0]:.4f} -> {np.mean(losses[-10:]):.5f}")
+    assert np.mean(losses[-10:]) < losses[0] / 10
+    # held-out check through inference
+    v = ref.random_vertices(4096, rng)
+    out = gpu.DeviceBuffer(4096 * 12)
+    nrc.infer(_dev(gpu, v).ptr, 4096, out.ptr)
+    gpu.synchronize()
+    y = out.to_numpy(np.float32, (4096, 3))
+    t = field(v)
+    rel = np.mean((y - t) ** 2 / (t * t + 0.01))
+    print(f"held-out relative L2 {rel:.5f}")
+    assert rel < 0.05
+    nrc.close()
