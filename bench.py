@@ -35,7 +35,8 @@ CONFIGS = {
     "cornell_box_256": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
                         "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
                  params={"spp": 16, "max_depth": 8, "rr": 0.8},
-                 desc="cornell_box_36tri_256x256_16spp_depth8"),
+                 desc="cornell_box_36tri_256x256_16spp_depth8",
+                 reference="tests/golden/cornell_256_fp64_65536spp.npz"),
     "volume_720p": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
                         "media": [{"type": "value_noise_grid", "res": 256, "octaves": 5, "seed": 11,
                                    "sigma_t_max": 20.0, "albedo": 0.8, "g": 0.3}],
@@ -124,6 +125,19 @@ def gpu_bench(cfg, steps, warmup, traffic):
         "counting pass traced different rays"
     img = film.to_numpy(np.float32, (h, w, 4))
     assert np.isfinite(img).all()
+    relmse = None
+    if cfg.get("reference"):
+        # relMSE of the last timed frame against the FP64 reference image
+        # (render/passes/errormeasure/metrics.cu:80-90), on the device
+        z = np.load(os.path.join(ROOT, cfg["reference"]))
+        ref = np.concatenate([z["image"], np.ones(z["image"].shape[:2] + (1,), np.float32)], -1)
+        refb = hip.DeviceBuffer.from_numpy(np.ascontiguousarray(ref, np.float32))
+        wf.begin_frame(cam, steps)
+        wf.render(film.ptr, stream.handle)
+        stream.synchronize()
+        relmse = {"value": round(krr.error_metric(3, film.ptr, refb.ptr, w * h), 6),
+                  "reference": f"{int(z['spp'])}-spp FP64 CPU oracle image ({cfg['reference']})",
+                  "image": f"frame {steps}, {cfg['params']['spp']} spp"}
     wfc.close()
     wf.close()
 
@@ -154,6 +168,7 @@ def gpu_bench(cfg, steps, warmup, traffic):
                                "gpu_events_total": round(gpu_ms / steps, 3)},
         "kernels": kernels,
         "roofline": roofline,
+        "relmse": relmse,
         "_scene": scene,
     }
     return out
@@ -288,6 +303,7 @@ def main():
             "mpaths_per_s": r["mpaths_per_s"], "closest_mrays_per_s": r["closest_mrays_per_s"],
             "rays_per_frame": r["rays_per_frame"], "kernel_ms_per_step": r["kernel_ms_per_step"],
             "kernels": r["kernels"],
+            "relmse": r["relmse"],
             "roofline": r["roofline"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, scene, budget_s=args.cpu_budget)

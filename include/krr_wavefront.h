@@ -283,6 +283,20 @@ enum krr_metric { KRR_METRIC_MSE = 0, KRR_METRIC_MAPE = 1, KRR_METRIC_SMAPE = 2,
 int krr_error_metric(int32_t metric, const float *img_dev, const float *ref_dev, int64_t n_pixels,
 					 double *out, void *hip_stream);
 
+/* ---------------------------------------------------------------------- */
+/* Frame accumulation (AccumulatePass, src/render/passes/accumulate/       */
+/* accumulate.cu:30-79)                                                     */
+/* ---------------------------------------------------------------------- */
+
+enum krr_accum_mode { KRR_ACCUM_SUM = 0, KRR_ACCUM_MOVING_AVERAGE = 1 };
+
+/* Folds frame_dev (n_pixels float4) into accum_dev (n_pixels float4 when
+ * precision is 32, double4 when 64), accum_count frames having been folded
+ * before; rewrites frame_dev with the accumulated mean.  max_count > 0 stops
+ * adding in sum mode after that many frames.  Queued on hip_stream. */
+int krr_accumulate(void *accum_dev, float *frame_dev, int64_t n_pixels, int64_t accum_count, int64_t max_count, int32_t mode,
+				   int32_t precision, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,7 +20,7 @@ from . import _abi
 from ._abi import (krr_camera, krr_hit, krr_material, krr_medium, krr_mesh, krr_scene_desc, krr_stats,
                    load_library)
 
-__all__ = ["KrrError", "Scene", "Accel", "WavefrontPathTracer", "RenderApp", "load_library", "error_metric",
+__all__ = ["KrrError", "Scene", "Accel", "WavefrontPathTracer", "RenderApp", "load_library", "error_metric", "accumulate",
            "krr_camera", "krr_hit", "krr_material", "krr_medium", "krr_mesh", "krr_scene_desc", "krr_stats"]
 
 
@@ -227,6 +227,13 @@ class RenderApp:
         from . import hip
         hip.synchronize()
         return self.film.to_numpy(np.float32, (self.height, self.width, 4))
+
+
+def accumulate(accum_ptr, frame_ptr, n_pixels, accum_count, max_count=0, mode=0, precision=32, stream=0):
+    """AccumulatePass: fold a float4 frame into the accumulation buffer and
+    rewrite the frame with the mean (mode 0 sum, 1 moving average)."""
+    _check(load_library().krr_accumulate(C.c_void_p(accum_ptr), C.c_void_p(frame_ptr), int(n_pixels), int(accum_count),
+                                         int(max_count), int(mode), int(precision), C.c_void_p(stream)))
 
 
 def error_metric(metric, img_ptr, ref_ptr, n_pixels, stream=0):

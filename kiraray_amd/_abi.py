@@ -104,6 +104,7 @@ PROTOTYPES = {
     "krr_wf_get_params": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "krr_wf_destroy": (C.c_int, [C.c_void_p]),
     "krr_error_metric": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_void_p]),
+    "krr_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p]),
     # neural radiance cache (include/krr_nrc.h)
     "krr_nrc_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "krr_nrc_destroy": (C.c_int, [C.c_void_p]),
