@@ -149,6 +149,11 @@ typedef struct krr_stats {
 	uint64_t megakernel_shadow_rays;  /* shadow rays traced inside it (part of shadow_rays) */
 	uint64_t megakernel_node_visits; /* BVH node visits of both ("count_traversal") */
 	uint64_t megakernel_tri_tests;	 /* their ray-triangle tests ("count_traversal") */
+	double nrc_ms;				/* time of the radiance-cache kernels ("time_kernels") */
+	uint64_t nrc_queries;		/* radiance-cache queries answered (paths ended by the cache) */
+	uint64_t nrc_train_vertices; /* training vertices recorded this frame (at most nrc.batch are used) */
+	uint64_t nrc_steps;			/* training steps taken so far */
+	double nrc_loss;			/* loss of the last training step (synchronises) */
 } krr_stats;
 
 /* Closest hit record: t along the (unnormalised) ray direction, primitive
@@ -238,6 +243,15 @@ typedef struct krr_wf krr_wf;
  *                            launch with the same per-path stage order, so
  *                            the image does not depend on this value
  *   leaf_size (4), sah_node_cost (0.25), stack_lds_entries (16)  as for krr_accel_build
+ *   nrc (false)              neural radiance cache (include/krr_nrc.h): true,
+ *                            or {"train_mod": 16, "batch": 65536, and the
+ *                            krr_nrc_create keys}.  Render paths end at the
+ *                            first vertex reached through a non-specular
+ *                            bounce and take the cache's radiance there;
+ *                            pixels with trainHash(pixel) % train_mod ==
+ *                            frame % train_mod trace unbiased training paths
+ *                            whose vertices train the cache once per frame.
+ *                            Needs wavefront_depths = 0 and no media.
  * NULL or "" means all defaults. */
 /* Replaces RenderPassFactory::deserialize<WavefrontPathTracer> + from_json
  * (src/core/renderpass.h:202-273, src/render/wavefront/integrator.h:96-103)

@@ -70,7 +70,9 @@ class krr_stats(C.Structure):
                 ("render_ms", C.c_double), ("megakernel_ms", C.c_double),
                 ("megakernel_closest_rays", C.c_uint64), ("megakernel_shadow_rays", C.c_uint64),
                 ("megakernel_node_visits", C.c_uint64),
-                ("megakernel_tri_tests", C.c_uint64)]
+                ("megakernel_tri_tests", C.c_uint64),
+                ("nrc_ms", C.c_double), ("nrc_queries", C.c_uint64), ("nrc_train_vertices", C.c_uint64),
+                ("nrc_steps", C.c_uint64), ("nrc_loss", C.c_double)]
 
 
 class krr_hit(C.Structure):

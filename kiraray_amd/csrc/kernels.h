@@ -56,8 +56,33 @@ struct Counters {
 	unsigned long long megaShadow;	   // shadow rays traced in the tail megakernel
 	unsigned long long megaNodeVisits; // closest + shadow traversal in the megakernel
 	unsigned long long megaTriTests;
-	int work[4]; // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
+	int work[4];	  // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
+	int nrcQueries;	  // radiance-cache queries of the current sample
+	int nrcTrain;	  // training vertices pushed this frame (may exceed the batch)
+	int nrcTrainDone; // training vertices whose targets are written
+	int nrcPad;
+	unsigned long long nrcQueriesTotal; // cache queries answered this frame
 };
+
+// Neural radiance cache queues of the path tracer.  A query is 5 float4:
+// the 64-byte krr_nrc_vertex and {thp.xyz, pixel}; a training vertex is 6
+// float4: the vertex, {thp.xyz, pixel}, {L.xyz so far, 0}.
+struct NrcDev {
+	float4 *query;
+	float4 *train;
+	int *queryCount, *trainCount; // Counters::nrcQueries, Counters::nrcTrain
+	int capQuery, capTrain;
+	int trainMod;	// pixel trains when trainHash(pixel) % trainMod == phase
+	int phase;		// frame index % trainMod
+};
+
+KRR_DEV uint32_t trainHash(uint32_t pixel) {
+	uint32_t h = pixel * 0x9E3779B1u;
+	h ^= h >> 15;
+	h *= 0x85EBCA77u;
+	h ^= h >> 13;
+	return h;
+}
 
 struct CameraDev {
 	float filmH, focalLength, focalDistance, lensRadius, aspect;

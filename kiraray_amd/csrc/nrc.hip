@@ -192,6 +192,57 @@ __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const floa
 	}
 }
 
+// Inference on a query queue whose length lives on the device; adds
+// thp * max(y, 0) to L[pixel].
+__global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, const float4 *queries, int stride, const int *countPtr, int capacity,
+													 float4 *L) {
+	extern __shared__ uint16_t ldsW[];
+	loadWeights(ldsW, wtAll, kFwdElems);
+	__syncthreads();
+	const int n	   = min(max(*countPtr, 0), capacity);
+	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
+	const int wavesPerBlock = blockDim.x >> 6;
+	const int tiles			= (n + 31) / 32;
+	for (int t = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); t < tiles; t += gridDim.x * wavesPerBlock) {
+		const int i = t * 32 + m;
+		float v[16];
+		float4 pay = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (i < n) {
+			const float4 *p = queries + (int64_t) i * stride;
+#pragma unroll
+			for (int q = 0; q < 4; q++) {
+				const float4 f = p[q];
+				v[4 * q]	   = f.x;
+				v[4 * q + 1]   = f.y;
+				v[4 * q + 2]   = f.z;
+				v[4 * q + 3]   = f.w;
+			}
+			pay = p[4];
+		} else {
+#pragma unroll
+			for (int q = 0; q < 16; q++) v[q] = 0.f;
+		}
+		short4v x[8], y[8];
+		encodeB(v, g, x);
+		float16v a0, a1;
+#pragma unroll
+		for (int l = 0; l < kHidden; l++) {
+			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
+#pragma unroll
+			for (int s = 0; s < 8; s++) x[s] = y[s];
+		}
+		const float16v o = outputForward(ldsW + kHidden * 4096, lane, g, x);
+		if (g == 0 && i < n) {
+			const int pixel = __float_as_int(pay.w);
+			float4 acc		= L[pixel];
+			acc.x += pay.x * fmaxf(o[0], 0.f);
+			acc.y += pay.y * fmaxf(o[1], 0.f);
+			acc.z += pay.z * fmaxf(o[2], 0.f);
+			L[pixel] = acc;
+		}
+	}
+}
+
 // Writes the B-layout bf16 operand (features 8s + 4g + j of sample m) to a
 // feature-major [feature][ld] array at column `col`.
 __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int64_t col, int g, const short4v x[8], int features = 64) {
@@ -208,8 +259,10 @@ __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int
 // dZ_l^T (bf16, feature-major, leading dimension ld = padded batch) and the
 // tile's loss sum.  Samples i >= n contribute nothing.
 __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const uint16_t *wAll, const float *vertices, const float *targets,
-													int64_t n, int64_t nTiles, int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT,
-													float *tileLoss) {
+													int64_t nHost, const int *nDev, int64_t nTiles, int64_t ld, float lossEps, uint16_t *xT,
+													uint16_t *dzT, float *tileLoss) {
+	// sample count: host value, or the device counter clamped to it
+	const int64_t n = nDev ? min((int64_t) max(*nDev, 0), nHost) : nHost;
 	// forward A operands from LDS; backward ones straight from global memory
 	// (45 KB, cache resident), which keeps the block within 64 KB of LDS
 	extern __shared__ uint16_t lds[];
@@ -219,7 +272,7 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 	__syncthreads();
 	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
 	const int wavesPerBlock = blockDim.x >> 6;
-	const float invCount	= 1.0f / (3.0f * (float) n);
+	const float invCount	= n > 0 ? 1.0f / (3.0f * (float) n) : 0.f;
 	for (int64_t t = (int64_t) blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); t < nTiles; t += (int64_t) gridDim.x * wavesPerBlock) {
 		const int64_t i	  = t * 32 + m;
 		const bool valid = i < n;
@@ -477,13 +530,13 @@ std::vector<float> initialWeights(uint64_t seed) {
 
 bool validN(int64_t n) { return n >= 0 && n <= ((int64_t) 1 << 31); }
 
-int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st) {
+int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st, const int *nDev = nullptr) {
 	const int64_t chunks = (n + kChunk - 1) / kChunk;
 	const int64_t nTiles = chunks * (kChunk / 32);
 	const int lds		 = kFwdElems * (int) sizeof(uint16_t);
 	const int blocks	 = (int) std::min<int64_t>(1024, (nTiles + 3) / 4);
 	hipLaunchKernelGGL(k_train_tile, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), (const float *) vert,
-					   targets, n, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(), h->tileLoss.as<float>());
+					   targets, n, nDev, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(), h->tileLoss.as<float>());
 	NRC_HIP(hipGetLastError());
 	for (int l = 0; l <= kHidden; l++) {
 		const int nOut = l < kHidden ? 64 : 3, nOutTiles = l < kHidden ? 2 : 1;
@@ -612,6 +665,47 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 	return KRR_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+int adamStep(krr_nrc *nrc, hipStream_t st) {
+	nrc->steps++;
+	const float c1 = 1.f - std::pow(nrc->beta1, (float) nrc->steps), c2 = 1.f - std::pow(nrc->beta2, (float) nrc->steps);
+	hipLaunchKernelGGL(k_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, nrc->w.as<float>(), nrc->m.as<float>(), nrc->v.as<float>(),
+					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, c1, c2, nrc->wt.as<uint16_t>(),
+					   nrc->wb.as<uint16_t>());
+	NRC_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+} // namespace
+
+int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const int *count, int capacity, float4 *L, hipStream_t st) {
+	if (capacity <= 0) return KRR_OK;
+	const int blocks = std::max(1, std::min(1024, (capacity + 127) / 128));
+	hipLaunchKernelGGL(k_infer_queue, dim3(blocks), dim3(256), kFwdElems * (int) sizeof(uint16_t), st, nrc->wt.as<uint16_t>(), queries, strideF4,
+					   count, capacity, L);
+	NRC_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
+int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, const int *count, int capacity, hipStream_t st) {
+	const int64_t cap = std::min<int64_t>(capacity, nrc->maxBatch);
+	if (cap <= 0) return KRR_OK;
+	if (int rc = gradientImpl(nrc, (const krr_nrc_vertex *) vertices, targets, cap, st, count)) return rc;
+	return adamStep(nrc, st);
+}
+
+float nrcLastLoss(krr_nrc *nrc, hipStream_t st) {
+	float l = 0.f;
+	if (hipMemcpyAsync(&l, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st) != hipSuccess) return -1.f;
+	if (hipStreamSynchronize(st) != hipSuccess) return -1.f;
+	return l;
+}
+
+extern "C" {
+
 int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const float *targets_dev, int64_t n, float *loss_out,
 					   void *hip_stream) {
 	krr::clearError();
@@ -619,12 +713,7 @@ int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const f
 	if (n > nrc->maxBatch) return krr::fail(KRR_ERR_CAPACITY, "krr_nrc_train_step: batch %lld exceeds max_batch %lld", (long long) n, (long long) nrc->maxBatch);
 	hipStream_t st = (hipStream_t) hip_stream;
 	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st)) return rc;
-	nrc->steps++;
-	const float c1 = 1.f - std::pow(nrc->beta1, (float) nrc->steps), c2 = 1.f - std::pow(nrc->beta2, (float) nrc->steps);
-	hipLaunchKernelGGL(k_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, nrc->w.as<float>(), nrc->m.as<float>(), nrc->v.as<float>(),
-					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, c1, c2, nrc->wt.as<uint16_t>(),
-					   nrc->wb.as<uint16_t>());
-	NRC_HIP(hipGetLastError());
+	if (int rc = adamStep(nrc, st)) return rc;
 	if (loss_out) {
 		NRC_HIP(hipMemcpyAsync(loss_out, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
 		NRC_HIP(hipStreamSynchronize(st));

@@ -80,6 +80,7 @@ struct DevScene {
 	const DevMedium *media;
 	int nMedia;
 	float sceneRadius;
+	float bmin[3], binv[3]; // scene box: p -> (p - bmin) * binv maps it to [0,1]^3
 };
 
 KRR_DEV f3 loadV(const float *a, int i) { return {a[3 * i], a[3 * i + 1], a[3 * i + 2]}; }

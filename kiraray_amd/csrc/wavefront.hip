@@ -27,6 +27,7 @@
 #include "errors.h"
 #include "json.h"
 #include "kernels.h"
+#include "nrc_dev.h"
 #include "krr_wavefront.h"
 
 namespace krr {
@@ -52,6 +53,10 @@ __global__ void k_reset_totals(Counters *cnt) {
 	cnt->megaShadow		  = 0;
 	cnt->megaNodeVisits	  = 0;
 	cnt->megaTriTests	  = 0;
+	cnt->nrcQueries		  = 0;
+	cnt->nrcTrain		  = 0;
+	cnt->nrcTrainDone	  = 0;
+	cnt->nrcQueriesTotal  = 0;
 	cnt->rayCount[0] = cnt->rayCount[1] = cnt->shadowCount = 0;
 }
 
@@ -270,9 +275,43 @@ struct ShadowOut {
 
 // One ray of the queue through the stages above; new work is returned in
 // (wantNext, nx) and (wantShadow, so), radiance and RNG state go to memory.
-template <bool MEDIA>
+// Spherical angles of a unit vector, mapped to [0,1]: theta / pi, phi / (2 pi) + 1/2.
+KRR_DEV void unitAngles(const f3 &w, float &th, float &ph) {
+	const float z = fminf(1.f, fmaxf(-1.f, w.z));
+	th			  = detmath::atan2f(sqrtf(w.x * w.x + w.y * w.y), z) * kInvPi;
+	ph			  = detmath::atan2f(w.y, w.x) * (0.5f * kInvPi) + 0.5f;
+}
+
+// The krr_nrc_vertex of a surface vertex: position in the scene box,
+// outgoing direction, shading normal, roughness, diffuse albedo (diffuse
+// materials) and specular albedo (the base colour of conductors and
+// dielectrics).
+KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float4 *out) {
+	const DevMaterial mat = sc.materials[intr.material];
+	const float px		  = fminf(1.f, fmaxf(0.f, (intr.p.x - sc.bmin[0]) * sc.binv[0]));
+	const float py		  = fminf(1.f, fmaxf(0.f, (intr.p.y - sc.bmin[1]) * sc.binv[1]));
+	const float pz		  = fminf(1.f, fmaxf(0.f, (intr.p.z - sc.bmin[2]) * sc.binv[2]));
+	float dt, dp, nt, np;
+	unitAngles(intr.wo, dt, dp);
+	unitAngles(intr.n, nt, np);
+	const bool diffuse = mat.type == KRR_MAT_DIFFUSE;
+	const float d0 = diffuse ? mat.diffuse[0] : 0.f, d1 = diffuse ? mat.diffuse[1] : 0.f, d2 = diffuse ? mat.diffuse[2] : 0.f;
+	const float s0 = diffuse ? 0.f : mat.diffuse[0], s1 = diffuse ? 0.f : mat.diffuse[1], s2 = diffuse ? 0.f : mat.diffuse[2];
+	out[0] = make_float4(px, py, pz, dt);
+	out[1] = make_float4(dp, nt, np, mat.roughness);
+	out[2] = make_float4(d0, d1, d2, s0);
+	out[3] = make_float4(s1, s2, 0.f, 0.f);
+}
+
+// NRC (template flag) in the shading stage, before generateScatterRays:
+// a render path ends at the first surface vertex reached through a
+// non-specular bounce (depth >= 1) and queues a cache query there (the
+// vertex's own emission is already added); a training path (pixel chosen by
+// trainHash) records every scattering vertex with its throughput and the
+// radiance accumulated so far, and continues unbiased.
+template <bool MEDIA, bool NRC = false>
 KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, const PixelStateDev &px, int loopDepth,
-					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so) {
+					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so, const NrcDev *nrc = nullptr, bool trainPath = false) {
 	const int pixel = r.pixel;
 	f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
 	int &sMedium = so.medium;
@@ -395,6 +434,28 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			} else Lm = Lm + Li / pu.mean();
 		}
 		L = L + thp * Lm;
+	}
+
+	if (NRC) {
+		const bool live	  = hasScatter && loopDepth != P.maxDepth;
+		const bool wantQ  = live && !trainPath && r.depth >= 1 && !(r.bsdfType & BSDF_DELTA);
+		const bool wantT  = live && trainPath;
+		const int qi	  = wavePush(nrc->queryCount, wantQ);
+		const int ti	  = wavePush(nrc->trainCount, wantT);
+		if (wantQ) {
+			hasScatter = false; // the cache stands for the rest of the path
+			if (qi < nrc->capQuery) {
+				float4 *q = nrc->query + (size_t) qi * 5;
+				nrcVertex(sc, intr, q);
+				q[4] = make_float4(thp.x, thp.y, thp.z, __int_as_float(pixel));
+			}
+		}
+		if (wantT && ti < nrc->capTrain) {
+			float4 *t = nrc->train + (size_t) ti * 6;
+			nrcVertex(sc, intr, t);
+			t[4] = make_float4(thp.x, thp.y, thp.z, __int_as_float(pixel));
+			t[5] = make_float4(L.x, L.y, L.z, 0.f);
+		}
 	}
 
 	if (loopDepth != P.maxDepth) {
@@ -684,7 +745,8 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 }
 
 __global__ void k_mega_begin(Counters *cnt) {
-	cnt->work[2] = 0;
+	cnt->work[2]	= 0;
+	cnt->nrcQueries = 0;
 	cnt->closestLaunches += 1;
 }
 
@@ -698,9 +760,9 @@ __global__ void k_mega_begin(Counters *cnt) {
 // Each loop iteration advances every lane's path by one depth; a lane whose
 // path has ended takes the next queued path at once (one atomic per wave for
 // all such lanes), so no lane idles while the queue still holds work.
-template <bool MEDIA, bool COUNT>
+template <bool MEDIA, bool COUNT, bool NRC = false>
 __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
-							 int *ovf, Counters *cnt) {
+							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}) {
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	const int count		  = *countPtr;
@@ -727,7 +789,8 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 		RayItem nx;
 		ShadowOut so;
 		so.medium = -1;
-		shadeCore<MEDIA>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so);
+		const bool trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
+		shadeCore<MEDIA, NRC>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath);
 		if (wantShadow) {
 			nShadow++;
 			if (MEDIA && P.enableMedium)
@@ -750,6 +813,29 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 		waveAddU64(&cnt->megaNodeVisits, nv + snv);
 		waveAddU64(&cnt->megaTriTests, nt + snt);
 	}
+}
+
+// Training targets of the vertices recorded in this sample: the radiance the
+// path gathered after the vertex, divided by the throughput at the vertex.
+__global__ void k_nrc_targets(NrcDev nrc, const int *donePtr, const float4 *L, float4 *batchV, float *batchT) {
+	const int count = min(*nrc.trainCount, nrc.capTrain), done = *donePtr;
+	for (int e = done + (int) (blockIdx.x * blockDim.x + threadIdx.x); e < count; e += (int) (gridDim.x * blockDim.x)) {
+		const float4 *t	 = nrc.train + (size_t) e * 6;
+		const float4 pay = t[4], li = t[5];
+		const float4 Lf	 = L[__float_as_int(pay.w)];
+		batchV[4 * e]		= t[0];
+		batchV[4 * e + 1]	= t[1];
+		batchV[4 * e + 2]	= t[2];
+		batchV[4 * e + 3]	= t[3];
+		batchT[3 * e]		= pay.x > 0.f ? fmaxf(0.f, (Lf.x - li.x) / pay.x) : 0.f;
+		batchT[3 * e + 1]	= pay.y > 0.f ? fmaxf(0.f, (Lf.y - li.y) / pay.y) : 0.f;
+		batchT[3 * e + 2]	= pay.z > 0.f ? fmaxf(0.f, (Lf.z - li.z) / pay.z) : 0.f;
+	}
+}
+
+__global__ void k_nrc_mark_done(Counters *cnt, int capTrain, int capQuery) {
+	cnt->nrcTrainDone = min(cnt->nrcTrain, capTrain);
+	cnt->nrcQueriesTotal += (unsigned long long) min(cnt->nrcQueries, capQuery);
 }
 
 // pixel += L (integrator.cpp:362-365).
@@ -952,6 +1038,13 @@ struct krr_wf {
 	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 0;
 	int stackLdsEntries = kShortStack;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
+	// neural radiance cache ("nrc" params)
+	bool nrcOn = false;
+	int nrcTrainMod = 16, nrcBatch = 65536;
+	std::string nrcConfig; // JSON for krr_nrc_create
+	krr_nrc *nrc = nullptr;
+	DevBuf nrcQuery, nrcTrainBuf, nrcBatchV, nrcBatchT;
+	int gMegaNrc = 1;
 	// scene
 	bool haveScene = false;
 	AccelData accel;
@@ -970,7 +1063,8 @@ struct krr_wf {
 	hipStream_t lastStream = nullptr;
 	krr_stats stats{};
 	// HIP-event timing of kernel launches ("time_kernels"): kind 0 closest
-	// trace, 1 shade, 2 shadow trace, 3 whole render, 4 tail megakernel.
+	// trace, 1 shade, 2 shadow trace, 3 whole render, 4 tail megakernel,
+	// 5 radiance cache (inference, targets, training).
 	struct Span {
 		int kind, begin, end;
 	};
@@ -994,6 +1088,7 @@ struct krr_wf {
 	}
 	~krr_wf() {
 		for (hipEvent_t e : events) (void) hipEventDestroy(e);
+		if (nrc) krr_nrc_destroy(nrc);
 	}
 };
 
@@ -1020,6 +1115,21 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
+		if (j.contains("nrc")) {
+			const Json &n = j.at("nrc");
+			if (n.isBool()) wf->nrcOn = n.asBool();
+			else if (n.isObject()) {
+				wf->nrcOn		= n.value("enable", true);
+				wf->nrcTrainMod = (int) n.value("train_mod", 16.0);
+				wf->nrcBatch	= (int) n.value("batch", 65536.0);
+				Json c			= Json::object();
+				for (const auto &kv : n.members())
+					if (kv.first != "enable" && kv.first != "train_mod" && kv.first != "batch") c.set(kv.first, kv.second);
+				c.set("max_batch", Json((double) wf->nrcBatch));
+				wf->nrcConfig = c.dump();
+			} else throw std::runtime_error("\"nrc\" must be a boolean or an object");
+			if (wf->nrcConfig.empty()) wf->nrcConfig = "{\"max_batch\": " + std::to_string(wf->nrcBatch) + "}";
+		}
 		wf->sahNodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
@@ -1032,6 +1142,8 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
 	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
+	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
+	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (!(wf->sahNodeCost >= 0.f && wf->sahNodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	return KRR_OK;
 }
@@ -1165,8 +1277,12 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		} else if (m.type != KRR_MEDIUM_HOMOGENEOUS) return krr::fail(KRR_ERR_UNSUPPORTED, "medium %d: unsupported type", i);
 	}
 	float sceneRadius = 1e5f;
+	float lo[3] = {0.f, 0.f, 0.f}, hi[3] = {1.f, 1.f, 1.f};
 	if (s->n_vertices > 0) {
-		float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+		for (int a = 0; a < 3; a++) {
+			lo[a] = INFINITY;
+			hi[a] = -INFINITY;
+		}
 		for (int64_t v = 0; v < s->n_vertices; v++)
 			for (int a = 0; a < 3; a++) {
 				lo[a] = std::min(lo[a], s->positions[3 * v + a]);
@@ -1198,6 +1314,10 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	ds.media		= wf->media.as<DevMedium>();
 	ds.nMedia		= s->n_media;
 	ds.sceneRadius	= sceneRadius;
+	for (int a = 0; a < 3; a++) {
+		ds.bmin[a] = lo[a];
+		ds.binv[a] = hi[a] > lo[a] ? 1.f / (hi[a] - lo[a]) : 1.f;
+	}
 	wf->sceneHasMedia = s->n_media > 0;
 	wf->haveScene	  = true;
 	return KRR_OK;
@@ -1218,6 +1338,12 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
 	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
 	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+	if (wf->nrcOn) {
+		KRR_HIP(wf->nrcQuery.alloc(N * 5 * sizeof(float4)));
+		KRR_HIP(wf->nrcTrainBuf.alloc((size_t) wf->nrcBatch * 6 * sizeof(float4)));
+		KRR_HIP(wf->nrcBatchV.alloc((size_t) wf->nrcBatch * 4 * sizeof(float4)));
+		KRR_HIP(wf->nrcBatchT.alloc((size_t) wf->nrcBatch * 3 * sizeof(float)));
+	}
 	return KRR_OK;
 }
 
@@ -1340,6 +1466,10 @@ int krr_wf_create(const char *json_params, krr_wf **out) {
 	auto wf = std::make_unique<krr_wf>();
 	int rc	= parseParams(wf.get(), json_params);
 	if (rc) return rc;
+	if (wf->nrcOn) {
+		rc = krr_nrc_create(wf->nrcConfig.c_str(), &wf->nrc);
+		if (rc) return rc;
+	}
 	*out = wf.release();
 	return KRR_OK;
 }
@@ -1364,6 +1494,12 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
+	if (wf->nrcOn) {
+		Json n = Json::object();
+		n.set("train_mod", Json((double) wf->nrcTrainMod));
+		n.set("batch", Json((double) wf->nrcBatch));
+		j.set("nrc", n);
+	} else j.set("nrc", Json(false));
 	std::string s = j.dump();
 	if (buf && cap > 0) {
 		size_t n = std::min((size_t) cap - 1, s.size());
@@ -1457,7 +1593,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
-							  (count ? 2 : 0) ^ (media ? 1 : 0);
+							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0);
 	if (shapeKey != wf->gridKey) {
 		// Counting variants use more registers: size each grid for the variant launched.
 		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
@@ -1466,7 +1602,9 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		wf->gShade = media ? persistentGrid(k_shade<true>, bs, 0, (int) gN) : persistentGrid(k_shade<false>, bs, 0, (int) gN);
 		if (media) wf->gMega = count ? persistentGrid(k_megakernel<true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<true, false>, tb, lds, gTmax);
 		else wf->gMega = count ? persistentGrid(k_megakernel<false, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false>, tb, lds, gTmax);
-		const int gMax = std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega);
+		if (wf->nrcOn)
+			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false, true>, tb, lds, gTmax);
+		const int gMax = std::max(std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega), wf->nrcOn ? wf->gMegaNrc : 0);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) gMax * tb)) return rc;
 		wf->gridKey = shapeKey;
 	}
@@ -1486,10 +1624,24 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		launch();
 		wf->spans.push_back({kind, b, wf->mark(st)});
 	};
+	int nrcRc = KRR_OK;
+	if (wf->nrcOn && (media || wf->wavefrontDepths != 0))
+		return krr::fail(KRR_ERR_UNSUPPORTED, "nrc: needs a scene without media and wavefront_depths = 0");
 	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
 	if (wf->beginPending) {
 		hipLaunchKernelGGL(k_begin_frame, dim3(gN), dim3(bs), 0, st, P, px);
 		wf->beginPending = false;
+	}
+	NrcDev nrc{};
+	if (wf->nrcOn) {
+		nrc.query	   = wf->nrcQuery.as<float4>();
+		nrc.train	   = wf->nrcTrainBuf.as<float4>();
+		nrc.queryCount = &cnt->nrcQueries;
+		nrc.trainCount = &cnt->nrcTrain;
+		nrc.capQuery   = N;
+		nrc.capTrain   = wf->nrcBatch;
+		nrc.trainMod   = wf->nrcTrainMod;
+		nrc.phase	   = (int) (wf->frameIndex % (uint64_t) wf->nrcTrainMod);
 	}
 	for (int s = 0; s < wf->spp; s++) {
 		hipLaunchKernelGGL(k_camera, dim3(gN), dim3(bs), 0, st, P, cam, px, rayQueue(wf, 0), cnt);
@@ -1501,7 +1653,11 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 				timed(4, [&] {
 					const unsigned g = (unsigned) wf->gMega;
 					const int *cp	 = (const int *) &cnt->rayCount[cur];
-					if (media) {
+					if (wf->nrcOn) {
+						const unsigned gn = (unsigned) wf->gMegaNrc;
+						if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+						else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+					} else if (media) {
 						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
 						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
 					} else {
@@ -1540,7 +1696,20 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 				});
 			}
 		}
+		if (wf->nrcOn) { // training targets of this sample, then the cache answers the queries
+			timed(5, [&] {
+				hipLaunchKernelGGL(k_nrc_targets, dim3(256), dim3(256), 0, st, nrc, (const int *) &cnt->nrcTrainDone, (const float4 *) px.L,
+								   wf->nrcBatchV.as<float4>(), wf->nrcBatchT.as<float>());
+				hipLaunchKernelGGL(k_nrc_mark_done, dim3(1), dim3(1), 0, st, cnt, wf->nrcBatch, N);
+				nrcRc = nrcInferAccumulate(wf->nrc, nrc.query, 5, &cnt->nrcQueries, N, px.L, st);
+			});
+			if (nrcRc) return nrcRc;
+		}
 		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
+	}
+	if (wf->nrcOn) { // one training step per frame on the recorded vertices
+		timed(5, [&] { nrcRc = nrcTrainDevice(wf->nrc, wf->nrcBatchV.as<float>(), wf->nrcBatchT.as<float>(), &cnt->nrcTrainDone, wf->nrcBatch, st); });
+		if (nrcRc) return nrcRc;
 	}
 	hipLaunchKernelGGL(k_film, dim3(gN), dim3(bs), 0, st, P, px, (float4 *) film_dev);
 	if (timing) wf->spans.push_back({3, renderBegin, wf->mark(st)});
@@ -1575,7 +1744,7 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 	out->megakernel_node_visits = c.megaNodeVisits;
 	out->megakernel_tri_tests	 = c.megaTriTests;
 	if (wf->timeKernels) {
-		double ms[5] = {0, 0, 0, 0, 0};
+		double ms[6] = {0, 0, 0, 0, 0, 0};
 		for (const auto &sp : wf->spans) {
 			float t = 0.f;
 			hipEvent_t a = wf->event(sp.begin), b = wf->event(sp.end);
@@ -1589,6 +1758,13 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 		out->shadow_trace_ms  = ms[2];
 		out->render_ms		  = ms[3];
 		out->megakernel_ms	  = ms[4];
+		out->nrc_ms			  = ms[5];
+	}
+	if (wf->nrcOn) {
+		out->nrc_queries		= c.nrcQueriesTotal;
+		out->nrc_train_vertices = (uint64_t) c.nrcTrain;
+		out->nrc_loss			= nrcLastLoss(wf->nrc, wf->lastStream);
+		out->nrc_steps			= (uint64_t) krr_nrc_steps(wf->nrc);
 	}
 	return KRR_OK;
 }

@@ -1,0 +1,79 @@
+"""The path tracer with the neural radiance cache ("nrc" pass params).
+
+The cache has no reference counterpart, so its render mode is checked
+against closed forms and against the plain tracer: the emissive furnace
+(radiance Le * sum_k a^k everywhere, test_oracle_integrator.py) must come
+out right once the cache has trained, and cache-terminated paths must trace
+fewer rays than full ones."""
+import numpy as np
+import pytest
+
+from conftest import CORNELL
+from test_oracle_integrator import _furnace_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _frames(krr, gpu, scene, w, h, params, frames, first=1):
+    wf = krr.WavefrontPathTracer(params)
+    wf.set_scene(scene)
+    wf.resize(w, h)
+    film = gpu.DeviceBuffer(w * h * 16)
+    acc = gpu.DeviceBuffer(w * h * 32)
+    stats = []
+    for f in range(frames):
+        wf.begin_frame(scene.camera(0), first + f)
+        wf.render(film.ptr, 0)
+        wf.end_frame()
+        stats.append(wf.stats())
+        krr.accumulate(acc.ptr, film.ptr, w * h, f, 0, 0, 64)
+    gpu.synchronize()
+    img = film.to_numpy(np.float32, (h, w, 4))
+    wf.close()
+    return img, stats
+
+
+def test_nrc_pass_runs_and_trains(krr, gpu):
+    print("case: nrc cornell 64x64, 20 frames", flush=True)
+    s = krr.Scene(CORNELL)
+    img, st = _frames(krr, gpu, s, 64, 64, {"spp": 1, "max_depth": 8, "nrc": {"train_mod": 4, "lr": 3e-3}}, 20)
+    last = st[-1]
+    print({k: last[k] for k in ("closest_rays", "shadow_rays", "nrc_queries", "nrc_train_vertices", "nrc_steps", "nrc_loss")})
+    assert last["nrc_steps"] == 20
+    assert last["nrc_queries"] > 0.3 * 64 * 64 * 3 / 4  # most render paths end in the cache
+    assert last["nrc_train_vertices"] > 64 * 64 / 4
+    assert np.isfinite(last["nrc_loss"]) and last["nrc_loss"] < st[0]["nrc_loss"]
+    assert np.isfinite(img).all() and (img[..., :3] >= 0).all()
+    # full paths of the plain tracer trace more rays than cache-terminated ones
+    _, plain = _frames(krr, gpu, s, 64, 64, {"spp": 1, "max_depth": 8}, 1, first=20)
+    assert last["closest_rays"] < 0.8 * plain[0]["closest_rays"]
+
+
+def test_nrc_furnace_matches_closed_form(krr, gpu):
+    """Emissive furnace, albedo 0.5, Le 1, max depth 16: the radiance is
+    sum_{k<=16} 0.5^k ~ 2 everywhere; after 150 frames of online training the
+    cache-terminated image averaged over 20 more frames is within 4 %."""
+    print("case: nrc emissive furnace 48x48", flush=True)
+    s = _furnace_scene(krr, 0.5, 1.0)
+    params = {"spp": 1, "max_depth": 16, "nrc": {"train_mod": 2, "lr": 5e-3, "seed": 3}}
+    wf = krr.WavefrontPathTracer(params)
+    wf.set_scene(s)
+    wf.resize(48, 48)
+    film = gpu.DeviceBuffer(48 * 48 * 16)
+    acc = gpu.DeviceBuffer(48 * 48 * 32)
+    for f in range(150):
+        wf.begin_frame(s.camera(0), 1 + f)
+        wf.render(film.ptr, 0)
+    for f in range(20):
+        wf.begin_frame(s.camera(0), 151 + f)
+        wf.render(film.ptr, 0)
+        krr.accumulate(acc.ptr, film.ptr, 48 * 48, f, 0, 0, 64)
+    gpu.synchronize()
+    st = wf.stats()
+    img = film.to_numpy(np.float32, (48, 48, 4))
+    wf.close()
+    expected = sum(0.5 ** k for k in range(17))
+    mean = float(img[..., :3].mean())
+    print(f"nrc furnace mean {mean:.4f} expected {expected:.4f} loss {st['nrc_loss']:.4f} queries {st['nrc_queries']}")
+    assert st["nrc_queries"] > 0
+    assert abs(mean - expected) < 0.04 * expected
