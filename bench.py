@@ -37,6 +37,10 @@ CONFIGS = {
                  params={"spp": 16, "max_depth": 8, "rr": 0.8},
                  desc="cornell_box_36tri_256x256_16spp_depth8",
                  reference="tests/golden/cornell_256_fp64_65536spp.npz"),
+    "cornell_box_1024": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [1024, 1024],
+                                    "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
+                             params={"spp": 1, "max_depth": 8, "rr": 0.8},
+                             desc="cornell_box_36tri_1024x1024_1spp_depth8"),
     "volume_720p": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
                         "media": [{"type": "value_noise_grid", "res": 256, "octaves": 5, "seed": 11,
                                    "sigma_t_max": 20.0, "albedo": 0.8, "g": 0.3}],
@@ -224,6 +228,93 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     return out
 
 
+NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 1e-3}
+
+
+def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc_params=None):
+    """Radiance cache against the plain tracer on the same workload.
+
+    Reference: the plain tracer with ref_frames x ref_spp samples per pixel
+    (frame indices 100001.., independent of the compared frames).  Plain:
+    frames 1..frames.  Cache: warm_frames of online training, then frames
+    more (training continues every frame).  Each frame is timed on its own
+    (host wall clock around begin_frame + render + sync, training included
+    for the cache).  Reports the relMSE of single frames, the per-frame time,
+    and the relMSE of the running average after equal wall time."""
+    import numpy as np
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    w, h = cfg["scene"]["resolution"]
+    n = w * h
+    scene = krr.Scene(cfg["scene"])
+    cam = scene.camera(0)
+    film = hip.DeviceBuffer(n * 16)
+    acc = hip.DeviceBuffer(n * 32)
+    ref = hip.DeviceBuffer(n * 16)
+    stream = hip.Stream()
+
+    def run(params, first, count, accumulate_into=None, warm=0):
+        wf = krr.WavefrontPathTracer(params)
+        wf.set_scene(scene)
+        wf.resize(w, h)
+        for i in range(warm):
+            wf.begin_frame(cam, first - warm + i)
+            wf.render(film.ptr, stream.handle)
+        stream.synchronize()
+        times, single, running = [], [], []
+        for i in range(count):
+            t0 = time.perf_counter()
+            wf.begin_frame(cam, first + i)
+            wf.render(film.ptr, stream.handle)
+            stream.synchronize()
+            times.append(time.perf_counter() - t0)
+            if accumulate_into is None:
+                single.append(krr.error_metric(3, film.ptr, ref.ptr, n, stream.handle))
+                krr.accumulate(acc.ptr, film.ptr, n, i, 0, 0, 64, stream.handle)
+                stream.synchronize()
+                running.append(krr.error_metric(3, film.ptr, ref.ptr, n, stream.handle))
+            else:
+                krr.accumulate(accumulate_into.ptr, film.ptr, n, i, 0, 0, 64, stream.handle)
+        stream.synchronize()
+        st = wf.stats()
+        wf.close()
+        return times, single, running, st
+
+    base = dict(cfg["params"])
+    log(f"nrc: reference {ref_frames} x {ref_spp} spp")
+    run(dict(base, spp=ref_spp), 100001, ref_frames, accumulate_into=acc)
+    hip.check(hip.runtime().hipMemcpy(C_void(ref.ptr), C_void(film.ptr), n * 16, 3))
+    log("nrc: plain tracer")
+    tp, sp, rp, _ = run(base, 1, frames)
+    log("nrc: radiance cache")
+    nrc_params = dict(NRC_PARAMS, **(nrc_params or {}))
+    tn, sn, rn, stn = run(dict(base, nrc=nrc_params, time_kernels=True), warm_frames + 1, frames, warm=warm_frames)
+    ms_p, ms_n = 1e3 * float(np.median(tp)), 1e3 * float(np.median(tn))
+    # running average at equal wall time: the cache's curve at the time plain needs for all its frames
+    t_plain = float(np.sum(tp))
+    k = int(np.searchsorted(np.cumsum(tn), t_plain, side="right"))
+    rel_n_equal = rn[min(max(k, 1), len(rn)) - 1]
+    # time the cache needs to reach plain's single-frame error, per frame
+    return {"reference": f"plain tracer, {ref_frames * ref_spp} spp", "warm_frames": warm_frames, "frames": frames,
+            "nrc_params": nrc_params,
+            "plain": {"ms_per_frame": round(ms_p, 3), "relmse_single_frame": round(float(np.median(sp)), 5),
+                      "relmse_running": [round(float(x), 5) for x in rp]},
+            "nrc": {"ms_per_frame": round(ms_n, 3), "relmse_single_frame": round(float(np.median(sn)), 5),
+                    "relmse_running": [round(float(x), 5) for x in rn], "loss": round(stn["nrc_loss"], 5),
+                    "queries_per_frame": int(stn["nrc_queries"]), "train_vertices_per_frame": int(stn["nrc_train_vertices"]),
+                    "kernel_ms_per_frame": round(stn["nrc_ms"], 3)},
+            "equal_time": {"wall_s": round(t_plain, 4), "plain_relmse": round(float(rp[-1]), 5),
+                           "nrc_frames": min(max(k, 1), len(rn)), "nrc_relmse": round(float(rel_n_equal), 5)},
+            "speed_ratio_per_frame": round(ms_p / ms_n, 3),
+            "single_frame_relmse_ratio": round(float(np.median(sp)) / float(np.median(sn)), 3)}
+
+
+def C_void(p):
+    import ctypes
+    return ctypes.c_void_p(p)
+
+
 def cpu_baseline(cfg, scene, budget_s=12.0, max_frames=20):
     import oracle.binding as ob
     w, h = cfg["scene"]["resolution"]
@@ -276,6 +367,7 @@ def main():
     ap.add_argument("--config", default="icosphere_field_1080p", choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -309,6 +401,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline(cfg, scene, budget_s=args.cpu_budget)
     else:
         line["cpu_baseline"] = None
+    if args.config == "icosphere_field_1080p" and not args.no_nrc:
+        line["nrc_vs_plain"] = nrc_equal_time(cfg)
     print(json.dumps(line), flush=True)
 
 

@@ -244,8 +244,10 @@ typedef struct krr_wf krr_wf;
  *                            the image does not depend on this value
  *   leaf_size (4), sah_node_cost (0.25), stack_lds_entries (16)  as for krr_accel_build
  *   nrc (false)              neural radiance cache (include/krr_nrc.h): true,
- *                            or {"train_mod": 16, "batch": 65536, and the
- *                            krr_nrc_create keys}.  Render paths end at the
+ *                            or {"train_mod": 16, "batch": 65536,
+ *                            "steps_per_frame": 4, and the krr_nrc_create
+ *                            keys}; the batch is split into steps_per_frame
+ *                            consecutive mini-batches, one Adam step each.  Render paths end at the
  *                            first vertex reached through a non-specular
  *                            bounce and take the cache's radiance there;
  *                            pixels with trainHash(pixel) % train_mod ==

@@ -56,12 +56,13 @@ struct Counters {
 	unsigned long long megaShadow;	   // shadow rays traced in the tail megakernel
 	unsigned long long megaNodeVisits; // closest + shadow traversal in the megakernel
 	unsigned long long megaTriTests;
-	int work[4];	  // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
-	int nrcQueries;	  // radiance-cache queries of the current sample
-	int nrcTrain;	  // training vertices pushed this frame (may exceed the batch)
-	int nrcTrainDone; // training vertices whose targets are written
-	int nrcPad;
 	unsigned long long nrcQueriesTotal; // cache queries answered this frame
+	int nrcTrainDone;					// training vertices whose targets are written
+	// Counters that take an atomic per wave during a launch, one 128-byte
+	// line each, so that their atomics do not serialise against each other.
+	alignas(128) int work[4]; // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
+	alignas(128) int nrcQueries; // radiance-cache queries of the current sample
+	alignas(128) int nrcTrain;	 // training vertices pushed this frame (may exceed the batch)
 };
 
 // Neural radiance cache queues of the path tracer.  A query is 5 float4:

@@ -20,7 +20,8 @@
 // Encoding (64 features, lane half g = 0/1 computes slots q = 4s + j of
 // feature 8s + 4g + j):
 //   half 0: q = 10c + 2f + t: sin (t = 0) / cos (t = 1) of 2^f pi pos_c,
-//           c < 3, f < 5; q = 30, 31: 1.
+//           c < 3, f < 5 (hardware sine/cosine of pi pos_c, doubled f
+//           times); q = 30, 31: 1.
 //   half 1: q 0..2 pos; 3..6 / 7..10 one-blob(4) of dir theta / phi;
 //           11..14 / 15..18 of normal theta / phi; 19..22 of roughness;
 //           23..25 diffuse; 26..28 specular; 29..31: 0.
@@ -61,9 +62,11 @@ __device__ __forceinline__ short4v pack4(float a, float b, float c, float d) {
 	return v;
 }
 
+// The encoding uses the hardware exp2 / sine / cosine (a learned input needs
+// no bit-exact agreement with the CPU; the checker compares to 2e-5).
 __device__ __forceinline__ float oneBlob(float x, int i) {
 	const float d = 4.0f * (x - ((float) i + 0.5f) * 0.25f);
-	return detmath::expf(-0.5f * d * d);
+	return __builtin_amdgcn_exp2f(-0.72134752044448170368f * d * d); // exp(-d^2 / 2)
 }
 
 // Encoding slot q (0..31) of lane half g for vertex record v (16 floats).
@@ -71,8 +74,13 @@ __device__ float encodeSlot(const float *v, int g, int q) {
 	if (g == 0) {
 		if (q >= 30) return 1.0f;
 		const int c = q / 10, f = (q % 10) / 2, t = q & 1;
-		const float a = (float) (1 << f) * 3.14159265358979323846f * v[c];
-		return t ? detmath::cosf(a) : detmath::sinf(a);
+		float sn = __builtin_amdgcn_sinf(0.5f * v[c]), cs = __builtin_amdgcn_cosf(0.5f * v[c]);
+		for (int k = 0; k < f; k++) {
+			const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+			sn = s2;
+			cs = c2;
+		}
+		return t ? cs : sn;
 	}
 	if (q < 3) return v[q];						  // pos
 	if (q < 7) return oneBlob(v[3], q - 3);		  // dir theta
@@ -84,10 +92,32 @@ __device__ float encodeSlot(const float *v, int g, int q) {
 	return 0.0f;
 }
 
-// B operands of layer 0 for sample record v: b[s] = features 8s + 4g + j.
+// B operands of layer 0 for sample record v: b[s] = features 8s + 4g + j
+// (slots q = 4s + j of half g).
 __device__ void encodeB(const float *v, int g, short4v b[8]) {
+	float e[32];
+	if (g == 0) {
+		// sin / cos of pi x from the hardware (argument in revolutions: x / 2),
+		// higher frequencies by the double-angle recurrence
 #pragma unroll
-	for (int s = 0; s < 8; s++) b[s] = pack4(encodeSlot(v, g, 4 * s), encodeSlot(v, g, 4 * s + 1), encodeSlot(v, g, 4 * s + 2), encodeSlot(v, g, 4 * s + 3));
+		for (int c = 0; c < 3; c++) {
+			float sn = __builtin_amdgcn_sinf(0.5f * v[c]), cs = __builtin_amdgcn_cosf(0.5f * v[c]);
+#pragma unroll
+			for (int f = 0; f < 5; f++) {
+				e[10 * c + 2 * f]	  = sn;
+				e[10 * c + 2 * f + 1] = cs;
+				const float s2 = 2.0f * sn * cs, c2 = (cs - sn) * (cs + sn);
+				sn = s2;
+				cs = c2;
+			}
+		}
+		e[30] = e[31] = 1.0f;
+	} else {
+#pragma unroll
+		for (int q = 0; q < 32; q++) e[q] = encodeSlot(v, 1, q);
+	}
+#pragma unroll
+	for (int s = 0; s < 8; s++) b[s] = pack4(e[4 * s], e[4 * s + 1], e[4 * s + 2], e[4 * s + 3]);
 }
 
 __device__ __forceinline__ float16v zero16() {
@@ -177,7 +207,8 @@ __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const floa
 		short4v x[8], y[8];
 		encodeB(v, g, x);
 		float16v a0, a1;
-#pragma unroll
+		// not unrolled: an unrolled loop hoists every layer's weight loads
+#pragma unroll 1
 		for (int l = 0; l < kHidden; l++) {
 			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
 #pragma unroll
@@ -225,7 +256,8 @@ __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, cons
 		short4v x[8], y[8];
 		encodeB(v, g, x);
 		float16v a0, a1;
-#pragma unroll
+		// not unrolled: an unrolled loop hoists every layer's weight loads
+#pragma unroll 1
 		for (int l = 0; l < kHidden; l++) {
 			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
 #pragma unroll
@@ -259,10 +291,18 @@ __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int
 // dZ_l^T (bf16, feature-major, leading dimension ld = padded batch) and the
 // tile's loss sum.  Samples i >= n contribute nothing.
 __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const uint16_t *wAll, const float *vertices, const float *targets,
-													int64_t nHost, const int *nDev, int64_t nTiles, int64_t ld, float lossEps, uint16_t *xT,
-													uint16_t *dzT, float *tileLoss) {
-	// sample count: host value, or the device counter clamped to it
-	const int64_t n = nDev ? min((int64_t) max(*nDev, 0), nHost) : nHost;
+													int64_t nHost, const int *nDev, int64_t capTotal, int step, int steps, int64_t nTiles,
+													int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT, float *tileLoss) {
+	// samples: nHost, or with a device count, the step-th of `steps` equal
+	// slices of the first min(*nDev, capTotal) samples
+	int64_t n = nHost;
+	if (nDev) {
+		const int64_t cnt = min((int64_t) max(*nDev, 0), capTotal);
+		const int64_t off = (int64_t) step * cnt / steps;
+		n				  = min((int64_t) (step + 1) * cnt / steps - off, nHost);
+		vertices += off * 16;
+		targets += off * 3;
+	}
 	// forward A operands from LDS; backward ones straight from global memory
 	// (45 KB, cache resident), which keeps the block within 64 KB of LDS
 	extern __shared__ uint16_t lds[];
@@ -291,21 +331,36 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 #pragma unroll
 			for (int q = 0; q < 16; q++) v[q] = 0.f;
 		}
-		// forward, keeping every layer input
-		short4v x[kHidden + 1][8];
-		encodeB(v, g, x[0]);
+		// forward: each layer input goes to X^T as it is produced; the
+		// backward pass needs only its ReLU mask, one bit per held feature
+		// (bit 4s + j of mask[l] <=> feature 8s + 4g + j of x_l is > 0)
+		short4v x[8], y[8];
+		uint32_t mask[kHidden + 1];
+		encodeB(v, g, x);
 		float16v a0, a1;
+#pragma unroll 1
+		for (int l = 0; l < kHidden; l++) {
+			storeFeatureMajor(xT + (int64_t) l * 64 * ld, ld, i, g, x);
+			hiddenForward(ldsWt + l * 4096, lane, g, x, y, a0, a1);
+			uint32_t mk = 0;
 #pragma unroll
-		for (int l = 0; l < kHidden; l++) hiddenForward(ldsWt + l * 4096, lane, g, x[l], x[l + 1], a0, a1);
-		const float16v o = outputForward(ldsWt + kHidden * 4096, lane, g, x[kHidden]);
+			for (int s = 0; s < 8; s++) {
+				x[s] = y[s];
+#pragma unroll
+				for (int j = 0; j < 4; j++) mk |= (fromBf16((uint16_t) y[s][j]) > 0.f ? 1u : 0u) << (4 * s + j);
+			}
+			mask[l + 1] = mk;
+		}
+		storeFeatureMajor(xT + (int64_t) kHidden * 64 * ld, ld, i, g, x);
+		const float16v o = outputForward(ldsWt + kHidden * 4096, lane, g, x);
 		// relative L2 loss and its gradient (lanes g == 0 hold the 3 outputs)
 		float d[3]	 = {0.f, 0.f, 0.f};
 		float lsum	 = 0.f;
 		if (g == 0 && valid) {
 #pragma unroll
 			for (int c = 0; c < 3; c++) {
-				const float y = o[c], r = y - targets[3 * i + c];
-				const float den = y * y + lossEps;
+				const float yv = o[c], r = yv - targets[3 * i + c];
+				const float den = yv * yv + lossEps;
 				lsum += r * r / den;
 				d[c] = 2.0f * r / den * invCount;
 			}
@@ -318,9 +373,7 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 #pragma unroll
 		for (int s = 0; s < 8; s++) dz[s] = pack4(0.f, 0.f, 0.f, 0.f);
 		if (g == 0) dz[0] = pack4(d[0], d[1], d[2], 0.f);
-		// store X_5^T, dZ_5^T (32 padded output rows)
-		storeFeatureMajor(xT + (int64_t) kHidden * 64 * ld, ld, i, g, x[kHidden]);
-		storeFeatureMajor(dzT + (int64_t) kHidden * 64 * ld, ld, i, g, dz, 32);
+		storeFeatureMajor(dzT + (int64_t) kHidden * 64 * ld, ld, i, g, dz, 32); // 32 padded output rows
 		// backward through the output layer: dX_5^T = W_5 dZ_5^T (K = 32 padded outputs)
 		const int r0 = lane & 31;
 		{
@@ -333,19 +386,17 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 				a1 = mfma(ldsA(w5, 32 + r0, s, g, 32), dz[s], a1);
 			}
 		}
-#pragma unroll
+#pragma unroll 1
 		for (int l = kHidden - 1; l >= 0; l--) {
 			// dZ_l = dX_{l+1} * (x_{l+1} > 0), as the next B operand
+			const uint32_t mk = mask[l + 1];
 #pragma unroll
 			for (int s = 0; s < 4; s++)
 #pragma unroll
 				for (int j = 0; j < 4; j++) {
-					const float m0 = fromBf16((uint16_t) x[l + 1][s][j]) > 0.f ? a0[4 * s + j] : 0.f;
-					const float m1 = fromBf16((uint16_t) x[l + 1][4 + s][j]) > 0.f ? a1[4 * s + j] : 0.f;
-					dz[s][j]	   = (short) toBf16(m0);
-					dz[4 + s][j]   = (short) toBf16(m1);
+					dz[s][j]	 = (short) toBf16((mk >> (4 * s + j)) & 1u ? a0[4 * s + j] : 0.f);
+					dz[4 + s][j] = (short) toBf16((mk >> (16 + 4 * s + j)) & 1u ? a1[4 * s + j] : 0.f);
 				}
-			storeFeatureMajor(xT + (int64_t) l * 64 * ld, ld, i, g, x[l]);
 			storeFeatureMajor(dzT + (int64_t) l * 64 * ld, ld, i, g, dz);
 			if (l > 0) {
 				const uint16_t *w = ldsW + l * 4096;
@@ -361,17 +412,22 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 	}
 }
 
-// Split-K weight gradient of one layer: partial[chunk][k][nOut] =
-// sum over the chunk's samples of X^T[k][m] dZ^T[n][m].  One wave per 32x32
-// output tile; blockIdx.y = chunk.
-__global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xT, const uint16_t *dzT, int64_t ld, int chunk, int nOutTiles,
-													 int nOut, float *partial) {
+// Split-K weight gradients of all layers in one launch: blockIdx.z = layer,
+// blockIdx.y = chunk of `chunk` samples; partial[layer][chunk][k][nOut] =
+// sum over the chunk of X_l^T[k][m] dZ_l^T[n][m].  One wave per 32x32
+// output tile (4 for hidden layers, 2 for the 64x3 output layer).
+__global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xTAll, const uint16_t *dzTAll, int64_t ld, int chunk, int chunks,
+													 float *partialAll) {
+	const int layer = blockIdx.z;
+	const int nOut = layer < kHidden ? 64 : 3, nOutTiles = layer < kHidden ? 2 : 1;
 	const int lane = threadIdx.x & 63, g = lane >> 5, r0 = lane & 31;
-	const int tile = threadIdx.x >> 6; // 0..(2 * nOutTiles - 1)
+	const int tile = threadIdx.x >> 6;
 	if (tile >= 2 * nOutTiles) return;
 	const int kt = tile / nOutTiles, nt = tile % nOutTiles;
-	const int64_t m0 = (int64_t) blockIdx.y * chunk;
-	float16v acc	 = zero16();
+	const int64_t m0	 = (int64_t) blockIdx.y * chunk;
+	const uint16_t *xT	 = xTAll + (int64_t) layer * 64 * ld;
+	const uint16_t *dzT	 = dzTAll + (int64_t) layer * 64 * ld;
+	float16v acc		 = zero16();
 	const uint16_t *arow = xT + (int64_t) (32 * kt + r0) * ld + m0;
 	const uint16_t *brow = dzT + (int64_t) (32 * nt + r0) * ld + m0;
 	for (int s = 0; s < chunk / 8; s++) {
@@ -380,7 +436,7 @@ __global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xT, const u
 		acc				= mfma(a, b, acc);
 	}
 	// D[i = k][j = n]: lane holds column n = r0, rows k = 8(r/4) + 4g + r%4
-	float *dst = partial + (int64_t) blockIdx.y * 64 * nOut;
+	float *dst = partialAll + (int64_t) layer * chunks * 4096 + (int64_t) blockIdx.y * 64 * nOut;
 #pragma unroll
 	for (int r = 0; r < 16; r++) {
 		const int k = 32 * kt + 8 * (r / 4) + 4 * g + (r % 4);
@@ -389,12 +445,17 @@ __global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xT, const u
 	}
 }
 
-// grad[p] = sum over chunks (fixed order) of partial[chunk][p].
-__global__ void k_reduce_chunks(const float *partial, int chunks, int count, float *grad) {
+// grad[p] = sum over chunks (fixed order) of the layer's partials, for every
+// weight p (layer-major like the master weights).
+__global__ void k_reduce_chunks(const float *partialAll, int chunks, float *grad) {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= count) return;
-	float s = 0.f;
-	for (int c = 0; c < chunks; c++) s += partial[(int64_t) c * count + p];
+	if (p >= KRR_NRC_PARAMS) return;
+	const int layer = min(p / 4096, kHidden);
+	const int q		= p - layer * 4096;
+	const int count = layer < kHidden ? 4096 : 64 * 3;
+	const float *src = partialAll + (int64_t) layer * chunks * 4096 + q;
+	float s			 = 0.f;
+	for (int c = 0; c < chunks; c++) s += src[(int64_t) c * count];
 	grad[p] = s;
 }
 
@@ -477,7 +538,8 @@ struct Buf {
 		if (e_ != hipSuccess) return krr::fail(KRR_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));                  \
 	} while (0)
 
-constexpr int kChunk = 1024; // samples per split-K chunk of the weight gradient
+constexpr int kChunk	= 1024; // batch granularity of the training buffers
+constexpr int kWChunk = 256;	// samples per split-K chunk of the weight gradient
 
 } // namespace
 
@@ -530,25 +592,23 @@ std::vector<float> initialWeights(uint64_t seed) {
 
 bool validN(int64_t n) { return n >= 0 && n <= ((int64_t) 1 << 31); }
 
-int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st, const int *nDev = nullptr) {
+int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st, const int *nDev = nullptr,
+				 int64_t capTotal = 0, int step = 0, int steps = 1) {
 	const int64_t chunks = (n + kChunk - 1) / kChunk;
 	const int64_t nTiles = chunks * (kChunk / 32);
 	const int lds		 = kFwdElems * (int) sizeof(uint16_t);
 	const int blocks	 = (int) std::min<int64_t>(1024, (nTiles + 3) / 4);
 	hipLaunchKernelGGL(k_train_tile, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), (const float *) vert,
-					   targets, n, nDev, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(), h->tileLoss.as<float>());
+					   targets, n, nDev, capTotal, step, steps, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(),
+					   h->tileLoss.as<float>());
 	NRC_HIP(hipGetLastError());
-	for (int l = 0; l <= kHidden; l++) {
-		const int nOut = l < kHidden ? 64 : 3, nOutTiles = l < kHidden ? 2 : 1;
-		hipLaunchKernelGGL(k_weight_grad, dim3(1, (unsigned) chunks), dim3(64 * 2 * nOutTiles), 0, st,
-						   h->xT.as<uint16_t>() + (int64_t) l * 64 * h->ld, h->dzT.as<uint16_t>() + (int64_t) l * 64 * h->ld, h->ld, kChunk,
-						   nOutTiles, nOut, h->partial.as<float>());
-		NRC_HIP(hipGetLastError());
-		const int count = 64 * nOut;
-		hipLaunchKernelGGL(k_reduce_chunks, dim3((count + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), (int) chunks, count,
-						   h->grad.as<float>() + l * 4096);
-		NRC_HIP(hipGetLastError());
-	}
+	const int wchunks = (int) ((nTiles * 32 + kWChunk - 1) / kWChunk);
+	hipLaunchKernelGGL(k_weight_grad, dim3(1, (unsigned) wchunks, kHidden + 1), dim3(256), 0, st, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(),
+					   h->ld, kWChunk, wchunks, h->partial.as<float>());
+	NRC_HIP(hipGetLastError());
+	hipLaunchKernelGGL(k_reduce_chunks, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), wchunks,
+					   h->grad.as<float>());
+	NRC_HIP(hipGetLastError());
 	hipLaunchKernelGGL(k_reduce_loss, dim3(1), dim3(256), 0, st, h->tileLoss.as<float>(), nTiles, h->loss.as<float>());
 	NRC_HIP(hipGetLastError());
 	return KRR_OK;
@@ -593,7 +653,7 @@ int krr_nrc_create(const char *json, krr_nrc **out) {
 	NRC_HIP(h->wb.alloc(sizeof(uint16_t) * kBwdElems));
 	NRC_HIP(h->xT.alloc(sizeof(uint16_t) * 64 * (kHidden + 1) * h->ld));
 	NRC_HIP(h->dzT.alloc(sizeof(uint16_t) * 64 * (kHidden + 1) * h->ld));
-	NRC_HIP(h->partial.alloc(sizeof(float) * 4096 * chunks));
+	NRC_HIP(h->partial.alloc(sizeof(float) * 4096 * (kHidden + 1) * (h->ld / kWChunk)));
 	NRC_HIP(h->tileLoss.alloc(sizeof(float) * (h->ld / 32)));
 	NRC_HIP(h->loss.alloc(sizeof(float)));
 	// padded rows of dZ_5^T (outputs 3..31) and of X^T are written per tile;
@@ -690,11 +750,15 @@ int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const 
 	return KRR_OK;
 }
 
-int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, const int *count, int capacity, hipStream_t st) {
+int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, const int *count, int capacity, int steps, hipStream_t st) {
 	const int64_t cap = std::min<int64_t>(capacity, nrc->maxBatch);
-	if (cap <= 0) return KRR_OK;
-	if (int rc = gradientImpl(nrc, (const krr_nrc_vertex *) vertices, targets, cap, st, count)) return rc;
-	return adamStep(nrc, st);
+	if (cap <= 0 || steps <= 0) return KRR_OK;
+	const int64_t m = (cap + steps - 1) / steps; // largest mini-batch
+	for (int b = 0; b < steps; b++) {
+		if (int rc = gradientImpl(nrc, (const krr_nrc_vertex *) vertices, targets, m, st, count, cap, b, steps)) return rc;
+		if (int rc = adamStep(nrc, st)) return rc;
+	}
+	return KRR_OK;
 }
 
 float nrcLastLoss(krr_nrc *nrc, hipStream_t st) {

@@ -32,9 +32,10 @@ typedef float float16v __attribute__((ext_vector_type(16))); // 32x32 accumulato
 // krr_nrc_vertex, float4 4 = {thp.xyz, pixel bits}.  Adds thp * max(cache, 0)
 // to L[pixel] (one query per pixel per sample, so no atomics).
 int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const int *count, int capacity, float4 *L, hipStream_t st);
-// One Adam step on min(*count, capacity) vertices (16 floats each) and
-// targets (3 floats each); the kernels are sized for `capacity`.
-int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, const int *count, int capacity, hipStream_t st);
+// `steps` Adam steps, one per equal consecutive slice of the first
+// min(*count, capacity) vertices (16 floats each) and targets (3 floats
+// each), the slices computed on the device; kernels sized for `capacity`.
+int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, const int *count, int capacity, int steps, hipStream_t st);
 // Loss of the last training step (synchronises the stream).
 float nrcLastLoss(krr_nrc *nrc, hipStream_t st);
 #endif

@@ -767,7 +767,7 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	const int count		  = *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
-	bool active = false, exhausted = false;
+	bool active = false, exhausted = false, trainPath = false;
 	int depth	= startDepth;
 	RayItem r;
 	while (true) {
@@ -775,9 +775,10 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				r	   = loadRayItem(q, i);
-				depth  = startDepth;
-				active = true;
+				r		  = loadRayItem(q, i);
+				depth	  = startDepth;
+				active	  = true;
+				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
 			}
 		}
 		if (__ballot(active) == 0) break;
@@ -789,7 +790,6 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 		RayItem nx;
 		ShadowOut so;
 		so.medium = -1;
-		const bool trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
 		shadeCore<MEDIA, NRC>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath);
 		if (wantShadow) {
 			nShadow++;
@@ -1040,7 +1040,7 @@ struct krr_wf {
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	// neural radiance cache ("nrc" params)
 	bool nrcOn = false;
-	int nrcTrainMod = 16, nrcBatch = 65536;
+	int nrcTrainMod = 16, nrcBatch = 65536, nrcSteps = 4;
 	std::string nrcConfig; // JSON for krr_nrc_create
 	krr_nrc *nrc = nullptr;
 	DevBuf nrcQuery, nrcTrainBuf, nrcBatchV, nrcBatchT;
@@ -1122,9 +1122,11 @@ int parseParams(krr_wf *wf, const char *json) {
 				wf->nrcOn		= n.value("enable", true);
 				wf->nrcTrainMod = (int) n.value("train_mod", 16.0);
 				wf->nrcBatch	= (int) n.value("batch", 65536.0);
+				wf->nrcSteps	= (int) n.value("steps_per_frame", 4.0);
 				Json c			= Json::object();
 				for (const auto &kv : n.members())
-					if (kv.first != "enable" && kv.first != "train_mod" && kv.first != "batch") c.set(kv.first, kv.second);
+					if (kv.first != "enable" && kv.first != "train_mod" && kv.first != "batch" && kv.first != "steps_per_frame")
+						c.set(kv.first, kv.second);
 				c.set("max_batch", Json((double) wf->nrcBatch));
 				wf->nrcConfig = c.dump();
 			} else throw std::runtime_error("\"nrc\" must be a boolean or an object");
@@ -1144,6 +1146,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
+	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
 	if (!(wf->sahNodeCost >= 0.f && wf->sahNodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	return KRR_OK;
 }
@@ -1498,6 +1501,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 		Json n = Json::object();
 		n.set("train_mod", Json((double) wf->nrcTrainMod));
 		n.set("batch", Json((double) wf->nrcBatch));
+		n.set("steps_per_frame", Json((double) wf->nrcSteps));
 		j.set("nrc", n);
 	} else j.set("nrc", Json(false));
 	std::string s = j.dump();
@@ -1708,7 +1712,8 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
 	}
 	if (wf->nrcOn) { // one training step per frame on the recorded vertices
-		timed(5, [&] { nrcRc = nrcTrainDevice(wf->nrc, wf->nrcBatchV.as<float>(), wf->nrcBatchT.as<float>(), &cnt->nrcTrainDone, wf->nrcBatch, st); });
+		timed(5, [&] { nrcRc = nrcTrainDevice(wf->nrc, wf->nrcBatchV.as<float>(), wf->nrcBatchT.as<float>(), &cnt->nrcTrainDone, wf->nrcBatch, wf->nrcSteps, st);
+		});
 		if (nrcRc) return nrcRc;
 	}
 	hipLaunchKernelGGL(k_film, dim3(gN), dim3(bs), 0, st, P, px, (float4 *) film_dev);
