@@ -228,7 +228,7 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     return out
 
 
-NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 1e-3}
+NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 3e-3}
 
 
 def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc_params=None):
@@ -402,7 +402,10 @@ def main():
     else:
         line["cpu_baseline"] = None
     if args.config == "icosphere_field_1080p" and not args.no_nrc:
-        line["nrc_vs_plain"] = nrc_equal_time(cfg)
+        # the open workload, and a closed scene where paths are long
+        line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg),
+                                "cornell_box_1024": nrc_equal_time(CONFIGS["cornell_box_1024"],
+                                                                   nrc_params={"steps_per_frame": 2, "batch": 32768})}
     print(json.dumps(line), flush=True)
 
 
