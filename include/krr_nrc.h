@@ -79,7 +79,8 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const float *targets_dev, int64_t n,
 					   float *loss_out, void *hip_stream);
 
-/* Adam steps taken since creation or the last krr_nrc_set_weights. */
+/* Adam steps taken since creation or the last krr_nrc_set_weights (the count
+ * is kept in device memory; this call waits for the device). */
 int64_t krr_nrc_steps(const krr_nrc *nrc);
 
 #ifdef __cplusplus

@@ -254,6 +254,10 @@ typedef struct krr_wf krr_wf;
  *                            frame % train_mod trace unbiased training paths
  *                            whose vertices train the cache once per frame.
  *                            Needs wavefront_depths = 0 and no media.
+ *   occupancy (1.0)          fraction of the resident blocks a persistent
+ *                            launch uses (in-flight path capacity)
+ *   graph (true)             record the multi-view frame into a HIP graph
+ *                            once and replay it (krr_wf_render_views)
  * NULL or "" means all defaults. */
 /* Replaces RenderPassFactory::deserialize<WavefrontPathTracer> + from_json
  * (src/core/renderpass.h:202-273, src/render/wavefront/integrator.h:96-103)
@@ -287,6 +291,37 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap);
 /* Frees every device allocation of the pass (~RenderPass,
  * src/core/renderpass.h:146; the reference never frees its queues). */
 int krr_wf_destroy(krr_wf *wf);
+
+/* ---------------------------------------------------------------------- */
+/* Many camera views in one frame.  No reference counterpart: KiRaRay        */
+/* renders one view per frame (integrator.cpp:205-267); this batches views   */
+/* of uneven sizes into one persistent launch per sample.                    */
+/* ---------------------------------------------------------------------- */
+
+typedef struct krr_view {
+	krr_camera camera;
+	int32_t width, height; /* 1..65536 */
+	float *film_dev;	   /* width x height float4 RGBA32F, row height-1-y, caller-owned */
+} krr_view;
+
+/* Copies the view list (cameras, sizes, film pointers) and sizes the pixel
+ * state for all views, at most 2^30 pixels in total. */
+int krr_wf_set_views(krr_wf *wf, const krr_view *views, int32_t n_views);
+/* One frame of every view: beginFrame + render (integrator.cpp:205-267) of
+ * each view, all views' pixels in one persistent launch per sample, films
+ * written.  View v seeds pixel (x, y) with PCG initstate
+ * interleave(x, y) | v << 32, so view 0 renders exactly what krr_wf_render
+ * renders for the same camera, size and frame index.  With "graph" the
+ * launches are recorded once (per views / scene / launch shape) and replayed;
+ * the frame index reaches the kernels through device memory. */
+int krr_wf_render_views(krr_wf *wf, uint64_t frame_index, void *hip_stream);
+/* Times persistent-launch shapes - trace_block_size {64, 128, 256} x
+ * stack_lds_entries {8, 16, 24} x occupancy {1, 0.75, 0.5} - on the first
+ * trial_paths (<= 0: 2^21) camera paths of the views (or of the single view,
+ * after krr_wf_begin_frame) and keeps the fastest.  Images do not depend on
+ * the choice.  Writes a JSON report (candidates with ms, best) into report
+ * (cap bytes) and returns its full length.  No reference counterpart. */
+int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, void *hip_stream);
 
 /* ---------------------------------------------------------------------- */
 /* Error metric (ErrorMeasurePass, src/render/passes/errormeasure/metrics.cu) */

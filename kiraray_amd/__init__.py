@@ -17,11 +17,11 @@ import ctypes as C
 import json as _json
 
 from . import _abi
-from ._abi import (krr_camera, krr_hit, krr_material, krr_medium, krr_mesh, krr_scene_desc, krr_stats,
+from ._abi import (krr_camera, krr_hit, krr_material, krr_medium, krr_mesh, krr_scene_desc, krr_stats, krr_view,
                    load_library)
 
 __all__ = ["KrrError", "Scene", "Accel", "WavefrontPathTracer", "RenderApp", "load_library", "error_metric", "accumulate",
-           "krr_camera", "krr_hit", "krr_material", "krr_medium", "krr_mesh", "krr_scene_desc", "krr_stats"]
+           "krr_camera", "krr_hit", "krr_material", "krr_medium", "krr_mesh", "krr_scene_desc", "krr_stats", "krr_view"]
 
 
 class KrrError(RuntimeError):
@@ -172,6 +172,26 @@ class WavefrontPathTracer:
 
     def end_frame(self):
         _check(self._lib.krr_wf_end_frame(self._h))
+
+    def set_views(self, views):
+        """views: iterable of (camera, width, height, film_ptr); all are
+        rendered by render_views in one persistent launch per sample."""
+        views = list(views)
+        arr = (krr_view * len(views))()
+        for i, (cam, w, h, film) in enumerate(views):
+            arr[i].camera = cam
+            arr[i].width, arr[i].height = int(w), int(h)
+            arr[i].film_dev = film
+        _check(self._lib.krr_wf_set_views(self._h, arr, len(views)))
+
+    def render_views(self, frame_index, stream=0):
+        _check(self._lib.krr_wf_render_views(self._h, int(frame_index), C.c_void_p(stream)))
+
+    def autotune(self, trial_paths=0, stream=0):
+        """Time persistent-launch shapes and keep the fastest; returns the report."""
+        buf = C.create_string_buffer(1 << 14)
+        _check(self._lib.krr_wf_autotune(self._h, int(trial_paths), buf, 1 << 14, C.c_void_p(stream)))
+        return _json.loads(buf.value.decode())
 
     def stats(self):
         s = krr_stats()

@@ -62,6 +62,10 @@ class krr_camera(C.Structure):
                 ("shutter_time", C.c_float), ("transform", C.c_float * 12), ("medium", C.c_int32)]
 
 
+class krr_view(C.Structure):
+    _fields_ = [("camera", krr_camera), ("width", C.c_int32), ("height", C.c_int32), ("film_dev", C.c_void_p)]
+
+
 class krr_stats(C.Structure):
     _fields_ = [("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64), ("paths", C.c_uint64),
                 ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
@@ -105,6 +109,9 @@ PROTOTYPES = {
     "krr_wf_stats": (C.c_int, [C.c_void_p, C.POINTER(krr_stats)]),
     "krr_wf_get_params": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "krr_wf_destroy": (C.c_int, [C.c_void_p]),
+    "krr_wf_set_views": (C.c_int, [C.c_void_p, C.POINTER(krr_view), C.c_int32]),
+    "krr_wf_render_views": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
+    "krr_wf_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int32, C.c_void_p]),
     "krr_error_metric": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.c_void_p]),
     "krr_accumulate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p]),
     # neural radiance cache (include/krr_nrc.h)

@@ -106,6 +106,28 @@ struct WfParams {
 	uint32_t sampleIndex; // frameIndex * spp (as uint)
 };
 
+// Values that change from frame to frame, read from device memory by the
+// multi-view kernels so that a captured graph can be replayed.
+struct FrameDev {
+	uint64_t inc;		  // PCG32 increment: (frameIndex * spp) << 1 | 1
+	uint32_t sampleIndex; // frameIndex * spp (as uint)
+	int phase;			  // radiance-cache training phase: frameIndex % trainMod
+};
+
+// One camera view of a multi-view frame; its film is W x H float4, row H-1-y.
+struct ViewDev {
+	CameraDev cam;
+	int W, H;
+	float4 *film;
+};
+
+struct ViewsDev {
+	const ViewDev *view;
+	const int *start; // n + 1 entries: first global pixel of each view, total last
+	int n;
+	const FrameDev *frame;
+};
+
 KRR_DEV int laneId() { return __lane_id(); }
 
 // Wave-aggregated queue push: one atomic per wave per queue, each active lane

@@ -493,10 +493,16 @@ __global__ void k_write_copies(const float *w, uint16_t *wt, uint16_t *wb) {
 }
 
 // Adam (Kingma & Ba 2015) with bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
-__global__ void k_adam(float *w, float *mo, float *ve, const float *grad, float lr, float b1, float b2, float eps, float c1, float c2,
+// Adam's step count lives in device memory, so that training steps recorded
+// in a graph count up on every replay.
+__global__ void k_adam_count(int *t) { *t += 1; }
+
+__global__ void k_adam(float *w, float *mo, float *ve, const float *grad, float lr, float b1, float b2, float eps, const int *tDev,
 					   uint16_t *wt, uint16_t *wb) {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p >= KRR_NRC_PARAMS) return;
+	const float t  = (float) *tDev;
+	const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t); // bias corrections
 	const float gr = grad[p];
 	const float m  = b1 * mo[p] + (1.f - b1) * gr;
 	const float v  = b2 * ve[p] + (1.f - b2) * gr * gr;
@@ -547,8 +553,7 @@ struct krr_nrc {
 	float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f, lossEps = 0.01f;
 	uint64_t seed	= 1;
 	int64_t maxBatch = 65536, ld = 0;
-	int64_t steps	= 0;
-	Buf w, m, v, grad, wt, wb, xT, dzT, partial, tileLoss, loss;
+	Buf w, m, v, grad, wt, wb, xT, dzT, partial, tileLoss, loss, step;
 };
 
 namespace {
@@ -563,7 +568,8 @@ int uploadWeights(krr_nrc *h, const std::vector<float> &host) {
 					   h->wb.as<uint16_t>());
 	NRC_HIP(hipGetLastError());
 	NRC_HIP(hipDeviceSynchronize());
-	h->steps = 0;
+	NRC_HIP(hipMemset(h->step.p, 0, sizeof(int)));
+	NRC_HIP(hipDeviceSynchronize());
 	return KRR_OK;
 }
 
@@ -656,6 +662,7 @@ int krr_nrc_create(const char *json, krr_nrc **out) {
 	NRC_HIP(h->partial.alloc(sizeof(float) * 4096 * (kHidden + 1) * (h->ld / kWChunk)));
 	NRC_HIP(h->tileLoss.alloc(sizeof(float) * (h->ld / 32)));
 	NRC_HIP(h->loss.alloc(sizeof(float)));
+	NRC_HIP(h->step.alloc(sizeof(int)));
 	// padded rows of dZ_5^T (outputs 3..31) and of X^T are written per tile;
 	// clear once so that unused feature rows are finite
 	NRC_HIP(hipMemset(h->xT.p, 0, h->xT.sz));
@@ -730,11 +737,10 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 namespace {
 
 int adamStep(krr_nrc *nrc, hipStream_t st) {
-	nrc->steps++;
-	const float c1 = 1.f - std::pow(nrc->beta1, (float) nrc->steps), c2 = 1.f - std::pow(nrc->beta2, (float) nrc->steps);
+	hipLaunchKernelGGL(k_adam_count, dim3(1), dim3(1), 0, st, nrc->step.as<int>());
 	hipLaunchKernelGGL(k_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, nrc->w.as<float>(), nrc->m.as<float>(), nrc->v.as<float>(),
-					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, c1, c2, nrc->wt.as<uint16_t>(),
-					   nrc->wb.as<uint16_t>());
+					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, (const int *) nrc->step.as<int>(),
+					   nrc->wt.as<uint16_t>(), nrc->wb.as<uint16_t>());
 	NRC_HIP(hipGetLastError());
 	return KRR_OK;
 }
@@ -785,6 +791,11 @@ int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const f
 	return KRR_OK;
 }
 
-int64_t krr_nrc_steps(const krr_nrc *nrc) { return nrc ? nrc->steps : -1; }
+int64_t krr_nrc_steps(const krr_nrc *nrc) {
+	if (!nrc) return -1;
+	int t = 0;
+	if (hipDeviceSynchronize() != hipSuccess || hipMemcpy(&t, nrc->step.p, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+	return t;
+}
 
 } // extern "C"

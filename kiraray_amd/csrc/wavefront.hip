@@ -78,22 +78,15 @@ __global__ void k_begin_frame(WfParams P, PixelStateDev px) {
 	px.rng[i]			= s.state;
 }
 
-// generateCameraRays (integrator.cpp:271-284) + CameraData::getRay
-// (camera.h:36-58).  Camera rays go to slot = pixelId, so the first trace
-// sees image-coherent rays.
-__global__ void k_camera(WfParams P, CameraDev cam, PixelStateDev px, RayQueueDev q, Counters *cnt) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i == 0) cnt->rayCount[0] = P.N;
-	if (i >= P.N) return;
-	if (!P.legacySpp) px.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-	Pcg s{px.rng[i], P.inc};
+// CameraData::generateSample + getRay (camera.h:32-58): the five camera draws
+// (pFilm 2, pLens 2, time 1) in order, pixel (x, y) of a W x H film.
+KRR_DEV RayItem cameraRay(const CameraDev &cam, int W, int H, int x, int y, Pcg &s) {
 	float cs[5];
 	for (int k = 0; k < 5; k++) cs[k] = s.get1D();
-	const int x = i % P.W, y = i / P.W;
 	const float p0	 = (float) x + 0.5f + cs[0];
 	const float p1	 = (float) y + 0.5f + cs[1];
-	const float ndc0 = (2.f * p0) / (float) P.W + (-1.f);
-	const float ndc1 = (2.f * p1) / (float) P.H + (-1.f);
+	const float ndc0 = (2.f * p0) / (float) W + (-1.f);
+	const float ndc1 = (2.f * p1) / (float) H + (-1.f);
 	const float fov	 = detmath::atan2f(cam.filmH * 0.5f, cam.focalLength);
 	const f3 fd		 = normalize(f3(detmath::tanf(fov) * cam.aspect * ndc0, detmath::tanf(fov) * ndc1, -1.f));
 	f3 o, d;
@@ -119,9 +112,94 @@ __global__ void k_camera(WfParams P, CameraDev cam, PixelStateDev px, RayQueueDe
 	r.depth	   = 0;
 	r.bsdfType = BSDF_UNSET;
 	r.medium   = cam.medium;
-	r.pixel	   = i;
+	r.pixel	   = 0;
+	return r;
+}
+
+// generateCameraRays (integrator.cpp:271-284).  Camera rays go to slot =
+// pixelId, so the first trace sees image-coherent rays.
+__global__ void k_camera(WfParams P, CameraDev cam, PixelStateDev px, RayQueueDev q, Counters *cnt) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0) cnt->rayCount[0] = P.N;
+	if (i >= P.N) return;
+	if (!P.legacySpp) px.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	Pcg s{px.rng[i], P.inc};
+	RayItem r = cameraRay(cam, P.W, P.H, i % P.W, i / P.W, s);
+	r.pixel	  = i;
 	storeRayItem(q, i, r);
 	px.rng[i] = s.state;
+}
+
+// ---------------------------------------------------------------------------
+// Many views in one launch.  The pixels of all views are numbered one after
+// the other (view v owns [start[v], start[v+1])); pixel state, queues and the
+// radiance cache work on that global number.  Frame-varying values come from
+// a FrameDev in device memory, so one captured graph serves every frame.
+// ---------------------------------------------------------------------------
+
+// View of the global pixel i (largest v with start[v] <= i).
+KRR_DEV int findView(const ViewsDev &V, int i) {
+	int lo = 0, hi = V.n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (V.start[mid] <= i) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+__global__ void k_set_frame(FrameDev *f, FrameDev v) { *f = v; }
+
+// beginFrame for every view.  View v seeds its pixel (x, y) with initstate
+// interleave(x, y) | v << 32, so view 0 gets exactly the single-view seed.
+__global__ void k_begin_views(WfParams P, ViewsDev V, PixelStateDev px) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.N) return;
+	const int v		= findView(V, i);
+	const int local = i - V.start[v];
+	const int W		= V.view[v].W;
+	const int x = local % W, y = local / W;
+	px.L[i]		= make_float4(0.f, 0.f, 0.f, 0.f);
+	px.pixel[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	Pcg s;
+	s.setSeed((uint64_t) interleave32((uint32_t) x, (uint32_t) y) | ((uint64_t) (uint32_t) v << 32), (uint64_t) V.frame->sampleIndex);
+	s.advance((int64_t) (256 * local));
+	const float u		= s.get1D();
+	const float lambda0 = (1.f - u) * 360.f + u * 830.f;
+	int ch				= (int) ((lambda0 - 360.f) / 470.f * 3);
+	px.channel[i]		= ch < 0 ? 0 : (ch > 2 ? 2 : ch);
+	px.rng[i]			= s.state;
+}
+
+// Camera ray of global pixel i for the current sample (k_camera per view).
+KRR_DEV RayItem viewCameraRay(const WfParams &P, const ViewsDev &V, const PixelStateDev &px, int i) {
+	const int v		= findView(V, i);
+	const int local = i - V.start[v];
+	const ViewDev &vw = V.view[v];
+	if (!P.legacySpp) px.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+	Pcg s{px.rng[i], P.inc};
+	RayItem r = cameraRay(vw.cam, vw.W, vw.H, local % vw.W, local / vw.W, s);
+	r.pixel	  = i;
+	px.rng[i] = s.state;
+	return r;
+}
+
+// Film write of every view (k_film per view).
+__global__ void k_film_views(WfParams P, ViewsDev V, PixelStateDev px) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= P.N) return;
+	const int v		  = findView(V, i);
+	const int local	  = i - V.start[v];
+	const ViewDev &vw = V.view[v];
+	const int x = local % vw.W, y = local / vw.W;
+	const float4 p = px.pixel[i];
+	float r = p.x / (float) P.spp, g = p.y / (float) P.spp, b = p.z / (float) P.spp;
+	if (P.enableClamp) {
+		r = clampf(r, 0.f, P.clampMax);
+		g = clampf(g, 0.f, P.clampMax);
+		b = clampf(b, 0.f, P.clampMax);
+	}
+	vw.film[(size_t) (vw.H - 1 - y) * vw.W + x] = make_float4(r, g, b, 1.f);
 }
 
 __global__ void k_depth_begin(Counters *cnt, int cur) {
@@ -760,12 +838,21 @@ __global__ void k_mega_begin(Counters *cnt) {
 // Each loop iteration advances every lane's path by one depth; a lane whose
 // path has ended takes the next queued path at once (one atomic per wave for
 // all such lanes), so no lane idles while the queue still holds work.
-template <bool MEDIA, bool COUNT, bool NRC = false>
-__global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
-							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}) {
+//
+// VIEWS: the paths are the camera samples of all views (global pixels
+// 0..P.N-1), generated at fetch instead of read from the ray queue; the
+// frame's PCG increment and training phase come from V.frame.
+template <bool MEDIA, bool COUNT, bool NRC = false, bool VIEWS = false>
+__global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
+							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{}) {
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
-	const int count		  = *countPtr;
+	WfParams P			  = P0;
+	if (VIEWS) {
+		P.inc = V.frame->inc;
+		if (NRC) nrc.phase = V.frame->phase;
+	}
+	const int count = VIEWS ? P.N : *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
 	bool active = false, exhausted = false, trainPath = false;
 	int depth	= startDepth;
@@ -775,7 +862,7 @@ __global__ void k_megakernel(WfParams P, DevScene sc, TraceBvh bvh, RayQueueDev 
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				r		  = loadRayItem(q, i);
+				r		  = VIEWS ? viewCameraRay(P, V, px, i) : loadRayItem(q, i);
 				depth	  = startDepth;
 				active	  = true;
 				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
@@ -1017,6 +1104,9 @@ template <class K> int persistentGrid(K kernel, int block, size_t lds, int maxBl
 size_t stackLds(const AccelData &a, int block) { return (size_t) (a.ldsEntries() + 1) * block * sizeof(int); }
 size_t stackOverflowInts(const AccelData &a) { return (size_t) (a.stackSize - a.ldsEntries()); }
 
+// A fraction of a persistent grid (at least one block).
+int scaledGrid(int g, float frac) { return std::max(1, (int) std::lround((double) g * (double) frac)); }
+
 // Grows buf to hold the traversal-stack overflow of `threads` grid threads.
 int reserveOverflow(DevBuf &buf, const AccelData &a, size_t threads) {
 	const size_t bytes = stackOverflowInts(a) * threads * sizeof(int);
@@ -1045,6 +1135,30 @@ struct krr_wf {
 	krr_nrc *nrc = nullptr;
 	DevBuf nrcQuery, nrcTrainBuf, nrcBatchV, nrcBatchT;
 	int gMegaNrc = 1;
+	// persistent launches: fraction of the resident blocks launched
+	float occupancy = 1.f;
+	// multi-view frame (krr_wf_set_views / krr_wf_render_views): pixel state
+	// of all views' pixels, numbered view after view
+	std::vector<krr_view> views;
+	int NV = 0;
+	uint64_t viewsVersion = 0, sceneVersion = 0;
+	DevBuf viewBuf, viewStart, frameBuf, vL, vPixel, vRng, vChannel, vNrcQuery;
+	uint64_t viewsGridKey = ~0ull;
+	int gMegaViews = 1;
+	// the multi-view frame is recorded once into a graph and replayed
+	bool useGraph = true;
+	hipStream_t capStream = nullptr;
+	hipGraph_t graph	  = nullptr;
+	hipGraphExec_t graphExec = nullptr;
+	uint64_t graphKey		 = ~0ull;
+	bool haveCamera			 = false;
+	void dropGraph() {
+		if (graphExec) (void) hipGraphExecDestroy(graphExec);
+		if (graph) (void) hipGraphDestroy(graph);
+		graphExec = nullptr;
+		graph	  = nullptr;
+		graphKey  = ~0ull;
+	}
 	// scene
 	bool haveScene = false;
 	AccelData accel;
@@ -1087,6 +1201,8 @@ struct krr_wf {
 		return k;
 	}
 	~krr_wf() {
+		dropGraph();
+		if (capStream) (void) hipStreamDestroy(capStream);
 		for (hipEvent_t e : events) (void) hipEventDestroy(e);
 		if (nrc) krr_nrc_destroy(nrc);
 	}
@@ -1133,6 +1249,8 @@ int parseParams(krr_wf *wf, const char *json) {
 			if (wf->nrcConfig.empty()) wf->nrcConfig = "{\"max_batch\": " + std::to_string(wf->nrcBatch) + "}";
 		}
 		wf->sahNodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
+		wf->occupancy		= (float) j.value("occupancy", 1.0);
+		wf->useGraph		= j.value("graph", true);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
 	}
@@ -1148,6 +1266,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
 	if (!(wf->sahNodeCost >= 0.f && wf->sahNodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
+	if (!(wf->occupancy > 0.f && wf->occupancy <= 1.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "occupancy must be in (0, 1]");
 	return KRR_OK;
 }
 
@@ -1326,6 +1445,50 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	return KRR_OK;
 }
 
+// Kernel parameters of one frame (integrator.cpp:310-372 values).
+WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex) {
+	WfParams P;
+	P.W			   = W;
+	P.H			   = H;
+	P.N			   = N;
+	P.spp		   = wf->spp;
+	P.maxDepth	   = wf->maxDepth;
+	P.rr		   = wf->rr;
+	P.rrMinDepth   = wf->rrMinDepth;
+	P.nee		   = wf->nee ? 1 : 0;
+	P.enableMedium = (wf->enableMedium && wf->sceneHasMedia) ? 1 : 0;
+	P.enableClamp  = wf->enableClamp ? 1 : 0;
+	P.clampMax	   = wf->clampMax;
+	P.legacySpp	   = wf->legacySpp ? 1 : 0;
+	P.sampleIndex  = (uint32_t) (frameIndex * (uint64_t) wf->spp);
+	P.inc		   = ((uint64_t) P.sampleIndex << 1u) | 1u;
+	return P;
+}
+
+CameraDev cameraDev(const krr_camera &c) {
+	CameraDev cam;
+	cam.filmH		  = c.film_size[1];
+	cam.focalLength	  = c.focal_length;
+	cam.focalDistance = c.focal_distance;
+	cam.lensRadius	  = c.lens_radius;
+	cam.aspect		  = c.aspect_ratio;
+	std::memcpy(cam.M, c.transform, sizeof(cam.M));
+	cam.medium = c.medium;
+	return cam;
+}
+
+// Radiance-cache queues: queries of one sample of `queryCap` pixels, and the
+// frame's training vertices.
+int ensureNrcBuffers(krr_wf *wf, DevBuf &query, size_t queryCap) {
+	if (!wf->nrcOn) return KRR_OK;
+	if (query.sz < queryCap * 5 * sizeof(float4)) KRR_HIP(query.alloc(queryCap * 5 * sizeof(float4)));
+	const size_t b = (size_t) wf->nrcBatch;
+	if (!wf->nrcTrainBuf.p) KRR_HIP(wf->nrcTrainBuf.alloc(b * 6 * sizeof(float4)));
+	if (!wf->nrcBatchV.p) KRR_HIP(wf->nrcBatchV.alloc(b * 4 * sizeof(float4)));
+	if (!wf->nrcBatchT.p) KRR_HIP(wf->nrcBatchT.alloc(b * 3 * sizeof(float)));
+	return KRR_OK;
+}
+
 int allocFrame(krr_wf *wf) {
 	const size_t N = (size_t) wf->W * wf->H;
 	for (int q = 0; q < 2; q++) {
@@ -1341,16 +1504,11 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
 	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
 	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
-	if (wf->nrcOn) {
-		KRR_HIP(wf->nrcQuery.alloc(N * 5 * sizeof(float4)));
-		KRR_HIP(wf->nrcTrainBuf.alloc((size_t) wf->nrcBatch * 6 * sizeof(float4)));
-		KRR_HIP(wf->nrcBatchV.alloc((size_t) wf->nrcBatch * 4 * sizeof(float4)));
-		KRR_HIP(wf->nrcBatchT.alloc((size_t) wf->nrcBatch * 3 * sizeof(float)));
-	}
-	return KRR_OK;
+	return ensureNrcBuffers(wf, wf->nrcQuery, N);
 }
 
 RayQueueDev rayQueue(krr_wf *wf, int q) { return RayQueueDev{wf->rayQ[q][0].as<float4>(), wf->rayQ[q][1].as<float4>()}; }
+
 
 } // namespace
 
@@ -1497,6 +1655,8 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
+	j.set("occupancy", Json((double) wf->occupancy));
+	j.set("graph", Json(wf->useGraph));
 	if (wf->nrcOn) {
 		Json n = Json::object();
 		n.set("train_mod", Json((double) wf->nrcTrainMod));
@@ -1519,6 +1679,8 @@ int krr_wf_set_scene(krr_wf *wf, const krr_scene_desc *scene) {
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_set_scene: no HIP device");
 	KRR_HIP(hipDeviceSynchronize());
 	wf->haveScene = false;
+	wf->sceneVersion++;
+	wf->dropGraph();
 	int rc		  = uploadScene(wf, scene);
 	if (rc) return rc;
 	if (wf->W > 0 && wf->H > 0) return allocFrame(wf);
@@ -1546,6 +1708,7 @@ int krr_wf_begin_frame(krr_wf *wf, const krr_camera *camera, uint64_t frame_inde
 	wf->cam			 = *camera;
 	wf->frameIndex	 = frame_index;
 	wf->beginPending = true;
+	wf->haveCamera	 = true;
 	return KRR_OK;
 }
 
@@ -1557,30 +1720,9 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	if (!film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: null film");
 	hipStream_t st = (hipStream_t) hip_stream;
 	wf->lastStream = st;
-	const int N	   = wf->W * wf->H;
-	WfParams P;
-	P.W			  = wf->W;
-	P.H			  = wf->H;
-	P.N			  = N;
-	P.spp		  = wf->spp;
-	P.maxDepth	  = wf->maxDepth;
-	P.rr		  = wf->rr;
-	P.rrMinDepth  = wf->rrMinDepth;
-	P.nee		  = wf->nee ? 1 : 0;
-	P.enableMedium	 = (wf->enableMedium && wf->sceneHasMedia) ? 1 : 0;
-	P.enableClamp	 = wf->enableClamp ? 1 : 0;
-	P.clampMax		 = wf->clampMax;
-	P.legacySpp		 = wf->legacySpp ? 1 : 0;
-	P.sampleIndex	 = (uint32_t) (wf->frameIndex * (uint64_t) wf->spp);
-	P.inc			 = ((uint64_t) P.sampleIndex << 1u) | 1u;
-	CameraDev cam;
-	cam.filmH		  = wf->cam.film_size[1];
-	cam.focalLength	  = wf->cam.focal_length;
-	cam.focalDistance = wf->cam.focal_distance;
-	cam.lensRadius	  = wf->cam.lens_radius;
-	cam.aspect		  = wf->cam.aspect_ratio;
-	std::memcpy(cam.M, wf->cam.transform, sizeof(cam.M));
-	cam.medium = wf->cam.medium;
+	const int N		   = wf->W * wf->H;
+	const WfParams P   = frameParams(wf, wf->W, wf->H, N, wf->frameIndex);
+	const CameraDev cam = cameraDev(wf->cam);
 
 	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
 	ShadowQueueDev sq{wf->shadowRay.as<float4>(), wf->shadowC.as<float4>(), wf->shadowExt.as<float4>()};
@@ -1597,7 +1739,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
-							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0);
+							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (shapeKey != wf->gridKey) {
 		// Counting variants use more registers: size each grid for the variant launched.
 		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
@@ -1608,6 +1750,8 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		else wf->gMega = count ? persistentGrid(k_megakernel<false, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false>, tb, lds, gTmax);
 		if (wf->nrcOn)
 			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false, true>, tb, lds, gTmax);
+		wf->gMega	 = scaledGrid(wf->gMega, wf->occupancy);
+		wf->gMegaNrc = scaledGrid(wf->gMegaNrc, wf->occupancy);
 		const int gMax = std::max(std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega), wf->nrcOn ? wf->gMegaNrc : 0);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) gMax * tb)) return rc;
 		wf->gridKey = shapeKey;
@@ -1726,6 +1870,362 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 int krr_wf_end_frame(krr_wf *wf) {
 	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_end_frame: null handle");
 	return KRR_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// The persistent kernel variants of the multi-view frame and of the
+// single-view frame's camera samples (autotuner trials).
+template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS> struct Mega {
+	static int grid(int tb, size_t lds, int maxBlocks) { return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS>, tb, lds, maxBlocks); }
+	static void launch(unsigned g, int tb, size_t lds, hipStream_t st, const WfParams &P, const DevScene &ds, const TraceBvh &bvh,
+					   const RayQueueDev &q, const int *countPtr, const PixelStateDev &px, int *ovf, Counters *cnt, const NrcDev &nrc,
+					   const ViewsDev &V) {
+		hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, 0, px, ovf, cnt, nrc, V);
+	}
+};
+
+template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, F &&f) {
+	if (nrc) return count ? f(Mega<false, true, true, VIEWS>{}) : f(Mega<false, false, true, VIEWS>{});
+	if (media) return count ? f(Mega<true, true, false, VIEWS>{}) : f(Mega<true, false, false, VIEWS>{});
+	return count ? f(Mega<false, true, false, VIEWS>{}) : f(Mega<false, false, false, VIEWS>{});
+}
+
+ViewsDev viewsDev(krr_wf *wf) { return ViewsDev{wf->viewBuf.as<ViewDev>(), wf->viewStart.as<int>(), (int) wf->views.size(), wf->frameBuf.as<FrameDev>()}; }
+PixelStateDev viewsPx(krr_wf *wf) { return PixelStateDev{wf->vL.as<float4>(), wf->vPixel.as<float4>(), wf->vRng.as<uint64_t>(), wf->vChannel.as<int>()}; }
+
+// Launch shape of the multi-view persistent kernel over N paths.
+struct ViewsLaunch {
+	TraceBvh bvh;
+	size_t lds;
+	unsigned grid;
+	bool media, count, nrc;
+};
+
+int viewsLaunch(krr_wf *wf, int N, bool nrc, ViewsLaunch &L) {
+	wf->accel.stackLds = wf->stackLdsEntries; // before dev(), as in krr_wf_render
+	L.bvh			   = wf->accel.dev();
+	const int tb	   = wf->traceBlockSize;
+	L.lds			   = stackLds(wf->accel, tb);
+	L.count			   = wf->countTraversal;
+	L.media			   = wf->enableMedium && wf->sceneHasMedia;
+	L.nrc			   = nrc;
+	const uint64_t key = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) wf->accel.stackSize << 50) ^
+						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^
+						 ((uint64_t) (wf->occupancy * 1024.f) << 32);
+	if (key != wf->viewsGridKey) {
+		const int maxBlocks = std::max(1, (N + tb - 1) / tb);
+		const int g = withMega<true>(L.media, L.count, nrc, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
+		wf->gMegaViews		= scaledGrid(g, wf->occupancy);
+		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) wf->gMegaViews * tb)) return rc;
+		wf->viewsGridKey = key;
+	}
+	L.grid = (unsigned) wf->gMegaViews;
+	return KRR_OK;
+}
+
+// Everything of one multi-view frame after the frame values are set:
+// beginFrame of all views, each sample in one persistent launch over all
+// views' pixels (plus the cache queries), accumulate, the cache's training,
+// the film write of every view.  `trial` stops after the first persistent
+// launch and writes no film (autotuner).
+int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_t st, bool timing, bool trial) {
+	const ViewsDev V	   = viewsDev(wf);
+	const PixelStateDev px = viewsPx(wf);
+	Counters *cnt		   = wf->counters.as<Counters>();
+	const int tb		   = wf->traceBlockSize;
+	const unsigned gN	   = (unsigned) ((P.N + wf->blockSize - 1) / wf->blockSize);
+	int *ovf			   = wf->ovf.as<int>();
+	NrcDev nrc{};
+	if (L.nrc) {
+		nrc.query	   = wf->vNrcQuery.as<float4>();
+		nrc.train	   = wf->nrcTrainBuf.as<float4>();
+		nrc.queryCount = &cnt->nrcQueries;
+		nrc.trainCount = &cnt->nrcTrain;
+		nrc.capQuery   = P.N;
+		nrc.capTrain   = wf->nrcBatch;
+		nrc.trainMod   = wf->nrcTrainMod; // phase comes from the frame values
+	}
+	auto timed = [&](int kind, auto &&launch) {
+		if (!timing) {
+			launch();
+			return;
+		}
+		const int b = wf->mark(st);
+		launch();
+		wf->spans.push_back({kind, b, wf->mark(st)});
+	};
+	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
+	hipLaunchKernelGGL(k_begin_views, dim3(gN), dim3(wf->blockSize), 0, st, P, V, px);
+	int rc = KRR_OK;
+	for (int s = 0; s < wf->spp; s++) {
+		hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
+		timed(4, [&] {
+			withMega<true>(L.media, L.count, L.nrc, [&](auto k) {
+				decltype(k)::launch(L.grid, tb, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, px, ovf, cnt, nrc, V);
+				return 0;
+			});
+		});
+		if (trial) return KRR_OK;
+		if (L.nrc) {
+			timed(5, [&] {
+				hipLaunchKernelGGL(k_nrc_targets, dim3(256), dim3(256), 0, st, nrc, (const int *) &cnt->nrcTrainDone, (const float4 *) px.L,
+								   wf->nrcBatchV.as<float4>(), wf->nrcBatchT.as<float>());
+				hipLaunchKernelGGL(k_nrc_mark_done, dim3(1), dim3(1), 0, st, cnt, wf->nrcBatch, P.N);
+				rc = nrcInferAccumulate(wf->nrc, nrc.query, 5, &cnt->nrcQueries, P.N, px.L, st);
+			});
+			if (rc) return rc;
+		}
+		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(wf->blockSize), 0, st, P.N, px);
+	}
+	if (L.nrc) {
+		timed(5, [&] { rc = nrcTrainDevice(wf->nrc, wf->nrcBatchV.as<float>(), wf->nrcBatchT.as<float>(), &cnt->nrcTrainDone, wf->nrcBatch, wf->nrcSteps, st); });
+		if (rc) return rc;
+	}
+	hipLaunchKernelGGL(k_film_views, dim3(gN), dim3(wf->blockSize), 0, st, P, V, px);
+	return KRR_OK;
+}
+
+FrameDev frameDev(const krr_wf *wf, uint64_t frameIndex) {
+	FrameDev f;
+	f.sampleIndex = (uint32_t) (frameIndex * (uint64_t) wf->spp);
+	f.inc		  = ((uint64_t) f.sampleIndex << 1u) | 1u;
+	f.phase		  = (int) (frameIndex % (uint64_t) std::max(1, wf->nrcTrainMod));
+	return f;
+}
+
+} // namespace
+
+extern "C" {
+
+int krr_wf_set_views(krr_wf *wf, const krr_view *views, int32_t n_views) {
+	krr::clearError();
+	if (!wf || !views || n_views < 1) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_views: need a handle and at least one view");
+	if (n_views > 1 << 20) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_set_views: at most 2^20 views");
+	std::vector<ViewDev> host((size_t) n_views);
+	std::vector<int> start((size_t) n_views + 1);
+	int64_t total = 0;
+	for (int v = 0; v < n_views; v++) {
+		const krr_view &kv = views[v];
+		if (kv.width < 1 || kv.height < 1 || kv.width > 65536 || kv.height > 65536)
+			return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_views: view %d size %dx%d outside [1, 65536]", v, kv.width, kv.height);
+		if (!kv.film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_views: view %d has no film", v);
+		start[v]	 = (int) total;
+		total		+= (int64_t) kv.width * kv.height;
+		if (total > (int64_t) 1 << 30) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_set_views: at most 2^30 pixels over all views");
+		host[v].cam	 = cameraDev(kv.camera);
+		host[v].W	 = kv.width;
+		host[v].H	 = kv.height;
+		host[v].film = (float4 *) kv.film_dev;
+	}
+	start[n_views] = (int) total;
+	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_set_views: no HIP device");
+	KRR_HIP(hipDeviceSynchronize());
+	wf->dropGraph();
+	const size_t N = (size_t) total;
+	KRR_HIP(wf->viewBuf.upload(host.data(), host.size() * sizeof(ViewDev)));
+	KRR_HIP(wf->viewStart.upload(start.data(), start.size() * sizeof(int)));
+	if (!wf->frameBuf.p) KRR_HIP(wf->frameBuf.alloc(sizeof(FrameDev)));
+	if (wf->vL.sz < N * sizeof(float4)) {
+		KRR_HIP(wf->vL.alloc(N * sizeof(float4)));
+		KRR_HIP(wf->vPixel.alloc(N * sizeof(float4)));
+		KRR_HIP(wf->vRng.alloc(N * sizeof(uint64_t)));
+		KRR_HIP(wf->vChannel.alloc(N * sizeof(int)));
+	}
+	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+	if (int rc = ensureNrcBuffers(wf, wf->vNrcQuery, N)) return rc;
+	wf->views.assign(views, views + n_views);
+	wf->NV = (int) total;
+	wf->viewsVersion++;
+	return KRR_OK;
+}
+
+int krr_wf_render_views(krr_wf *wf, uint64_t frame_index, void *hip_stream) {
+	krr::clearError();
+	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render_views: null handle");
+	if (!wf->haveScene) return krr::fail(KRR_ERR_NO_SCENE, "krr_wf_render_views: no scene set");
+	if (wf->NV == 0) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render_views: no views set");
+	for (const krr_view &v : wf->views)
+		if (v.camera.medium >= wf->ds.nMedia) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render_views: camera medium out of range");
+	const bool media = wf->enableMedium && wf->sceneHasMedia;
+	if (wf->nrcOn && media) return krr::fail(KRR_ERR_UNSUPPORTED, "nrc: needs a scene without media");
+	hipStream_t st = (hipStream_t) hip_stream;
+	wf->lastStream = st;
+	const WfParams P = frameParams(wf, 0, 0, wf->NV, frame_index);
+	ViewsLaunch L;
+	if (int rc = viewsLaunch(wf, wf->NV, wf->nrcOn, L)) return rc;
+	hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, st, wf->frameBuf.as<FrameDev>(), frameDev(wf, frame_index));
+	const bool timing = wf->timeKernels;
+	wf->spans.clear();
+	wf->eventsUsed = 0;
+	if (wf->useGraph && !timing) {
+		// every value the recorded launches take is fixed until the views,
+		// the scene, the launch shape or the overflow buffer change
+		const uint64_t key = wf->viewsVersion * 0x9E3779B97F4A7C15ull ^ wf->sceneVersion * 0xC2B2AE3D27D4EB4Full ^ wf->viewsGridKey ^
+							 (uint64_t) (uintptr_t) wf->ovf.p ^ ((uint64_t) wf->blockSize << 56);
+		if (!wf->graphExec || key != wf->graphKey) {
+			wf->dropGraph();
+			if (!wf->capStream) KRR_HIP(hipStreamCreateWithFlags(&wf->capStream, hipStreamNonBlocking));
+			KRR_HIP(hipStreamBeginCapture(wf->capStream, hipStreamCaptureModeThreadLocal));
+			int rc		 = enqueueViews(wf, P, L, wf->capStream, false, false);
+			hipGraph_t g = nullptr;
+			hipError_t e = hipStreamEndCapture(wf->capStream, &g);
+			if (rc || e != hipSuccess) {
+				if (g) (void) hipGraphDestroy(g);
+				return rc ? rc : krr::fail(KRR_ERR_HIP, "krr_wf_render_views: graph capture: %s", hipGetErrorString(e));
+			}
+			wf->graph = g;
+			e		  = hipGraphInstantiate(&wf->graphExec, g, nullptr, nullptr, 0);
+			if (e != hipSuccess) {
+				wf->graphExec = nullptr;
+				wf->dropGraph();
+				return krr::fail(KRR_ERR_HIP, "krr_wf_render_views: graph instantiate: %s", hipGetErrorString(e));
+			}
+			wf->graphKey = key;
+		}
+		KRR_HIP(hipGraphLaunch(wf->graphExec, st));
+	} else {
+		const int b = timing ? wf->mark(st) : -1;
+		if (int rc = enqueueViews(wf, P, L, st, timing, false)) return rc;
+		if (timing) wf->spans.push_back({3, b, wf->mark(st)});
+	}
+	KRR_HIP(hipGetLastError());
+	wf->stats.paths = (uint64_t) wf->NV * wf->spp;
+	return KRR_OK;
+}
+
+int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, void *hip_stream) {
+	krr::clearError();
+	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_autotune: null handle");
+	if (!wf->haveScene) return krr::fail(KRR_ERR_NO_SCENE, "krr_wf_autotune: no scene set");
+	const bool views = wf->NV > 0;
+	if (!views && (wf->W * wf->H == 0 || !wf->haveCamera))
+		return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_autotune: needs views, or a frame size and a camera from krr_wf_begin_frame");
+	if (trial_paths <= 0) trial_paths = (int64_t) 1 << 21;
+	hipStream_t st	= (hipStream_t) hip_stream;
+	wf->lastStream	= st;
+	const int total = views ? wf->NV : wf->W * wf->H;
+	const int N		= (int) std::min<int64_t>(total, trial_paths);
+	const bool media = wf->enableMedium && wf->sceneHasMedia;
+	const int saveTb = wf->traceBlockSize, saveLds = wf->stackLdsEntries;
+	const float saveOcc = wf->occupancy;
+	const bool saveCount = wf->countTraversal;
+	wf->countTraversal	 = false;
+	hipEvent_t e0 = wf->event(0), e1 = wf->event(1);
+	if (!e0 || !e1) return krr::fail(KRR_ERR_HIP, "krr_wf_autotune: no events");
+	struct Trial {
+		int tb, lds;
+		float occ;
+		float ms;
+	};
+	std::vector<Trial> trials;
+	int rc = KRR_OK;
+	// One candidate: `runs` timed persistent launches over the first N paths
+	// of the frame (the views' pixels in order, or the first N pixels of the
+	// single view); the fastest run counts.
+	auto measure = [&](Trial &t, int runs) -> int {
+		wf->traceBlockSize	= t.tb;
+		wf->stackLdsEntries = t.lds;
+		wf->occupancy		= t.occ;
+		t.ms				= 1e30f;
+		for (int r = 0; r < runs; r++) {
+			if (views) {
+				const WfParams P = frameParams(wf, 0, 0, N, 1);
+				ViewsLaunch L;
+				if (int e = viewsLaunch(wf, N, false, L)) return e;
+				hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, st, wf->frameBuf.as<FrameDev>(), frameDev(wf, 1));
+				// begin + one persistent launch, timed from its start
+				hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
+				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), viewsPx(wf));
+				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
+				KRR_HIP(hipEventRecord(e0, st));
+				withMega<true>(L.media, false, false, [&](auto k) {
+					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, viewsPx(wf),
+										wf->ovf.as<int>(), wf->counters.as<Counters>(), NrcDev{}, viewsDev(wf));
+					return 0;
+				});
+			} else {
+				WfParams P = frameParams(wf, wf->W, wf->H, N, wf->frameIndex);
+				wf->accel.stackLds = wf->stackLdsEntries;
+				const TraceBvh bvh = wf->accel.dev();
+				const size_t lds   = stackLds(wf->accel, t.tb);
+				const int maxBlocks = std::max(1, (N + t.tb - 1) / t.tb);
+				int g = withMega<false>(media, false, false, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
+				g	  = scaledGrid(g, t.occ);
+				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
+				const PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
+				Counters *cnt = wf->counters.as<Counters>();
+				const unsigned gN = (unsigned) ((N + 255) / 256);
+				hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
+				hipLaunchKernelGGL(k_begin_frame, dim3(gN), dim3(256), 0, st, P, px);
+				hipLaunchKernelGGL(k_camera, dim3(gN), dim3(256), 0, st, P, cameraDev(wf->cam), px, rayQueue(wf, 0), cnt);
+				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
+				KRR_HIP(hipEventRecord(e0, st));
+				withMega<false>(media, false, false, [&](auto k) {
+					decltype(k)::launch((unsigned) g, t.tb, lds, st, P, wf->ds, bvh, rayQueue(wf, 0), (const int *) &cnt->rayCount[0], px,
+										wf->ovf.as<int>(), cnt, NrcDev{}, ViewsDev{});
+					return 0;
+				});
+			}
+			KRR_HIP(hipEventRecord(e1, st));
+			KRR_HIP(hipGetLastError());
+			KRR_HIP(hipEventSynchronize(e1));
+			float ms = 0.f;
+			KRR_HIP(hipEventElapsedTime(&ms, e0, e1));
+			if (r > 0 || runs == 1) t.ms = std::min(t.ms, ms); // the first run warms caches
+		}
+		return KRR_OK;
+	};
+	const int tbs[]		= {64, 128, 256};
+	const int ldss[]	= {8, 16, 24};
+	const float occs[]	= {1.f, 0.75f, 0.5f};
+	for (int tb : tbs)
+		for (int l : ldss)
+			for (float o : occs) {
+				Trial t{tb, l, o, 0.f};
+				if ((rc = measure(t, 3))) break;
+				trials.push_back(t);
+			}
+	wf->countTraversal = saveCount;
+	if (rc) {
+		wf->traceBlockSize	= saveTb;
+		wf->stackLdsEntries = saveLds;
+		wf->occupancy		= saveOcc;
+		return rc;
+	}
+	const Trial best = *std::min_element(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.ms < b.ms; });
+	wf->traceBlockSize	= best.tb;
+	wf->stackLdsEntries = best.lds;
+	wf->occupancy		= best.occ;
+	wf->gridKey = wf->viewsGridKey = ~0ull; // grids and the recorded frame follow the new shape
+	wf->dropGraph();
+	if (!views) wf->beginPending = true; // the trials used the frame's pixel state: seed it again
+	Json j = Json::object(), arr = Json::array();
+	for (const Trial &t : trials) {
+		Json o = Json::object();
+		o.set("trace_block_size", Json((double) t.tb));
+		o.set("stack_lds_entries", Json((double) t.lds));
+		o.set("occupancy", Json((double) t.occ));
+		o.set("ms", Json((double) t.ms));
+		arr.push(o);
+	}
+	Json b = Json::object();
+	b.set("trace_block_size", Json((double) best.tb));
+	b.set("stack_lds_entries", Json((double) best.lds));
+	b.set("occupancy", Json((double) best.occ));
+	b.set("ms", Json((double) best.ms));
+	j.set("trial_paths", Json((double) N));
+	j.set("candidates", arr);
+	j.set("best", b);
+	const std::string out = j.dump();
+	if (report && cap > 0) {
+		const size_t n = std::min((size_t) cap - 1, out.size());
+		std::memcpy(report, out.data(), n);
+		report[n] = 0;
+	}
+	return (int) out.size();
 }
 
 int krr_wf_stats(krr_wf *wf, krr_stats *out) {

@@ -20,7 +20,7 @@ class oracle_params(C.Structure):
                 ("rr_min_depth", C.c_int32), ("enable_clamp", C.c_int32), ("clamp_max", C.c_float),
                 ("spp", C.c_int32), ("legacy_spp_accumulate", C.c_int32), ("precision", C.c_int32),
                 ("n_threads", C.c_int32), ("row_begin", C.c_int32), ("row_end", C.c_int32),
-                ("row_step", C.c_int32), ("pixel_step", C.c_int32)]
+                ("row_step", C.c_int32), ("pixel_step", C.c_int32), ("view_id", C.c_int32)]
 
 
 class oracle_stats(C.Structure):

@@ -93,7 +93,9 @@ template <class R> struct Integrator {
 	void renderPixel(int px, int py, int W, int H, uint32_t sampleIndex, const krr_camera &cam, V3<R> &pixel) {
 		const int pixelId = py * W + px;
 		Pcg sampler;
-		sampler.setSeed(interleave32((uint32_t) px, (uint32_t) py), (uint64_t) sampleIndex);
+		// view 0 is the reference's seed; view v > 0 starts its pixels at other
+		// points of the same PCG stream (initstate bits 32..63)
+		sampler.setSeed((uint64_t) interleave32((uint32_t) px, (uint32_t) py) | ((uint64_t) (uint32_t) P.view_id << 32), (uint64_t) sampleIndex);
 		sampler.advance((int64_t) (256 * pixelId));
 		const float ul		= sampler.get1D();
 		const float lambda0 = (1.f - ul) * 360.f + ul * 830.f;		  // SampledWavelengths::sampleUniform

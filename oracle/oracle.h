@@ -38,6 +38,7 @@ typedef struct oracle_params {
 	int32_t n_threads;			  /* 0 = all OpenMP threads */
 	int32_t row_begin, row_end, row_step; /* rendered rows; row_end 0 = height */
 	int32_t pixel_step;			  /* render every pixel_step-th pixel of a row (1 = all) */
+	int32_t view_id;			  /* multi-view seeding: bits 32..63 of the PCG initstate (0 = reference) */
 } oracle_params;
 
 typedef struct oracle_stats {

@@ -49,10 +49,12 @@ def test_pass_params_roundtrip(krr):
     assert d["nee"] is True and d["enable_medium"] is True and d["max_depth"] == 10
     assert abs(d["rr"] - 0.8) < 1e-7 and d["enable_clamp"] is False and d["clamp_max"] == 1000
     assert d["wavefront_depths"] == 0 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
+    assert d["occupancy"] == 1 and d["graph"] is True
 
 
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',
-                                 '{"stack_lds_entries": -1}', '{"stack_lds_entries": 97}', '{"wavefront_depths": -1}'])
+                                 '{"stack_lds_entries": -1}', '{"stack_lds_entries": 97}', '{"wavefront_depths": -1}',
+                                 '{"occupancy": 0}', '{"occupancy": 1.5}'])
 def test_bad_params_are_reported(krr, bad):
     with pytest.raises(krr.KrrError) as e:
         krr.WavefrontPathTracer(bad)
@@ -122,3 +124,20 @@ def test_no_device_is_reported_not_crashed(krr):
     with pytest.raises(krr.KrrError) as e:
         krr.Accel(s.desc)
     assert e.value.code == -8
+
+
+def test_views_argument_errors(krr):
+    """set_views / render_views / autotune reject bad arguments before any
+    device work."""
+    import ctypes as C
+    lib = krr.load_library()
+    wf = krr.WavefrontPathTracer(None)
+    assert lib.krr_wf_set_views(wf._h, None, 0) == krr._abi.KRR_ERR_INVALID_ARG
+    v = (krr.krr_view * 1)()
+    v[0].width, v[0].height, v[0].film_dev = 0, 4, 16
+    assert lib.krr_wf_set_views(wf._h, v, 1) == krr._abi.KRR_ERR_INVALID_ARG
+    v[0].width, v[0].film_dev = 4, None
+    assert lib.krr_wf_set_views(wf._h, v, 1) == krr._abi.KRR_ERR_INVALID_ARG
+    assert lib.krr_wf_render_views(wf._h, 1, None) == krr._abi.KRR_ERR_NO_SCENE
+    assert lib.krr_wf_autotune(wf._h, 0, None, 0, None) == krr._abi.KRR_ERR_NO_SCENE
+    wf.close()
