@@ -77,6 +77,12 @@ def gpu_bench(cfg, steps, warmup, traffic):
     film = hip.DeviceBuffer(w * h * 16)
     stream = hip.Stream()
     cam = scene.camera(0)
+    # launch shape (block size, LDS stack depth, occupancy) from the autotuner
+    wf.begin_frame(cam, 1)
+    t0 = time.time()
+    tune = wf.autotune(stream=stream.handle)
+    log(f"autotune {time.time() - t0:.2f}s: best {tune['best']}")
+    prm = wf.to_json()
 
     def frame(idx):
         wf.begin_frame(cam, idx)
@@ -113,7 +119,8 @@ def gpu_bench(cfg, steps, warmup, traffic):
     log(f"timed {steps} frames: wall {wall * 1e3:.1f} ms, events {gpu_ms:.1f} ms")
     # Node / triangle counts of exactly these rays: same frame indices again,
     # with the device counters on (outside the timed region).
-    wfc = krr.WavefrontPathTracer(dict(cfg["params"], count_traversal=True))
+    wfc = krr.WavefrontPathTracer(dict(cfg["params"], count_traversal=True,
+                                       **{k: prm[k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")}))
     wfc.set_scene(scene)
     wfc.resize(w, h)
     keys = ("closest_rays", "shadow_rays", "node_visits", "tri_tests", "shadow_node_visits", "shadow_tri_tests",
@@ -173,6 +180,7 @@ def gpu_bench(cfg, steps, warmup, traffic):
         "kernels": kernels,
         "roofline": roofline,
         "relmse": relmse,
+        "autotune": {"best": tune["best"], "candidates": len(tune["candidates"]), "trial_paths": tune["trial_paths"]},
         "_scene": scene,
     }
     return out
@@ -310,6 +318,114 @@ def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc
             "single_frame_relmse_ratio": round(float(np.median(sp)) / float(np.median(sn)), 3)}
 
 
+MULTIVIEW = dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
+                        "views": {"count": 256, "seed": 5, "radius": 2.0, "center": [0.5, 0.5, 0.5],
+                                  "sizes": [128, 256, 512, 1024]}},
+                 params={"spp": 2, "max_depth": 12, "rr": 0.8, "rr_min_depth": 3},
+                 desc="icosphere_field_1.31Mtri_256views_128..1024sq_2spp_depth12")
+
+
+def multiview_bench(frames=4, warmup=1, ref_frames=16, nrc_warm=8, nrc_params=None):
+    """256 cameras on a radius-2 sphere around the icosphere field, square
+    views of 128..1024 pixels, all rendered per frame by one persistent
+    launch per sample (krr_wf_render_views), replayed from a recorded graph,
+    launch shape from the autotuner.  Aggregate Mrays/s with the radiance
+    cache off and on, and per-view relMSE of the last frame against a plain
+    reference of ref_frames x 2 spp."""
+    import numpy as np
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    cfg = MULTIVIEW
+    t0 = time.time()
+    scene = krr.Scene(cfg["scene"])
+    nv = scene.n_views
+    sizes = [scene.view_size(v) for v in range(nv)]
+    offs = np.concatenate([[0], np.cumsum([w * h for w, h in sizes])]).astype(np.int64)
+    total = int(offs[-1])
+    log(f"multiview: {nv} views, {total} pixels, scene {time.time() - t0:.1f}s")
+    films = hip.DeviceBuffer(total * 16)  # all views' films back to back
+    acc = hip.DeviceBuffer(total * 32)
+    ref = hip.DeviceBuffer(total * 16)
+    stream = hip.Stream()
+    views = [(scene.camera(v), w, h, films.ptr + int(offs[v]) * 16) for v, (w, h) in enumerate(sizes)]
+
+    def make(params):
+        wf = krr.WavefrontPathTracer(params)
+        t1 = time.time()
+        wf.set_scene(scene)
+        wf.set_views(views)
+        return wf, time.time() - t1
+
+    def timed(wf, first, count):
+        rays = paths = 0
+        stream.synchronize()
+        w0 = time.perf_counter()
+        per = []
+        for i in range(count):
+            t1 = time.perf_counter()
+            wf.render_views(first + i, stream.handle)
+            st = wf.stats()  # synchronises the stream
+            per.append(time.perf_counter() - t1)
+            rays += st["closest_rays"] + st["shadow_rays"]
+            paths += st["paths"]
+        return time.perf_counter() - w0, rays, paths, per, st
+
+    def per_view_relmse():
+        out = []
+        for v, (w, h) in enumerate(sizes):
+            o = int(offs[v]) * 16
+            out.append(krr.error_metric(3, films.ptr + o, ref.ptr + o, w * h, stream.handle))
+        a = np.array(out)
+        return {"mean": round(float(a.mean()), 5), "median": round(float(np.median(a)), 5),
+                "max": round(float(a.max()), 5), "min": round(float(a.min()), 5)}
+
+    base = dict(cfg["params"])
+    wf, t_set = make(base)
+    t1 = time.time()
+    tune = wf.autotune(stream=stream.handle)
+    t_tune = time.time() - t1
+    log(f"multiview: set_scene+views {t_set:.1f}s, autotune {t_tune:.2f}s best {tune['best']}")
+    # reference: plain frames 100001.., accumulated
+    for i in range(ref_frames):
+        wf.render_views(100001 + i, stream.handle)
+        krr.accumulate(acc.ptr, films.ptr, total, i, 0, 0, 64, stream.handle)
+    stream.synchronize()
+    hip.check(hip.runtime().hipMemcpy(C_void(ref.ptr), C_void(films.ptr), total * 16, 3))
+    for i in range(warmup):
+        wf.render_views(1000 + i, stream.handle)
+    secs, rays, paths, per, st = timed(wf, 1, frames)
+    plain_rel = per_view_relmse()
+    eager, _ = make(dict(base, graph=False, **{k: tune["best"][k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")}))
+    eager.render_views(1000, stream.handle)
+    e_secs, e_rays, _, _, _ = timed(eager, 1, 2)
+    eager.close()
+    plain = {"value": round(rays / secs / 1e6, 2), "ms_per_frame": round(secs * 1e3 / frames, 2),
+             "mpaths_per_s": round(paths / secs / 1e6, 2), "rays_per_frame": rays // frames,
+             "eager_ms_per_frame": round(e_secs * 1e3 / 2, 2), "relmse_per_view": plain_rel}
+    prm = wf.to_json()
+    wf.close()
+    log(f"multiview plain: {plain}")
+    nrc_params = dict(NRC_PARAMS, **(nrc_params or {}))
+    wn, _ = make(dict(base, nrc=nrc_params, **{k: tune["best"][k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")}))
+    for i in range(nrc_warm):
+        wn.render_views(2000 + i, stream.handle)
+    n_secs, n_rays, n_paths, _, nst = timed(wn, 1 + nrc_warm, frames)
+    nrc = {"value": round(n_rays / n_secs / 1e6, 2), "ms_per_frame": round(n_secs * 1e3 / frames, 2),
+           "mpaths_per_s": round(n_paths / n_secs / 1e6, 2), "rays_per_frame": n_rays // frames,
+           "relmse_per_view": per_view_relmse(), "queries_per_frame": int(nst["nrc_queries"]),
+           "loss": round(nst["nrc_loss"], 5), "warm_frames": nrc_warm, "nrc_params": nrc_params}
+    wn.close()
+    log(f"multiview nrc: {nrc}")
+    return {"workload": cfg["desc"], "views": nv, "pixels": total,
+            "sizes": {str(s_): sum(1 for w, h in sizes if w == s_) for s_ in sorted(set(w for w, h in sizes))},
+            "params": base, "graph": True, "autotune": {"seconds": round(t_tune, 2), "trial_paths": tune["trial_paths"],
+                                                        "best": tune["best"], "candidates": len(tune["candidates"])},
+            "launch": {k: prm[k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")},
+            "reference": f"plain tracer, {ref_frames * base['spp']} spp per pixel",
+            "plain": plain, "nrc": nrc}
+
+
 def C_void(p):
     import ctypes
     return ctypes.c_void_p(p)
@@ -368,6 +484,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
+    ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -395,7 +512,7 @@ def main():
             "mpaths_per_s": r["mpaths_per_s"], "closest_mrays_per_s": r["closest_mrays_per_s"],
             "rays_per_frame": r["rays_per_frame"], "kernel_ms_per_step": r["kernel_ms_per_step"],
             "kernels": r["kernels"],
-            "relmse": r["relmse"],
+            "relmse": r["relmse"], "autotune": r["autotune"],
             "roofline": r["roofline"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, scene, budget_s=args.cpu_budget)
@@ -406,6 +523,8 @@ def main():
         line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg),
                                 "cornell_box_1024": nrc_equal_time(CONFIGS["cornell_box_1024"],
                                                                    nrc_params={"steps_per_frame": 2, "batch": 32768})}
+    if args.config == "icosphere_field_1080p" and not args.no_multiview:
+        line["multiview"] = multiview_bench()
     print(json.dumps(line), flush=True)
 
 
