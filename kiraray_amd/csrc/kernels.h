@@ -61,8 +61,14 @@ struct Counters {
 	// Counters that take an atomic per wave during a launch, one 128-byte
 	// line each, so that their atomics do not serialise against each other.
 	alignas(128) int work[4]; // chunk counters of the persistent kernels: 0 closest, 1 shadow, 2 megakernel
-	alignas(128) int nrcQueries; // radiance-cache queries of the current sample
-	alignas(128) int nrcTrain;	 // training vertices pushed this frame (may exceed the batch)
+	// radiance-cache queries of the current sample and training vertices
+	// pushed this frame (may exceed the batch): adjacent, so that one 64-bit
+	// atomic reserves slots in both
+	alignas(128) int nrcQueries;
+	int nrcTrain;
+	alignas(128) int nrcCandidates; // training-path vertices seen this frame (kept or not)
+	int nrcPrevCandidates;			// ... in the previous frame
+	float nrcKeep;					// probability of keeping a candidate this frame
 };
 
 // Neural radiance cache queues of the path tracer.  A query is 5 float4:
@@ -71,10 +77,14 @@ struct Counters {
 struct NrcDev {
 	float4 *query;
 	float4 *train;
-	int *queryCount, *trainCount; // Counters::nrcQueries, Counters::nrcTrain
+	int *queryCount, *trainCount; // Counters::nrcQueries, Counters::nrcTrain (adjacent, 8-byte aligned)
+	int *candCount;				  // Counters::nrcCandidates
+	const float *keep;			  // Counters::nrcKeep
+	float keepProb;				  // *keep, loaded once per launch
 	int capQuery, capTrain;
 	int trainMod;	// pixel trains when trainHash(pixel) % trainMod == phase
 	int phase;		// frame index % trainMod
+	float spread;	// path-spread threshold c of the termination rule (0: first vertex after a bounce)
 };
 
 KRR_DEV uint32_t trainHash(uint32_t pixel) {
@@ -144,6 +154,30 @@ KRR_DEV int wavePush(int *counter, bool want) {
 	return base + __popcll(below);
 }
 
+// Two wave-aggregated pushes with one 64-bit atomic on the pair of int
+// counters {a, b} at `pair` (a in the low word): ia / ib are the slots of the
+// lanes that want them, -1 elsewhere.
+KRR_DEV void wavePush2(int *pair, bool wantA, bool wantB, int &ia, int &ib) {
+	const uint64_t ma = __ballot(wantA), mb = __ballot(wantB);
+	ia = ib = -1;
+	if ((ma | mb) == 0) return;
+	const int leader			 = __ffsll((unsigned long long) (ma | mb)) - 1;
+	unsigned long long base = 0;
+	if (laneId() == leader)
+		base = atomicAdd(reinterpret_cast<unsigned long long *>(pair),
+						 (unsigned long long) __popcll(ma) | ((unsigned long long) __popcll(mb) << 32));
+	base				 = __shfl(base, leader);
+	const uint64_t below = laneId() == 0 ? 0ull : (~0ull >> (64 - laneId()));
+	if (wantA) ia = (int) (uint32_t) base + __popcll(ma & below);
+	if (wantB) ib = (int) (uint32_t) (base >> 32) + __popcll(mb & below);
+}
+
+// Wave-aggregated count (no slot returned, so the atomic is not waited for).
+KRR_DEV void waveCount(int *counter, bool want) {
+	const uint64_t mask = __ballot(want);
+	if (mask != 0 && laneId() == __ffsll((unsigned long long) mask) - 1) atomicAdd(counter, __popcll(mask));
+}
+
 // Persistent-kernel work distribution: each wave takes the next 64 items of
 // the queue with one atomic (wave-uniform result).
 KRR_DEV int waveFetch(int *work) {
@@ -172,6 +206,10 @@ struct RayItem {
 	f3 thp, pu, pl, cp, cn;
 	int depth, bsdfType, medium;
 	int pixel;
+	// path spread of the radiance-cache termination rule (registers only,
+	// not queued): primary footprint a0, running sum of the secondary
+	// footprints' square roots, solid-angle pdf of the last bounce (0 = delta)
+	float sp0, spSum, pdfPrev;
 };
 
 KRR_DEV RayItem loadRayItem(const RayQueueDev &q, int i) {
@@ -190,6 +228,9 @@ KRR_DEV RayItem loadRayItem(const RayQueueDev &q, int i) {
 	r.depth	   = metaDepth(m);
 	r.bsdfType = metaBsdf(m);
 	r.medium   = metaMedium(m);
+	r.sp0	   = 0.f;
+	r.spSum	   = 0.f;
+	r.pdfPrev  = 0.f;
 	return r;
 }
 

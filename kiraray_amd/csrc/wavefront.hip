@@ -113,6 +113,9 @@ KRR_DEV RayItem cameraRay(const CameraDev &cam, int W, int H, int x, int y, Pcg 
 	r.bsdfType = BSDF_UNSET;
 	r.medium   = cam.medium;
 	r.pixel	   = 0;
+	r.sp0	   = 0.f;
+	r.spSum	   = 0.f;
+	r.pdfPrev  = 0.f;
 	return r;
 }
 
@@ -381,15 +384,23 @@ KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float
 	out[3] = make_float4(s1, s2, 0.f, 0.f);
 }
 
-// NRC (template flag) in the shading stage, before generateScatterRays:
-// a render path ends at the first surface vertex reached through a
-// non-specular bounce (depth >= 1) and queues a cache query there (the
-// vertex's own emission is already added); a training path (pixel chosen by
-// trainHash) records every scattering vertex with its throughput and the
-// radiance accumulated so far, and continues unbiased.
+// NRC (template flag) in the shading stage, before generateScatterRays.
+// A render path ends at the first surface vertex (depth >= 1) where the
+// path's spread has grown past c * a0 (Mueller et al. 2021, sec. 5: a0 =
+// |x0 - x1|^2 / (4 pi |cos1|), spread = (sum_i>=2 sqrt(|x_i-1 - x_i|^2 /
+// (pdf_i-1 |cos_i|)))^2, delta bounces add nothing): the vertex's emission is
+// added, its light sample (NEE) is traced as usual, and a cache query stands
+// for the rest - the radiance the BSDF-sampled continuation brings.  A
+// training path (pixel chosen by trainHash) offers every scattering vertex;
+// a uniform hash keeps each with probability Counters::nrcKeep, so the kept
+// vertices spread over the whole frame.  A kept vertex records its throughput
+// now and the radiance gathered up to and including its own NEE afterwards
+// (*trainSlot, written by the caller after the shadow ray), and the path
+// continues unbiased.
 template <bool MEDIA, bool NRC = false>
 KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, const PixelStateDev &px, int loopDepth,
-					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so, const NrcDev *nrc = nullptr, bool trainPath = false) {
+					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so, const NrcDev *nrc = nullptr, bool trainPath = false,
+					   int *trainSlot = nullptr) {
 	const int pixel = r.pixel;
 	f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
 	int &sMedium = so.medium;
@@ -514,14 +525,25 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 		L = L + thp * Lm;
 	}
 
+	bool nrcEnd = false; // this vertex ends the path in the cache
+	float sp0 = r.sp0, spSum = r.spSum;
 	if (NRC) {
-		const bool live	  = hasScatter && loopDepth != P.maxDepth;
-		const bool wantQ  = live && !trainPath && r.depth >= 1 && !(r.bsdfType & BSDF_DELTA);
-		const bool wantT  = live && trainPath;
-		const int qi	  = wavePush(nrc->queryCount, wantQ);
-		const int ti	  = wavePush(nrc->trainCount, wantT);
+		const bool live = hasScatter && loopDepth != P.maxDepth;
+		if (live) { // path spread at this vertex
+			const float t  = h.x;
+			const float cs = fmaxf(fabsf(dot(intr.n, r.d)), 1e-4f);
+			if (r.depth == 0) sp0 = t * t / (4.f * kPi * cs);
+			else if (r.pdfPrev > 0.f) spSum += sqrtf(t * t / (r.pdfPrev * cs));
+		}
+		const bool wantQ = live && !trainPath && r.depth >= 1 && spSum * spSum > nrc->spread * sp0;
+		const bool cand	 = live && trainPath;
+		uint32_t hk		 = trainHash((uint32_t) pixel ^ ((uint32_t) r.depth * 0x9E3779B9u) ^ (uint32_t) (P.inc >> 1) * 0x85EBCA6Bu);
+		const bool wantT = cand && (float) (hk >> 8) * (1.f / 16777216.f) < nrc->keepProb;
+		waveCount(nrc->candCount, cand);
+		int qi, ti;
+		wavePush2(nrc->queryCount, wantQ, wantT, qi, ti);
 		if (wantQ) {
-			hasScatter = false; // the cache stands for the rest of the path
+			nrcEnd = true;
 			if (qi < nrc->capQuery) {
 				float4 *q = nrc->query + (size_t) qi * 5;
 				nrcVertex(sc, intr, q);
@@ -532,7 +554,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			float4 *t = nrc->train + (size_t) ti * 6;
 			nrcVertex(sc, intr, t);
 			t[4] = make_float4(thp.x, thp.y, thp.z, __int_as_float(pixel));
-			t[5] = make_float4(L.x, L.y, L.z, 0.f);
+			*trainSlot = ti;
 		}
 	}
 
@@ -588,7 +610,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 		}
 		if (hasScatter) { // generateScatterRays
 			bool alive = true;
-			if (r.depth >= P.rrMinDepth) {
+			if (r.depth >= P.rrMinDepth && !nrcEnd) {
 				if (rng.get1D() >= P.rr) alive = false;
 				else thp = thp / P.rr;
 			}
@@ -623,8 +645,8 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 						}
 					}
 				}
-				const BSDFSample s = bsdf.sample(woL, rng);
-				if (s.pdf != 0.f && s.f.any()) {
+				const BSDFSample s = nrcEnd ? BSDFSample{} : bsdf.sample(woL, rng);
+				if (!nrcEnd && s.pdf != 0.f && s.f.any()) {
 					const f3 wiW  = intr.toWorld(s.wi);
 					const f3 nthp = thp * s.f * fabsf(s.wi.z) / s.pdf;
 					if (nthp.any()) {
@@ -640,6 +662,9 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 						nx.depth	= r.depth + 1;
 						nx.thp		= nthp;
 						nx.pixel	= pixel;
+						nx.sp0		= sp0;
+						nx.spSum	= spSum;
+						nx.pdfPrev	= (s.flags & BSDF_DELTA) ? 0.f : s.pdf;
 					}
 				}
 			}
@@ -852,6 +877,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		P.inc = V.frame->inc;
 		if (NRC) nrc.phase = V.frame->phase;
 	}
+	if (NRC) nrc.keepProb = *nrc.keep;
 	const int count = VIEWS ? P.N : *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
 	bool active = false, exhausted = false, trainPath = false;
@@ -877,13 +903,18 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		RayItem nx;
 		ShadowOut so;
 		so.medium = -1;
-		shadeCore<MEDIA, NRC>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath);
+		int trainSlot = -1;
+		shadeCore<MEDIA, NRC>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
 		if (wantShadow) {
 			nShadow++;
 			if (MEDIA && P.enableMedium)
 				transmittanceItem<COUNT>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
 			else
 				occlusionItem<COUNT>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+		}
+		if (NRC && trainSlot >= 0) { // radiance gathered up to and including this vertex's NEE
+			const float4 Lv = px.L[r.pixel];
+			nrc.train[(size_t) trainSlot * 6 + 5] = make_float4(Lv.x, Lv.y, Lv.z, 0.f);
 		}
 		if (wantNext && depth < P.maxDepth) {
 			r = nx;
@@ -918,6 +949,15 @@ __global__ void k_nrc_targets(NrcDev nrc, const int *donePtr, const float4 *L, f
 		batchT[3 * e + 1]	= pay.y > 0.f ? fmaxf(0.f, (Lf.y - li.y) / pay.y) : 0.f;
 		batchT[3 * e + 2]	= pay.z > 0.f ? fmaxf(0.f, (Lf.z - li.z) / pay.z) : 0.f;
 	}
+}
+
+// Keep probability of this frame's training candidates from the previous
+// frame's count, so that about capTrain vertices are kept, spread uniformly.
+__global__ void k_nrc_begin(Counters *cnt, int capTrain) {
+	const int prev		   = cnt->nrcCandidates;
+	cnt->nrcPrevCandidates = prev;
+	cnt->nrcKeep		   = prev > 0 ? fminf(1.f, 0.95f * (float) capTrain / (float) prev) : 1.f;
+	cnt->nrcCandidates	   = 0;
 }
 
 __global__ void k_nrc_mark_done(Counters *cnt, int capTrain, int capQuery) {
@@ -1131,6 +1171,7 @@ struct krr_wf {
 	// neural radiance cache ("nrc" params)
 	bool nrcOn = false;
 	int nrcTrainMod = 16, nrcBatch = 65536, nrcSteps = 4;
+	float nrcSpread = 0.01f;
 	std::string nrcConfig; // JSON for krr_nrc_create
 	krr_nrc *nrc = nullptr;
 	DevBuf nrcQuery, nrcTrainBuf, nrcBatchV, nrcBatchT;
@@ -1239,9 +1280,11 @@ int parseParams(krr_wf *wf, const char *json) {
 				wf->nrcTrainMod = (int) n.value("train_mod", 16.0);
 				wf->nrcBatch	= (int) n.value("batch", 65536.0);
 				wf->nrcSteps	= (int) n.value("steps_per_frame", 4.0);
+				wf->nrcSpread	= (float) n.value("spread", 0.01);
 				Json c			= Json::object();
 				for (const auto &kv : n.members())
-					if (kv.first != "enable" && kv.first != "train_mod" && kv.first != "batch" && kv.first != "steps_per_frame")
+					if (kv.first != "enable" && kv.first != "train_mod" && kv.first != "batch" && kv.first != "steps_per_frame" &&
+						kv.first != "spread")
 						c.set(kv.first, kv.second);
 				c.set("max_batch", Json((double) wf->nrcBatch));
 				wf->nrcConfig = c.dump();
@@ -1265,6 +1308,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
+	if (!(wf->nrcSpread >= 0.f && wf->nrcSpread <= 1e6f)) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.spread must be >= 0");
 	if (!(wf->sahNodeCost >= 0.f && wf->sahNodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	if (!(wf->occupancy > 0.f && wf->occupancy <= 1.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "occupancy must be in (0, 1]");
 	return KRR_OK;
@@ -1503,7 +1547,10 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(wf->pixel.alloc(N * sizeof(float4)));
 	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
 	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
-	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+	if (!wf->counters.p) {
+		KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+		KRR_HIP(hipMemset(wf->counters.p, 0, sizeof(Counters)));
+	}
 	return ensureNrcBuffers(wf, wf->nrcQuery, N);
 }
 
@@ -1662,6 +1709,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 		n.set("train_mod", Json((double) wf->nrcTrainMod));
 		n.set("batch", Json((double) wf->nrcBatch));
 		n.set("steps_per_frame", Json((double) wf->nrcSteps));
+		n.set("spread", Json((double) wf->nrcSpread));
 		j.set("nrc", n);
 	} else j.set("nrc", Json(false));
 	std::string s = j.dump();
@@ -1776,6 +1824,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	if (wf->nrcOn && (media || wf->wavefrontDepths != 0))
 		return krr::fail(KRR_ERR_UNSUPPORTED, "nrc: needs a scene without media and wavefront_depths = 0");
 	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
+	if (wf->nrcOn) hipLaunchKernelGGL(k_nrc_begin, dim3(1), dim3(1), 0, st, cnt, wf->nrcBatch);
 	if (wf->beginPending) {
 		hipLaunchKernelGGL(k_begin_frame, dim3(gN), dim3(bs), 0, st, P, px);
 		wf->beginPending = false;
@@ -1786,6 +1835,9 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		nrc.train	   = wf->nrcTrainBuf.as<float4>();
 		nrc.queryCount = &cnt->nrcQueries;
 		nrc.trainCount = &cnt->nrcTrain;
+		nrc.candCount  = &cnt->nrcCandidates;
+		nrc.keep	   = &cnt->nrcKeep;
+		nrc.spread	   = wf->nrcSpread;
 		nrc.capQuery   = N;
 		nrc.capTrain   = wf->nrcBatch;
 		nrc.trainMod   = wf->nrcTrainMod;
@@ -1944,6 +1996,9 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 		nrc.train	   = wf->nrcTrainBuf.as<float4>();
 		nrc.queryCount = &cnt->nrcQueries;
 		nrc.trainCount = &cnt->nrcTrain;
+		nrc.candCount  = &cnt->nrcCandidates;
+		nrc.keep	   = &cnt->nrcKeep;
+		nrc.spread	   = wf->nrcSpread;
 		nrc.capQuery   = P.N;
 		nrc.capTrain   = wf->nrcBatch;
 		nrc.trainMod   = wf->nrcTrainMod; // phase comes from the frame values
@@ -1958,6 +2013,7 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 		wf->spans.push_back({kind, b, wf->mark(st)});
 	};
 	hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, cnt);
+	if (L.nrc) hipLaunchKernelGGL(k_nrc_begin, dim3(1), dim3(1), 0, st, cnt, wf->nrcBatch);
 	hipLaunchKernelGGL(k_begin_views, dim3(gN), dim3(wf->blockSize), 0, st, P, V, px);
 	int rc = KRR_OK;
 	for (int s = 0; s < wf->spp; s++) {
@@ -2034,7 +2090,10 @@ int krr_wf_set_views(krr_wf *wf, const krr_view *views, int32_t n_views) {
 		KRR_HIP(wf->vRng.alloc(N * sizeof(uint64_t)));
 		KRR_HIP(wf->vChannel.alloc(N * sizeof(int)));
 	}
-	if (!wf->counters.p) KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+	if (!wf->counters.p) {
+		KRR_HIP(wf->counters.alloc(sizeof(Counters)));
+		KRR_HIP(hipMemset(wf->counters.p, 0, sizeof(Counters)));
+	}
 	if (int rc = ensureNrcBuffers(wf, wf->vNrcQuery, N)) return rc;
 	wf->views.assign(views, views + n_views);
 	wf->NV = (int) total;
