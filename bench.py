@@ -303,7 +303,10 @@ def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc
     t_plain = float(np.sum(tp))
     k = int(np.searchsorted(np.cumsum(tn), t_plain, side="right"))
     rel_n_equal = rn[min(max(k, 1), len(rn)) - 1]
-    # time the cache needs to reach plain's single-frame error, per frame
+    # wall time each needs to bring its running average to plain's final relMSE
+    target = float(rp[-1])
+    hit = [i for i, x in enumerate(rn) if x <= target]
+    t_nrc_reach = float(np.cumsum(tn)[hit[0]]) if hit else None
     return {"reference": f"plain tracer, {ref_frames * ref_spp} spp", "warm_frames": warm_frames, "frames": frames,
             "nrc_params": nrc_params,
             "plain": {"ms_per_frame": round(ms_p, 3), "relmse_single_frame": round(float(np.median(sp)), 5),
@@ -314,6 +317,10 @@ def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc
                     "kernel_ms_per_frame": round(stn["nrc_ms"], 3)},
             "equal_time": {"wall_s": round(t_plain, 4), "plain_relmse": round(float(rp[-1]), 5),
                            "nrc_frames": min(max(k, 1), len(rn)), "nrc_relmse": round(float(rel_n_equal), 5)},
+            "time_to_plain_relmse": {"relmse": round(target, 5), "plain_s": round(t_plain, 4),
+                                     "nrc_s": round(t_nrc_reach, 4) if t_nrc_reach else None,
+                                     "nrc_frames": hit[0] + 1 if hit else None,
+                                     "speedup": round(t_plain / t_nrc_reach, 3) if t_nrc_reach else None},
             "speed_ratio_per_frame": round(ms_p / ms_n, 3),
             "single_frame_relmse_ratio": round(float(np.median(sp)) / float(np.median(sn)), 3)}
 
