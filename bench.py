@@ -492,6 +492,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
+    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview"],
+                    help="nrc / multiview: run only that comparison and print its JSON (profiling runs)")
+    ap.add_argument("--nrc-warm", type=int, default=96, help="training frames before the NRC comparison")
+    ap.add_argument("--nrc-frames", type=int, default=24, help="compared frames of the NRC comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -504,6 +508,16 @@ def main():
               "and not launched (use --gpus 1)", file=sys.stderr)
         raise SystemExit(2)
     cfg = CONFIGS[args.config]
+    if args.section != "all":
+        from kiraray_amd import hip
+        hip.check(hip.runtime().hipSetDevice(0))
+        if args.section == "nrc":
+            out = {"icosphere_field_1080p": nrc_equal_time(CONFIGS["icosphere_field_1080p"], warm_frames=args.nrc_warm,
+                                                           frames=args.nrc_frames)}
+        else:
+            out = multiview_bench()
+        print(json.dumps(out), flush=True)
+        return
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"hbm_traffic_{args.config}.json")
     if os.path.exists(tpath):
@@ -527,7 +541,7 @@ def main():
         line["cpu_baseline"] = None
     if args.config == "icosphere_field_1080p" and not args.no_nrc:
         # the open workload, and a closed scene where paths are long
-        line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg),
+        line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg, warm_frames=args.nrc_warm, frames=args.nrc_frames),
                                 "cornell_box_1024": nrc_equal_time(CONFIGS["cornell_box_1024"],
                                                                    nrc_params={"steps_per_frame": 2, "batch": 32768})}
     if args.config == "icosphere_field_1080p" and not args.no_multiview:
