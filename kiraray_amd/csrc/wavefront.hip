@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "bvh_build.h"
 #include "errors.h"
 #include "json.h"
 #include "kernels.h"
@@ -1078,6 +1079,12 @@ struct DevBuf {
 		return src && bytes ? hipMemcpy(p, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
 	}
 	template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+	// takes ownership of a hipMalloc'd block
+	void adopt(void *q, size_t bytes) {
+		release();
+		p  = q;
+		sz = bytes;
+	}
 };
 
 struct AccelData {
@@ -1088,11 +1095,12 @@ struct AccelData {
 	int64_t nNodes = 0, nLeafTris = 0;
 	int maxDepth	= 0;
 	double buildSeconds = 0;
+	bool onDevice		= false; // built by the GPU builder
 	int ldsEntries() const { return std::min(stackSize, stackLds); }
 	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes}; }
 };
 
-int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCost) {
+int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCost, bool onDevice) {
 	if (!s || s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
 		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing triangle arrays");
 	if (s->n_triangles > (int64_t) 0x0fffffff) return krr::fail(KRR_ERR_CAPACITY, "scene: at most 2^28 triangles");
@@ -1109,6 +1117,27 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCo
 	BvhBuildOptions opt;
 	opt.maxLeafSize = leafSize;
 	opt.nodeCost	= nodeCost;
+	a.onDevice		= onDevice && s->n_triangles > 0;
+	if (a.onDevice) {
+		DeviceBvh d;
+		std::string err;
+		if (buildBvh4Device(s->positions, s->n_vertices, s->indices, s->n_triangles, nullFlag.data(), opt, d, err))
+			return krr::fail(KRR_ERR_HIP, "device BVH build: %s", err.c_str());
+		if ((size_t) d.nNodes * sizeof(Bvh4Node) >= ((size_t) 1 << 32)) {
+			(void) hipFree(d.nodes);
+			(void) hipFree(d.tris);
+			return krr::fail(KRR_ERR_CAPACITY, "BVH of %lld nodes exceeds the 4 GiB node array limit", (long long) d.nNodes);
+		}
+		a.nodes.adopt(d.nodes, (size_t) d.nNodes * sizeof(Bvh4Node));
+		a.tris.adopt(d.tris, (size_t) d.nTris * sizeof(LeafTri));
+		a.maxDepth	   = d.maxDepth;
+		a.stackSize	   = 3 * d.maxDepth + 4; // bvhStackSize
+		a.nNodes	   = d.nNodes;
+		a.nLeafTris	   = d.nTris;
+		a.buildSeconds = d.buildSeconds;
+		if (a.stackSize > 96) return krr::fail(KRR_ERR_CAPACITY, "BVH depth %d needs a %d-entry stack (max 96)", a.maxDepth, a.stackSize);
+		return KRR_OK;
+	}
 	buildBvh4(s->positions, s->indices, s->n_triangles, nullFlag.data(), opt, a.host);
 	// traversal addresses nodes with 32-bit byte offsets
 	if (a.host.nodes.size() * sizeof(Bvh4Node) >= ((size_t) 1 << 32))
@@ -1168,6 +1197,7 @@ struct krr_wf {
 	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 0;
 	int stackLdsEntries = kShortStack;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
+	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
 	// neural radiance cache ("nrc" params)
 	bool nrcOn = false;
 	int nrcTrainMod = 16, nrcBatch = 65536, nrcSteps = 4;
@@ -1293,6 +1323,9 @@ int parseParams(krr_wf *wf, const char *json) {
 		}
 		wf->sahNodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 		wf->occupancy		= (float) j.value("occupancy", 1.0);
+		const std::string bb = j.value("bvh_builder", "device");
+		if (bb != "device" && bb != "host") throw std::runtime_error("bvh_builder must be \"device\" or \"host\"");
+		wf->deviceBuilder = bb == "device";
 		wf->useGraph		= j.value("graph", true);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
@@ -1466,7 +1499,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	KRR_HIP(wf->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial)));
 	KRR_HIP(wf->lights.upload(lights.data(), lights.size() * sizeof(DevLight)));
 	KRR_HIP(wf->media.upload(media.data(), media.size() * sizeof(DevMedium)));
-	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost);
+	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost, wf->deviceBuilder);
 	if (rc) return rc;
 	DevScene &ds	= wf->ds;
 	ds.pos			= wf->pos.as<float>();
@@ -1581,10 +1614,14 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	*out = nullptr;
 	int leafSize = 4, stackLdsEntries = kShortStack;
 	float nodeCost = kSahNodeCost;
+	bool deviceBuilder = true;
 	if (json_opts && *json_opts) {
 		try {
 			Json j			= Json::parse(json_opts);
 			leafSize		= (int) j.value("leaf_size", 4.0);
+			const std::string b = j.value("builder", "device");
+			if (b != "device" && b != "host") throw std::runtime_error("builder must be \"device\" or \"host\"");
+			deviceBuilder = b == "device";
 			stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 			nodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 		} catch (const std::exception &e) {
@@ -1598,7 +1635,7 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	for (int m = 0; m < scene->n_meshes; m++)
 		if (scene->meshes[m].material >= scene->n_materials) return krr::fail(KRR_ERR_INVALID_ARG, "mesh %d material out of range", m);
 	auto a = std::make_unique<krr_accel>();
-	int rc = buildAccel(scene, a->a, leafSize, nodeCost);
+	int rc = buildAccel(scene, a->a, leafSize, nodeCost, deviceBuilder);
 	if (rc) return rc;
 	a->a.stackLds = stackLdsEntries;
 	KRR_HIP(a->counters.alloc(sizeof(Counters)));
@@ -1703,6 +1740,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	j.set("occupancy", Json((double) wf->occupancy));
+	j.set("bvh_builder", Json(std::string(wf->deviceBuilder ? "device" : "host")));
 	j.set("graph", Json(wf->useGraph));
 	if (wf->nrcOn) {
 		Json n = Json::object();
