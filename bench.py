@@ -73,6 +73,17 @@ def gpu_bench(cfg, steps, warmup, traffic):
     t0 = time.time()
     wf.set_scene(scene)
     log(f"set_scene (BVH build + upload): {time.time() - t0:.1f}s")
+    # BVH build time of both builders (the same tree; the device one is the default)
+    bvh = {"device_ms": round(wf.stats()["bvh_build_ms"], 2)}
+    wh = krr.WavefrontPathTracer(dict(cfg["params"], bvh_builder="host"))
+    wh.set_scene(scene)
+    bvh["host_ms"] = round(wh.stats()["bvh_build_ms"], 2)
+    wh.close()
+    wd = krr.WavefrontPathTracer(dict(cfg["params"], bvh_builder="device"))
+    wd.set_scene(scene)  # again, with the device code loaded and warm
+    bvh["device_warm_ms"] = round(wd.stats()["bvh_build_ms"], 2)
+    wd.close()
+    log(f"bvh build: {bvh}")
     wf.resize(w, h)
     film = hip.DeviceBuffer(w * h * 16)
     stream = hip.Stream()
@@ -181,6 +192,7 @@ def gpu_bench(cfg, steps, warmup, traffic):
         "roofline": roofline,
         "relmse": relmse,
         "autotune": {"best": tune["best"], "candidates": len(tune["candidates"]), "trial_paths": tune["trial_paths"]},
+        "bvh_build": bvh,
         "_scene": scene,
     }
     return out
@@ -533,7 +545,7 @@ def main():
             "mpaths_per_s": r["mpaths_per_s"], "closest_mrays_per_s": r["closest_mrays_per_s"],
             "rays_per_frame": r["rays_per_frame"], "kernel_ms_per_step": r["kernel_ms_per_step"],
             "kernels": r["kernels"],
-            "relmse": r["relmse"], "autotune": r["autotune"],
+            "relmse": r["relmse"], "autotune": r["autotune"], "bvh_build": r["bvh_build"],
             "roofline": r["roofline"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(cfg, scene, budget_s=args.cpu_budget)

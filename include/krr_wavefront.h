@@ -154,6 +154,7 @@ typedef struct krr_stats {
 	uint64_t nrc_train_vertices; /* training vertices recorded this frame (at most nrc.batch are used) */
 	uint64_t nrc_steps;			/* training steps taken so far */
 	double nrc_loss;			/* loss of the last training step (synchronises) */
+	double bvh_build_ms;		/* BVH build of the last set_scene (host clock; "bvh_builder" picks the builder) */
 } krr_stats;
 
 /* Closest hit record: t along the (unnormalised) ray direction, primitive

@@ -76,7 +76,7 @@ class krr_stats(C.Structure):
                 ("megakernel_node_visits", C.c_uint64),
                 ("megakernel_tri_tests", C.c_uint64),
                 ("nrc_ms", C.c_double), ("nrc_queries", C.c_uint64), ("nrc_train_vertices", C.c_uint64),
-                ("nrc_steps", C.c_uint64), ("nrc_loss", C.c_double)]
+                ("nrc_steps", C.c_uint64), ("nrc_loss", C.c_double), ("bvh_build_ms", C.c_double)]
 
 
 class krr_hit(C.Structure):

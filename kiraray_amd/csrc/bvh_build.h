@@ -17,6 +17,7 @@ struct DeviceBvh {
 	int maxDepth = 0, largeLevels = 0;
 	double seconds = 0;		 // whole call, including the upload of the triangle arrays
 	double buildSeconds = 0; // after the upload
+	double largeSeconds = 0, smallSeconds = 0, collapseSeconds = 0; // phases of buildSeconds
 };
 
 // pos: 3 floats per vertex, idx: 3 ints per triangle (validated by the

@@ -880,6 +880,7 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 		lCount.swap(nc);
 		lNode.swap(nn);
 	}
+	const auto tLarge = std::chrono::steady_clock::now();
 	// ---- small subtrees, one wave each; descendants numbered per subtree
 	for (SmallTask &t : small) {
 		t.nodeBase = nextNode;
@@ -899,6 +900,7 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 		BVHD_HIP(hipGetLastError());
 		BVHD_HIP(hipStreamSynchronize(st));
 	}
+	const auto tSmall = std::chrono::steady_clock::now();
 	// ---- collapse to 4-wide nodes, level by level
 	Buf lvl[2], inner, off, total, nodes4, tris;
 	BVHD_HIP(lvl[0].alloc(maxNodes2 * sizeof(int)));
@@ -952,6 +954,9 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 	const auto t1  = std::chrono::steady_clock::now();
 	out.seconds	   = std::chrono::duration<double>(t1 - t0).count();
 	out.buildSeconds = std::chrono::duration<double>(t1 - tUp).count();
+	out.largeSeconds = std::chrono::duration<double>(tLarge - tUp).count();
+	out.smallSeconds = std::chrono::duration<double>(tSmall - tLarge).count();
+	out.collapseSeconds = std::chrono::duration<double>(t1 - tSmall).count();
 	return 0;
 }
 

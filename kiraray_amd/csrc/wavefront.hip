@@ -2329,6 +2329,7 @@ int krr_wf_stats(krr_wf *wf, krr_stats *out) {
 	krr::clearError();
 	if (!wf || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_stats: null argument");
 	std::memset(out, 0, sizeof(*out));
+	out->bvh_build_ms = wf->accel.buildSeconds * 1e3;
 	if (!wf->counters.p) return KRR_OK;
 	KRR_HIP(hipStreamSynchronize(wf->lastStream));
 	Counters c;
