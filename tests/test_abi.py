@@ -49,12 +49,13 @@ def test_pass_params_roundtrip(krr):
     assert d["nee"] is True and d["enable_medium"] is True and d["max_depth"] == 10
     assert abs(d["rr"] - 0.8) < 1e-7 and d["enable_clamp"] is False and d["clamp_max"] == 1000
     assert d["wavefront_depths"] == 0 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
-    assert d["occupancy"] == 1 and d["graph"] is True
+    assert d["occupancy"] == 1 and d["graph"] is True and d["bvh_builder"] == "device"
 
 
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',
                                  '{"stack_lds_entries": -1}', '{"stack_lds_entries": 97}', '{"wavefront_depths": -1}',
-                                 '{"occupancy": 0}', '{"occupancy": 1.5}'])
+                                 '{"occupancy": 0}', '{"occupancy": 1.5}', '{"bvh_builder": "gpu"}',
+                                 '{"nrc": {"spread": -1}}'])
 def test_bad_params_are_reported(krr, bad):
     with pytest.raises(krr.KrrError) as e:
         krr.WavefrontPathTracer(bad)
@@ -141,3 +142,11 @@ def test_views_argument_errors(krr):
     assert lib.krr_wf_render_views(wf._h, 1, None) == krr._abi.KRR_ERR_NO_SCENE
     assert lib.krr_wf_autotune(wf._h, 0, None, 0, None) == krr._abi.KRR_ERR_NO_SCENE
     wf.close()
+
+
+def test_accel_builder_option_is_checked(krr):
+    """An unknown "builder" fails as a bad config before any device work."""
+    s = krr.Scene(CORNELL)
+    with pytest.raises(krr.KrrError) as e:
+        krr.Accel(s.desc, {"builder": "gpu"})
+    assert e.value.code == -2
