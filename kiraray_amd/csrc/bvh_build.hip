@@ -437,49 +437,55 @@ struct SmallTask {
 	int node, first, count, nodeBase; // descendants take nodeBase, nodeBase + 1, ...
 };
 
-// One wave builds the subtree of one task depth first.  Its pending right
-// children wait in stack[first .. first + count) (at most count - 1 of them).
-__global__ void __launch_bounds__(kThreads) k_small(const SmallTask *tasks, int nTasks, Prims P, int *refs, int *tmp, int4 *stack, Nodes2 N,
-													float nodeCost, int maxLeaf) {
-	__shared__ Bins wb[kThreads / 64];
-	const int w	   = threadIdx.x / 64, lane = __lane_id();
-	const int ti   = blockIdx.x * (kThreads / 64) + w;
-	if (ti >= nTasks) return; // whole waves leave together; no block-wide barrier follows
-	Bins &B			   = wb[w];
+// One wave (one 64-thread block) builds the subtree of one task depth first,
+// with the task's references, the partition scratch and the stack of pending
+// right children in LDS.  Per node: one pass bins the references, one pass
+// partitions them (the left count comes from the bins) and accumulates both
+// children's bounds; the left child is processed next with those bounds, a
+// popped right child first recomputes its own.
+__device__ inline void waveBox(Box &b) {
+	for (int a = 0; a < 3; a++) {
+		b.lo[a] = waveMin(b.lo[a]);
+		b.hi[a] = waveMax(b.hi[a]);
+	}
+}
+
+__global__ void __launch_bounds__(64) k_small(const SmallTask *tasks, int nTasks, Prims P, int *refs, Nodes2 N, float nodeCost, int maxLeaf) {
+	__shared__ int sRef[kSmall], sTmp[kSmall];
+	__shared__ int2 sStack[kSmall]; // {node, first << 11 | count}, first relative to the task
+	__shared__ Bins B;
+	const int ti = blockIdx.x, lane = threadIdx.x;
+	if (ti >= nTasks) return;
 	const SmallTask tk = tasks[ti];
-	int4 *stk		   = stack + tk.first;
-	int sp			   = 0;
-	int nextNode	   = tk.nodeBase;
-	int cur[3]		   = {tk.node, tk.first, tk.count};
+	for (int i = lane; i < tk.count; i += 64) sRef[i] = refs[tk.first + i];
+	__syncthreads();
+	int sp = 0, nextNode = tk.nodeBase;
+	int node = tk.node, first = 0, count = tk.count; // first relative to the task
+	bool haveBounds = false;
+	Box bb, cb;
 	while (true) {
-		const int node = cur[0], first = cur[1], count = cur[2];
-		// bounds and centroid bounds
-		Box bb, cb;
-		bb.reset();
-		cb.reset();
-		for (int i = first + lane; i < first + count; i += 64) {
-			Box pb;
-			float c[3];
-			primBox(P, refs[i], pb, c);
-			bb.grow(pb);
-			for (int a = 0; a < 3; a++) {
-				cb.lo[a] = fminf(cb.lo[a], c[a]);
-				cb.hi[a] = fmaxf(cb.hi[a], c[a]);
+		if (!haveBounds) { // bounds and centroid bounds of the node
+			bb.reset();
+			cb.reset();
+			for (int i = first + lane; i < first + count; i += 64) {
+				Box pb;
+				float c[3];
+				primBox(P, sRef[i], pb, c);
+				bb.grow(pb);
+				for (int a = 0; a < 3; a++) {
+					cb.lo[a] = fminf(cb.lo[a], c[a]);
+					cb.hi[a] = fmaxf(cb.hi[a], c[a]);
+				}
 			}
-		}
-		for (int a = 0; a < 3; a++) {
-			bb.lo[a] = waveMin(bb.lo[a]);
-			bb.hi[a] = waveMax(bb.hi[a]);
-			cb.lo[a] = waveMin(cb.lo[a]);
-			cb.hi[a] = waveMax(cb.hi[a]);
+			waveBox(bb);
+			waveBox(cb);
 		}
 		int d = 1;
 		Split s{-1, -1, 0.f};
+		float scale[3];
 		if (count > 1) {
 			binsReset(B, lane, 64);
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-			__builtin_amdgcn_wave_barrier();
-			float scale[3];
+			__syncthreads();
 			for (int a = 0; a < 3; a++) {
 				const float ext = cb.hi[a] - cb.lo[a];
 				scale[a]		= ext > 0 ? NB / ext : 0.f;
@@ -487,7 +493,7 @@ __global__ void __launch_bounds__(kThreads) k_small(const SmallTask *tasks, int 
 			for (int i = first + lane; i < first + count; i += 64) {
 				Box pb;
 				float c[3];
-				primBox(P, refs[i], pb, c);
+				primBox(P, sRef[i], pb, c);
 				for (int a = 0; a < 3; a++) {
 					if (!(scale[a] > 0)) continue;
 					const int k = binOf(c[a], cb.lo[a], scale[a]);
@@ -498,87 +504,82 @@ __global__ void __launch_bounds__(kThreads) k_small(const SmallTask *tasks, int 
 					}
 				}
 			}
-			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-			__builtin_amdgcn_wave_barrier();
+			__syncthreads();
 			s = chooseSplit(B, cb, bb.area(), nodeCost); // every lane, same result
 			d = decide(s, count, maxLeaf);
-			__builtin_amdgcn_wave_barrier(); // bins read before the next node resets them
 		}
 		if (d == 1) {
-			if (lane == 0) writeNode(N, node, bb, -1, -1, first, count);
-		} else {
-			// stable partition into tmp, then back into refs
-			const float ext	  = d == 0 ? cb.hi[s.axis] - cb.lo[s.axis] : 0.f;
-			const float scale = d == 0 ? NB / ext : 0.f;
-			int nLeft		  = 0;
-			for (int base = first; base < first + count; base += 64) {
-				const int i	 = base + lane;
-				const bool v = i < first + count;
-				bool left	 = false;
-				int r		 = 0;
-				if (v) {
-					r = refs[i];
-					if (d == 2) left = i < first + count / 2;
-					else {
-						Box pb;
-						float c[3];
-						primBox(P, r, pb, c);
-						left = binOf(c[s.axis], cb.lo[s.axis], scale) <= s.bin;
-					}
-				}
-				const uint64_t lm = __ballot(v && left);
-				const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-				if (v && left) tmp[first + nLeft + __popcll(lm & below)] = r;
-				nLeft += __popcll(lm);
-			}
-			// a binned split never leaves a side empty; guard anyway
-			const bool mid = nLeft == 0 || nLeft == count;
-			if (mid) nLeft = count / 2;
-			int nRight = 0;
-			for (int base = first; base < first + count; base += 64) {
-				const int i	 = base + lane;
-				const bool v = i < first + count;
-				bool left	 = false;
-				int r		 = 0;
-				if (v) {
-					r = refs[i];
-					if (mid || d == 2) left = i < first + count / 2;
-					else {
-						Box pb;
-						float c[3];
-						primBox(P, r, pb, c);
-						left = binOf(c[s.axis], cb.lo[s.axis], scale) <= s.bin;
-					}
-				}
-				const uint64_t rm	 = __ballot(v && !left);
-				const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-				if (mid && v && left) tmp[i] = r;
-				if (v && !left) tmp[first + nLeft + nRight + __popcll(rm & below)] = r;
-				nRight += __popcll(rm);
-			}
-			__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-			for (int i = first + lane; i < first + count; i += 64) refs[i] = tmp[i];
-			__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-			const int l = nextNode, rgt = nextNode + 1;
-			nextNode += 2;
-			if (lane == 0) {
-				writeNode(N, node, bb, l, rgt, 0, 0);
-				stk[sp] = make_int4(rgt, first + nLeft, count - nLeft, 0);
-			}
-			sp++;
-			cur[0] = l;
-			cur[1] = first;
-			cur[2] = nLeft;
+			if (lane == 0) writeNode(N, node, bb, -1, -1, tk.first + first, count);
+			__syncthreads(); // bins read before the next node resets them
+			if (sp == 0) break;
+			const int2 e = sStack[--sp];
+			node		 = e.x;
+			first		 = e.y >> 11;
+			count		 = e.y & 2047;
+			haveBounds	 = false;
 			continue;
 		}
-		if (sp == 0) break;
-		sp--;
-		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
-		const int4 e = stk[sp];
-		cur[0]		 = e.x;
-		cur[1]		 = e.y;
-		cur[2]		 = e.z;
+		// left count from the bins (binned split) or half (middle split)
+		int nLeft = count / 2;
+		if (d == 0) {
+			nLeft = 0;
+			for (int k = 0; k <= s.bin; k++) nLeft += B.cnt[s.axis][k];
+		}
+		__syncthreads(); // bins read before the next node resets them
+		// stable partition into sTmp, with the left child's bounds
+		Box lb, lc;
+		lb.reset();
+		lc.reset();
+		int nl = 0, nr = 0;
+		for (int base = first; base < first + count; base += 64) {
+			const int i	 = base + lane;
+			const bool v = i < first + count;
+			bool left	 = false;
+			int r		 = 0;
+			Box pb;
+			float c[3];
+			if (v) {
+				r = sRef[i];
+				primBox(P, r, pb, c);
+				left = d == 2 ? i < first + count / 2 : binOf(c[s.axis], cb.lo[s.axis], scale[s.axis]) <= s.bin;
+				if (left) {
+					lb.grow(pb);
+					for (int a = 0; a < 3; a++) {
+						lc.lo[a] = fminf(lc.lo[a], c[a]);
+						lc.hi[a] = fmaxf(lc.hi[a], c[a]);
+					}
+				}
+			}
+			const uint64_t lm = __ballot(v && left), rm = __ballot(v && !left);
+			const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+			const int slot = v ? (left ? first + nl + __popcll(lm & below) : first + nLeft + nr + __popcll(rm & below)) : -1;
+			if (slot >= first && slot < first + count) sTmp[slot] = r; // always in range when nl ends equal to nLeft
+			nl += __popcll(lm);
+			nr += __popcll(rm);
+		}
+		__syncthreads();
+		for (int i = first + lane; i < first + count; i += 64) sRef[i] = sTmp[i];
+		__syncthreads();
+		// nl == nLeft by construction; a side is never empty (binned splits
+		// skip empty sides, middle splits halve at least 9 references)
+		const int l = nextNode, rgt = nextNode + 1;
+		nextNode += 2;
+		if (lane == 0) {
+			writeNode(N, node, bb, l, rgt, 0, 0);
+			sStack[sp] = make_int2(rgt, ((first + nLeft) << 11) | (count - nLeft));
+		}
+		sp++;
+		__syncthreads();
+		waveBox(lb);
+		waveBox(lc);
+		node	   = l;
+		count	   = nLeft;
+		bb		   = lb;
+		cb		   = lc;
+		haveBounds = true;
 	}
+	__syncthreads();
+	for (int i = lane; i < tk.count; i += 64) refs[tk.first + i] = sRef[i];
 }
 
 // ------------------------------------------------------------- collapse
@@ -768,7 +769,7 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 		hipStream_t s;
 		~StreamGuard() { (void) hipStreamDestroy(s); }
 	} guard{st};
-	Buf dPos, dIdx, dNull, pLo, pHi, refs, tmp, stack, nLo, nHi, nRange;
+	Buf dPos, dIdx, dNull, pLo, pHi, refs, tmp, nLo, nHi, nRange;
 	BVHD_HIP(dPos.alloc((size_t) nVerts * 3 * sizeof(float)));
 	BVHD_HIP(dIdx.alloc((size_t) n * 3 * sizeof(int)));
 	BVHD_HIP(dNull.alloc((size_t) n));
@@ -782,7 +783,6 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 	BVHD_HIP(pHi.alloc((size_t) n * sizeof(float4)));
 	BVHD_HIP(refs.alloc((size_t) n * sizeof(int)));
 	BVHD_HIP(tmp.alloc((size_t) n * sizeof(int)));
-	BVHD_HIP(stack.alloc((size_t) n * sizeof(int4)));
 	const size_t maxNodes2 = 2 * (size_t) n;
 	BVHD_HIP(nLo.alloc(maxNodes2 * sizeof(float4)));
 	BVHD_HIP(nHi.alloc(maxNodes2 * sizeof(float4)));
@@ -893,10 +893,8 @@ int buildBvh4Device(const float *pos, int64_t nVerts, const int32_t *idx, int64_
 	if (!small.empty()) {
 		Buf bSmall;
 		BVHD_HIP(upload(bSmall, small, st));
-		const int wavesPerBlock = kThreads / 64;
-		hipLaunchKernelGGL(k_small, dim3(((int) small.size() + wavesPerBlock - 1) / wavesPerBlock), dim3(kThreads), 0, st,
-						   (const SmallTask *) bSmall.as<SmallTask>(), (int) small.size(), P, refs.as<int>(), tmp.as<int>(), stack.as<int4>(), N,
-						   cost, maxLeaf);
+		hipLaunchKernelGGL(k_small, dim3((unsigned) small.size()), dim3(64), 0, st, (const SmallTask *) bSmall.as<SmallTask>(),
+						   (int) small.size(), P, refs.as<int>(), N, cost, maxLeaf);
 		BVHD_HIP(hipGetLastError());
 		BVHD_HIP(hipStreamSynchronize(st));
 	}
