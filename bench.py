@@ -248,7 +248,7 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     return out
 
 
-NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 3e-3}
+NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 1e-3}
 
 
 def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc_params=None):
