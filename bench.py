@@ -31,7 +31,9 @@ CONFIGS = {
                         "resolution": [1920, 1080],
                         "camera": {"eye": [0.5, 0.75, 2.3], "target": [0.5, 0.3, 0.45]}},
                  params={"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3},
-                 desc="icosphere_field_1.31Mtri_1920x1080_1spp_depth16_rr3"),
+                 desc="icosphere_field_1.31Mtri_1920x1080_1spp_depth16_rr3",
+                 # FP64 reference of the same view at 1/8 resolution (a 1080p one would take hours of CPU time)
+                 reference_lowres=("tests/golden/icosphere_field_240x135_fp64_4096spp.npz", 240, 135)),
     "cornell_box_256": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
                         "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
                  params={"spp": 16, "max_depth": 8, "rr": 0.8},
@@ -160,6 +162,29 @@ def gpu_bench(cfg, steps, warmup, traffic):
         relmse = {"value": round(krr.error_metric(3, film.ptr, refb.ptr, w * h), 6),
                   "reference": f"{int(z['spp'])}-spp FP64 CPU oracle image ({cfg['reference']})",
                   "image": f"frame {steps}, {cfg['params']['spp']} spp"}
+    if cfg.get("reference_lowres"):
+        # per-image relMSE against the FP64 CPU oracle, at the reference's resolution
+        path, lw, lh = cfg["reference_lowres"]
+        z = np.load(os.path.join(ROOT, path))
+        ref = np.concatenate([z["image"], np.ones(z["image"].shape[:2] + (1,), np.float32)], -1)
+        refb = hip.DeviceBuffer.from_numpy(np.ascontiguousarray(ref, np.float32))
+        wl = krr.WavefrontPathTracer(dict(cfg["params"]))
+        wl.set_scene(scene)
+        wl.resize(lw, lh)
+        lf, la = hip.DeviceBuffer(lw * lh * 16), hip.DeviceBuffer(lw * lh * 32)
+        single = []
+        for f in range(16):
+            wl.begin_frame(cam, 1 + f)
+            wl.render(lf.ptr, stream.handle)
+            stream.synchronize()
+            single.append(krr.error_metric(3, lf.ptr, refb.ptr, lw * lh, stream.handle))
+            krr.accumulate(la.ptr, lf.ptr, lw * lh, f, 0, 0, 64, stream.handle)
+        stream.synchronize()
+        relmse = {"value": round(float(np.median(single)), 5),
+                  "reference": f"{int(z['spp'])}-spp FP64 CPU oracle image ({path})",
+                  "image": f"median over frames 1..16 of single {cfg['params']['spp']}-spp frames at {lw}x{lh} (the 1080p view at 1/8 resolution)",
+                  "running_16_frames": round(krr.error_metric(3, lf.ptr, refb.ptr, lw * lh, stream.handle), 5)}
+        wl.close()
     wfc.close()
     wf.close()
 

@@ -100,3 +100,63 @@ def test_gpu_accumulate_pass(krr, gpu, mode, precision, max_count):
             a = a + c
         exp = a if mode == 1 else a * w
         np.testing.assert_allclose(out, exp.astype(np.float32), rtol=2e-6, atol=1e-7)
+
+
+ICO_REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "icosphere_field_240x135_fp64_4096spp.npz")
+
+
+def icosphere_reference_rgba():
+    z = np.load(ICO_REF)
+    img = z["image"]
+    assert img.shape == (135, 240, 3) and int(z["spp"]) == 4096
+    return np.concatenate([img, np.ones((135, 240, 1), np.float32)], -1)
+
+
+def test_icosphere_reference_is_plausible():
+    """The FP64 4,096-spp image of the 1.31 M-triangle field: finite,
+    non-negative, lit (16 lights over a ground plane), darker than the
+    Cornell box; the black pixels are sky (no environment light), above
+    the horizon only."""
+    ref = icosphere_reference_rgba()
+    assert np.isfinite(ref).all() and (ref[..., :3] >= 0).all()
+    assert 0.01 < ref[..., :3].mean() < 0.2
+    lit = ref[..., :3].sum(-1) > 0
+    assert 0.5 < lit.mean() < 0.95
+    # pixel row y (0 = bottom of the view, ndc -1) is stored at row H-1-y:
+    # the ground in front of the camera fills the last stored rows, the sky
+    # the first ones
+    assert lit[-20:].mean() > 0.99 and lit[:20].mean() < 0.5
+
+
+@pytest.mark.gpu
+def test_gpu_icosphere_converges_to_fp64_reference(krr, oracle, gpu):
+    """The 1.31 M-triangle field at 240x135 (depth 16, RR from depth 3), one
+    frame of 16 and one of 64 spp: the GPU relMSE against the FP64 reference
+    equals the checker's (identical images), falls by at least a fifth from
+    16 to 64 spp, and the 64-spp mean is within 1 % of the reference's.  Glass
+    and conductor paths give the clamped metric a heavy tail, so it falls
+    slower than 1/spp here (the diffuse Cornell box shows 1/spp)."""
+    from conftest import icosphere_scene
+    print("case: icosphere 240x135 convergence to FP64", flush=True)
+    s = krr.Scene(icosphere_scene())
+    ref = icosphere_reference_rgba()
+    refb = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(ref, np.float32))
+    film = gpu.DeviceBuffer(240 * 135 * 16)
+    r, rc = {}, {}
+    for spp in (16, 64):
+        wf = krr.WavefrontPathTracer({"spp": spp, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3})
+        wf.set_scene(s)
+        wf.resize(240, 135)
+        wf.begin_frame(s.camera(0), 3)
+        wf.render(film.ptr, 0)
+        gpu.synchronize()
+        r[spp] = krr.error_metric(3, film.ptr, refb.ptr, 240 * 135)
+        img = film.to_numpy(np.float32, (135, 240, 4))
+        wf.close()
+        o, _ = oracle.render(s.desc, s.camera(0), 240, 135, 3, spp=spp, max_depth=16, rr=0.8, rr_min_depth=3)
+        rc[spp] = relmse_np(o, ref)
+    print(f"relMSE 16 spp {r[16]:.5f} (checker {rc[16]:.5f}), 64 spp {r[64]:.5f} (checker {rc[64]:.5f})")
+    for n in (16, 64):
+        assert abs(r[n] - rc[n]) <= 1e-6 * rc[n]
+    assert r[64] < 0.8 * r[16]
+    assert abs(img[..., :3].mean() - ref[..., :3].mean()) < 0.01 * ref[..., :3].mean()
