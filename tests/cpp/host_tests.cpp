@@ -8,12 +8,18 @@
 //                               writes the colour texture (float32 RGBA,
 //                               row-major as stored) to OUT and prints the
 //                               ray counts of the last frame
+//   host_tests measure F REF OUT IMG  Cornell 256x256 through the passes
+//                               [WavefrontPathTracer, AccumulatePass,
+//                               ErrorMeasurePass(REF .npy, every 4 frames,
+//                               saved to OUT)] for F frames; writes the
+//                               accumulated colour texture to IMG
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
 
+#include "passes.h"
 #include "render_app.h"
 #include "wavefront_pass.h"
 
@@ -39,7 +45,9 @@ static int testFactory() {
 	EXPECT(p != nullptr);
 	EXPECT(p && p->getName() == "WavefrontPathTracer");
 	EXPECT(p && p->enabled() && p->isCudaPass());
-	EXPECT(RenderPassFactory::createInstance("AccumulatePass") == nullptr);
+	EXPECT(RenderPassFactory::createInstance("AccumulatePass") != nullptr);
+	EXPECT(RenderPassFactory::createInstance("ErrorMeasurePass") != nullptr);
+	EXPECT(RenderPassFactory::createInstance("ToneMappingPass") == nullptr);
 	EXPECT(RenderPassFactory::deserizeInstance("NoSuchPass", json::object()) == nullptr);
 	auto q = RenderPassFactory::deserizeInstance("WavefrontPathTracer", json::parse(R"({"max_depth": 5, "nee": false})"));
 	auto *w = dynamic_cast<WavefrontPathTracer *>(q.get());
@@ -72,6 +80,34 @@ static int testParams() {
 		code = e.code();
 	}
 	EXPECT(code == KRR_ERR_BAD_CONFIG);
+	// AccumulatePass / ErrorMeasurePass: the reference's enum strings
+	AccumulatePass a;
+	from_json(json::parse(R"({"spp": 64, "mode": "moving average", "precision": "double"})"), a);
+	json ja;
+	to_json(ja, a);
+	EXPECT(ja.value("spp", 0.0) == 64.0 && ja.value("mode", "") == "moving average" && ja.value("precision", "") == "double");
+	from_json(json::parse(R"({"mode": 0, "precision": 1})"), a);
+	to_json(ja, a);
+	EXPECT(ja.value("mode", "") == "accumulate" && ja.value("precision", "") == "double" && ja.value("spp", 1.0) == 0.0);
+	ErrorMeasurePass e;
+	from_json(json::parse(R"({"metric": "mse", "continuous": true, "interval": 8})"), e);
+	json je;
+	to_json(je, e);
+	EXPECT(je.value("metric", "") == "mse" && je.value("interval", 0.0) == 8.0 && je.value("continuous", false));
+	code = 0;
+	try {
+		from_json(json::parse(R"({"interval": 0})"), e);
+	} catch (const Error &err) {
+		code = err.code();
+	}
+	EXPECT(code == KRR_ERR_BAD_CONFIG);
+	code = 0;
+	try {
+		from_json(json::parse(R"({"mode": 5})"), a);
+	} catch (const Error &err) {
+		code = err.code();
+	}
+	EXPECT(code == KRR_ERR_BAD_CONFIG);
 	return 0;
 }
 
@@ -82,7 +118,8 @@ static int testConfig() {
 		{"enable": true, "name": "AccumulatePass", "params": {"spp": 0, "mode": "accumulate"}},
 		{"enable": false, "name": "DenoisePass"}],
 		"scene": )") + kCornell + "}"));
-	EXPECT(app.getRenderPasses().size() == 1);
+	EXPECT(app.getRenderPasses().size() == 2);
+	EXPECT(app.getRenderPasses().size() == 2 && app.getRenderPasses()[1]->getName() == "AccumulatePass");
 	EXPECT(app.getFrameSize() == Vector2i(96, 64));
 	EXPECT(app.getScene() && app.getScene()->getDesc().n_triangles == 36);
 	auto *w = dynamic_cast<WavefrontPathTracer *>(app.getRenderPasses()[0].get());
@@ -118,6 +155,27 @@ static int testRender(int w, int h, int frames, const char *out) {
 	return 0;
 }
 
+static int testMeasure(int frames, const char *ref, const char *out, const char *imgOut) {
+	RenderApp app;
+	app.loadConfig(json::parse(std::string(R"({"resolution": [256, 256], "passes": [
+		{"enable": true, "name": "WavefrontPathTracer", "params": {"max_depth": 8, "spp": 1}},
+		{"enable": true, "name": "AccumulatePass", "params": {"spp": 0, "mode": "accumulate", "precision": "double"}},
+		{"enable": true, "name": "ErrorMeasurePass", "params": {"metric": "rel_mse", "continuous": true, "interval": 4,
+		 "save": true, "reference": ")") + ref + R"(", "output": ")" + out + R"("}}],
+		"scene": )" + kCornell + "}"));
+	app.initialize();
+	app.run(frames);
+	const std::vector<float> img = app.getRenderContext()->readColor();
+	FILE *f						  = std::fopen(imgOut, "wb");
+	if (!f) return 2;
+	std::fwrite(img.data(), sizeof(float), img.size(), f);
+	std::fclose(f);
+	auto *acc = dynamic_cast<AccumulatePass *>(app.getRenderPasses()[1].get());
+	EXPECT(acc && acc->getAccumCount() == (size_t) frames);
+	app.finalize(); // writes OUT
+	return 0;
+}
+
 int main(int argc, char **argv) {
 	if (argc < 2) {
 		std::fprintf(stderr, "usage: host_tests factory|params|config|render W H FRAMES OUT\n");
@@ -130,6 +188,7 @@ int main(int argc, char **argv) {
 		else if (cmd == "params") rc = testParams();
 		else if (cmd == "config") rc = testConfig();
 		else if (cmd == "render" && argc == 6) rc = testRender(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
+		else if (cmd == "measure" && argc == 6) rc = testMeasure(std::atoi(argv[2]), argv[3], argv[4], argv[5]);
 		else {
 			std::fprintf(stderr, "unknown command %s\n", cmd.c_str());
 			return 2;

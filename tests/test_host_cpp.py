@@ -48,3 +48,37 @@ def test_render_app_matches_checker(oracle, krr, tmp_path):
     d = np.abs(g[..., :3] - o[..., :3])
     assert (d <= 1e-4 * np.abs(o[..., :3]) + 1e-6).all(), d.max()
     assert np.all(g[..., 3] == 1.0)
+
+
+@pytest.mark.gpu
+def test_accumulate_and_error_measure_passes(oracle, krr, tmp_path):
+    """[WavefrontPathTracer, AccumulatePass (double), ErrorMeasurePass (relMSE
+    against the 65,536-spp FP64 Cornell image every 4 frames, saved)] through
+    RenderApp for 32 frames: the saved series has timesteps 4, 8, .., 32; the
+    accumulated image equals the checker's frames 1..32 averaged in double;
+    the last value equals the relMSE of that image; the error falls ~1/frames."""
+    print("case: RenderApp cornell 256 + AccumulatePass + ErrorMeasurePass, 32 frames", flush=True)
+    from test_relmse import reference_rgba, relmse_np
+    ref = reference_rgba()
+    ref_path = tmp_path / "ref.npy"
+    np.save(ref_path, np.ascontiguousarray(ref[..., :3], np.float32))
+    out = tmp_path / "error" / "series.json"
+    img_path = tmp_path / "accum.bin"
+    r = _run("measure", 32, ref_path, out, img_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    series = json.loads(out.read_text())
+    assert series["timesteps"] == [4, 8, 12, 16, 20, 24, 28, 32]
+    assert len(series["timepoints"]) == 8 and all(b >= a for a, b in zip(series["timepoints"], series["timepoints"][1:]))
+    vals = [d["RelMSE"] for d in series["data"]]
+    g = np.fromfile(img_path, np.float32).reshape(256, 256, 4)
+    s = krr.Scene(CORNELL)
+    acc = np.zeros((256, 256, 3), np.float64)
+    for f in range(1, 33):
+        o, _ = oracle.render(s.desc, s.camera(0), 256, 256, f, spp=1, max_depth=8)
+        acc += o[..., :3].astype(np.float64)
+    mean = (acc / 32).astype(np.float32)
+    assert np.array_equal(g[..., :3], mean)
+    rel = relmse_np(g, ref)
+    print("series", [round(v, 4) for v in vals], "host", rel)
+    assert abs(vals[-1] - rel) <= 1e-6 * rel
+    assert 4.0 < vals[0] / vals[-1] < 12.0  # 4 -> 32 frames: ~8x
