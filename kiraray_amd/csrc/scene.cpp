@@ -19,10 +19,21 @@
 // Views: {"count", "seed", "radius", "center", "sizes": [...]} creates many
 // cameras on a sphere looking at "center", each with a square resolution
 // drawn from "sizes".
+//
+// KiRaRay's own scene JSON (SceneImporter::import, krrscene.cpp:253-350) is
+// read as well: "model" entries {"model": "<file>.obj", "name", "translate",
+// "rotate" [w,x,y,z], "scale"} load Wavefront OBJ + MTL files (paths relative
+// to "base_dir", default the working directory), "materials" entries
+// {"name", "bsdf", "params"} override the file's materials of that name
+// (krrscene.cpp:160-238), and "camera": {"mData": {...}} with
+// "cameraController": {"mData": {"target", "radius", "pitch", "yaw"}} places
+// the camera as the orbit controller does (camera.cpp:48-61).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <memory>
 #include <string>
 #include <thread>
@@ -430,32 +441,314 @@ krr_camera lookAtCamera(V eye, V target, V up, int w, int h, const Json &j) {
 	return c;
 }
 
-void buildScene(krr_scene &s, const Json &j) {
-	Builder b{s};
-	if (j.contains("materials")) {
-		for (const Json &m : j.at("materials").items()) {
-			const Json &p = m.contains("params") ? m.at("params") : m;
-			std::string bsdf = m.value("bsdf", "diffuse");
-			int type		 = bsdf == "diffuse" ? KRR_MAT_DIFFUSE : bsdf == "dielectric" ? KRR_MAT_DIELECTRIC
-												: bsdf == "conductor" ? KRR_MAT_CONDUCTOR : bsdf == "null" ? KRR_MAT_NULL : -1;
-			if (type < 0) throw std::runtime_error("unsupported bsdf \"" + bsdf + "\"");
-			krr_material mm;
-			std::memset(&mm, 0, sizeof(mm));
-			mm.type		= type;
-			float dif[3] = {1, 1, 1}, spec[3] = {0, 0, 0};
-			p.vec<3>("diffuse", dif);
-			p.vec<3>("specular", spec);
-			for (int a = 0; a < 3; a++) {
-				mm.diffuse[a]  = dif[a];
-				mm.specular[a] = spec[a];
-			}
-			mm.specular[3]			 = 1.f - (float) p.value("roughness", 1.0);
-			mm.specular_transmission = (float) p.value("specular_transmission", 0.0);
-			mm.anisotropic			 = (float) p.value("anisotropic", 0.0);
-			mm.ior					 = (float) p.value("eta", 1.5);
-			s.materials.push_back(mm);
+// ---------------------------------------------------------------- OBJ/MTL
+
+struct ObjMaterial {
+	V kd{1, 1, 1}, ks{0, 0, 0}, ke{0, 0, 0}, tf{1, 1, 1};
+	float ns = -1.f, ni = -1.f, d = 1.f;
+	bool hasTf = false;
+};
+
+std::string joinPath(const std::string &dir, const std::string &p) {
+	if (p.empty() || p[0] == '/' || dir.empty()) return p;
+	return dir.back() == '/' ? dir + p : dir + "/" + p;
+}
+std::string dirOf(const std::string &p) {
+	const size_t k = p.find_last_of('/');
+	return k == std::string::npos ? std::string() : p.substr(0, k);
+}
+
+void readMtl(const std::string &path, std::map<std::string, ObjMaterial> &out) {
+	std::ifstream f(path);
+	if (!f) throw std::runtime_error("cannot open material library " + path);
+	std::string line, cur;
+	auto rd3 = [](std::istringstream &ss) {
+		V v{0, 0, 0};
+		ss >> v.x >> v.y >> v.z;
+		return v;
+	};
+	while (std::getline(f, line)) {
+		std::istringstream ss(line);
+		std::string k;
+		if (!(ss >> k) || k[0] == '#') continue;
+		if (k == "newmtl") {
+			ss >> cur;
+			out[cur] = ObjMaterial{};
+			continue;
+		}
+		if (cur.empty()) continue;
+		ObjMaterial &m = out[cur];
+		if (k == "Kd") m.kd = rd3(ss);
+		else if (k == "Ks") m.ks = rd3(ss);
+		else if (k == "Ke") m.ke = rd3(ss);
+		else if (k == "Tf") {
+			m.tf	= rd3(ss);
+			m.hasTf = true;
+		} else if (k == "Ns") ss >> m.ns;
+		else if (k == "Ni") ss >> m.ni;
+		else if (k == "d") ss >> m.d;
+		else if (k == "Tr") {
+			float tr = 0.f;
+			ss >> tr;
+			m.d = 1.f - tr;
 		}
 	}
+}
+
+// AssimpImporter's OBJ material (assimp.cpp:60-65,96-224): shininess Ns ->
+// roughness sqrt(2 / (Ns + 2)) (0 from 1000 on), specular.a = 1 - roughness;
+// Ni -> IoR; opacity 0 or a fully transparent Tf -> dielectric with
+// specular transmission; Kd, Ks; Ke -> emission.  The reference's default BSDF
+// for file materials is Disney, which this build does not have: such
+// materials become Lambertian with the same diffuse colour.
+krr_material objMaterial(const ObjMaterial &o) {
+	krr_material m = makeMaterial(KRR_MAT_DIFFUSE, o.kd, 1.f, 1.5f, 0.f);
+	m.specular[0]  = o.ks.x;
+	m.specular[1]  = o.ks.y;
+	m.specular[2]  = o.ks.z;
+	m.specular[3]  = 0.f;
+	if (o.ns >= 0.f) {
+		const float r = o.ns >= 1000.f ? 0.f : std::min(1.f, std::max(0.f, std::sqrt(2.f / (o.ns + 2.f))));
+		m.specular[3] = 1.f - r;
+	}
+	if (o.ni >= 0.f) m.ior = o.ni;
+	if (o.d == 0.f) {
+		m.specular_transmission = 1.f;
+		m.type					= KRR_MAT_DIELECTRIC;
+	}
+	if (o.hasTf) {
+		const float t = 0.2126f * (1.f - o.tf.x) + 0.7152f * (1.f - o.tf.y) + 0.0722f * (1.f - o.tf.z); // luminance of 1 - Tf
+		m.specular_transmission = t;
+		if (t > 1.f - 1e-6f) m.type = KRR_MAT_DIELECTRIC;
+	}
+	return m;
+}
+
+// Node transform of a model entry (krrscene.cpp:317-333): translate, rotate
+// (quaternion [w, x, y, z]), scale; p' = T + R (S p).
+struct NodeXform {
+	float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+	V t{0, 0, 0}, sc{1, 1, 1};
+	V point(V p) const {
+		p = {p.x * sc.x, p.y * sc.y, p.z * sc.z};
+		return {R[0] * p.x + R[1] * p.y + R[2] * p.z + t.x, R[3] * p.x + R[4] * p.y + R[5] * p.z + t.y, R[6] * p.x + R[7] * p.y + R[8] * p.z + t.z};
+	}
+	V normal(V n) const { // inverse transpose of R S
+		n = {n.x / sc.x, n.y / sc.y, n.z / sc.z};
+		return nrmz({R[0] * n.x + R[1] * n.y + R[2] * n.z, R[3] * n.x + R[4] * n.y + R[5] * n.z, R[6] * n.x + R[7] * n.y + R[8] * n.z});
+	}
+};
+
+NodeXform nodeXform(const Json &m) {
+	NodeXform x;
+	float tr[3] = {0, 0, 0}, sc[3] = {1, 1, 1}, q[4] = {1, 0, 0, 0};
+	m.vec<3>("translate", tr);
+	m.vec<3>("scale", sc);
+	m.vec<4>("rotate", q);
+	x.t	 = {tr[0], tr[1], tr[2]};
+	x.sc = {sc[0], sc[1], sc[2]};
+	const float w = q[0], a = q[1], b = q[2], c = q[3];
+	const float n = std::sqrt(w * w + a * a + b * b + c * c);
+	const float W = w / n, X = a / n, Y = b / n, Z = c / n;
+	const float R[9] = {1 - 2 * (Y * Y + Z * Z), 2 * (X * Y - Z * W),	  2 * (X * Z + Y * W),
+						2 * (X * Y + Z * W),	 1 - 2 * (X * X + Z * Z), 2 * (Y * Z - X * W),
+						2 * (X * Z - Y * W),	 2 * (Y * Z + X * W),	  1 - 2 * (X * X + Y * Y)};
+	std::memcpy(x.R, R, sizeof(R));
+	return x;
+}
+
+// Wavefront OBJ: v, vn, f (v, v/t, v//n, v/t/n, negative indices, polygons
+// as fans), usemtl / mtllib.  One mesh per run of faces with one material.
+// Meshes without vn get smooth normals as assimp's GenSmoothNormals
+// (assimp.cpp:248-261; it runs before vertices are joined, on one vertex per
+// face corner): per position, the normalised sum of the unit normals of the
+// mesh's faces (OBJ polygons) that touch it.  "materials" entries override
+// file materials by name.
+void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std::map<std::string, krr_material> &overrides) {
+	std::ifstream f(path);
+	if (!f) throw std::runtime_error("cannot open model " + path);
+	std::map<std::string, ObjMaterial> mtl;
+	std::map<std::string, int> matIndex;
+	std::vector<V> P, N;
+	struct Corner {
+		int p, n;
+	};
+	struct Run {
+		std::string mat;
+		std::vector<Corner> tri;  // 3 per triangle
+		std::map<int, V> smooth; // position -> summed unit polygon normals
+	};
+	std::vector<Run> runs(1);
+	std::string line;
+	auto resolve = [](int i, size_t n) { return i > 0 ? i - 1 : (int) n + i; };
+	while (std::getline(f, line)) {
+		std::istringstream ss(line);
+		std::string k;
+		if (!(ss >> k) || k[0] == '#') continue;
+		if (k == "v") {
+			V v{0, 0, 0};
+			ss >> v.x >> v.y >> v.z;
+			P.push_back(v);
+		} else if (k == "vn") {
+			V v{0, 0, 0};
+			ss >> v.x >> v.y >> v.z;
+			N.push_back(v);
+		} else if (k == "mtllib") {
+			std::string lib;
+			std::getline(ss, lib);
+			lib.erase(0, lib.find_first_not_of(" \t"));
+			lib.erase(lib.find_last_not_of(" \t\r") + 1);
+			readMtl(joinPath(dirOf(path), lib), mtl);
+		} else if (k == "usemtl") {
+			std::string name;
+			ss >> name;
+			if (runs.back().tri.empty()) runs.back().mat = name;
+			else runs.push_back(Run{name, {}, {}});
+		} else if (k == "f") {
+			std::vector<Corner> poly;
+			std::string tok;
+			while (ss >> tok) {
+				int vi = 0, ni = 0;
+				const size_t s1 = tok.find('/');
+				vi				= std::atoi(tok.substr(0, s1).c_str());
+				if (s1 != std::string::npos) {
+					const size_t s2 = tok.find('/', s1 + 1);
+					if (s2 != std::string::npos && s2 + 1 < tok.size()) ni = std::atoi(tok.substr(s2 + 1).c_str());
+				}
+				Corner c{resolve(vi, P.size()), ni ? resolve(ni, N.size()) : -1};
+				if (c.p < 0 || c.p >= (int) P.size() || c.n >= (int) N.size()) throw std::runtime_error(path + ": face index out of range");
+				poly.push_back(c);
+			}
+			if (poly.size() < 3) continue;
+			for (size_t i = 2; i < poly.size(); i++) { // fan
+				runs.back().tri.push_back(poly[0]);
+				runs.back().tri.push_back(poly[i - 1]);
+				runs.back().tri.push_back(poly[i]);
+			}
+			// the polygon's normal (of its first triangle), once per position
+			const V p0 = xf.point(P[poly[0].p]), p1 = xf.point(P[poly[1].p]), p2 = xf.point(P[poly[2].p]);
+			const V fn = nrmz(crs(sub(p1, p0), sub(p2, p0)));
+			for (size_t k = 0; k < poly.size(); k++) {
+				bool dup = false;
+				for (size_t q = 0; q < k; q++) dup |= poly[q].p == poly[k].p;
+				if (!dup) runs.back().smooth[poly[k].p] = add(runs.back().smooth[poly[k].p], fn);
+			}
+		}
+	}
+	for (const Run &r : runs) {
+		if (r.tri.empty()) continue;
+		// material: JSON override by name, else the MTL entry, else default
+		int mi;
+		auto mit = matIndex.find(r.mat);
+		if (mit != matIndex.end()) mi = mit->second;
+		else {
+			auto ov = overrides.find(r.mat);
+			const auto ot = mtl.find(r.mat);
+			const ObjMaterial om = ot != mtl.end() ? ot->second : ObjMaterial{};
+			mi				= b.addMaterial(ov != overrides.end() ? ov->second : objMaterial(om));
+			matIndex[r.mat] = mi;
+		}
+		const auto ot	 = mtl.find(r.mat);
+		const V emission = ot != mtl.end() ? ot->second.ke : V{0, 0, 0};
+		// vertices of the run: one per (position, normal) corner pair
+		std::map<std::pair<int, int>, int> vid;
+		std::vector<V> verts, nrms;
+		std::vector<int> tris;
+		bool haveN = true;
+		for (const Corner &c : r.tri) haveN &= c.n >= 0;
+		const std::map<int, V> &smooth = r.smooth;
+		for (const Corner &c : r.tri) {
+			const auto key = std::make_pair(c.p, haveN ? c.n : -1);
+			auto it		   = vid.find(key);
+			if (it == vid.end()) {
+				it = vid.emplace(key, (int) verts.size()).first;
+				verts.push_back(xf.point(P[c.p]));
+				nrms.push_back(haveN ? xf.normal(N[c.n]) : nrmz(smooth.at(c.p)));
+			}
+			tris.push_back(it->second);
+		}
+		b.addMesh(verts, nrms, tris, mi, emission);
+	}
+}
+
+// A "materials" entry (krrscene.cpp:198-229).
+krr_material jsonMaterial(const Json &m) {
+	const Json &p	 = m.contains("params") ? m.at("params") : m;
+	std::string bsdf = m.value("bsdf", "diffuse");
+	int type		 = bsdf == "diffuse" ? KRR_MAT_DIFFUSE : bsdf == "dielectric" ? KRR_MAT_DIELECTRIC
+									   : bsdf == "conductor" ? KRR_MAT_CONDUCTOR : bsdf == "null" ? KRR_MAT_NULL : -1;
+	if (type < 0) throw std::runtime_error("unsupported bsdf \"" + bsdf + "\"");
+	krr_material mm;
+	std::memset(&mm, 0, sizeof(mm));
+	mm.type		 = type;
+	float dif[3] = {1, 1, 1}, spec[3] = {0, 0, 0};
+	p.vec<3>("diffuse", dif);
+	p.vec<3>("specular", spec);
+	for (int a = 0; a < 3; a++) {
+		mm.diffuse[a]  = dif[a];
+		mm.specular[a] = spec[a];
+	}
+	mm.specular[3]			 = 1.f - (float) p.value("roughness", 1.0);
+	mm.specular_transmission = (float) p.value("specular_transmission", 0.0);
+	mm.anisotropic			 = (float) p.value("anisotropic", 0.0);
+	mm.ior					 = (float) p.value("eta", 1.5);
+	return mm;
+}
+
+// Camera of KiRaRay's scene JSON: CameraData keys under "mData"
+// (camera.h:64-81) and the orbit controller (camera.cpp:48-61): rotation
+// Ry(yaw) Rx(pitch), forward = R (0, 0, -1), position = target - forward *
+// radius.  Without a controller the camera sits at the origin looking down -z.
+krr_camera referenceCamera(const Json &cam, const Json *ctrl) {
+	krr_camera c;
+	std::memset(&c, 0, sizeof(c));
+	c.film_size[0]	 = 42.666667f;
+	c.film_size[1]	 = 24.0f;
+	c.focal_length	 = 21.f;
+	c.focal_distance = 10.f;
+	c.lens_radius	 = 0.f;
+	c.aspect_ratio	 = 1.777777f;
+	c.medium		 = -1;
+	const Json d	 = cam.contains("mData") ? cam.at("mData") : Json::object();
+	c.focal_length	 = (float) d.value("focalLength", (double) c.focal_length);
+	c.focal_distance = (float) d.value("focalDistance", (double) c.focal_distance);
+	c.lens_radius	 = (float) d.value("lensRadius", (double) c.lens_radius);
+	c.aspect_ratio	 = (float) d.value("aspectRatio", (double) c.aspect_ratio);
+	c.shutter_open	 = (float) d.value("shutterOpen", 0.0);
+	c.shutter_time	 = (float) d.value("shutterTime", 0.0);
+	float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, pos[3] = {0, 0, 0};
+	if (ctrl) {
+		const Json cd	 = ctrl->contains("mData") ? ctrl->at("mData") : Json::object();
+		float target[3] = {0, 0, 0};
+		cd.vec<3>("target", target);
+		const float radius = (float) cd.value("radius", 5.0), pitch = (float) cd.value("pitch", 0.0), yaw = (float) cd.value("yaw", 0.0);
+		const float cy = std::cos(yaw), sy = std::sin(yaw), cp = std::cos(pitch), sp = std::sin(pitch);
+		// Ry(yaw) * Rx(pitch)
+		const float M[9] = {cy, sy * sp, sy * cp, 0, cp, -sp, -sy, cy * sp, cy * cp};
+		std::memcpy(R, M, sizeof(M));
+		const V fwd = {-R[2], -R[5], -R[8]};
+		pos[0]		= target[0] - fwd.x * radius;
+		pos[1]		= target[1] - fwd.y * radius;
+		pos[2]		= target[2] - fwd.z * radius;
+	}
+	const float T[12] = {R[0], R[1], R[2], pos[0], R[3], R[4], R[5], pos[1], R[6], R[7], R[8], pos[2]};
+	std::memcpy(c.transform, T, sizeof(T));
+	return c;
+}
+
+void buildScene(krr_scene &s, const Json &j) {
+	Builder b{s};
+	// "materials": by index for the procedural models, by name for the
+	// materials of model files (krrscene.cpp:198-229)
+	std::map<std::string, krr_material> overrides;
+	if (j.contains("materials")) {
+		for (const Json &m : j.at("materials").items()) {
+			const krr_material mm = jsonMaterial(m);
+			s.materials.push_back(mm);
+			if (m.contains("name")) overrides[m.value("name", "")] = mm;
+		}
+	}
+	const std::string baseDir = j.value("base_dir", "");
 	if (j.contains("media")) {
 		for (const Json &m : j.at("media").items()) {
 			std::string t = m.value("type", "homogeneous");
@@ -471,7 +764,15 @@ void buildScene(krr_scene &s, const Json &j) {
 		if (models.isArray()) list = models.items();
 		else list.push_back(models);
 		for (const Json &m : list) {
+			if (m.contains("model")) { // a model file (krrscene.cpp:317-340)
+				const std::string file = m.value("model", "");
+				const bool obj		   = file.size() > 4 && (file.compare(file.size() - 4, 4, ".obj") == 0 || file.compare(file.size() - 4, 4, ".OBJ") == 0);
+				if (!obj) throw std::runtime_error("model file \"" + file + "\": only Wavefront .obj files are read");
+				loadObj(b, joinPath(baseDir, file), nodeXform(m), overrides);
+				continue;
+			}
 			std::string t = m.value("type", "mesh");
+			if (t == "light") throw std::runtime_error("light entries (point / environment lights of the scene graph) are not supported");
 			if (t == "cornell_box") cornellBox(b, m);
 			else if (t == "icosphere_field") icosphereField(b, m);
 			else if (t == "mesh") explicitMesh(b, m);
@@ -507,7 +808,12 @@ void buildScene(krr_scene &s, const Json &j) {
 	cj.vec<3>("eye", eye);
 	cj.vec<3>("target", target);
 	cj.vec<3>("up", up);
-	if (j.contains("views")) {
+	if (cj.contains("mData")) { // KiRaRay's camera (and orbit controller)
+		const Json ctrl = j.contains("cameraController") ? j.at("cameraController") : Json::object();
+		krr_camera c	= referenceCamera(cj, j.contains("cameraController") ? &ctrl : nullptr);
+		s.cameras.push_back(c);
+		s.viewSizes.push_back({w, h});
+	} else if (j.contains("views")) {
 		const Json &vj = j.at("views");
 		int count	   = (int) vj.value("count", 256.0);
 		Rng rng((uint64_t) vj.value("seed", 5.0), 3);
