@@ -21,14 +21,21 @@
 // drawn from "sizes".
 //
 // KiRaRay's own scene JSON (SceneImporter::import, krrscene.cpp:253-350) is
-// read as well: "model" entries {"model": "<file>.obj", "name", "translate",
-// "rotate" [w,x,y,z], "scale"} load Wavefront OBJ + MTL files (paths relative
-// to "base_dir", default the working directory), "materials" entries
-// {"name", "bsdf", "params"} override the file's materials of that name
-// (krrscene.cpp:160-238), and "camera": {"mData": {...}} with
-// "cameraController": {"mData": {"target", "radius", "pitch", "yaw"}} places
-// the camera as the orbit controller does (camera.cpp:48-61).
+// read as well.  "model" is a node tree (arrays, file paths, objects with
+// "name", "translate", "rotate" [w,x,y,z], "scale", "params" and a nested
+// "model"); transforms compose down the tree.  Wavefront OBJ + MTL files
+// are loaded (paths relative to "base_dir", default the working directory);
+// "type": "medium" nodes add homogeneous media with the reference's keys,
+// a "bound" box and mesh bindings by name; "type": "light" nodes add the
+// constant infinite light.  "materials" entries {"name", "bsdf", "params"}
+// override the file's materials of that name (krrscene.cpp:160-238), and
+// "camera": {"mData": {...}} with "cameraController": {"mData": {"target",
+// "radius", "pitch", "yaw"}} places the camera as the orbit controller does
+// (camera.cpp:48-61).  What this build cannot render is an error: other
+// model formats, OpenVDB volumes, point / spot / directional lights and
+// textured environments.
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -47,6 +54,7 @@ struct krr_scene {
 	std::vector<float> positions, normals;
 	std::vector<int32_t> indices;
 	std::vector<krr_mesh> meshes;
+	std::vector<std::string> meshNames; // one per mesh ("" when unnamed)
 	std::vector<krr_material> materials;
 	std::vector<krr_medium> media;
 	std::vector<std::unique_ptr<std::vector<float>>> densities;
@@ -113,7 +121,7 @@ struct Builder {
 	}
 	// Appends a mesh; normals may be empty.
 	int addMesh(const std::vector<V> &verts, const std::vector<V> &nrms, const std::vector<int> &tris, int material,
-				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1) {
+				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1, const std::string &name = "") {
 		krr_mesh m;
 		std::memset(&m, 0, sizeof(m));
 		int32_t base	 = (int32_t) (s.positions.size() / 3);
@@ -138,6 +146,7 @@ struct Builder {
 		if (!nrms.empty()) s.anyNormals = true;
 		for (int t : tris) s.indices.push_back(base + t);
 		s.meshes.push_back(m);
+		s.meshNames.push_back(name);
 		return (int) s.meshes.size() - 1;
 	}
 	// Quad p0 p1 p2 p3 (counter-clockwise seen from the side its normal faces).
@@ -292,7 +301,7 @@ void explicitMesh(Builder &b, const Json &j) {
 	int mat	 = (int) j.value("material", 0.0);
 	int mIn	 = (int) j.value("medium_inside", -1.0);
 	int mOut = (int) j.value("medium_outside", -1.0);
-	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut);
+	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut, j.value("name", ""));
 }
 
 void quadModel(Builder &b, const Json &j) {
@@ -395,14 +404,27 @@ void valueNoiseMedium(krr_scene &s, const Json &j) {
 	s.media.push_back(m);
 }
 
+// Homogeneous medium with the reference's keys and defaults
+// (krrscene.cpp:72-86): sigma_t 1, albedo 0.5, g 0, Le 0; "sigma_a" /
+// "sigma_s" (each 0.5 when only the other is given) replace them with
+// sigma_t = sigma_a + sigma_s and albedo = sigma_s / sigma_t.
 void homogeneousMedium(krr_scene &s, const Json &j) {
 	krr_medium m;
 	std::memset(&m, 0, sizeof(m));
 	m.type = KRR_MEDIUM_HOMOGENEOUS;
-	float st[3] = {1, 1, 1}, al[3] = {0.8f, 0.8f, 0.8f}, le[3] = {0, 0, 0};
+	float st[3] = {1, 1, 1}, al[3] = {0.5f, 0.5f, 0.5f}, le[3] = {0, 0, 0};
 	j.vec<3>("sigma_t", st);
 	j.vec<3>("albedo", al);
 	j.vec<3>("Le", le);
+	if (j.contains("sigma_a") || j.contains("sigma_s")) {
+		float sa[3] = {0.5f, 0.5f, 0.5f}, ss[3] = {0.5f, 0.5f, 0.5f};
+		j.vec<3>("sigma_a", sa);
+		j.vec<3>("sigma_s", ss);
+		for (int a = 0; a < 3; a++) {
+			st[a] = sa[a] + ss[a];
+			al[a] = ss[a] / st[a];
+		}
+	}
 	for (int a = 0; a < 3; a++) {
 		m.sigma_t[a] = st[a];
 		m.albedo[a]	 = al[a];
@@ -524,18 +546,33 @@ krr_material objMaterial(const ObjMaterial &o) {
 	return m;
 }
 
-// Node transform of a model entry (krrscene.cpp:317-333): translate, rotate
-// (quaternion [w, x, y, z]), scale; p' = T + R (S p).
+// Node transform (krrscene.cpp:317-333, scene graph nodes): translate,
+// rotate (quaternion [w, x, y, z]), scale, p' = T + R (S p) for one node;
+// nested nodes compose parent * child as the scene graph does.  Stored as
+// the affine map p' = L p + t.
 struct NodeXform {
-	float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-	V t{0, 0, 0}, sc{1, 1, 1};
-	V point(V p) const {
-		p = {p.x * sc.x, p.y * sc.y, p.z * sc.z};
-		return {R[0] * p.x + R[1] * p.y + R[2] * p.z + t.x, R[3] * p.x + R[4] * p.y + R[5] * p.z + t.y, R[6] * p.x + R[7] * p.y + R[8] * p.z + t.z};
+	float L[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+	V t{0, 0, 0};
+	bool identity() const {
+		static const float I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+		return std::memcmp(L, I, sizeof(I)) == 0 && t.x == 0.f && t.y == 0.f && t.z == 0.f;
 	}
-	V normal(V n) const { // inverse transpose of R S
-		n = {n.x / sc.x, n.y / sc.y, n.z / sc.z};
-		return nrmz({R[0] * n.x + R[1] * n.y + R[2] * n.z, R[3] * n.x + R[4] * n.y + R[5] * n.z, R[6] * n.x + R[7] * n.y + R[8] * n.z});
+	V linear(V p) const { return {L[0] * p.x + L[1] * p.y + L[2] * p.z, L[3] * p.x + L[4] * p.y + L[5] * p.z, L[6] * p.x + L[7] * p.y + L[8] * p.z}; }
+	V point(V p) const { return add(linear(p), t); }
+	V normal(V n) const { // inverse transpose of L: its cofactor matrix over det(L)
+		const float c[9] = {L[4] * L[8] - L[5] * L[7], L[5] * L[6] - L[3] * L[8], L[3] * L[7] - L[4] * L[6],
+							L[2] * L[7] - L[1] * L[8], L[0] * L[8] - L[2] * L[6], L[1] * L[6] - L[0] * L[7],
+							L[1] * L[5] - L[2] * L[4], L[2] * L[3] - L[0] * L[5], L[0] * L[4] - L[1] * L[3]};
+		const float det = L[0] * c[0] + L[1] * c[1] + L[2] * c[2];
+		const float sg	= det < 0.f ? -1.f : 1.f;
+		return nrmz({sg * (c[0] * n.x + c[1] * n.y + c[2] * n.z), sg * (c[3] * n.x + c[4] * n.y + c[5] * n.z), sg * (c[6] * n.x + c[7] * n.y + c[8] * n.z)});
+	}
+	NodeXform then(const NodeXform &child) const { // this * child
+		NodeXform r;
+		for (int i = 0; i < 3; i++)
+			for (int j = 0; j < 3; j++) r.L[3 * i + j] = L[3 * i] * child.L[j] + L[3 * i + 1] * child.L[3 + j] + L[3 * i + 2] * child.L[6 + j];
+		r.t = point(child.t);
+		return r;
 	}
 };
 
@@ -545,20 +582,23 @@ NodeXform nodeXform(const Json &m) {
 	m.vec<3>("translate", tr);
 	m.vec<3>("scale", sc);
 	m.vec<4>("rotate", q);
-	x.t	 = {tr[0], tr[1], tr[2]};
-	x.sc = {sc[0], sc[1], sc[2]};
 	const float w = q[0], a = q[1], b = q[2], c = q[3];
 	const float n = std::sqrt(w * w + a * a + b * b + c * c);
+	if (!(n > 0.f)) throw std::runtime_error("rotate: the quaternion has zero length");
 	const float W = w / n, X = a / n, Y = b / n, Z = c / n;
 	const float R[9] = {1 - 2 * (Y * Y + Z * Z), 2 * (X * Y - Z * W),	  2 * (X * Z + Y * W),
 						2 * (X * Y + Z * W),	 1 - 2 * (X * X + Z * Z), 2 * (Y * Z - X * W),
 						2 * (X * Z - Y * W),	 2 * (Y * Z + X * W),	  1 - 2 * (X * X + Y * Y)};
-	std::memcpy(x.R, R, sizeof(R));
+	for (int i = 0; i < 3; i++)
+		for (int j = 0; j < 3; j++) x.L[3 * i + j] = R[3 * i + j] * sc[j];
+	x.t = {tr[0], tr[1], tr[2]};
 	return x;
 }
 
 // Wavefront OBJ: v, vn, f (v, v/t, v//n, v/t/n, negative indices, polygons
-// as fans), usemtl / mtllib.  One mesh per run of faces with one material.
+// as fans), usemtl / mtllib, o / g.  One mesh per run of faces with one
+// material and one object or group name; the mesh takes that name (the name
+// a medium's "meshes" list refers to).
 // Meshes without vn get smooth normals as assimp's GenSmoothNormals
 // (assimp.cpp:248-261; it runs before vertices are joined, on one vertex per
 // face corner): per position, the normalised sum of the unit normals of the
@@ -574,7 +614,7 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 		int p, n;
 	};
 	struct Run {
-		std::string mat;
+		std::string mat, name;
 		std::vector<Corner> tri;  // 3 per triangle
 		std::map<int, V> smooth; // position -> summed unit polygon normals
 	};
@@ -603,7 +643,14 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			std::string name;
 			ss >> name;
 			if (runs.back().tri.empty()) runs.back().mat = name;
-			else runs.push_back(Run{name, {}, {}});
+			else runs.push_back(Run{name, runs.back().name, {}, {}});
+		} else if (k == "o" || k == "g") {
+			std::string name;
+			std::getline(ss, name);
+			name.erase(0, name.find_first_not_of(" \t"));
+			name.erase(name.find_last_not_of(" \t\r") + 1);
+			if (runs.back().tri.empty()) runs.back().name = name;
+			else runs.push_back(Run{runs.back().mat, name, {}, {}});
 		} else if (k == "f") {
 			std::vector<Corner> poly;
 			std::string tok;
@@ -667,7 +714,7 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			}
 			tris.push_back(it->second);
 		}
-		b.addMesh(verts, nrms, tris, mi, emission);
+		b.addMesh(verts, nrms, tris, mi, emission, -1, -1, r.name);
 	}
 }
 
@@ -736,6 +783,143 @@ krr_camera referenceCamera(const Json &cam, const Json *ctrl) {
 	return c;
 }
 
+// The scene graph of KiRaRay's scene JSON (SceneImporter::importNode,
+// krrscene.cpp:305-343): "model" is an array, a model file path, or a node
+// {"name", "translate", "rotate", "scale", "params"} holding a nested
+// "model" or, by "type", a medium (krrscene.cpp:68-143) or a light
+// (krrscene.cpp:28-66).  Node transforms compose down the tree.  The
+// procedural model types of this build are leaves without a transform.
+struct Importer {
+	Builder &b;
+	krr_scene &s;
+	std::string baseDir;
+	const std::map<std::string, krr_material> &overrides;
+	std::vector<std::pair<int, Json>> bindings; // media bound to meshes by name
+
+	static bool endsWith(const std::string &f, const char *ext) {
+		const size_t n = std::strlen(ext);
+		if (f.size() < n) return false;
+		for (size_t i = 0; i < n; i++)
+			if (std::tolower((unsigned char) f[f.size() - n + i]) != ext[i]) return false;
+		return true;
+	}
+
+	void modelFile(const std::string &file, const NodeXform &xf) {
+		if (endsWith(file, ".vdb")) throw std::runtime_error("model file \"" + file + "\": OpenVDB volumes are not read by this build");
+		if (!endsWith(file, ".obj")) throw std::runtime_error("model file \"" + file + "\": only Wavefront .obj files are read");
+		loadObj(b, joinPath(baseDir, file), xf, overrides);
+	}
+
+	void node(const Json &j, const NodeXform &parent) {
+		if (j.isArray()) {
+			for (const Json &c : j.items()) node(c, parent);
+			return;
+		}
+		if (j.isString()) {
+			modelFile(j.asString(), parent);
+			return;
+		}
+		if (!j.isObject()) throw std::runtime_error("model entries are arrays, file paths or objects, not " + j.dump());
+		const NodeXform xf = parent.then(nodeXform(j));
+		const Json params  = j.contains("params") ? j.at("params") : Json::object();
+		if (j.contains("model")) {
+			node(j.at("model"), xf);
+			return;
+		}
+		const std::string t = j.value("type", "mesh");
+		if (t == "medium") {
+			medium(params, params.value("type", "homogeneous"), xf);
+			return;
+		}
+		if (t == "light") {
+			light(params);
+			return;
+		}
+		if (!xf.identity()) throw std::runtime_error("translate / rotate / scale apply to model files, media and nested nodes, not to \"" + t + "\" entries");
+		if (t == "cornell_box") cornellBox(b, j);
+		else if (t == "icosphere_field") icosphereField(b, j);
+		else if (t == "mesh") explicitMesh(b, j);
+		else if (t == "quad") quadModel(b, j);
+		else throw std::runtime_error("unsupported model type \"" + t + "\"");
+	}
+
+	// A medium; "bound": [[lo], [hi]] adds a box without material (a medium
+	// interface, material -1) with the medium inside (krrscene.cpp:90-111), "meshes" / "meshes_outside" put
+	// it inside / outside the meshes of those names (krrscene.cpp:113-129).
+	int medium(const Json &p, const std::string &t, const NodeXform &xf) {
+		if (t == "homogeneous") homogeneousMedium(s, p);
+		else if (t == "heterogeneous" && p.contains("file"))
+			throw std::runtime_error("heterogeneous medium \"" + p.value("file", "") + "\": OpenVDB volumes are not read by this build");
+		else if (t == "heterogeneous" || t == "value_noise_grid") valueNoiseMedium(s, p);
+		else throw std::runtime_error("unsupported medium type \"" + t + "\"");
+		const int mi = (int) s.media.size() - 1;
+		if (p.contains("bound")) {
+			const auto &B = p.at("bound").items();
+			if (B.size() != 2 || B[0].items().size() != 3 || B[1].items().size() != 3) throw std::runtime_error("medium bound must be [[x, y, z], [x, y, z]]");
+			V lo{(float) B[0][0].asNumber(), (float) B[0][1].asNumber(), (float) B[0][2].asNumber()};
+			V hi{(float) B[1][0].asNumber(), (float) B[1][1].asNumber(), (float) B[1][2].asNumber()};
+			box(lo, hi, xf, mi);
+		}
+		if (p.contains("meshes") || p.contains("meshes_outside")) bindings.push_back({mi, p});
+		return mi;
+	}
+
+	// Twelve triangles wound so that their normals face out of the box.
+	void box(V lo, V hi, const NodeXform &xf, int mi) {
+		const V c{0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
+		const V h{0.5f * (hi.x - lo.x), 0.5f * (hi.y - lo.y), 0.5f * (hi.z - lo.z)};
+		const V X{h.x, 0, 0}, Y{0, h.y, 0}, Z{0, 0, h.z};
+		const V nx{-h.x, 0, 0}, ny{0, -h.y, 0}, nz{0, 0, -h.z};
+		// (outward offset, u, v) with u x v along the outward normal
+		const V faces[6][3] = {{X, Y, Z}, {nx, Z, Y}, {Y, Z, X}, {ny, X, Z}, {Z, X, Y}, {nz, Y, X}};
+		std::vector<V> verts;
+		std::vector<int> tris;
+		for (const auto &f : faces) {
+			const V o = add(c, f[0]);
+			const int base = (int) verts.size();
+			verts.push_back(xf.point(sub(sub(o, f[1]), f[2])));
+			verts.push_back(xf.point(sub(add(o, f[1]), f[2])));
+			verts.push_back(xf.point(add(add(o, f[1]), f[2])));
+			verts.push_back(xf.point(add(sub(o, f[1]), f[2])));
+			for (int k : {0, 1, 2, 0, 2, 3}) tris.push_back(base + k);
+		}
+		b.addMesh(verts, {}, tris, -1, {0, 0, 0}, mi, -1, "Medium box");
+	}
+
+	void bindMedia() {
+		for (const auto &bd : bindings) {
+			for (int side = 0; side < 2; side++) {
+				const char *key = side ? "meshes_outside" : "meshes";
+				if (!bd.second.contains(key)) continue;
+				for (const Json &n : bd.second.at(key).items()) {
+					bool found = false;
+					for (size_t k = 0; k < s.meshes.size(); k++) {
+						if (s.meshNames[k] != n.asString()) continue;
+						(side ? s.meshes[k].medium_outside : s.meshes[k].medium_inside) = bd.first;
+						found = true;
+					}
+					if (!found) throw std::runtime_error("medium: no mesh is named \"" + n.asString() + "\"");
+				}
+			}
+		}
+	}
+
+	// Lights of the scene graph: the constant infinite light only; point,
+	// directional and spot lights and textured environments are not part
+	// of this build's light sampling.
+	void light(const Json &p) {
+		const std::string t = p.value("type", "infinite");
+		if (t != "infinite") throw std::runtime_error("light type \"" + t + "\" is not supported (infinite lights only)");
+		if (!p.value("texture", "").empty()) throw std::runtime_error("textured infinite lights are not supported");
+		krr_infinite_light il;
+		float col[3] = {1, 1, 1};
+		p.vec<3>("color", col);
+		for (int a = 0; a < 3; a++) il.tint[a] = col[a];
+		il.scale = (float) p.value("scale", 1.0);
+		s.infinite.push_back(il);
+	}
+};
+
 void buildScene(krr_scene &s, const Json &j) {
 	Builder b{s};
 	// "materials": by index for the procedural models, by name for the
@@ -748,38 +932,17 @@ void buildScene(krr_scene &s, const Json &j) {
 			if (m.contains("name")) overrides[m.value("name", "")] = mm;
 		}
 	}
-	const std::string baseDir = j.value("base_dir", "");
+	Importer im{b, s, j.value("base_dir", ""), overrides};
+	// "media" before "model": procedural models refer to media by index
 	if (j.contains("media")) {
 		for (const Json &m : j.at("media").items()) {
-			std::string t = m.value("type", "homogeneous");
 			const Json &p = m.contains("params") ? m.at("params") : m;
-			if (t == "value_noise_grid" || t == "heterogeneous") valueNoiseMedium(s, p);
-			else if (t == "homogeneous") homogeneousMedium(s, p);
-			else throw std::runtime_error("unsupported medium type \"" + t + "\"");
+			const std::string outer = m.value("type", "homogeneous");
+			im.medium(p, p.value("type", outer.c_str()), NodeXform{});
 		}
 	}
-	if (j.contains("model")) {
-		const Json &models = j.at("model");
-		std::vector<Json> list;
-		if (models.isArray()) list = models.items();
-		else list.push_back(models);
-		for (const Json &m : list) {
-			if (m.contains("model")) { // a model file (krrscene.cpp:317-340)
-				const std::string file = m.value("model", "");
-				const bool obj		   = file.size() > 4 && (file.compare(file.size() - 4, 4, ".obj") == 0 || file.compare(file.size() - 4, 4, ".OBJ") == 0);
-				if (!obj) throw std::runtime_error("model file \"" + file + "\": only Wavefront .obj files are read");
-				loadObj(b, joinPath(baseDir, file), nodeXform(m), overrides);
-				continue;
-			}
-			std::string t = m.value("type", "mesh");
-			if (t == "light") throw std::runtime_error("light entries (point / environment lights of the scene graph) are not supported");
-			if (t == "cornell_box") cornellBox(b, m);
-			else if (t == "icosphere_field") icosphereField(b, m);
-			else if (t == "mesh") explicitMesh(b, m);
-			else if (t == "quad") quadModel(b, m);
-			else throw std::runtime_error("unsupported model type \"" + t + "\"");
-		}
-	}
+	if (j.contains("model")) im.node(j.at("model"), NodeXform{});
+	im.bindMedia();
 	for (const krr_mesh &m : s.meshes) {
 		if (m.material >= (int) s.materials.size()) throw std::runtime_error("mesh material index out of range");
 		if (m.medium_inside >= (int) s.media.size() || m.medium_outside >= (int) s.media.size())
@@ -789,6 +952,7 @@ void buildScene(krr_scene &s, const Json &j) {
 	}
 	if (j.contains("environment")) {
 		const Json &e = j.at("environment");
+		if (e.isString()) throw std::runtime_error("environment \"" + e.asString() + "\": image environments are not supported");
 		krr_infinite_light il;
 		float tint[3] = {1, 1, 1};
 		e.vec<3>("tint", tint);

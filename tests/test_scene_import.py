@@ -15,6 +15,8 @@ import pytest
 from conftest import CORNELL
 
 REFERENCE_CONFIG = "/root/reference/common/configs/example_cbox.json"
+MAT = [{"bsdf": "diffuse", "params": {"diffuse": [0.5, 0.5, 0.5]}}]
+REFERENCE_MIST = "/root/reference/common/configs/volumetric/example_cboxmist.json"
 
 
 def _desc_arrays(krr, s):
@@ -120,12 +122,113 @@ def test_reference_camera_and_orbit_controller(krr, tmp_path):
 
 
 def test_unsupported_entries_are_errors(krr, tmp_path):
-    with pytest.raises(krr.KrrError):
-        krr.Scene({"model": [{"model": str(tmp_path / "x.gltf")}]})
-    with pytest.raises(krr.KrrError):
-        krr.Scene({"model": [{"type": "light", "name": "Point Light"}]})
-    with pytest.raises(krr.KrrError):
-        krr.Scene({"model": [{"model": str(tmp_path / "missing.obj")}]})
+    quad = {"type": "quad", "corners": [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], "material": 0}
+    bad = [
+        {"model": [{"model": str(tmp_path / "x.gltf")}]},
+        {"model": [{"model": str(tmp_path / "missing.obj")}]},
+        {"model": [{"model": "smoke.vdb", "params": {"g": 0.5}}]},
+        {"model": [quad, {"type": "light", "params": {"type": "point", "position": [0, 1, 0]}}]},
+        {"model": [quad, {"type": "light", "params": {"type": "infinite", "texture": "sky.exr"}}]},
+        {"model": [quad, {"type": "medium", "params": {"type": "heterogeneous", "file": "smoke.vdb"}}]},
+        {"model": [quad, {"type": "medium", "params": {"type": "homogeneous", "meshes": ["nothing"]}}]},
+        {"model": [dict(quad, translate=[1, 0, 0])]},
+        {"model": [quad], "environment": "sky.exr"},
+    ]
+    krr.Scene({"model": [quad], "materials": MAT})
+    for cfg in bad:
+        with pytest.raises(krr.KrrError):
+            krr.Scene(dict(cfg, materials=MAT))
+
+
+def test_light_node_infinite(krr):
+    """A light node without a type is the constant infinite light of colour
+    x scale (krrscene.cpp:28-47)."""
+    quad = {"type": "quad", "corners": [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], "material": 0}
+    s = krr.Scene({"model": [quad, {"type": "light", "name": "Sky", "params": {"color": [1, 2, 3], "scale": 2}},
+                             {"type": "light"}], "materials": MAT})
+    d = s.desc
+    assert d.n_infinite_lights == 2
+    assert list(d.infinite_lights[0].tint) == [1, 2, 3] and d.infinite_lights[0].scale == 2
+    assert list(d.infinite_lights[1].tint) == [1, 1, 1] and d.infinite_lights[1].scale == 1
+
+
+def test_nested_nodes_compose_transforms(krr, tmp_path):
+    """A node inside a node: the child's scale, then the parent's rotation
+    (90 degrees about +y) and translation, as the scene graph composes
+    them; a file path given as a bare string is a model too."""
+    obj = _write(tmp_path / "t.obj", "v 1 0 0\nv 0 1 0\nv 0 0 1\nf 1 2 3\n")
+    h = math.sqrt(0.5)
+    s = krr.Scene({"model": [{"translate": [10, 0, 0], "rotate": [h, 0, h, 0],
+                              "model": [{"scale": [2, 3, 4], "model": obj}, obj]}]})
+    pos, idx, nrm, meshes, _ = _desc_arrays(krr, s)
+    assert len(meshes) == 2
+    # R_y(90): (x, y, z) -> (z, y, -x)
+    assert np.allclose(pos[idx[0]], [[10, 0, -2], [10, 3, 0], [14, 0, 0]], atol=1e-5)
+    assert np.allclose(pos[idx[1]], [[10, 0, -1], [10, 1, 0], [11, 0, 0]], atol=1e-5)
+    # smooth normals follow the (non-uniformly scaled) face
+    P = pos[idx[0]].astype(np.float64)
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    assert np.allclose(nrm[idx[0][0]], n / np.linalg.norm(n), atol=1e-5)
+
+
+def test_medium_node_box_and_bindings(krr, tmp_path):
+    """A homogeneous medium node with sigma_a / sigma_s and a bound becomes
+    sigma_t = a + s, albedo = s / t and a box without material (a medium
+    interface) with the medium inside, moved by the node's transform, its faces wound outward; a second
+    medium is put inside, a third outside, meshes named in the config."""
+    tri = {"type": "mesh", "name": "ball", "positions": [0, 0, 0, 1, 0, 0, 0, 1, 0], "indices": [0, 1, 2]}
+    other = dict(tri, name="other")
+    s = krr.Scene({"model": [tri, other,
+                             {"type": "medium", "name": "Mist", "translate": [0, 5, 0],
+                              "params": {"g": 0.2, "sigma_a": [0.4, 0.1, 0.3], "sigma_s": [0.4, 0.3, 0.1],
+                                         "bound": [[-2, -1, -3], [2, 1, 3]]}},
+                             {"type": "medium", "params": {"sigma_t": [2, 2, 2], "meshes": ["ball"]}},
+                             {"type": "medium", "params": {"meshes_outside": ["other", "ball"]}}], "materials": MAT})
+    pos, idx, _, meshes, mats = _desc_arrays(krr, s)
+    d = s.desc
+    assert d.n_media == 3
+    m0 = d.media[0]
+    assert np.allclose(list(m0.sigma_t), [0.8, 0.4, 0.4]) and np.allclose(list(m0.albedo), [0.5, 0.75, 0.25])
+    assert abs(m0.g - 0.2) < 1e-7
+    assert list(d.media[1].sigma_t) == [2, 2, 2] and list(d.media[1].albedo) == [0.5, 0.5, 0.5]
+    box = meshes[2]
+    assert box.n_triangles == 12 and box.material == -1
+    assert box.medium_inside == 0 and box.medium_outside == -1
+    P = pos[idx[box.first_triangle:box.first_triangle + 12]].astype(np.float64)
+    assert np.allclose(P.min((0, 1)), [-2, 4, -3]) and np.allclose(P.max((0, 1)), [2, 6, 3])
+    centre = np.array([0, 5, 0])
+    n = np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0])
+    assert (np.einsum("ij,ij->i", n, P.mean(1) - centre) > 0).all()
+    assert abs(np.linalg.norm(n, axis=1).sum() / 2 - 2 * (4 * 2 + 2 * 6 + 4 * 6)) < 1e-3  # surface area
+    assert (meshes[0].medium_inside, meshes[0].medium_outside) == (1, 2)
+    assert (meshes[1].medium_inside, meshes[1].medium_outside) == (-1, 2)
+
+
+def test_reference_media_entry_form(krr):
+    """Top-level "media" entries in the reference's form, the type inside
+    "params" (krrscene.cpp:145-157), and this build's form with the type
+    beside the parameters."""
+    quad = {"type": "quad", "corners": [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], "material": 0}
+    s = krr.Scene({"model": [quad], "media": [{"name": "fog", "params": {"type": "homogeneous", "sigma_t": [3, 3, 3]}},
+                                              {"type": "homogeneous", "albedo": [0.1, 0.2, 0.3]}], "materials": MAT})
+    d = s.desc
+    assert d.n_media == 2 and list(d.media[0].sigma_t) == [3, 3, 3]
+    assert np.allclose(list(d.media[1].albedo), [0.1, 0.2, 0.3])
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_MIST), reason="KiRaRay tree not present")
+def test_reference_example_cboxmist_scene(krr):
+    """The reference's volumetric/example_cboxmist.json scene as it is: cbox.obj
+    and a mist node (sigma_a = sigma_s = 0.4, g 0.2, bound [-2, 2]^3)."""
+    cfg = json.load(open(REFERENCE_MIST))["scene"]
+    cfg["base_dir"] = "/root/reference"
+    s = krr.Scene(cfg)
+    _, _, _, meshes, mats = _desc_arrays(krr, s)
+    d = s.desc
+    assert s.n_triangles == 36 + 12 and d.n_media == 1
+    assert np.allclose(list(d.media[0].sigma_t), [0.8] * 3) and np.allclose(list(d.media[0].albedo), [0.5] * 3)
+    box = [m for m in meshes if m.medium_inside == 0]
+    assert len(box) == 1 and box[0].material == -1
 
 
 @pytest.mark.skipif(not os.path.exists(REFERENCE_CONFIG), reason="KiRaRay tree not present")
@@ -251,3 +354,27 @@ def test_obj_copy_has_the_same_arrays(krr, tmp_path):
     assert np.array_equal(a2.arrays()[2][a2.arrays()[1]], nb[ib])
     assert [list(m.emission) for m in ma] == [list(m.emission) for m in mb]
     assert [list(mta[m.material].diffuse) for m in ma] == [list(mtb[m.material].diffuse) for m in mb]
+
+
+@pytest.mark.gpu
+def test_mist_node_render_parity(krr, oracle, gpu):
+    """The Cornell box inside a mist node (sigma_a = sigma_s = 0.4, g 0.2, a
+    bound box without material that the camera looks into from outside), as the
+    reference's cbox mist scene sets it up: GPU and checker take the same
+    paths through the box interface and the medium."""
+    from test_gpu_render import assert_images_match, gpu_render
+    print("case: cornell box + homogeneous mist node 96x96 spp 2", flush=True)
+    cfg = dict(CORNELL)
+    cfg["model"] = list(CORNELL["model"]) + [
+        {"type": "medium", "name": "Mist",
+         "params": {"g": 0.2, "sigma_a": [0.4, 0.4, 0.4], "sigma_s": [0.4, 0.4, 0.4], "bound": [[-2, -2, -2], [2, 2, 2]]}}]
+    s = krr.Scene(cfg)
+    assert s.desc.n_media == 1
+    params = {"spp": 2, "max_depth": 10}
+    g, gs = gpu_render(krr, gpu, s, 96, 96, params, frame_index=3)
+    o, os_ = oracle.render(s.desc, s.camera(0), 96, 96, 3, spp=2, max_depth=10)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+    clear = krr.Scene(CORNELL)  # kept alive: desc points into the scene's arrays
+    plain, _ = oracle.render(clear.desc, s.camera(0), 96, 96, 3, spp=2, max_depth=10)
+    assert o[..., :3].mean() < plain[..., :3].mean()  # the mist absorbs half of what it extinguishes
