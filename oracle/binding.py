@@ -34,6 +34,8 @@ _PROTO = {
     "oracle_default_params": (None, [C.POINTER(oracle_params)]),
     "oracle_render": (C.c_int, [C.POINTER(krr_scene_desc), C.POINTER(krr_camera), C.POINTER(oracle_params), C.c_uint64,
                                 C.c_int32, C.c_int32, C.c_void_p, C.POINTER(oracle_stats)]),
+    "oracle_render_megakernel": (C.c_int, [C.POINTER(krr_scene_desc), C.POINTER(krr_camera), C.POINTER(oracle_params),
+                                           C.c_uint64, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(oracle_stats)]),
     "oracle_pcg32_sequence": (None, [C.c_uint64, C.c_uint64, C.c_int32, C.c_void_p]),
     "oracle_pcg32_advance": (None, [C.c_uint64, C.c_uint64, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "oracle_pixel_samples": (None, [C.c_int32, C.c_int32, C.c_uint32, C.c_int64, C.c_int32, C.c_void_p]),
@@ -113,6 +115,21 @@ def render(desc, cam, width, height, frame_index=1, film=None, **kw):
     rc = lib.oracle_render(C.byref(desc), C.byref(cam), C.byref(p), frame_index, width, height, _ptr(film), C.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
+    return film, {"closest_rays": st.closest_rays, "shadow_rays": st.shadow_rays, "paths": st.paths,
+                  "seconds": st.seconds, "threads": st.threads}
+
+
+def render_megakernel(desc, cam, width, height, frame_index=1, film=None, **kw):
+    """One frame of the megakernel path tracer -> (film[h, w, 4] float32, stats dict)."""
+    lib = load()
+    if film is None:
+        film = np.zeros((height, width, 4), np.float32)
+    st = oracle_stats()
+    p = params(**kw)
+    rc = lib.oracle_render_megakernel(C.byref(desc), C.byref(cam), C.byref(p), frame_index, width, height, _ptr(film),
+                                      C.byref(st))
+    if rc != 0:
+        raise RuntimeError(f"oracle_render_megakernel failed: {rc}")
     return film, {"closest_rays": st.closest_rays, "shadow_rays": st.shadow_rays, "paths": st.paths,
                   "seconds": st.seconds, "threads": st.threads}
 

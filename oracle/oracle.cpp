@@ -380,6 +380,106 @@ template <class R> struct Integrator {
 	}
 };
 
+// MegakernelPathTracer (src/render/megakernel/device.cu:50-195): each sample
+// runs its whole path in one loop.  Emission found by BSDF sampling and the
+// light sample of each smooth vertex are weighted by the power heuristic
+// (evalMIS(1, p0, 1, p1), src/render/sampling.h:8-28); Russian roulette with
+// probability rr from the first vertex on; no media.  Draw order per sample:
+// the camera sample (5), the wavelength (1), then per vertex the roulette
+// draw, the light choice (1) and position (2), and the BSDF sample.
+template <class R> struct MkIntegrator {
+	Integrator<R> base;
+	const SceneView &sv;
+	const oracle_params &P;
+	int nLights;
+	uint64_t closestRays = 0, shadowRays = 0;
+
+	MkIntegrator(const SceneView &s, const Tracer &t, const oracle_params &p) : base(s, t, p), sv(s), P(p), nLights((int) s.lights.size()) {}
+
+	static R mis(R p0, R p1) { // power heuristic, one sample each
+		R q0 = p0 * p0, q1 = p1 * p1;
+		return q0 / (q0 + q1);
+	}
+	static bool finite(R w) { return !(std::isnan((double) w) || std::isinf((double) w)); }
+
+	void renderPixel(int px, int py, int W, int H, uint32_t sampleIndex, const krr_camera &cam, V3<R> &color) {
+		Pcg sampler;
+		sampler.setSeed((uint64_t) interleave32((uint32_t) px, (uint32_t) py), (uint64_t) sampleIndex);
+		color = V3<R>();
+		const R selPdf = R(1.f / (float) std::max(nLights, 1));
+		for (int s = 0; s < P.spp; s++) {
+			V3<R> thp(R(1)), L;
+			float cs[5];
+			for (int i = 0; i < 5; i++) cs[i] = sampler.get1D();
+			Ray<R> ray = base.cameraRay(cam, px, py, W, H, cs);
+			(void) sampler.get1D(); // SampledWavelengths::sampleUniform
+			R pdf		 = R(0);
+			int bsdfType = BSDF_UNSET;
+			Ctx<R> ctx{};
+			for (int depth = 0;; depth++) {
+				closestRays++;
+				const Hit h = base.closest(ray, INFINITY);
+				if (h.prim < 0) { // handleMiss (device.cu:67-79)
+					for (const LightRec &Lr : sv.lights) {
+						if (Lr.type != 1) continue;
+						R w = R(1);
+						if (P.nee && depth > 0 && !(bsdfType & BSDF_SPECULAR)) {
+							w = mis(pdf, kInv4Pi<R>() * selPdf);
+							if (!finite(w)) w = R(1);
+						}
+						L += thp * w * (R(Lr.scale) * V3<R>(R(Lr.Le[0]), R(Lr.Le[1]), R(Lr.Le[2])));
+					}
+					break;
+				}
+				const SurfInt<R> intr = prepareSurfaceInteraction<R>(sv, h, ray);
+				if (intr.light >= 0) { // handleHit (device.cu:52-65)
+					const LightRec &Lr = sv.lights[intr.light];
+					const V3<R> Le	   = areaL<R>(Lr, intr.n, intr.wo);
+					R w				   = R(1);
+					if (P.nee && depth > 0) {
+						const R lightPdf = areaPdfLi<R>(sv, Lr, intr.p, intr.n, ctx) * selPdf;
+						if (!(bsdfType & BSDF_SPECULAR)) w = mis(pdf, lightPdf);
+						if (!finite(w)) w = R(1);
+					}
+					L += Le * w * thp;
+				}
+				if (depth == P.max_depth || (P.rr < 1.f && sampler.get1D() > P.rr)) break;
+				thp /= R(P.rr);
+				const V3<R> woLocal = intr.toLocal(intr.wo);
+				BSDF<R> bsdf(intr.sd);
+				if (P.nee && (intr.sd.bsdfType() & BSDF_SMOOTH) && nLights > 0) { // evalDirect (device.cu:101-106)
+					const float ul	  = sampler.get1D();
+					const int li	  = std::min((int) (uint32_t) (ul * (float) nLights), nLights - 1);
+					const R u0 = R(sampler.get1D()), u1 = R(sampler.get1D());
+					LightSample<R> ls = sampleLi<R>(sv, sv.lights[li], u0, u1, Ctx<R>{intr.p, intr.n});
+					const V3<R> wiLocal = intr.toLocal(normalize(ls.intr.p - intr.p));
+					const R lightPdf	= selPdf * ls.pdf;
+					if (lightPdf != R(0)) {
+						const R bsdfPdf		= bsdf.pdf(woLocal, wiLocal);
+						const V3<R> bsdfVal = bsdf.f(woLocal, wiLocal) * m_abs(wiLocal.z);
+						const R w			= mis(lightPdf, bsdfPdf);
+						if (finite(w) && bsdfVal.any()) {
+							shadowRays++;
+							if (base.unoccluded(intr.spawnRayTo(ls.intr), 1.f)) L += thp * bsdfVal * w / lightPdf * ls.L;
+						}
+					}
+				}
+				// generateScatterRay (device.cu:108-122)
+				const BSDFSample<R> bs = bsdf.sample(woLocal, sampler);
+				if (bs.pdf == R(0) || !bs.f.any()) break;
+				const V3<R> wiWorld = intr.toWorld(bs.wi);
+				bsdfType			= bs.flags;
+				ray					= intr.spawnRayTowards(wiWorld);
+				ctx					= {intr.p, intr.n};
+				pdf					= bs.pdf;
+				thp *= bs.f * m_abs(bs.wi.z) / bs.pdf;
+				if (!thp.any()) break;
+			}
+			color += L;
+		}
+	}
+};
+
 static void sceneRadius(const krr_scene_desc *d, SceneView &sv) {
 	if (d->n_vertices <= 0) return;
 	float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -392,10 +492,10 @@ static void sceneRadius(const krr_scene_desc *d, SceneView &sv) {
 	sv.sceneRadius = std::sqrt(e0 * e0 + e1 * e1 + e2 * e2);
 }
 
-template <class R>
+template <class R, class Integ = Integrator<R>>
 static void renderT(const SceneView &sv, const Tracer &tr, const krr_camera &cam, const oracle_params &p, uint64_t frameIndex,
-					int W, int H, float *film, oracle_stats *st) {
-	const uint32_t sampleIndex = (uint32_t) (frameIndex * (uint64_t) p.spp);
+					int W, int H, float *film, oracle_stats *st, uint32_t sampleIndexOverride = 0, bool useOverride = false) {
+	const uint32_t sampleIndex = useOverride ? sampleIndexOverride : (uint32_t) (frameIndex * (uint64_t) p.spp);
 	int rowEnd				   = p.row_end > 0 ? std::min(p.row_end, H) : H;
 	int rowStep				   = p.row_step > 0 ? p.row_step : 1;
 	int pixStep				   = p.pixel_step > 0 ? p.pixel_step : 1;
@@ -409,7 +509,7 @@ static void renderT(const SceneView &sv, const Tracer &tr, const krr_camera &cam
 #pragma omp single
 		threads = omp_get_num_threads();
 #endif
-		Integrator<R> integ(sv, tr, p);
+		Integ integ(sv, tr, p);
 #pragma omp for schedule(dynamic, 1)
 		for (int ri = 0; ri < (int) rows.size(); ri++) {
 			int y = rows[ri];
@@ -482,6 +582,41 @@ int oracle_render(const krr_scene_desc *scene, const krr_camera *camera, const o
 	auto t2 = std::chrono::steady_clock::now();
 	if (stats) stats->seconds = std::chrono::duration<double>(t2 - t1).count();
 	(void) t0;
+#ifdef _OPENMP
+	omp_set_num_threads(prevThreads);
+#endif
+	return KRR_OK;
+}
+
+int oracle_render_megakernel(const krr_scene_desc *scene, const krr_camera *camera, const oracle_params *p, uint64_t frame_index,
+							 int32_t width, int32_t height, float *film, oracle_stats *stats) {
+	if (!scene || !camera || !p || !film || width <= 0 || height <= 0) return KRR_ERR_INVALID_ARG;
+	for (int i = 0; i < scene->n_meshes; i++)
+		if (scene->meshes[i].material < 0) return KRR_ERR_UNSUPPORTED; // surfaces only
+#ifdef _OPENMP
+	int prevThreads = omp_get_max_threads();
+	if (p->n_threads > 0) omp_set_num_threads(p->n_threads);
+#endif
+	SceneView sv;
+	buildSceneView(scene, sv);
+	sceneRadius(scene, sv);
+	Bvh bvh;
+	Tracer tr;
+	tr.sv = &sv;
+	if (scene->n_triangles > 64) {
+		buildBvh(scene, bvh);
+		tr.bvh = &bvh;
+	}
+	oracle_params q	 = *p;
+	q.enable_clamp	 = 0;
+	q.legacy_spp_accumulate = 0;
+	// setPixelSample(pixel, frameID * 512) (device.cu:157)
+	const uint32_t sampleIndex = (uint32_t) frame_index * 512u;
+	auto t1 = std::chrono::steady_clock::now();
+	if (p->precision == 64) renderT<double, MkIntegrator<double>>(sv, tr, *camera, q, frame_index, width, height, film, stats, sampleIndex, true);
+	else renderT<float, MkIntegrator<float>>(sv, tr, *camera, q, frame_index, width, height, film, stats, sampleIndex, true);
+	auto t2 = std::chrono::steady_clock::now();
+	if (stats) stats->seconds = std::chrono::duration<double>(t2 - t1).count();
 #ifdef _OPENMP
 	omp_set_num_threads(prevThreads);
 #endif

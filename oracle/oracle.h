@@ -57,6 +57,16 @@ int oracle_render(const krr_scene_desc *scene, const krr_camera *camera, const o
 				  uint64_t frame_index, int32_t width, int32_t height, float *film,
 				  oracle_stats *stats);
 
+/* MegakernelPathTracer (src/render/megakernel/device.cu:50-195): one loop
+ * per sample, power-heuristic MIS, Russian roulette from the first vertex,
+ * no media; RNG seeded setPixelSample(pixel, frame_index * 512).  Uses nee,
+ * max_depth, rr, spp, precision, n_threads and the row/pixel subset of p;
+ * the film holds the mean of the spp samples.  Meshes without material are
+ * rejected (KRR_ERR_UNSUPPORTED). */
+int oracle_render_megakernel(const krr_scene_desc *scene, const krr_camera *camera, const oracle_params *p,
+							 uint64_t frame_index, int32_t width, int32_t height, float *film,
+							 oracle_stats *stats);
+
 /* Per-pixel ray counts of the last oracle_render call on this thread are not
  * kept; use the totals in oracle_stats. */
 
