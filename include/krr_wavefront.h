@@ -294,6 +294,49 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap);
 int krr_wf_destroy(krr_wf *wf);
 
 /* ---------------------------------------------------------------------- */
+/* MegakernelPathTracer render pass (src/render/megakernel/pathtracer.h:14-50, */
+/* device.cu:50-195): one lane per pixel runs each sample's whole path, with */
+/* the power heuristic between light and BSDF sampling, Russian roulette from */
+/* the first vertex, no media.  Same scene, camera and film conventions as   */
+/* the wavefront pass; an A/B baseline for it.                               */
+/* ---------------------------------------------------------------------- */
+
+typedef struct krr_mk krr_mk;
+
+/* json_params: the pass "params" object of the reference (pathtracer.h:37-44):
+ *   nee (true), max_depth (10), rr (0.8)
+ * plus keys this library adds:
+ *   spp (1)                  samples per pixel per frame (the film holds
+ *                            their mean; UI-only in the reference)
+ *   block_size (256)         threads per block, a multiple of 64
+ *   occupancy (1.0)          fraction of the resident blocks launched
+ *   time_kernels (false)     time the launch with HIP events
+ *   leaf_size, sah_node_cost, stack_lds_entries, bvh_builder  as for krr_wf_create
+ * NULL or "" means all defaults.  Replaces RenderPassFactory::deserialize<
+ * MegakernelPathTracer> + from_json (pathtracer.h:45-49). */
+int krr_mk_create(const char *json_params, krr_mk **out);
+/* Uploads the scene and builds the acceleration structure
+ * (MegakernelPathTracer::setScene, pathtracer.cpp:25-39).  Meshes without
+ * material (medium interfaces) are rejected with KRR_ERR_UNSUPPORTED. */
+int krr_mk_set_scene(krr_mk *mk, const krr_scene_desc *scene);
+/* MegakernelPathTracer::resize (pathtracer.cpp:20-23). */
+int krr_mk_resize(krr_mk *mk, int32_t width, int32_t height);
+/* Camera and 1-based frame index of the next render; the samplers of frame f
+ * start at setPixelSample(pixel, f * 512) (device.cu:157). */
+int krr_mk_begin_frame(krr_mk *mk, const krr_camera *camera, uint64_t frame_index);
+/* MegakernelPathTracer::render (pathtracer.cpp:52-69): RGBA32F, the mean of
+ * the spp samples, into film_dev (W x H float4, row H-1-y), queued on
+ * hip_stream; returns without waiting. */
+int krr_mk_render(krr_mk *mk, float *film_dev, void *hip_stream);
+/* closest_rays, shadow_rays, paths, render_ms (with time_kernels) and
+ * bvh_build_ms of the last render; synchronises its stream. */
+int krr_mk_stats(krr_mk *mk, krr_stats *out);
+/* Current params as JSON (to_json, pathtracer.h:37-44); as krr_wf_get_params. */
+int krr_mk_get_params(const krr_mk *mk, char *buf, int32_t cap);
+/* Frees every device allocation of the pass. */
+int krr_mk_destroy(krr_mk *mk);
+
+/* ---------------------------------------------------------------------- */
 /* Many camera views in one frame.  No reference counterpart: KiRaRay        */
 /* renders one view per frame (integrator.cpp:205-267); this batches views   */
 /* of uneven sizes into one persistent launch per sample.                    */

@@ -20,7 +20,7 @@ from . import _abi
 from ._abi import (krr_camera, krr_hit, krr_material, krr_medium, krr_mesh, krr_scene_desc, krr_stats, krr_view,
                    load_library)
 
-__all__ = ["KrrError", "Scene", "Accel", "WavefrontPathTracer", "RenderApp", "load_library", "error_metric", "accumulate",
+__all__ = ["KrrError", "Scene", "Accel", "WavefrontPathTracer", "MegakernelPathTracer", "RenderApp", "load_library", "error_metric", "accumulate",
            "krr_camera", "krr_hit", "krr_material", "krr_medium", "krr_mesh", "krr_scene_desc", "krr_stats", "krr_view"]
 
 
@@ -201,6 +201,64 @@ class WavefrontPathTracer:
     def close(self):
         if getattr(self, "_h", None):
             self._lib.krr_wf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MegakernelPathTracer:
+    """The MegakernelPathTracer render pass over the C ABI
+    (src/render/megakernel/pathtracer.h:14-50): one lane per pixel runs each
+    sample's whole path, power-heuristic MIS, no media.  Same hooks as
+    WavefrontPathTracer; end_frame does nothing."""
+
+    name = "MegakernelPathTracer"
+
+    def __init__(self, params=None):
+        self._lib = load_library()
+        h = C.c_void_p()
+        _check(self._lib.krr_mk_create(_json_arg(params), C.byref(h)))
+        self._h = h
+        self.size = (0, 0)
+
+    @classmethod
+    def from_json(cls, params):
+        return cls(params)
+
+    def to_json(self):
+        buf = C.create_string_buffer(4096)
+        _check(self._lib.krr_mk_get_params(self._h, buf, 4096))
+        return _json.loads(buf.value.decode())
+
+    def set_scene(self, scene):
+        desc = scene.desc if isinstance(scene, Scene) else scene
+        _check(self._lib.krr_mk_set_scene(self._h, C.byref(desc)))
+
+    def resize(self, width, height):
+        _check(self._lib.krr_mk_resize(self._h, int(width), int(height)))
+        self.size = (int(width), int(height))
+
+    def begin_frame(self, camera, frame_index):
+        _check(self._lib.krr_mk_begin_frame(self._h, C.byref(camera), int(frame_index)))
+
+    def render(self, film_ptr, stream=0):
+        _check(self._lib.krr_mk_render(self._h, C.c_void_p(film_ptr), C.c_void_p(stream)))
+
+    def end_frame(self):
+        pass
+
+    def stats(self):
+        s = krr_stats()
+        _check(self._lib.krr_mk_stats(self._h, C.byref(s)))
+        return {f: (float(getattr(s, f)) if t is C.c_double else int(getattr(s, f))) for f, t in krr_stats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.krr_mk_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -109,6 +109,14 @@ PROTOTYPES = {
     "krr_wf_stats": (C.c_int, [C.c_void_p, C.POINTER(krr_stats)]),
     "krr_wf_get_params": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
     "krr_wf_destroy": (C.c_int, [C.c_void_p]),
+    "krr_mk_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "krr_mk_set_scene": (C.c_int, [C.c_void_p, C.POINTER(krr_scene_desc)]),
+    "krr_mk_resize": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "krr_mk_begin_frame": (C.c_int, [C.c_void_p, C.POINTER(krr_camera), C.c_uint64]),
+    "krr_mk_render": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "krr_mk_stats": (C.c_int, [C.c_void_p, C.POINTER(krr_stats)]),
+    "krr_mk_get_params": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int32]),
+    "krr_mk_destroy": (C.c_int, [C.c_void_p]),
     "krr_wf_set_views": (C.c_int, [C.c_void_p, C.POINTER(krr_view), C.c_int32]),
     "krr_wf_render_views": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p]),
     "krr_wf_autotune": (C.c_int, [C.c_void_p, C.c_int64, C.c_char_p, C.c_int32, C.c_void_p]),

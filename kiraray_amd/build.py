@@ -25,7 +25,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contra
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
-HIP_SOURCES = ["wavefront.hip", "nrc.hip", "passes.hip", "bvh_build.hip"]
+HIP_SOURCES = ["wavefront.hip", "nrc.hip", "passes.hip", "bvh_build.hip", "pathtracer.hip"]
 CXX_SOURCES = ["scene.cpp", "bvh.cpp"]
 
 # C++ render-pass layer over the C ABI (libkrr_host.so) and its test driver.

@@ -243,4 +243,52 @@ KRR_DEV void storeRayItem(const RayQueueDev &q, int i, const RayItem &r) {
 	q.st[4 * i + 3]	 = make_float4(r.cn.x, r.cn.y, r.cn.z, __int_as_float((int) packMeta(r.depth, r.bsdfType, r.medium)));
 }
 
+// CameraData::generateSample + getRay (camera.h:32-58): the five camera draws
+// (pFilm 2, pLens 2, time 1) in order, pixel (x, y) of a W x H film.
+KRR_DEV RayItem cameraRay(const CameraDev &cam, int W, int H, int x, int y, Pcg &s) {
+	float cs[5];
+	for (int k = 0; k < 5; k++) cs[k] = s.get1D();
+	const float p0	 = (float) x + 0.5f + cs[0];
+	const float p1	 = (float) y + 0.5f + cs[1];
+	const float ndc0 = (2.f * p0) / (float) W + (-1.f);
+	const float ndc1 = (2.f * p1) / (float) H + (-1.f);
+	const float fov	 = detmath::atan2f(cam.filmH * 0.5f, cam.focalLength);
+	const f3 fd		 = normalize(f3(detmath::tanf(fov) * cam.aspect * ndc0, detmath::tanf(fov) * ndc1, -1.f));
+	f3 o, d;
+	if (cam.lensRadius > 1e-5f) {
+		float ax, ay;
+		uniformSampleDisk(cs[2], cs[3], ax, ay);
+		o = f3(cam.lensRadius * ax, cam.lensRadius * ay, 0.f);
+		d = normalize(fd * cam.focalDistance - o);
+	} else {
+		o = f3();
+		d = fd;
+	}
+	const float *M = cam.M;
+	RayItem r;
+	r.o		   = f3(M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
+					M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]);
+	r.d		   = f3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z, M[8] * d.x + M[9] * d.y + M[10] * d.z);
+	r.thp	   = f3(1.f);
+	r.pu	   = f3(1.f);
+	r.pl	   = f3(1.f);
+	r.cp	   = f3();
+	r.cn	   = f3();
+	r.depth	   = 0;
+	r.bsdfType = BSDF_UNSET;
+	r.medium   = cam.medium;
+	r.pixel	   = 0;
+	r.sp0	   = 0.f;
+	r.spSum	   = 0.f;
+	r.pdfPrev  = 0.f;
+	return r;
+}
+
+// This lane's traversal stack: LDS slots of the block, overflow slots of the
+// grid (ovf holds (stack size - kShortStack) * grid threads ints).
+KRR_DEV LaneStack laneStack(const TraceBvh &bvh, int *lds, int *ovf) {
+	return LaneStack{lds + threadIdx.x, (int) blockDim.x, ovf + (size_t) blockIdx.x * blockDim.x + threadIdx.x, (int) (gridDim.x * blockDim.x),
+					 bvh.stackLds};
+}
+
 } // namespace krr

@@ -29,6 +29,7 @@
 #include "json.h"
 #include "kernels.h"
 #include "nrc_dev.h"
+#include "pathtracer.h"
 #include "krr_wavefront.h"
 
 namespace krr {
@@ -77,47 +78,6 @@ __global__ void k_begin_frame(WfParams P, PixelStateDev px) {
 	int ch				= (int) ((lambda0 - 360.f) / 470.f * 3);
 	px.channel[i]		= ch < 0 ? 0 : (ch > 2 ? 2 : ch);
 	px.rng[i]			= s.state;
-}
-
-// CameraData::generateSample + getRay (camera.h:32-58): the five camera draws
-// (pFilm 2, pLens 2, time 1) in order, pixel (x, y) of a W x H film.
-KRR_DEV RayItem cameraRay(const CameraDev &cam, int W, int H, int x, int y, Pcg &s) {
-	float cs[5];
-	for (int k = 0; k < 5; k++) cs[k] = s.get1D();
-	const float p0	 = (float) x + 0.5f + cs[0];
-	const float p1	 = (float) y + 0.5f + cs[1];
-	const float ndc0 = (2.f * p0) / (float) W + (-1.f);
-	const float ndc1 = (2.f * p1) / (float) H + (-1.f);
-	const float fov	 = detmath::atan2f(cam.filmH * 0.5f, cam.focalLength);
-	const f3 fd		 = normalize(f3(detmath::tanf(fov) * cam.aspect * ndc0, detmath::tanf(fov) * ndc1, -1.f));
-	f3 o, d;
-	if (cam.lensRadius > 1e-5f) {
-		float ax, ay;
-		uniformSampleDisk(cs[2], cs[3], ax, ay);
-		o = f3(cam.lensRadius * ax, cam.lensRadius * ay, 0.f);
-		d = normalize(fd * cam.focalDistance - o);
-	} else {
-		o = f3();
-		d = fd;
-	}
-	const float *M = cam.M;
-	RayItem r;
-	r.o		   = f3(M[0] * o.x + M[1] * o.y + M[2] * o.z + M[3], M[4] * o.x + M[5] * o.y + M[6] * o.z + M[7],
-					M[8] * o.x + M[9] * o.y + M[10] * o.z + M[11]);
-	r.d		   = f3(M[0] * d.x + M[1] * d.y + M[2] * d.z, M[4] * d.x + M[5] * d.y + M[6] * d.z, M[8] * d.x + M[9] * d.y + M[10] * d.z);
-	r.thp	   = f3(1.f);
-	r.pu	   = f3(1.f);
-	r.pl	   = f3(1.f);
-	r.cp	   = f3();
-	r.cn	   = f3();
-	r.depth	   = 0;
-	r.bsdfType = BSDF_UNSET;
-	r.medium   = cam.medium;
-	r.pixel	   = 0;
-	r.sp0	   = 0.f;
-	r.spSum	   = 0.f;
-	r.pdfPrev  = 0.f;
-	return r;
 }
 
 // generateCameraRays (integrator.cpp:271-284).  Camera rays go to slot =
@@ -213,13 +173,6 @@ __global__ void k_depth_begin(Counters *cnt, int cur) {
 	cnt->rayCount[nxt] = 0;
 	cnt->shadowCount   = 0;
 	cnt->work[0] = cnt->work[1] = 0;
-}
-
-// This lane's traversal stack: LDS slots of the block, overflow slots of the
-// grid (ovf holds (stack size - kShortStack) * grid threads ints).
-KRR_DEV LaneStack laneStack(const TraceBvh &bvh, int *lds, int *ovf) {
-	return LaneStack{lds + threadIdx.x, (int) blockDim.x, ovf + (size_t) blockIdx.x * blockDim.x + threadIdx.x, (int) (gridDim.x * blockDim.x),
-					 bvh.stackLds};
 }
 
 // Closest-hit trace of one ray queue entry into its hit record.
@@ -2400,6 +2353,222 @@ int krr_error_metric(int32_t metric, const float *img_dev, const float *ref_dev,
 	(void) hipFree(acc);
 	if (e != hipSuccess) return krr::fail(KRR_ERR_HIP, "krr_error_metric: %s", hipGetErrorString(e));
 	*out = sum / (double) n_pixels;
+	return KRR_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// MegakernelPathTracer (src/render/megakernel/pathtracer.{h,cpp}): the scene
+// and acceleration structure live in an inner wavefront pass that never
+// allocates frame state; the kernel is in pathtracer.hip.
+// ---------------------------------------------------------------------------
+struct krr_mk {
+	krr_wf *wf = nullptr;
+	bool nee = true, timeKernels = false;
+	int maxDepth = 10, spp = 1, blockSize = 256;
+	float rr = 0.8f, occupancy = 1.f;
+	int W = 0, H = 0;
+	krr_camera cam{};
+	bool haveCamera	 = false;
+	uint64_t frameIndex = 0;
+	DevBuf rays, ovf;
+	int grid		 = 0;
+	uint64_t gridKey = ~0ull;
+	hipStream_t lastStream = nullptr;
+	hipEvent_t ev[2]	   = {nullptr, nullptr};
+	bool timed			   = false;
+	uint64_t paths		   = 0;
+	~krr_mk() {
+		for (hipEvent_t e : ev)
+			if (e) (void) hipEventDestroy(e);
+		if (wf) krr_wf_destroy(wf);
+	}
+};
+
+namespace {
+
+int parseMkParams(krr_mk *mk, const char *json, std::string &inner) {
+	Json in = Json::object();
+	if (json && *json) {
+		try {
+			Json j = Json::parse(json);
+			if (!j.isObject()) return krr::fail(KRR_ERR_BAD_CONFIG, "pass params must be a JSON object");
+			mk->nee			= j.value("nee", true);
+			mk->maxDepth	= (int) j.value("max_depth", 10.0);
+			mk->rr			= (float) j.value("rr", 0.8);
+			mk->spp			= (int) j.value("spp", 1.0);
+			mk->blockSize	= (int) j.value("block_size", 256.0);
+			mk->occupancy	= (float) j.value("occupancy", 1.0);
+			mk->timeKernels = j.value("time_kernels", false);
+			for (const char *k : {"leaf_size", "sah_node_cost", "stack_lds_entries", "bvh_builder"})
+				if (j.contains(k)) in.set(k, j.at(k));
+		} catch (const std::exception &e) {
+			return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
+		}
+	}
+	if (mk->maxDepth < 0 || mk->maxDepth > 250) return krr::fail(KRR_ERR_BAD_CONFIG, "max_depth must be in [0, 250]");
+	if (!(mk->rr > 0.f && mk->rr <= 1.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "rr must be in (0, 1]");
+	if (mk->spp < 1 || mk->spp > 1 << 20) return krr::fail(KRR_ERR_BAD_CONFIG, "spp must be >= 1");
+	if (mk->blockSize < 64 || mk->blockSize > 256 || mk->blockSize % 64) return krr::fail(KRR_ERR_BAD_CONFIG, "block_size must be 64, 128, 192 or 256");
+	if (!(mk->occupancy > 0.f && mk->occupancy <= 1.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "occupancy must be in (0, 1]");
+	inner = in.dump();
+	return KRR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int krr_mk_create(const char *json_params, krr_mk **out) {
+	krr::clearError();
+	if (!out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_create: null out");
+	*out	= nullptr;
+	auto mk = std::make_unique<krr_mk>();
+	std::string inner;
+	if (int rc = parseMkParams(mk.get(), json_params, inner)) return rc;
+	if (int rc = krr_wf_create(inner.c_str(), &mk->wf)) return rc;
+	*out = mk.release();
+	return KRR_OK;
+}
+
+int krr_mk_get_params(const krr_mk *mk, char *buf, int32_t cap) {
+	if (!mk) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_get_params: null handle");
+	Json j = Json::object();
+	j.set("nee", Json(mk->nee));
+	j.set("max_depth", Json((double) mk->maxDepth));
+	j.set("rr", Json((double) mk->rr));
+	j.set("spp", Json((double) mk->spp));
+	j.set("block_size", Json((double) mk->blockSize));
+	j.set("occupancy", Json((double) mk->occupancy));
+	j.set("time_kernels", Json(mk->timeKernels));
+	j.set("leaf_size", Json((double) mk->wf->leafSize));
+	j.set("sah_node_cost", Json((double) mk->wf->sahNodeCost));
+	j.set("stack_lds_entries", Json((double) mk->wf->stackLdsEntries));
+	j.set("bvh_builder", Json(std::string(mk->wf->deviceBuilder ? "device" : "host")));
+	std::string s = j.dump();
+	if (buf && cap > 0) {
+		size_t n = std::min((size_t) cap - 1, s.size());
+		std::memcpy(buf, s.data(), n);
+		buf[n] = 0;
+	}
+	return (int) s.size();
+}
+
+int krr_mk_set_scene(krr_mk *mk, const krr_scene_desc *scene) {
+	krr::clearError();
+	if (!mk || !scene) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_set_scene: null argument");
+	if (scene->n_meshes > 0 && !scene->meshes) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing meshes");
+	for (int i = 0; i < scene->n_meshes; i++)
+		if (scene->meshes[i].material < 0)
+			return krr::fail(KRR_ERR_UNSUPPORTED, "krr_mk_set_scene: mesh %d has no material (the megakernel path tracer renders surfaces only)", i);
+	mk->gridKey = ~0ull;
+	return krr_wf_set_scene(mk->wf, scene);
+}
+
+int krr_mk_resize(krr_mk *mk, int32_t width, int32_t height) {
+	krr::clearError();
+	if (!mk || width < 0 || height < 0) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_resize: bad argument");
+	if ((int64_t) width * height > (int64_t) 1 << 26) return krr::fail(KRR_ERR_CAPACITY, "krr_mk_resize: at most 2^26 pixels per frame");
+	if (width > 65536 || height > 65536) return krr::fail(KRR_ERR_CAPACITY, "krr_mk_resize: width and height must be <= 65536");
+	mk->W = width;
+	mk->H = height;
+	return KRR_OK;
+}
+
+int krr_mk_begin_frame(krr_mk *mk, const krr_camera *camera, uint64_t frame_index) {
+	krr::clearError();
+	if (!mk || !camera) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_begin_frame: null argument");
+	mk->cam		   = *camera;
+	mk->frameIndex = frame_index;
+	mk->haveCamera = true;
+	return KRR_OK;
+}
+
+int krr_mk_render(krr_mk *mk, float *film_dev, void *hip_stream) {
+	krr::clearError();
+	if (!mk) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_render: null handle");
+	krr_wf *wf = mk->wf;
+	if (!wf->haveScene) return krr::fail(KRR_ERR_NO_SCENE, "krr_mk_render: no scene set");
+	if (mk->W == 0 || mk->H == 0) return KRR_OK; // pathtracer.cpp:53 early return
+	if (!film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_render: null film");
+	if (!mk->haveCamera) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_render: no camera (call krr_mk_begin_frame)");
+	hipStream_t st	   = (hipStream_t) hip_stream;
+	mk->lastStream	   = st;
+	wf->accel.stackLds = wf->stackLdsEntries;
+	const TraceBvh bvh = wf->accel.dev();
+	const int tb	   = mk->blockSize;
+	const size_t lds   = stackLds(wf->accel, tb);
+	const int tilesX = (mk->W + 7) / 8, tilesY = (mk->H + 7) / 8;
+	const int nTiles = tilesX * tilesY;
+	const int needed = (nTiles + tb / 64 - 1) / (tb / 64);
+	const uint64_t key = ((uint64_t) nTiles << 20) ^ ((uint64_t) tb << 8) ^ ((uint64_t) wf->accel.stackSize << 44) ^
+						 ((uint64_t) wf->stackLdsEntries << 36) ^ ((uint64_t) (mk->occupancy * 1024.f) << 52);
+	if (key != mk->gridKey) {
+		int dev = 0, cus = 0;
+		KRR_HIP(hipGetDevice(&dev));
+		KRR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+		const int resident = scaledGrid(pathTracerBlocksPerCU(tb, lds) * std::max(cus, 1), mk->occupancy);
+		mk->grid		   = std::max(1, std::min(needed, resident));
+		if (int rc = reserveOverflow(mk->ovf, wf->accel, (size_t) mk->grid * tb)) return rc;
+		if (!mk->rays.p) KRR_HIP(mk->rays.alloc(2 * sizeof(unsigned long long)));
+		mk->gridKey = key;
+	}
+	if ((size_t) (bvh.stackLds + 1) * tb * sizeof(int) != lds) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_render: stack layout mismatch");
+	PtArgs a;
+	a.sc		  = wf->ds;
+	a.bvh		  = bvh;
+	a.cam		  = cameraDev(mk->cam);
+	a.W			  = mk->W;
+	a.H			  = mk->H;
+	a.spp		  = mk->spp;
+	a.maxDepth	  = mk->maxDepth;
+	a.nee		  = mk->nee ? 1 : 0;
+	a.rr		  = mk->rr;
+	a.sampleIndex = (uint32_t) mk->frameIndex * 512u;
+	a.film		  = (float4 *) film_dev;
+	a.ovf		  = mk->ovf.as<int>();
+	a.rays		  = mk->rays.as<unsigned long long>();
+	a.tilesX	  = tilesX;
+	a.nTiles	  = nTiles;
+	mk->timed	  = mk->timeKernels;
+	if (mk->timed) {
+		for (hipEvent_t &e : mk->ev)
+			if (!e) KRR_HIP(hipEventCreate(&e));
+		KRR_HIP(hipEventRecord(mk->ev[0], st));
+	}
+	KRR_HIP(launchPathTracer(a, mk->grid, tb, lds, st));
+	if (mk->timed) KRR_HIP(hipEventRecord(mk->ev[1], st));
+	mk->paths = (uint64_t) mk->W * mk->H * mk->spp;
+	return KRR_OK;
+}
+
+int krr_mk_stats(krr_mk *mk, krr_stats *out) {
+	krr::clearError();
+	if (!mk || !out) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_stats: null argument");
+	std::memset(out, 0, sizeof(*out));
+	out->bvh_build_ms = mk->wf->accel.buildSeconds * 1e3;
+	if (!mk->rays.p) return KRR_OK;
+	KRR_HIP(hipStreamSynchronize(mk->lastStream));
+	unsigned long long r[2] = {0, 0};
+	KRR_HIP(hipMemcpy(r, mk->rays.p, sizeof(r), hipMemcpyDeviceToHost));
+	out->closest_rays = r[0];
+	out->shadow_rays  = r[1];
+	out->paths		  = mk->paths;
+	if (mk->timed) {
+		float t = 0.f;
+		KRR_HIP(hipEventSynchronize(mk->ev[1]));
+		KRR_HIP(hipEventElapsedTime(&t, mk->ev[0], mk->ev[1]));
+		out->render_ms = t;
+	}
+	return KRR_OK;
+}
+
+int krr_mk_destroy(krr_mk *mk) {
+	if (mk) {
+		if (deviceAvailable()) (void) hipDeviceSynchronize();
+		delete mk;
+	}
 	return KRR_OK;
 }
 

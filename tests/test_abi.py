@@ -150,3 +150,34 @@ def test_accel_builder_option_is_checked(krr):
     with pytest.raises(krr.KrrError) as e:
         krr.Accel(s.desc, {"builder": "gpu"})
     assert e.value.code == -2
+
+
+def test_megakernel_pass_params(krr):
+    """MegakernelPathTracer params: the reference's keys and defaults
+    (pathtracer.h:37-49), this library's additions, and their checks."""
+    d = krr.MegakernelPathTracer(None).to_json()
+    assert d["nee"] is True and d["max_depth"] == 10 and abs(d["rr"] - 0.8) < 1e-7
+    assert d["spp"] == 1 and d["block_size"] == 256 and d["stack_lds_entries"] == 16 and d["bvh_builder"] == "device"
+    p = krr.MegakernelPathTracer({"nee": False, "max_depth": 5, "rr": 1.0, "spp": 2, "leaf_size": 2}).to_json()
+    assert p["nee"] is False and p["max_depth"] == 5 and p["rr"] == 1.0 and p["spp"] == 2 and p["leaf_size"] == 2
+    for bad in ["{", '{"max_depth": -1}', '{"rr": 0}', '{"rr": 1.5}', '{"spp": 0}', '{"block_size": 96}',
+                '{"occupancy": 0}', '{"stack_lds_entries": 97}', '{"bvh_builder": "gpu"}']:
+        with pytest.raises(krr.KrrError) as e:
+            krr.MegakernelPathTracer(bad)
+        assert e.value.code == -2, bad
+    mk = krr.MegakernelPathTracer(None)
+    with pytest.raises(krr.KrrError) as e:
+        mk.render(0)
+    assert e.value.code == -3
+
+
+def test_megakernel_rejects_medium_interfaces(krr):
+    """Meshes without material are medium interfaces, which the megakernel
+    integrator does not handle: set_scene refuses them before any device
+    work."""
+    s = krr.Scene({"model": [{"type": "cornell_box"},
+                             {"type": "medium", "params": {"bound": [[-1, -1, -1], [1, 1, 1]]}}]})
+    mk = krr.MegakernelPathTracer(None)
+    with pytest.raises(krr.KrrError) as e:
+        mk.set_scene(s)
+    assert e.value.code == -7  # KRR_ERR_UNSUPPORTED
