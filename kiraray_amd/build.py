@@ -30,7 +30,7 @@ CXX_SOURCES = ["scene.cpp", "bvh.cpp"]
 
 # C++ render-pass layer over the C ABI (libkrr_host.so) and its test driver.
 HOST_DIR = os.path.join(HERE, "host")
-HOST_SOURCES = ["renderpass.cpp", "wavefront_pass.cpp", "render_app.cpp", "passes.cpp"]
+HOST_SOURCES = ["renderpass.cpp", "wavefront_pass.cpp", "megakernel_pass.cpp", "render_app.cpp", "passes.cpp"]
 HOST_TEST = os.path.join(ROOT, "tests", "cpp", "host_tests.cpp")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HOST_FLAGS = CXX_FLAGS + ["-D__HIP_PLATFORM_AMD__", "-isystem", os.path.join(ROCM, "include"), "-I" + HOST_DIR]

@@ -8,6 +8,7 @@
 //                               writes the colour texture (float32 RGBA,
 //                               row-major as stored) to OUT and prints the
 //                               ray counts of the last frame
+//   host_tests render_mk W H F OUT  the same through MegakernelPathTracer
 //   host_tests measure F REF OUT IMG  Cornell 256x256 through the passes
 //                               [WavefrontPathTracer, AccumulatePass,
 //                               ErrorMeasurePass(REF .npy, every 4 frames,
@@ -19,6 +20,7 @@
 #include <cstring>
 #include <string>
 
+#include "megakernel_pass.h"
 #include "passes.h"
 #include "render_app.h"
 #include "wavefront_pass.h"
@@ -53,6 +55,13 @@ static int testFactory() {
 	auto *w = dynamic_cast<WavefrontPathTracer *>(q.get());
 	EXPECT(w != nullptr);
 	EXPECT(w && w->maxDepth == 5 && !w->enableNEE && w->enableMedium && std::fabs(w->probRR - 0.8f) < 1e-7f);
+	// MegakernelPathTracer: the reference's keys and defaults (pathtracer.h:45-49)
+	auto m	 = RenderPassFactory::deserizeInstance("MegakernelPathTracer", json::parse(R"({"rr": 0.5, "spp": 2})"));
+	auto *mk = dynamic_cast<MegakernelPathTracer *>(m.get());
+	EXPECT(mk && mk->getName() == "MegakernelPathTracer");
+	EXPECT(mk && mk->enableNEE && mk->maxDepth == 10 && std::fabs(mk->probRR - 0.5f) < 1e-7f);
+	EXPECT(mk && mk->getParams().value("spp", 0.0) == 2.0);
+	EXPECT(mk && mk->getParams().value("rr", 0.0) == 0.5);
 	return 0;
 }
 
@@ -131,12 +140,21 @@ static int testConfig() {
 		code = e.code();
 	}
 	EXPECT(code == KRR_ERR_BAD_CONFIG);
+	// the pass list of the reference's example_megakernel.json
+	RenderApp mapp;
+	mapp.loadConfig(json::parse(std::string(R"({"resolution": [750, 750], "passes": [
+		{"enable": true, "name": "MegakernelPathTracer", "params": {"nee": true, "rr": 0.8, "max_depth": 10}},
+		{"enable": true, "name": "AccumulatePass", "params": {"spp": 0, "mode": "accumulate"}},
+		{"enable": false, "name": "DenoisePass"},
+		{"enable": true, "name": "ToneMappingPass", "params": {"exposure": 0.5, "operator": "aces"}}],
+		"scene": )") + kCornell + "}"));
+	EXPECT(mapp.getRenderPasses().size() == 2 && mapp.getRenderPasses()[0]->getName() == "MegakernelPathTracer");
 	return 0;
 }
 
-static int testRender(int w, int h, int frames, const char *out) {
+template <class Pass> static int testRender(int w, int h, int frames, const char *out) {
 	RenderApp app;
-	auto pass = std::make_shared<WavefrontPathTracer>();
+	auto pass = std::make_shared<Pass>();
 	from_json(json::parse(R"({"max_depth": 8, "spp": 2})"), *pass);
 	app.addRenderPassToBack(pass);
 	app.setScene(Scene::fromJson(kCornell));
@@ -187,7 +205,10 @@ int main(int argc, char **argv) {
 		if (cmd == "factory") rc = testFactory();
 		else if (cmd == "params") rc = testParams();
 		else if (cmd == "config") rc = testConfig();
-		else if (cmd == "render" && argc == 6) rc = testRender(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
+		else if (cmd == "render" && argc == 6)
+			rc = testRender<WavefrontPathTracer>(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
+		else if (cmd == "render_mk" && argc == 6)
+			rc = testRender<MegakernelPathTracer>(std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]), argv[5]);
 		else if (cmd == "measure" && argc == 6) rc = testMeasure(std::atoi(argv[2]), argv[3], argv[4], argv[5]);
 		else {
 			std::fprintf(stderr, "unknown command %s\n", cmd.c_str());

@@ -82,3 +82,23 @@ def test_accumulate_and_error_measure_passes(oracle, krr, tmp_path):
     print("series", [round(v, 4) for v in vals], "host", rel)
     assert abs(vals[-1] - rel) <= 1e-6 * rel
     assert 4.0 < vals[0] / vals[-1] < 12.0  # 4 -> 32 frames: ~8x
+
+
+@pytest.mark.gpu
+def test_render_app_megakernel_matches_checker(oracle, krr, tmp_path):
+    """Three frames of the Cornell box through RenderApp ->
+    MegakernelPathTracer: the colour texture holds frame 3, equal to the
+    checker's megakernel frame 3."""
+    print("case: RenderApp cornell 80x60 MegakernelPathTracer frames 1..3 spp 2", flush=True)
+    out = tmp_path / "film_mk.bin"
+    r = _run("render_mk", 80, 60, 3, out)
+    assert r.returncode == 0, r.stderr
+    st = json.loads(r.stdout.strip().splitlines()[-1])
+    g = np.fromfile(out, np.float32).reshape(60, 80, 4)
+    s = krr.Scene(CORNELL)
+    cam = s.camera(0)
+    cam.aspect_ratio = 80 / 60
+    o, ost = oracle.render_megakernel(s.desc, cam, 80, 60, 3, spp=2, max_depth=8)
+    assert st["closest_rays"] == ost["closest_rays"] and st["shadow_rays"] == ost["shadow_rays"]
+    d = np.abs(g[..., :3] - o[..., :3])
+    assert (d <= 1e-4 * np.abs(o[..., :3]) + 1e-6).all(), d.max()
