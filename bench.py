@@ -470,6 +470,73 @@ def multiview_bench(frames=4, warmup=1, ref_frames=16, nrc_warm=8, nrc_params=No
             "plain": plain, "nrc": nrc}
 
 
+def megakernel_ab(cfg, frames=5, warmup=2):
+    """The megakernel path tracer (one lane per pixel, MegakernelPathTracer)
+    against the wavefront pass on the same scene, view, spp and max depth,
+    both with the reference's Russian roulette (rr 0.8 from the first vertex,
+    as MegakernelPathTracer has it): ms per frame and Mrays/s of each.  Each
+    side gets its best launch shape (the wavefront pass from its autotuner,
+    the megakernel from block sizes 64 / 128 / 256)."""
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    w, h = cfg["scene"]["resolution"]
+    scene = krr.Scene(cfg["scene"])
+    cam = scene.camera(0)
+    base = {"spp": cfg["params"]["spp"], "max_depth": cfg["params"]["max_depth"], "rr": 0.8}
+    film = hip.DeviceBuffer(w * h * 16)
+    stream = hip.Stream()
+
+    def run(p, first, count):
+        st_sum = dict(closest=0, shadow=0, paths=0)
+        ev0, ev1 = hip.Event(), hip.Event()
+        ev0.record(stream)
+        for i in range(count):
+            p.begin_frame(cam, first + i)
+            p.render(film.ptr, stream.handle)
+            p.end_frame()
+            st = p.stats()
+            st_sum["closest"] += st["closest_rays"]
+            st_sum["shadow"] += st["shadow_rays"]
+            st_sum["paths"] += st["paths"]
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_ms(ev1) / count
+        rays = (st_sum["closest"] + st_sum["shadow"]) / count
+        return {"ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / ms / 1e3, 1),
+                "mpaths_per_s": round(st_sum["paths"] / count / ms / 1e3, 1), "rays_per_frame": int(rays)}
+
+    out = {"workload": cfg["desc"].replace("_rr3", "") + "_rr0.8_from_depth0", "frames": frames}
+    wf = krr.WavefrontPathTracer(dict(base, rr_min_depth=0))
+    wf.set_scene(scene)
+    wf.resize(w, h)
+    wf.begin_frame(cam, 1)
+    wf.autotune(stream=stream.handle)
+    run(wf, 500, warmup)
+    out["wavefront"] = run(wf, 1, frames)
+    wf.close()
+    best = None
+    for tb in (64, 128, 256):
+        mk = krr.MegakernelPathTracer(dict(base, block_size=tb))
+        mk.set_scene(scene)
+        mk.resize(w, h)
+        run(mk, 500, 1)
+        r = run(mk, 1, max(2, frames // 2))
+        log(f"megakernel block {tb}: {r}")
+        if best is None or r["ms_per_frame"] < best[1]["ms_per_frame"]:
+            best = (tb, r)
+        mk.close()
+    mk = krr.MegakernelPathTracer(dict(base, block_size=best[0]))
+    mk.set_scene(scene)
+    mk.resize(w, h)
+    run(mk, 500, warmup)
+    out["megakernel"] = dict(run(mk, 1, frames), block_size=best[0])
+    mk.close()
+    out["wavefront_speedup"] = round(out["megakernel"]["ms_per_frame"] / out["wavefront"]["ms_per_frame"], 2)
+    log(f"megakernel A/B: {out}")
+    return out
+
+
 def C_void(p):
     import ctypes
     return ctypes.c_void_p(p)
@@ -529,7 +596,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
-    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview"],
+    ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel path tracer A/B")
+    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel"],
                     help="nrc / multiview: run only that comparison and print its JSON (profiling runs)")
     ap.add_argument("--nrc-warm", type=int, default=96, help="training frames before the NRC comparison")
     ap.add_argument("--nrc-frames", type=int, default=24, help="compared frames of the NRC comparison")
@@ -551,6 +619,8 @@ def main():
         if args.section == "nrc":
             out = {"icosphere_field_1080p": nrc_equal_time(CONFIGS["icosphere_field_1080p"], warm_frames=args.nrc_warm,
                                                            frames=args.nrc_frames)}
+        elif args.section == "megakernel":
+            out = megakernel_ab(CONFIGS["icosphere_field_1080p"])
         else:
             out = multiview_bench()
         print(json.dumps(out), flush=True)
@@ -583,6 +653,8 @@ def main():
                                                                    nrc_params={"steps_per_frame": 2, "batch": 32768})}
     if args.config == "icosphere_field_1080p" and not args.no_multiview:
         line["multiview"] = multiview_bench()
+    if args.config == "icosphere_field_1080p" and not args.no_megakernel:
+        line["megakernel_ab"] = megakernel_ab(cfg)
     print(json.dumps(line), flush=True)
 
 
