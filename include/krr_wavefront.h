@@ -92,11 +92,20 @@ typedef struct krr_medium {
 	float bounds_max[3];
 } krr_medium;
 
-/* Constant-radiance infinite light (rt::InfiniteLight without an image,
- * src/core/light.h:208-259). */
+/* Infinite light (rt::InfiniteLight, src/core/light.h:208-259): radiance
+ * scale * tint, times image(worldToLatLong(rotation^T * w)) when an image is
+ * given (math_utils.h:133-139: u = atan2(x, -z) / 2pi + 1/2, v = acos(y) / pi).
+ * The image is RGBA float32, image_height rows of image_width texels, row 0
+ * at v = 0 (the +y pole), sampled bilinearly with wrap-around in both
+ * directions as a normalised-coordinate texture.  rotation is row-major 3x3;
+ * all zeros means identity.  Light samples are uniform over the sphere, as
+ * in the reference. */
 typedef struct krr_infinite_light {
 	float tint[3];
 	float scale;
+	float rotation[9];
+	int32_t image_width, image_height;
+	const float *image_rgba; /* host memory, copied by krr_wf_set_scene; NULL = constant */
 } krr_infinite_light;
 
 /* Scene handed to the tracer: flattened world-space triangle soup. */

@@ -44,7 +44,8 @@ class krr_medium(C.Structure):
 
 
 class krr_infinite_light(C.Structure):
-    _fields_ = [("tint", C.c_float * 3), ("scale", C.c_float)]
+    _fields_ = [("tint", C.c_float * 3), ("scale", C.c_float), ("rotation", C.c_float * 9),
+                ("image_width", C.c_int32), ("image_height", C.c_int32), ("image_rgba", C.POINTER(C.c_float))]
 
 
 class krr_scene_desc(C.Structure):

@@ -249,4 +249,18 @@ DM_HD inline float atan2f(float yf, float xf) {
 	return (float) (y < 0 ? -a : a);
 }
 
+// acos(x) for x in [-1, 1] (clamped), from the binary64 arctangent:
+// acos(x) = atan(sqrt(1 - x^2) / x), reflected for x < 0.
+DM_HD inline float acosf(float xf) {
+	const double PI = 3.14159265358979323846;
+	if (is_nan((double) xf)) return __builtin_nanf("");
+	double x = xf;
+	if (x > 1.0) x = 1.0;
+	if (x < -1.0) x = -1.0;
+	const double ax = x < 0 ? -x : x;
+	const double s	= __builtin_sqrt(1.0 - x * x);
+	const double a	= ax == 0.0 ? 1.57079632679489661923 : atan_pos(s / ax);
+	return (float) (x < 0 ? PI - a : a);
+}
+
 } // namespace detmath

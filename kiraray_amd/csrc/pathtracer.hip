@@ -50,7 +50,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 						w = misPower(pdf, kInv4Pi * selPdf);
 						if (!finiteWeight(w)) w = 1.f;
 					}
-					L = L + thp * w * (Lr.scale * f3(Lr.Le[0], Lr.Le[1], Lr.Le[2]));
+					L = L + thp * w * infiniteLi(Lr, d);
 				}
 				break;
 			}

@@ -39,6 +39,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <sstream>
 #include <memory>
@@ -59,6 +60,7 @@ struct krr_scene {
 	std::vector<krr_medium> media;
 	std::vector<std::unique_ptr<std::vector<float>>> densities;
 	std::vector<krr_infinite_light> infinite;
+	std::vector<std::unique_ptr<std::vector<float>>> images; // RGBA, held for the infinite lights
 	std::vector<krr_camera> cameras;
 	std::vector<std::pair<int32_t, int32_t>> viewSizes;
 	bool anyNormals = false;
@@ -783,6 +785,90 @@ krr_camera referenceCamera(const Json &cam, const Json *ctrl) {
 	return c;
 }
 
+// Float images for infinite lights, as RGBA rows from the top of the image
+// (the reference loads images without a vertical flip, texture.cpp:27-60).
+// Portable float maps (.pfm, "PF" RGB or "Pf" grey, rows stored bottom-up)
+// and NumPy arrays (.npy, little-endian float32, H x W, H x W x 3 or
+// H x W x 4, C order).  The reference also reads .exr / .hdr / .png through
+// image libraries this build does not link.
+void readImage(const std::string &path, std::vector<float> &rgba, int &w, int &h) {
+	std::ifstream f(path, std::ios::binary);
+	if (!f) throw std::runtime_error("cannot open image " + path);
+	const std::string data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+	auto fail = [&](const char *why) { throw std::runtime_error("image " + path + ": " + why); };
+	int ch = 0;
+	const char *px = nullptr;
+	bool bottomUp = false, bigEndian = false;
+	if (data.size() >= 2 && data[0] == 'P' && (data[1] == 'F' || data[1] == 'f')) {
+		ch = data[1] == 'F' ? 3 : 1;
+		std::istringstream hs(data);
+		std::string magic;
+		double scale = 0;
+		hs >> magic >> w >> h >> scale;
+		if (!hs || w < 1 || h < 1) fail("bad PFM header");
+		const size_t off = (size_t) hs.tellg() + 1; // one whitespace byte after the scale
+		if (off + (size_t) w * h * ch * 4 > data.size()) fail("truncated PFM data");
+		px		  = data.data() + off;
+		bottomUp  = true;
+		bigEndian = scale > 0;
+	} else if (data.size() >= 10 && (unsigned char) data[0] == 0x93 && data.compare(1, 5, "NUMPY") == 0) {
+		const size_t hl	 = data[6] == 1 ? (size_t) (unsigned char) data[8] | (size_t) (unsigned char) data[9] << 8
+										: (size_t) (unsigned char) data[8] | (size_t) (unsigned char) data[9] << 8 | (size_t) (unsigned char) data[10] << 16;
+		const size_t off = data[6] == 1 ? 10 + hl : 12 + hl;
+		const std::string hdr = data.substr(data[6] == 1 ? 10 : 12, hl);
+		if (hdr.find("'<f4'") == std::string::npos) fail("NPY data must be little-endian float32");
+		if (hdr.find("'fortran_order': False") == std::string::npos) fail("NPY data must be in C order");
+		const size_t a = hdr.find('('), z = hdr.find(')');
+		if (a == std::string::npos || z == std::string::npos) fail("bad NPY shape");
+		std::vector<long> dims;
+		std::istringstream ds(hdr.substr(a + 1, z - a - 1));
+		std::string tok;
+		while (std::getline(ds, tok, ','))
+			if (tok.find_first_not_of(" ") != std::string::npos) dims.push_back(std::stol(tok));
+		if (dims.size() == 2) ch = 1;
+		else if (dims.size() == 3 && (dims[2] == 3 || dims[2] == 4)) ch = (int) dims[2];
+		else fail("NPY shape must be (H, W), (H, W, 3) or (H, W, 4)");
+		h = (int) dims[0];
+		w = (int) dims[1];
+		if (w < 1 || h < 1 || off + (size_t) w * h * ch * 4 > data.size()) fail("truncated NPY data");
+		px = data.data() + off;
+	} else fail("only .pfm and .npy float images are read");
+	if (w > 65536 || h > 65536) fail("larger than 65536 texels on a side");
+	rgba.assign((size_t) w * h * 4, 1.f);
+	for (int y = 0; y < h; y++) {
+		const int sy = bottomUp ? h - 1 - y : y;
+		for (int x = 0; x < w; x++)
+			for (int c = 0; c < 4; c++) {
+				const int sc = ch == 1 ? 0 : c;
+				if (c >= ch && !(ch == 1 && c < 3)) continue;
+				unsigned char b[4];
+				std::memcpy(b, px + 4 * (((size_t) sy * w + x) * ch + sc), 4);
+				if (bigEndian) std::swap(b[0], b[3]), std::swap(b[1], b[2]);
+				float v;
+				std::memcpy(&v, b, 4);
+				rgba[4 * ((size_t) y * w + x) + c] = v;
+			}
+	}
+}
+
+// An infinite light (light.h:208-259): tint * scale, with an image file and
+// the rotation R (row-major, world from light) when given.
+void addInfinite(krr_scene &s, const float tint[3], float scale, const std::string &image, const float *R) {
+	krr_infinite_light il;
+	std::memset(&il, 0, sizeof(il));
+	for (int a = 0; a < 3; a++) il.tint[a] = tint[a];
+	il.scale = scale;
+	if (R)
+		for (int k = 0; k < 9; k++) il.rotation[k] = R[k];
+	if (!image.empty()) {
+		auto img = std::make_unique<std::vector<float>>();
+		readImage(image, *img, il.image_width, il.image_height);
+		il.image_rgba = img->data();
+		s.images.push_back(std::move(img));
+	}
+	s.infinite.push_back(il);
+}
+
 // The scene graph of KiRaRay's scene JSON (SceneImporter::importNode,
 // krrscene.cpp:305-343): "model" is an array, a model file path, or a node
 // {"name", "translate", "rotate", "scale", "params"} holding a nested
@@ -832,7 +918,7 @@ struct Importer {
 			return;
 		}
 		if (t == "light") {
-			light(params);
+			light(params, xf);
 			return;
 		}
 		if (!xf.identity()) throw std::runtime_error("translate / rotate / scale apply to model files, media and nested nodes, not to \"" + t + "\" entries");
@@ -904,19 +990,16 @@ struct Importer {
 		}
 	}
 
-	// Lights of the scene graph: the constant infinite light only; point,
-	// directional and spot lights and textured environments are not part
-	// of this build's light sampling.
-	void light(const Json &p) {
+	// Lights of the scene graph: infinite lights (constant, or an image
+	// "texture" turned by the node's rotation); point, directional and spot
+	// lights are not part of this build's light sampling.
+	void light(const Json &p, const NodeXform &xf) {
 		const std::string t = p.value("type", "infinite");
 		if (t != "infinite") throw std::runtime_error("light type \"" + t + "\" is not supported (infinite lights only)");
-		if (!p.value("texture", "").empty()) throw std::runtime_error("textured infinite lights are not supported");
-		krr_infinite_light il;
 		float col[3] = {1, 1, 1};
 		p.vec<3>("color", col);
-		for (int a = 0; a < 3; a++) il.tint[a] = col[a];
-		il.scale = (float) p.value("scale", 1.0);
-		s.infinite.push_back(il);
+		const std::string tex = p.value("texture", "");
+		addInfinite(s, col, (float) p.value("scale", 1.0), tex.empty() ? "" : joinPath(baseDir, tex), xf.L);
 	}
 };
 
@@ -950,15 +1033,19 @@ void buildScene(krr_scene &s, const Json &j) {
 		bool emissive = m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0;
 		if (emissive && !m.has_normals) throw std::runtime_error("emissive meshes need vertex normals (Triangle::sample reads them)");
 	}
-	if (j.contains("environment")) {
-		const Json &e = j.at("environment");
-		if (e.isString()) throw std::runtime_error("environment \"" + e.asString() + "\": image environments are not supported");
-		krr_infinite_light il;
-		float tint[3] = {1, 1, 1};
-		e.vec<3>("tint", tint);
-		for (int a = 0; a < 3; a++) il.tint[a] = tint[a];
-		il.scale = (float) e.value("scale", 1.0);
-		s.infinite.push_back(il);
+	if (j.contains("environment")) { // krrscene.cpp:268-274: an image path; or {"tint", "scale", "image", "rotate"}
+		const Json &e	 = j.at("environment");
+		const std::string base = j.value("base_dir", "");
+		float tint[3]	 = {1, 1, 1};
+		if (e.isString()) addInfinite(s, tint, 1.f, joinPath(base, e.asString()), nullptr);
+		else {
+			e.vec<3>("tint", tint);
+			const std::string img = e.value("image", "");
+			Json rot			  = Json::object();
+			if (e.contains("rotate")) rot.set("rotate", e.at("rotate"));
+			const NodeXform xf = nodeXform(rot);
+			addInfinite(s, tint, (float) e.value("scale", 1.0), img.empty() ? "" : joinPath(base, img), xf.L);
+		}
 	}
 	int w = 0, h = 0;
 	if (j.contains("resolution")) {

@@ -56,6 +56,11 @@ struct DevLight {
 	int prim;
 	float Le[3];
 	float scale;
+	// infinite light image (null: constant), rows from v = 0, and the
+	// light's rotation (row-major, world from light)
+	const float4 *img;
+	int iw, ih;
+	float R[9];
 };
 
 struct DevMedium {
@@ -661,6 +666,40 @@ KRR_DEV f3 areaLightL(const DevLight &L, const f3 &n, const f3 &w) {
 	return L.scale * f3(L.Le[0], L.Le[1], L.Le[2]);
 }
 
+// Bilinear lookup with wrap-around of a normalised-coordinate RGBA float
+// texture (texture.cpp:236-243: cudaAddressModeWrap, cudaFilterModeLinear;
+// weights in binary32).
+KRR_DEV f3 texBilinear(const float4 *img, int w, int h, float u, float v) {
+	float x = u * (float) w - 0.5f, y = v * (float) h - 0.5f;
+	if (!(x > -1e6f && x < 1e6f)) x = 0.f; // NaN / far out of range: any texel keeps the index in bounds
+	if (!(y > -1e6f && y < 1e6f)) y = 0.f;
+	const float x0 = floorf(x), y0 = floorf(y);
+	const float fx = x - x0, fy = y - y0;
+	int i0 = (int) x0 % w, j0 = (int) y0 % h;
+	if (i0 < 0) i0 += w;
+	if (j0 < 0) j0 += h;
+	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+	const float4 a = img[(size_t) j0 * w + i0], b = img[(size_t) j0 * w + i1];
+	const float4 c = img[(size_t) j1 * w + i0], d = img[(size_t) j1 * w + i1];
+	const float gx = 1.f - fx, gy = 1.f - fy;
+	return f3(gy * (gx * a.x + fx * b.x) + fy * (gx * c.x + fx * d.x), gy * (gx * a.y + fx * b.y) + fy * (gx * c.y + fx * d.y),
+			  gy * (gx * a.z + fx * b.z) + fy * (gx * c.z + fx * d.z));
+}
+
+// InfiniteLight::Li (light.h:241-246): scale * tint * image(worldToLatLong(
+// R^T w)), worldToLatLong of math_utils.h:133-139.
+KRR_DEV f3 infiniteLi(const DevLight &L, const f3 &w) {
+	f3 c(L.Le[0], L.Le[1], L.Le[2]);
+	if (L.img) {
+		const float *R = L.R;
+		const f3 p	   = normalize(f3(R[0] * w.x + R[3] * w.y + R[6] * w.z, R[1] * w.x + R[4] * w.y + R[7] * w.z, R[2] * w.x + R[5] * w.y + R[8] * w.z));
+		const float u  = detmath::atan2f(p.x, -p.z) * kInv2Pi + 0.5f;
+		const float v  = detmath::acosf(p.y) * kInvPi;
+		c			   = c * texBilinear(L.img, L.iw, L.ih, u, v);
+	}
+	return L.scale * c;
+}
+
 // Light::sampleLi for a DiffuseAreaLight (light.h:167-180 + shape.h:54-108)
 // or an InfiniteLight (light.h:226-235).
 KRR_DEV LightSample sampleLi(const DevScene &sc, const DevLight &L, float u0, float u1, const f3 &ctxP) {
@@ -669,7 +708,7 @@ KRR_DEV LightSample sampleLi(const DevScene &sc, const DevLight &L, float u0, fl
 		f3 wi  = uniformSampleSphere(u0, u1);
 		ls.p   = ctxP + wi * 2.f * sc.sceneRadius;
 		ls.n   = f3();
-		ls.L   = L.scale * f3(L.Le[0], L.Le[1], L.Le[2]);
+		ls.L   = infiniteLi(L, wi);
 		ls.pdf = kInv4Pi;
 		return ls;
 	}

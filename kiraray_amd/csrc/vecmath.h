@@ -63,6 +63,7 @@ KRR_HD float minf(float a, float b) { return a < b ? a : b; }
 constexpr float kPi		 = 3.14159265358979323846f;
 constexpr float k2Pi	 = 6.28318530717958647693f;
 constexpr float kInvPi	 = 0.318309886183790671538f;
+constexpr float kInv2Pi	 = 0.15915494309189533577f;
 constexpr float kInv4Pi	 = 0.07957747154594766788f;
 constexpr float kRayEps	 = 1e-4f; // KRR_RAY_EPS
 constexpr float kFloatMax = 3.402823466e+38f;

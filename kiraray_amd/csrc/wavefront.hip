@@ -470,7 +470,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 		for (int li = 0; li < sc.nLights; li++) {
 			const DevLight Lr = sc.lights[li];
 			if (Lr.type != 1) continue;
-			const f3 Li = Lr.scale * f3(Lr.Le[0], Lr.Le[1], Lr.Le[2]);
+			const f3 Li = infiniteLi(Lr, r.d);
 			if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
 				const float lp = kInv4Pi * lightSelectPdf;
 				Lm			   = Lm + Li / (pu + pl * lp).mean();
@@ -1187,7 +1187,7 @@ struct krr_wf {
 	bool haveScene = false;
 	AccelData accel;
 	DevBuf pos, nrm, idx, triMesh, meshes, materials, lights, media;
-	std::vector<std::unique_ptr<DevBuf>> mediumBufs;
+	std::vector<std::unique_ptr<DevBuf>> mediumBufs, imageBufs;
 	DevScene ds{};
 	bool sceneHasMedia = false;
 	// frame
@@ -1358,7 +1358,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 			d.lightBase		  = (int) lights.size();
 			const float scale = std::max(std::max(m.emission[0], m.emission[1]), m.emission[2]);
 			for (int t = 0; t < m.n_triangles; t++) {
-				DevLight L;
+				DevLight L{};
 				L.type	= 0;
 				L.prim	= m.first_triangle + t;
 				L.Le[0] = m.emission[0] / scale;
@@ -1372,12 +1372,27 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	}
 	for (int64_t t = 0; t < s->n_triangles; t++)
 		if (triMesh[t] < 0) return krr::fail(KRR_ERR_INVALID_ARG, "scene: triangle %lld belongs to no mesh", (long long) t);
+	wf->imageBufs.clear();
 	for (int i = 0; i < s->n_infinite_lights; i++) {
-		DevLight L;
+		const krr_infinite_light &il = s->infinite_lights[i];
+		DevLight L{};
 		L.type = 1;
 		L.prim = -1;
-		for (int a = 0; a < 3; a++) L.Le[a] = s->infinite_lights[i].tint[a];
-		L.scale = s->infinite_lights[i].scale;
+		for (int a = 0; a < 3; a++) L.Le[a] = il.tint[a];
+		L.scale	 = il.scale;
+		bool any = false;
+		for (int k = 0; k < 9; k++) any |= il.rotation[k] != 0.f;
+		for (int k = 0; k < 9; k++) L.R[k] = any ? il.rotation[k] : (k % 4 == 0 ? 1.f : 0.f);
+		if (il.image_rgba) {
+			if (il.image_width < 1 || il.image_height < 1 || il.image_width > 65536 || il.image_height > 65536)
+				return krr::fail(KRR_ERR_INVALID_ARG, "infinite light %d: image size %d x %d", i, il.image_width, il.image_height);
+			auto buf = std::make_unique<DevBuf>();
+			KRR_HIP(buf->upload(il.image_rgba, (size_t) il.image_width * il.image_height * 4 * sizeof(float)));
+			L.img = buf->as<float4>();
+			L.iw  = il.image_width;
+			L.ih  = il.image_height;
+			wf->imageBufs.push_back(std::move(buf));
+		}
 		lights.push_back(L);
 	}
 	std::vector<DevMaterial> mats((size_t) s->n_materials);

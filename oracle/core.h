@@ -46,6 +46,10 @@ inline float m_tan(float x) { return detmath::tanf(x); }
 inline double m_tan(double x) { return std::tan(x); }
 inline float m_atan2(float y, float x) { return detmath::atan2f(y, x); }
 inline double m_atan2(double y, double x) { return std::atan2(y, x); }
+inline float m_acos(float x) { return detmath::acosf(x); }
+inline double m_acos(double x) { return std::acos(x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x)); }
+inline float m_floor(float x) { return std::floor(x); }
+inline double m_floor(double x) { return std::floor(x); }
 template <class R> inline R m_abs(R x) { return std::fabs(x); }
 template <class R> inline R m_min(R a, R b) { return a < b ? a : b; }
 template <class R> inline R m_max(R a, R b) { return a > b ? a : b; }
@@ -59,6 +63,7 @@ template <class R> inline R safe_sqrt(R x) { return m_sqrt(m_max(R(0), x)); }
 template <class R> constexpr R kPi() { if constexpr (sizeof(R) == 4) return 3.14159265358979323846f; else return 3.14159265358979323846; }
 template <class R> constexpr R k2Pi() { if constexpr (sizeof(R) == 4) return 6.28318530717958647693f; else return 6.28318530717958647693; }
 template <class R> constexpr R kInvPi() { if constexpr (sizeof(R) == 4) return 0.318309886183790671538f; else return 0.318309886183790671538; }
+template <class R> constexpr R kInv2Pi() { if constexpr (sizeof(R) == 4) return 0.15915494309189533577f; else return 0.15915494309189533577; }
 template <class R> constexpr R kInv4Pi() { if constexpr (sizeof(R) == 4) return 0.07957747154594766788f; else return 0.07957747154594766788; }
 constexpr float kRayEps = 1e-4f;   // KRR_RAY_EPS, src/core/raytracing.h:10
 
@@ -693,6 +698,9 @@ struct LightRec {
 	int prim;
 	float Le[3];
 	float scale;
+	const float *img = nullptr; // infinite light image: RGBA rows from v = 0 (null: constant)
+	int iw = 0, ih = 0;
+	float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}; // row-major, world from light
 };
 
 struct SceneView {
@@ -758,6 +766,15 @@ static void buildSceneView(const krr_scene_desc *d, SceneView &sv) {
 	for (int i = 0; i < d->n_infinite_lights; i++) {
 		const krr_infinite_light &il = d->infinite_lights[i];
 		LightRec L{1, -1, {il.tint[0], il.tint[1], il.tint[2]}, il.scale};
+		bool any = false;
+		for (int k = 0; k < 9; k++) any |= il.rotation[k] != 0.f;
+		if (any)
+			for (int k = 0; k < 9; k++) L.R[k] = il.rotation[k];
+		if (il.image_rgba && il.image_width > 0 && il.image_height > 0) {
+			L.img = il.image_rgba;
+			L.iw  = il.image_width;
+			L.ih  = il.image_height;
+		}
 		sv.lights.push_back(L);
 	}
 	sv.media.resize(d->n_media);

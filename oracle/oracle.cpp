@@ -80,7 +80,7 @@ template <class R> struct Integrator {
 		V3<R> Lm;
 		for (const LightRec &Lr : sv.lights) {
 			if (Lr.type != 1) continue;
-			V3<R> Li = R(Lr.scale) * V3<R>(R(Lr.Le[0]), R(Lr.Le[1]), R(Lr.Le[2]));
+			V3<R> Li = infiniteLi<R>(Lr, w.ray.d);
 			if (P.nee && w.depth && !(w.bsdfType & BSDF_DELTA)) {
 				R lightPdf = kInv4Pi<R>() * lightSelectPdf();
 				Lm += Li / (w.pu + w.pl * lightPdf).mean();
@@ -427,7 +427,7 @@ template <class R> struct MkIntegrator {
 							w = mis(pdf, kInv4Pi<R>() * selPdf);
 							if (!finite(w)) w = R(1);
 						}
-						L += thp * w * (R(Lr.scale) * V3<R>(R(Lr.Le[0]), R(Lr.Le[1]), R(Lr.Le[2])));
+						L += thp * w * infiniteLi<R>(Lr, ray.d);
 					}
 					break;
 				}

@@ -205,13 +205,47 @@ template <class R> V3<R> areaL(const LightRec &L, const V3<R> &n, const V3<R> &w
 	return R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
 }
 
+// Bilinear lookup with wrap-around of a normalised-coordinate RGBA texture
+// (texture.cpp:236-243).
+template <class R> V3<R> texBilinear(const float *img, int w, int h, R u, R v) {
+	R x = u * R((float) w) - R(0.5f), y = v * R((float) h) - R(0.5f);
+	if (!(x > R(-1e6f) && x < R(1e6f))) x = R(0);
+	if (!(y > R(-1e6f) && y < R(1e6f))) y = R(0);
+	const R x0 = m_floor(x), y0 = m_floor(y);
+	const R fx = x - x0, fy = y - y0;
+	int i0 = (int) x0 % w, j0 = (int) y0 % h;
+	if (i0 < 0) i0 += w;
+	if (j0 < 0) j0 += h;
+	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+	const float *a = img + 4 * ((size_t) j0 * w + i0), *b = img + 4 * ((size_t) j0 * w + i1);
+	const float *c = img + 4 * ((size_t) j1 * w + i0), *d = img + 4 * ((size_t) j1 * w + i1);
+	const R gx = R(1) - fx, gy = R(1) - fy;
+	V3<R> r;
+	for (int k = 0; k < 3; k++) r[k] = gy * (gx * R(a[k]) + fx * R(b[k])) + fy * (gx * R(c[k]) + fx * R(d[k]));
+	return r;
+}
+
+// InfiniteLight::Li (light.h:241-246) with worldToLatLong (math_utils.h:133-139).
+template <class R> V3<R> infiniteLi(const LightRec &L, const V3<R> &w) {
+	V3<R> c(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
+	if (L.img) {
+		const float *M = L.R;
+		const V3<R> p  = normalize(V3<R>(R(M[0]) * w.x + R(M[3]) * w.y + R(M[6]) * w.z, R(M[1]) * w.x + R(M[4]) * w.y + R(M[7]) * w.z,
+										 R(M[2]) * w.x + R(M[5]) * w.y + R(M[8]) * w.z));
+		const R u	   = m_atan2(p.x, -p.z) * kInv2Pi<R>() + R(0.5f);
+		const R v	   = m_acos(p.y) * kInvPi<R>();
+		c			   = c * texBilinear<R>(L.img, L.iw, L.ih, u, v);
+	}
+	return R(L.scale) * c;
+}
+
 template <class R> LightSample<R> sampleLi(const SceneView &sv, const LightRec &L, R u0, R u1, const Ctx<R> &ctx) {
 	LightSample<R> ls;
 	const krr_scene_desc *d = sv.d;
 	if (L.type == 1) { // InfiniteLight::sampleLi
 		V3<R> wi	 = uniformSampleSphere(u0, u1);
 		ls.intr.p	 = ctx.p + wi * R(2) * R(sv.sceneRadius);
-		ls.L		 = R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
+		ls.L		 = infiniteLi<R>(L, wi);
 		ls.pdf		 = kInv4Pi<R>();
 		return ls;
 	}
