@@ -46,8 +46,12 @@ enum krr_material_type {
 };
 
 /* Material::MaterialParams under the SpecularGlossiness shading model that the
- * JSON importer always selects (src/scene/krrscene.cpp:204-228):
- * roughness = 1 - specular[3]. */
+ * JSON and OBJ importers select (src/scene/krrscene.cpp:204-228,
+ * assimp.cpp:222-224): roughness = 1 - specular[3].  Textures
+ * (prepareSurfaceInteraction, src/render/shading.h:187-226) replace the
+ * parameter they stand for at the hit's texture coordinates: the diffuse
+ * texture's rgb the diffuse colour, the specular texture's alpha the
+ * glossiness (roughness = 1 - a). */
 typedef struct krr_material {
 	int32_t type;			   /* krr_material_type */
 	float diffuse[3];		   /* "diffuse", default (1,1,1) */
@@ -55,7 +59,17 @@ typedef struct krr_material {
 	float ior;				   /* "eta", default 1.5 */
 	float specular_transmission; /* "specular_transmission", default 0 */
 	float anisotropic;		   /* "anisotropic", default 0 */
+	int32_t diffuse_texture;   /* 0 = none, else 1 + index into krr_scene_desc.textures */
+	int32_t specular_texture;  /* 0 = none, else 1 + index into krr_scene_desc.textures */
 } krr_material;
+
+/* An RGBA float32 image used as a normalised-coordinate texture: height rows
+ * of width texels, row 0 at v = 0, sampled bilinearly with wrap-around
+ * (texture.cpp:236-243). */
+typedef struct krr_texture {
+	int32_t width, height;
+	const float *rgba; /* host memory, copied by krr_wf_set_scene */
+} krr_texture;
 
 /* One mesh instance with identity transform: a contiguous triangle range of
  * the scene arrays (rt::MeshData + rt::InstanceData, src/core/mesh.h:25-56). */
@@ -67,6 +81,7 @@ typedef struct krr_mesh {
 	int32_t medium_outside; /* index into media or -1 */
 	int32_t has_normals;	   /* per-vertex normals in krr_scene_desc.normals are valid */
 	float emission[3];	   /* Le; any component > 0 makes every triangle a DiffuseAreaLight */
+	int32_t has_texcoords;  /* per-vertex texture coordinates in krr_scene_desc.texcoords are valid (else uv = 0) */
 } krr_mesh;
 
 enum krr_medium_type {
@@ -123,6 +138,9 @@ typedef struct krr_scene_desc {
 	const krr_medium *media;
 	int32_t n_infinite_lights;
 	const krr_infinite_light *infinite_lights;
+	const float *texcoords; /* 2 * n_vertices, may be NULL if no mesh has texture coordinates */
+	int32_t n_textures;
+	const krr_texture *textures;
 } krr_scene_desc;
 
 /* rt::CameraData (src/core/camera.h:20-30).  transform is the camera-to-world

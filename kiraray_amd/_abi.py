@@ -28,13 +28,18 @@ KRR_METRIC_MSE, KRR_METRIC_MAPE, KRR_METRIC_SMAPE, KRR_METRIC_RELMSE = 0, 1, 2, 
 
 class krr_material(C.Structure):
     _fields_ = [("type", C.c_int32), ("diffuse", C.c_float * 3), ("specular", C.c_float * 4),
-                ("ior", C.c_float), ("specular_transmission", C.c_float), ("anisotropic", C.c_float)]
+                ("ior", C.c_float), ("specular_transmission", C.c_float), ("anisotropic", C.c_float),
+                ("diffuse_texture", C.c_int32), ("specular_texture", C.c_int32)]
+
+
+class krr_texture(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("rgba", C.POINTER(C.c_float))]
 
 
 class krr_mesh(C.Structure):
     _fields_ = [("first_triangle", C.c_int32), ("n_triangles", C.c_int32), ("material", C.c_int32),
                 ("medium_inside", C.c_int32), ("medium_outside", C.c_int32), ("has_normals", C.c_int32),
-                ("emission", C.c_float * 3)]
+                ("emission", C.c_float * 3), ("has_texcoords", C.c_int32)]
 
 
 class krr_medium(C.Structure):
@@ -54,7 +59,8 @@ class krr_scene_desc(C.Structure):
                 ("n_meshes", C.c_int32), ("meshes", C.POINTER(krr_mesh)),
                 ("n_materials", C.c_int32), ("materials", C.POINTER(krr_material)),
                 ("n_media", C.c_int32), ("media", C.POINTER(krr_medium)),
-                ("n_infinite_lights", C.c_int32), ("infinite_lights", C.POINTER(krr_infinite_light))]
+                ("n_infinite_lights", C.c_int32), ("infinite_lights", C.POINTER(krr_infinite_light)),
+                ("texcoords", C.POINTER(C.c_float)), ("n_textures", C.c_int32), ("textures", C.POINTER(krr_texture))]
 
 
 class krr_camera(C.Structure):

@@ -69,7 +69,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 			}
 			if (depth == A.maxDepth || (A.rr < 1.f && rng.get1D() > A.rr)) break;
 			thp = thp / A.rr;
-			const DevMaterial mat = sc.materials[intr.material];
+			const DevMaterial mat = materialAt(sc, intr);
 			BSDF bsdf;
 			bsdf.setup(mat);
 			const f3 woL = intr.toLocal(intr.wo);

@@ -45,6 +45,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "errors.h"
@@ -52,7 +53,7 @@
 #include "krr_wavefront.h"
 
 struct krr_scene {
-	std::vector<float> positions, normals;
+	std::vector<float> positions, normals, texcoords;
 	std::vector<int32_t> indices;
 	std::vector<krr_mesh> meshes;
 	std::vector<std::string> meshNames; // one per mesh ("" when unnamed)
@@ -60,10 +61,12 @@ struct krr_scene {
 	std::vector<krr_medium> media;
 	std::vector<std::unique_ptr<std::vector<float>>> densities;
 	std::vector<krr_infinite_light> infinite;
-	std::vector<std::unique_ptr<std::vector<float>>> images; // RGBA, held for the infinite lights
+	std::vector<std::unique_ptr<std::vector<float>>> images; // RGBA, held for infinite lights and textures
+	std::vector<krr_texture> textures;
+	std::map<std::string, int32_t> textureByPath; // 1 + index into textures
 	std::vector<krr_camera> cameras;
 	std::vector<std::pair<int32_t, int32_t>> viewSizes;
-	bool anyNormals = false;
+	bool anyNormals = false, anyTexcoords = false;
 };
 
 namespace krr {
@@ -123,7 +126,8 @@ struct Builder {
 	}
 	// Appends a mesh; normals may be empty.
 	int addMesh(const std::vector<V> &verts, const std::vector<V> &nrms, const std::vector<int> &tris, int material,
-				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1, const std::string &name = "") {
+				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1, const std::string &name = "",
+				const std::vector<float> &uvs = {}) {
 		krr_mesh m;
 		std::memset(&m, 0, sizeof(m));
 		int32_t base	 = (int32_t) (s.positions.size() / 3);
@@ -133,6 +137,7 @@ struct Builder {
 		m.medium_inside	 = mIn;
 		m.medium_outside = mOut;
 		m.has_normals	 = nrms.empty() ? 0 : 1;
+		m.has_texcoords	 = uvs.empty() ? 0 : 1;
 		m.emission[0]	 = emission.x;
 		m.emission[1]	 = emission.y;
 		m.emission[2]	 = emission.z;
@@ -144,8 +149,11 @@ struct Builder {
 			s.normals.push_back(n.x);
 			s.normals.push_back(n.y);
 			s.normals.push_back(n.z);
+			s.texcoords.push_back(uvs.empty() ? 0.f : uvs[2 * i]);
+			s.texcoords.push_back(uvs.empty() ? 0.f : uvs[2 * i + 1]);
 		}
 		if (!nrms.empty()) s.anyNormals = true;
+		if (!uvs.empty()) s.anyTexcoords = true;
 		for (int t : tris) s.indices.push_back(base + t);
 		s.meshes.push_back(m);
 		s.meshNames.push_back(name);
@@ -298,12 +306,18 @@ void explicitMesh(Builder &b, const Json &j) {
 		if (v < 0 || (size_t) v >= verts.size()) throw std::runtime_error("mesh: index out of range");
 		tris.push_back(v);
 	}
+	std::vector<float> uvs;
+	if (j.contains("texcoords")) {
+		const auto &T = j.at("texcoords").items();
+		if (T.size() != 2 * verts.size()) throw std::runtime_error("mesh: texcoords must hold 2 values per vertex");
+		for (const Json &t : T) uvs.push_back((float) t.asNumber());
+	}
 	float em[3] = {0, 0, 0};
 	j.vec<3>("emission", em);
 	int mat	 = (int) j.value("material", 0.0);
 	int mIn	 = (int) j.value("medium_inside", -1.0);
 	int mOut = (int) j.value("medium_outside", -1.0);
-	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut, j.value("name", ""));
+	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut, j.value("name", ""), uvs);
 }
 
 void quadModel(Builder &b, const Json &j) {
@@ -471,6 +485,7 @@ struct ObjMaterial {
 	V kd{1, 1, 1}, ks{0, 0, 0}, ke{0, 0, 0}, tf{1, 1, 1};
 	float ns = -1.f, ni = -1.f, d = 1.f;
 	bool hasTf = false;
+	std::string mapKd, mapKs; // texture files (map_Kd, map_Ks), full paths
 };
 
 std::string joinPath(const std::string &dir, const std::string &p) {
@@ -515,8 +530,29 @@ void readMtl(const std::string &path, std::map<std::string, ObjMaterial> &out) {
 			float tr = 0.f;
 			ss >> tr;
 			m.d = 1.f - tr;
+		} else if (k == "map_Kd" || k == "map_Ks") { // options first, the file name last
+			std::string tok, file;
+			while (ss >> tok) file = tok;
+			if (!file.empty()) (k == "map_Kd" ? m.mapKd : m.mapKs) = joinPath(dirOf(path), file);
 		}
 	}
+}
+
+void readImage(const std::string &path, std::vector<float> &rgba, int &w, int &h);
+
+// 1 + the index of the texture read from this image file (each file once).
+int32_t textureFor(krr_scene &s, const std::string &path) {
+	auto it = s.textureByPath.find(path);
+	if (it != s.textureByPath.end()) return it->second;
+	auto img = std::make_unique<std::vector<float>>();
+	krr_texture t;
+	readImage(path, *img, t.width, t.height);
+	t.rgba = img->data();
+	s.images.push_back(std::move(img));
+	s.textures.push_back(t);
+	const int32_t id	 = (int32_t) s.textures.size();
+	s.textureByPath[path] = id;
+	return id;
 }
 
 // AssimpImporter's OBJ material (assimp.cpp:60-65,96-224): shininess Ns ->
@@ -612,8 +648,9 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 	std::map<std::string, ObjMaterial> mtl;
 	std::map<std::string, int> matIndex;
 	std::vector<V> P, N;
+	std::vector<float> T; // vt: u, 1 - v (aiProcess_FlipUVs, assimp.cpp:264)
 	struct Corner {
-		int p, n;
+		int p, n, t;
 	};
 	struct Run {
 		std::string mat, name;
@@ -635,6 +672,11 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			V v{0, 0, 0};
 			ss >> v.x >> v.y >> v.z;
 			N.push_back(v);
+		} else if (k == "vt") {
+			float u = 0.f, v = 0.f;
+			ss >> u >> v;
+			T.push_back(u);
+			T.push_back(1.f - v);
 		} else if (k == "mtllib") {
 			std::string lib;
 			std::getline(ss, lib);
@@ -657,15 +699,17 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			std::vector<Corner> poly;
 			std::string tok;
 			while (ss >> tok) {
-				int vi = 0, ni = 0;
+				int vi = 0, ni = 0, ti = 0;
 				const size_t s1 = tok.find('/');
 				vi				= std::atoi(tok.substr(0, s1).c_str());
 				if (s1 != std::string::npos) {
 					const size_t s2 = tok.find('/', s1 + 1);
+					ti				= std::atoi(tok.substr(s1 + 1, s2 == std::string::npos ? std::string::npos : s2 - s1 - 1).c_str());
 					if (s2 != std::string::npos && s2 + 1 < tok.size()) ni = std::atoi(tok.substr(s2 + 1).c_str());
 				}
-				Corner c{resolve(vi, P.size()), ni ? resolve(ni, N.size()) : -1};
-				if (c.p < 0 || c.p >= (int) P.size() || c.n >= (int) N.size()) throw std::runtime_error(path + ": face index out of range");
+				Corner c{resolve(vi, P.size()), ni ? resolve(ni, N.size()) : -1, ti ? resolve(ti, T.size() / 2) : -1};
+				if (c.p < 0 || c.p >= (int) P.size() || c.n >= (int) N.size() || 2 * c.t >= (int) T.size() || (ti && c.t < 0) || (ni && c.n < 0))
+					throw std::runtime_error(path + ": face index out of range");
 				poly.push_back(c);
 			}
 			if (poly.size() < 3) continue;
@@ -694,29 +738,46 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			auto ov = overrides.find(r.mat);
 			const auto ot = mtl.find(r.mat);
 			const ObjMaterial om = ot != mtl.end() ? ot->second : ObjMaterial{};
-			mi				= b.addMaterial(ov != overrides.end() ? ov->second : objMaterial(om));
+			krr_material fm	  = objMaterial(om);
+			fm.diffuse_texture	= om.mapKd.empty() ? 0 : textureFor(b.s, om.mapKd);
+			fm.specular_texture = om.mapKs.empty() ? 0 : textureFor(b.s, om.mapKs);
+			krr_material mm = fm;
+			if (ov != overrides.end()) { // the JSON entry sets the params; the file's textures stay (krrscene.cpp:199-205)
+				mm = ov->second;
+				if (!mm.diffuse_texture) mm.diffuse_texture = fm.diffuse_texture;
+				if (!mm.specular_texture) mm.specular_texture = fm.specular_texture;
+			}
+			mi				= b.addMaterial(mm);
 			matIndex[r.mat] = mi;
 		}
 		const auto ot	 = mtl.find(r.mat);
 		const V emission = ot != mtl.end() ? ot->second.ke : V{0, 0, 0};
 		// vertices of the run: one per (position, normal) corner pair
-		std::map<std::pair<int, int>, int> vid;
+		std::map<std::tuple<int, int, int>, int> vid;
 		std::vector<V> verts, nrms;
+		std::vector<float> uvs;
 		std::vector<int> tris;
-		bool haveN = true;
-		for (const Corner &c : r.tri) haveN &= c.n >= 0;
+		bool haveN = true, haveT = true;
+		for (const Corner &c : r.tri) {
+			haveN &= c.n >= 0;
+			haveT &= c.t >= 0;
+		}
 		const std::map<int, V> &smooth = r.smooth;
 		for (const Corner &c : r.tri) {
-			const auto key = std::make_pair(c.p, haveN ? c.n : -1);
+			const auto key = std::make_tuple(c.p, haveN ? c.n : -1, haveT ? c.t : -1);
 			auto it		   = vid.find(key);
 			if (it == vid.end()) {
 				it = vid.emplace(key, (int) verts.size()).first;
 				verts.push_back(xf.point(P[c.p]));
 				nrms.push_back(haveN ? xf.normal(N[c.n]) : nrmz(smooth.at(c.p)));
+				if (haveT) {
+					uvs.push_back(T[2 * c.t]);
+					uvs.push_back(T[2 * c.t + 1]);
+				}
 			}
 			tris.push_back(it->second);
 		}
-		b.addMesh(verts, nrms, tris, mi, emission, -1, -1, r.name);
+		b.addMesh(verts, nrms, tris, mi, emission, -1, -1, r.name, uvs);
 	}
 }
 
@@ -1010,7 +1071,11 @@ void buildScene(krr_scene &s, const Json &j) {
 	std::map<std::string, krr_material> overrides;
 	if (j.contains("materials")) {
 		for (const Json &m : j.at("materials").items()) {
-			const krr_material mm = jsonMaterial(m);
+			krr_material mm = jsonMaterial(m);
+			// this build's keys: image files standing for the diffuse colour and the glossiness
+			const Json &p = m.contains("params") ? m.at("params") : m;
+			for (const char *k : {"diffuse_texture", "specular_texture"})
+				if (p.contains(k)) (k[0] == 'd' ? mm.diffuse_texture : mm.specular_texture) = textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
 			s.materials.push_back(mm);
 			if (m.contains("name")) overrides[m.value("name", "")] = mm;
 		}
@@ -1135,6 +1200,9 @@ int krr_scene_get_desc(const krr_scene *s, krr_scene_desc *d) {
 	d->media			  = s->media.data();
 	d->n_infinite_lights  = (int32_t) s->infinite.size();
 	d->infinite_lights	  = s->infinite.data();
+	d->texcoords		  = s->anyTexcoords ? s->texcoords.data() : nullptr;
+	d->n_textures		  = (int32_t) s->textures.size();
+	d->textures			  = s->textures.data();
 	return KRR_OK;
 }
 

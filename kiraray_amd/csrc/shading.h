@@ -41,6 +41,12 @@ struct DevMaterial {
 	float specTrans;
 	float anisotropic;
 	float diffuse[3];
+	int texDiffuse, texSpecular; // textures replacing diffuse / specular.a (-1 none)
+};
+
+struct DevTexture {
+	const float4 *img; // rows from v = 0
+	int w, h;
 };
 
 struct DevMesh {
@@ -49,6 +55,7 @@ struct DevMesh {
 	int firstTri;
 	int mediumIn, mediumOut; // interface media (-1 none); transition iff different
 	int hasNormals;
+	int hasUV;
 };
 
 struct DevLight {
@@ -84,6 +91,8 @@ struct DevScene {
 	int nLights;
 	const DevMedium *media;
 	int nMedia;
+	const float *uv; // 2 per vertex (may be null)
+	const DevTexture *textures;
 	float sceneRadius;
 	float bmin[3], binv[3]; // scene box: p -> (p - bmin) * binv maps it to [0,1]^3
 };
@@ -613,6 +622,7 @@ struct SurfaceInteraction {
 	int medium;	  // medium of the incoming ray
 	int mIn, mOut;
 	bool interface_;
+	float u, v; // texture coordinates (0 without them)
 
 	KRR_DEV f3 toLocal(const f3 &v) const { return {dot(tangent, v), dot(bitangent, v), dot(n, v)}; }
 	KRR_DEV f3 toWorld(const f3 &v) const { return tangent * v.x + bitangent * v.y + n * v.z; }
@@ -645,6 +655,11 @@ KRR_DEV void prepareSurfaceInteraction(const DevScene &sc, int prim, float b1, f
 	it.n		  = normalize(it.n);
 	it.tangent	  = normalize(it.tangent);
 	it.bitangent  = normalize(it.bitangent);
+	it.u = it.v = 0.f;
+	if (m.hasUV) { // shading.h:147-149: b0 t0 + b1 t1 + b2 t2
+		it.u = b0 * sc.uv[2 * v0] + b1 * sc.uv[2 * v1] + b2 * sc.uv[2 * v2];
+		it.v = b0 * sc.uv[2 * v0 + 1] + b1 * sc.uv[2 * v1 + 1] + b2 * sc.uv[2 * v2 + 1];
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -669,7 +684,7 @@ KRR_DEV f3 areaLightL(const DevLight &L, const f3 &n, const f3 &w) {
 // Bilinear lookup with wrap-around of a normalised-coordinate RGBA float
 // texture (texture.cpp:236-243: cudaAddressModeWrap, cudaFilterModeLinear;
 // weights in binary32).
-KRR_DEV f3 texBilinear(const float4 *img, int w, int h, float u, float v) {
+KRR_DEV float4 texBilinear4(const float4 *img, int w, int h, float u, float v) {
 	float x = u * (float) w - 0.5f, y = v * (float) h - 0.5f;
 	if (!(x > -1e6f && x < 1e6f)) x = 0.f; // NaN / far out of range: any texel keeps the index in bounds
 	if (!(y > -1e6f && y < 1e6f)) y = 0.f;
@@ -682,8 +697,32 @@ KRR_DEV f3 texBilinear(const float4 *img, int w, int h, float u, float v) {
 	const float4 a = img[(size_t) j0 * w + i0], b = img[(size_t) j0 * w + i1];
 	const float4 c = img[(size_t) j1 * w + i0], d = img[(size_t) j1 * w + i1];
 	const float gx = 1.f - fx, gy = 1.f - fy;
-	return f3(gy * (gx * a.x + fx * b.x) + fy * (gx * c.x + fx * d.x), gy * (gx * a.y + fx * b.y) + fy * (gx * c.y + fx * d.y),
-			  gy * (gx * a.z + fx * b.z) + fy * (gx * c.z + fx * d.z));
+	return make_float4(gy * (gx * a.x + fx * b.x) + fy * (gx * c.x + fx * d.x), gy * (gx * a.y + fx * b.y) + fy * (gx * c.y + fx * d.y),
+					   gy * (gx * a.z + fx * b.z) + fy * (gx * c.z + fx * d.z), gy * (gx * a.w + fx * b.w) + fy * (gx * c.w + fx * d.w));
+}
+
+KRR_DEV f3 texBilinear(const float4 *img, int w, int h, float u, float v) {
+	const float4 c = texBilinear4(img, w, h, u, v);
+	return f3(c.x, c.y, c.z);
+}
+
+// The material at a hit with its textures applied (shading.h:187-226,
+// SpecularGlossiness): the diffuse texture's rgb replaces the diffuse
+// colour, the specular texture's alpha the glossiness (roughness = 1 - a).
+KRR_DEV DevMaterial materialAt(const DevScene &sc, const SurfaceInteraction &it) {
+	DevMaterial m = sc.materials[it.material];
+	if (m.texDiffuse >= 0) {
+		const DevTexture t = sc.textures[m.texDiffuse];
+		const float4 c	   = texBilinear4(t.img, t.w, t.h, it.u, it.v);
+		m.diffuse[0]	   = c.x;
+		m.diffuse[1]	   = c.y;
+		m.diffuse[2]	   = c.z;
+	}
+	if (m.texSpecular >= 0) {
+		const DevTexture t = sc.textures[m.texSpecular];
+		m.roughness		   = 1.f - texBilinear4(t.img, t.w, t.h, it.u, it.v).w;
+	}
+	return m;
 }
 
 // InfiniteLight::Li (light.h:241-246): scale * tint * image(worldToLatLong(

@@ -322,7 +322,7 @@ KRR_DEV void unitAngles(const f3 &w, float &th, float &ph) {
 // materials) and specular albedo (the base colour of conductors and
 // dielectrics).
 KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float4 *out) {
-	const DevMaterial mat = sc.materials[intr.material];
+	const DevMaterial mat = materialAt(sc, intr);
 	const float px		  = fminf(1.f, fmaxf(0.f, (intr.p.x - sc.bmin[0]) * sc.binv[0]));
 	const float py		  = fminf(1.f, fmaxf(0.f, (intr.p.y - sc.bmin[1]) * sc.binv[1]));
 	const float pz		  = fminf(1.f, fmaxf(0.f, (intr.p.z - sc.bmin[2]) * sc.binv[2]));
@@ -570,7 +570,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			}
 			if (alive) {
 				const f3 woL		  = intr.toLocal(intr.wo);
-				const DevMaterial mat = sc.materials[intr.material];
+				const DevMaterial mat = materialAt(sc, intr);
 				const int bt		  = bsdfTypeOf(mat.type, mat.roughness);
 				BSDF bsdf;
 				bsdf.setup(mat);
@@ -1187,7 +1187,8 @@ struct krr_wf {
 	bool haveScene = false;
 	AccelData accel;
 	DevBuf pos, nrm, idx, triMesh, meshes, materials, lights, media;
-	std::vector<std::unique_ptr<DevBuf>> mediumBufs, imageBufs;
+	std::vector<std::unique_ptr<DevBuf>> mediumBufs, imageBufs, textureBufs;
+	DevBuf uv, textures;
 	DevScene ds{};
 	bool sceneHasMedia = false;
 	// frame
@@ -1351,6 +1352,8 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		d.mediumOut	 = m.medium_outside < 0 ? -1 : m.medium_outside;
 		d.hasNormals = m.has_normals ? 1 : 0;
 		if (m.has_normals && !s->normals) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_normals but normals is NULL", mi);
+		d.hasUV = m.has_texcoords ? 1 : 0;
+		if (m.has_texcoords && !s->texcoords) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_texcoords but texcoords is NULL", mi);
 		const bool emissive = m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0;
 		d.lightBase			= -1;
 		if (emissive) {
@@ -1405,6 +1408,23 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		mats[i].specTrans	= m.specular_transmission;
 		mats[i].anisotropic = m.anisotropic;
 		for (int a = 0; a < 3; a++) mats[i].diffuse[a] = m.diffuse[a];
+		if (m.diffuse_texture < 0 || m.diffuse_texture > s->n_textures || m.specular_texture < 0 || m.specular_texture > s->n_textures)
+			return krr::fail(KRR_ERR_INVALID_ARG, "material %d: texture index out of range", i);
+		mats[i].texDiffuse	= m.diffuse_texture - 1;
+		mats[i].texSpecular = m.specular_texture - 1;
+	}
+	// textures
+	if (s->n_textures < 0 || (s->n_textures > 0 && !s->textures)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing textures");
+	wf->textureBufs.clear();
+	std::vector<DevTexture> texs((size_t) s->n_textures);
+	for (int i = 0; i < s->n_textures; i++) {
+		const krr_texture &t = s->textures[i];
+		if (t.width < 1 || t.height < 1 || t.width > 65536 || t.height > 65536 || !t.rgba)
+			return krr::fail(KRR_ERR_INVALID_ARG, "texture %d: bad image (%d x %d)", i, t.width, t.height);
+		auto buf = std::make_unique<DevBuf>();
+		KRR_HIP(buf->upload(t.rgba, (size_t) t.width * t.height * 4 * sizeof(float)));
+		texs[i] = DevTexture{buf->as<float4>(), t.width, t.height};
+		wf->textureBufs.push_back(std::move(buf));
 	}
 	// media
 	wf->mediumBufs.clear();
@@ -1467,6 +1487,10 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	KRR_HIP(wf->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial)));
 	KRR_HIP(wf->lights.upload(lights.data(), lights.size() * sizeof(DevLight)));
 	KRR_HIP(wf->media.upload(media.data(), media.size() * sizeof(DevMedium)));
+	if (s->texcoords) KRR_HIP(wf->uv.upload(s->texcoords, (size_t) s->n_vertices * 2 * sizeof(float)));
+	else wf->uv.release();
+	if (!texs.empty()) KRR_HIP(wf->textures.upload(texs.data(), texs.size() * sizeof(DevTexture)));
+	else wf->textures.release();
 	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost, wf->deviceBuilder);
 	if (rc) return rc;
 	DevScene &ds	= wf->ds;
@@ -1480,6 +1504,8 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	ds.nLights		= (int) lights.size();
 	ds.media		= wf->media.as<DevMedium>();
 	ds.nMedia		= s->n_media;
+	ds.uv			= s->texcoords ? wf->uv.as<float>() : nullptr;
+	ds.textures		= texs.empty() ? nullptr : wf->textures.as<DevTexture>();
 	ds.sceneRadius	= sceneRadius;
 	for (int a = 0; a < 3; a++) {
 		ds.bmin[a] = lo[a];

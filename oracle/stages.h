@@ -121,6 +121,7 @@ template <class R> struct SurfInt {
 	int material = -1, light = -1, medium = -1;
 	bool hasInterface = false;
 	int inside = -1, outside = -1;
+	R u = R(0), v = R(0); // texture coordinates
 	ShadingData<R> sd;
 
 	V3<R> toLocal(const V3<R> &v) const { return {dot(tangent, v), dot(bitangent, v), dot(n, v)}; }
@@ -140,6 +141,30 @@ template <class R> struct SurfInt {
 		return spawnRayTo(to);
 	}
 };
+
+// Bilinear lookup with wrap-around of a normalised-coordinate RGBA texture
+// (texture.cpp:236-243).
+template <class R> void texBilinear4(const float *img, int w, int h, R u, R v, R out[4]) {
+	R x = u * R((float) w) - R(0.5f), y = v * R((float) h) - R(0.5f);
+	if (!(x > R(-1e6f) && x < R(1e6f))) x = R(0);
+	if (!(y > R(-1e6f) && y < R(1e6f))) y = R(0);
+	const R x0 = m_floor(x), y0 = m_floor(y);
+	const R fx = x - x0, fy = y - y0;
+	int i0 = (int) x0 % w, j0 = (int) y0 % h;
+	if (i0 < 0) i0 += w;
+	if (j0 < 0) j0 += h;
+	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+	const float *a = img + 4 * ((size_t) j0 * w + i0), *b = img + 4 * ((size_t) j0 * w + i1);
+	const float *c = img + 4 * ((size_t) j1 * w + i0), *d = img + 4 * ((size_t) j1 * w + i1);
+	const R gx = R(1) - fx, gy = R(1) - fy;
+	for (int k = 0; k < 4; k++) out[k] = gy * (gx * R(a[k]) + fx * R(b[k])) + fy * (gx * R(c[k]) + fx * R(d[k]));
+}
+
+template <class R> V3<R> texBilinear(const float *img, int w, int h, R u, R v) {
+	R c[4];
+	texBilinear4<R>(img, w, h, u, v, c);
+	return V3<R>(c[0], c[1], c[2]);
+}
 
 // getHitInfo + prepareSurfaceInteraction (src/render/shading.h:78-87,115-226)
 // for an identity instance transform (p unchanged, frame vectors normalised
@@ -172,6 +197,11 @@ template <class R> SurfInt<R> prepareSurfaceInteraction(const SceneView &sv, con
 	it.n		 = normalize(it.n);
 	it.tangent	 = normalize(it.tangent);
 	it.bitangent = normalize(it.bitangent);
+	if (m.has_texcoords) { // shading.h:147-149
+		const float *t = d->texcoords;
+		it.u		   = b0 * R(t[2 * v0]) + bb1 * R(t[2 * v1]) + bb2 * R(t[2 * v2]);
+		it.v		   = b0 * R(t[2 * v0 + 1]) + bb1 * R(t[2 * v1 + 1]) + bb2 * R(t[2 * v2 + 1]);
+	}
 	if (it.material < 0) return it;
 	const krr_material &mat	 = d->materials[it.material];
 	it.sd.type				 = mat.type;
@@ -181,6 +211,17 @@ template <class R> SurfInt<R> prepareSurfaceInteraction(const SceneView &sv, con
 	it.sd.specular			 = V3<R>(R(mat.specular[0]), R(mat.specular[1]), R(mat.specular[2]));
 	it.sd.roughness			 = R(1.f - mat.specular[3]);
 	it.sd.anisotropic		 = R(mat.anisotropic);
+	// textures (shading.h:187-226, SpecularGlossiness)
+	if (mat.diffuse_texture > 0) {
+		const krr_texture &t = d->textures[mat.diffuse_texture - 1];
+		it.sd.diffuse		 = texBilinear<R>(t.rgba, t.width, t.height, it.u, it.v);
+	}
+	if (mat.specular_texture > 0) {
+		const krr_texture &t = d->textures[mat.specular_texture - 1];
+		R c[4];
+		texBilinear4<R>(t.rgba, t.width, t.height, it.u, it.v, c);
+		it.sd.roughness = R(1) - c[3];
+	}
 	return it;
 }
 
@@ -203,26 +244,6 @@ template <class R> R triArea(const krr_scene_desc *d, int prim) {
 template <class R> V3<R> areaL(const LightRec &L, const V3<R> &n, const V3<R> &w) {
 	if (dot(n, w) < R(0)) return V3<R>(); // one-sided (twoSided = false for mesh lights)
 	return R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
-}
-
-// Bilinear lookup with wrap-around of a normalised-coordinate RGBA texture
-// (texture.cpp:236-243).
-template <class R> V3<R> texBilinear(const float *img, int w, int h, R u, R v) {
-	R x = u * R((float) w) - R(0.5f), y = v * R((float) h) - R(0.5f);
-	if (!(x > R(-1e6f) && x < R(1e6f))) x = R(0);
-	if (!(y > R(-1e6f) && y < R(1e6f))) y = R(0);
-	const R x0 = m_floor(x), y0 = m_floor(y);
-	const R fx = x - x0, fy = y - y0;
-	int i0 = (int) x0 % w, j0 = (int) y0 % h;
-	if (i0 < 0) i0 += w;
-	if (j0 < 0) j0 += h;
-	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
-	const float *a = img + 4 * ((size_t) j0 * w + i0), *b = img + 4 * ((size_t) j0 * w + i1);
-	const float *c = img + 4 * ((size_t) j1 * w + i0), *d = img + 4 * ((size_t) j1 * w + i1);
-	const R gx = R(1) - fx, gy = R(1) - fy;
-	V3<R> r;
-	for (int k = 0; k < 3; k++) r[k] = gy * (gx * R(a[k]) + fx * R(b[k])) + fy * (gx * R(c[k]) + fx * R(d[k]));
-	return r;
 }
 
 // InfiniteLight::Li (light.h:241-246) with worldToLatLong (math_utils.h:133-139).

@@ -1,0 +1,46 @@
+"""GPU parity with textured materials: the Cornell box with two textured
+quads (a diffuse one with a random colour texture, a rough conductor whose
+glossiness comes from a random specular texture), through the wavefront
+pass and the megakernel path tracer, against the checker."""
+import numpy as np
+import pytest
+
+from conftest import CORNELL
+from test_gpu_megakernel import mk_render
+from test_gpu_render import assert_images_match, gpu_render
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene(krr, tmp_path):
+    rng = np.random.default_rng(9)
+    np.save(tmp_path / "kd.npy", rng.random((16, 24, 3), dtype=np.float32))
+    spec = rng.random((8, 8, 4), dtype=np.float32)
+    spec[..., 3] = 0.3 + 0.6 * spec[..., 3]
+    np.save(tmp_path / "ks.npy", spec)
+    quad = lambda x0, mat: {"type": "mesh", "material": mat, "indices": [0, 1, 2, 0, 2, 3],
+                            "positions": [x0, 0.2, -0.5, x0 + 0.7, 0.2, -0.5, x0 + 0.7, 1.2, -0.9, x0, 1.2, -0.9],
+                            "texcoords": [0, 1.5, 1.5, 1.5, 1.5, -0.25, 0, -0.25]}
+    cfg = dict(CORNELL)
+    cfg["materials"] = [{"bsdf": "diffuse", "params": {"diffuse_texture": str(tmp_path / "kd.npy")}},
+                        {"bsdf": "conductor", "params": {"diffuse": [0.9, 0.8, 0.6], "specular_texture": str(tmp_path / "ks.npy")}}]
+    cfg["model"] = list(CORNELL["model"]) + [quad(-0.85, 0), quad(0.1, 1)]
+    return krr.Scene(cfg)
+
+
+def test_wavefront_texture_parity(krr, oracle, gpu, tmp_path):
+    print("case: textured quads in the Cornell box, wavefront 96x96", flush=True)
+    s = _scene(krr, tmp_path)
+    g, gs = gpu_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=3)
+    o, os_ = oracle.render(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
+
+
+def test_megakernel_texture_parity(krr, oracle, gpu, tmp_path):
+    print("case: textured quads in the Cornell box, megakernel 96x96", flush=True)
+    s = _scene(krr, tmp_path)
+    g, gs = mk_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=3)
+    o, os_ = oracle.render_megakernel(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
