@@ -709,8 +709,10 @@ KRR_DEV f3 texBilinear(const float4 *img, int w, int h, float u, float v) {
 // The material at a hit with its textures applied (shading.h:187-226,
 // SpecularGlossiness): the diffuse texture's rgb replaces the diffuse
 // colour, the specular texture's alpha the glossiness (roughness = 1 - a).
-KRR_DEV DevMaterial materialAt(const DevScene &sc, const SurfaceInteraction &it) {
+// TEX = false: the scene has no textures (the plain material).
+template <bool TEX = true> KRR_DEV DevMaterial materialAt(const DevScene &sc, const SurfaceInteraction &it) {
 	DevMaterial m = sc.materials[it.material];
+	if (!TEX) return m;
 	if (m.texDiffuse >= 0) {
 		const DevTexture t = sc.textures[m.texDiffuse];
 		const float4 c	   = texBilinear4(t.img, t.w, t.h, it.u, it.v);
@@ -727,9 +729,10 @@ KRR_DEV DevMaterial materialAt(const DevScene &sc, const SurfaceInteraction &it)
 
 // InfiniteLight::Li (light.h:241-246): scale * tint * image(worldToLatLong(
 // R^T w)), worldToLatLong of math_utils.h:133-139.
-KRR_DEV f3 infiniteLi(const DevLight &L, const f3 &w) {
+// TEX = false: no infinite light of the scene has an image.
+template <bool TEX = true> KRR_DEV f3 infiniteLi(const DevLight &L, const f3 &w) {
 	f3 c(L.Le[0], L.Le[1], L.Le[2]);
-	if (L.img) {
+	if (TEX && L.img) {
 		const float *R = L.R;
 		const f3 p	   = normalize(f3(R[0] * w.x + R[3] * w.y + R[6] * w.z, R[1] * w.x + R[4] * w.y + R[7] * w.z, R[2] * w.x + R[5] * w.y + R[8] * w.z));
 		const float u  = detmath::atan2f(p.x, -p.z) * kInv2Pi + 0.5f;
@@ -741,13 +744,13 @@ KRR_DEV f3 infiniteLi(const DevLight &L, const f3 &w) {
 
 // Light::sampleLi for a DiffuseAreaLight (light.h:167-180 + shape.h:54-108)
 // or an InfiniteLight (light.h:226-235).
-KRR_DEV LightSample sampleLi(const DevScene &sc, const DevLight &L, float u0, float u1, const f3 &ctxP) {
+template <bool TEX = true> KRR_DEV LightSample sampleLi(const DevScene &sc, const DevLight &L, float u0, float u1, const f3 &ctxP) {
 	LightSample ls;
 	if (L.type == 1) {
 		f3 wi  = uniformSampleSphere(u0, u1);
 		ls.p   = ctxP + wi * 2.f * sc.sceneRadius;
 		ls.n   = f3();
-		ls.L   = infiniteLi(L, wi);
+		ls.L   = infiniteLi<TEX>(L, wi);
 		ls.pdf = kInv4Pi;
 		return ls;
 	}

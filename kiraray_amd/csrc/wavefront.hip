@@ -321,8 +321,8 @@ KRR_DEV void unitAngles(const f3 &w, float &th, float &ph) {
 // outgoing direction, shading normal, roughness, diffuse albedo (diffuse
 // materials) and specular albedo (the base colour of conductors and
 // dielectrics).
-KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float4 *out) {
-	const DevMaterial mat = materialAt(sc, intr);
+template <bool TEX> KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float4 *out) {
+	const DevMaterial mat = materialAt<TEX>(sc, intr);
 	const float px		  = fminf(1.f, fmaxf(0.f, (intr.p.x - sc.bmin[0]) * sc.binv[0]));
 	const float py		  = fminf(1.f, fmaxf(0.f, (intr.p.y - sc.bmin[1]) * sc.binv[1]));
 	const float pz		  = fminf(1.f, fmaxf(0.f, (intr.p.z - sc.bmin[2]) * sc.binv[2]));
@@ -351,7 +351,9 @@ KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInteraction &intr, float
 // now and the radiance gathered up to and including its own NEE afterwards
 // (*trainSlot, written by the caller after the shadow ray), and the path
 // continues unbiased.
-template <bool MEDIA, bool NRC = false>
+// PLAIN: no textures and no image-lit infinite lights in the scene; the
+// texture lookups are compiled out.
+template <bool MEDIA, bool NRC = false, bool PLAIN = false>
 KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, const PixelStateDev &px, int loopDepth,
 					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so, const NrcDev *nrc = nullptr, bool trainPath = false,
 					   int *trainSlot = nullptr) {
@@ -470,7 +472,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 		for (int li = 0; li < sc.nLights; li++) {
 			const DevLight Lr = sc.lights[li];
 			if (Lr.type != 1) continue;
-			const f3 Li = infiniteLi(Lr, r.d);
+			const f3 Li = infiniteLi<!PLAIN>(Lr, r.d);
 			if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
 				const float lp = kInv4Pi * lightSelectPdf;
 				Lm			   = Lm + Li / (pu + pl * lp).mean();
@@ -500,13 +502,13 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			nrcEnd = true;
 			if (qi < nrc->capQuery) {
 				float4 *q = nrc->query + (size_t) qi * 5;
-				nrcVertex(sc, intr, q);
+				nrcVertex<!PLAIN>(sc, intr, q);
 				q[4] = make_float4(thp.x, thp.y, thp.z, __int_as_float(pixel));
 			}
 		}
 		if (wantT && ti < nrc->capTrain) {
 			float4 *t = nrc->train + (size_t) ti * 6;
-			nrcVertex(sc, intr, t);
+			nrcVertex<!PLAIN>(sc, intr, t);
 			t[4] = make_float4(thp.x, thp.y, thp.z, __int_as_float(pixel));
 			*trainSlot = ti;
 		}
@@ -518,7 +520,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 				const float ul	 = rng.get1D();
 				const int li	 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
 				const float u0	 = rng.get1D(), u1 = rng.get1D();
-				const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, mscP);
+				const LightSample ls = sampleLi<!PLAIN>(sc, sc.lights[li], u0, u1, mscP);
 				const f3 to		 = offsetRayOrigin(ls.p, ls.n, mscP - ls.p);
 				const f3 p_o	 = offsetRayOrigin(mscP, f3(), to - mscP);
 				const f3 sdir	 = to - p_o;
@@ -570,7 +572,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			}
 			if (alive) {
 				const f3 woL		  = intr.toLocal(intr.wo);
-				const DevMaterial mat = materialAt(sc, intr);
+				const DevMaterial mat = materialAt<!PLAIN>(sc, intr);
 				const int bt		  = bsdfTypeOf(mat.type, mat.roughness);
 				BSDF bsdf;
 				bsdf.setup(mat);
@@ -578,7 +580,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 					const float ul		 = rng.get1D();
 					const int li		 = min((int) (uint32_t) (ul * (float) sc.nLights), sc.nLights - 1);
 					const float u0		 = rng.get1D(), u1 = rng.get1D();
-					const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
+					const LightSample ls = sampleLi<!PLAIN>(sc, sc.lights[li], u0, u1, intr.p);
 					const f3 to			 = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
 					const f3 p_o		 = offsetRayOrigin(intr.p, intr.n, to - intr.p);
 					const f3 sdir		 = to - p_o;
@@ -821,7 +823,7 @@ __global__ void k_mega_begin(Counters *cnt) {
 // VIEWS: the paths are the camera samples of all views (global pixels
 // 0..P.N-1), generated at fetch instead of read from the ray queue; the
 // frame's PCG increment and training phase come from V.frame.
-template <bool MEDIA, bool COUNT, bool NRC = false, bool VIEWS = false>
+template <bool MEDIA, bool COUNT, bool NRC = false, bool VIEWS = false, bool PLAIN = false>
 __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
 							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{}) {
 	extern __shared__ int ldsStack[];
@@ -858,7 +860,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		ShadowOut so;
 		so.medium = -1;
 		int trainSlot = -1;
-		shadeCore<MEDIA, NRC>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
+		shadeCore<MEDIA, NRC, PLAIN>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
 		if (wantShadow) {
 			nShadow++;
 			if (MEDIA && P.enableMedium)
@@ -1191,6 +1193,7 @@ struct krr_wf {
 	DevBuf uv, textures;
 	DevScene ds{};
 	bool sceneHasMedia = false;
+	bool sceneTextured = false; // textures or image-lit infinite lights: the kernels with texture lookups
 	// frame
 	int W = 0, H = 0;
 	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters, ovf;
@@ -1512,6 +1515,8 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		ds.binv[a] = hi[a] > lo[a] ? 1.f / (hi[a] - lo[a]) : 1.f;
 	}
 	wf->sceneHasMedia = s->n_media > 0;
+	wf->sceneTextured = s->n_textures > 0;
+	for (int i = 0; i < s->n_infinite_lights; i++) wf->sceneTextured |= s->infinite_lights[i].image_rgba != nullptr;
 	wf->haveScene	  = true;
 	return KRR_OK;
 }
@@ -1817,9 +1822,10 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	if ((size_t) (bvh.stackLds + 1) * tb * sizeof(int) != lds) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: stack layout mismatch");
 	const bool count   = wf->countTraversal;
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
+	const bool plain   = !wf->sceneTextured;
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
-							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
+							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ (plain ? 8 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (shapeKey != wf->gridKey) {
 		// Counting variants use more registers: size each grid for the variant launched.
 		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
@@ -1827,8 +1833,14 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		else wf->gShadow = count ? persistentGrid(k_trace_shadow<true>, tb, lds, gTmax) : persistentGrid(k_trace_shadow<false>, tb, lds, gTmax);
 		wf->gShade = media ? persistentGrid(k_shade<true>, bs, 0, (int) gN) : persistentGrid(k_shade<false>, bs, 0, (int) gN);
 		if (media) wf->gMega = count ? persistentGrid(k_megakernel<true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<true, false>, tb, lds, gTmax);
+		else if (plain)
+			wf->gMega = count ? persistentGrid(k_megakernel<false, true, false, false, true>, tb, lds, gTmax)
+							  : persistentGrid(k_megakernel<false, false, false, false, true>, tb, lds, gTmax);
 		else wf->gMega = count ? persistentGrid(k_megakernel<false, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false>, tb, lds, gTmax);
-		if (wf->nrcOn)
+		if (wf->nrcOn && plain)
+			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true, false, true>, tb, lds, gTmax)
+								 : persistentGrid(k_megakernel<false, false, true, false, true>, tb, lds, gTmax);
+		else if (wf->nrcOn)
 			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false, true>, tb, lds, gTmax);
 		wf->gMega	 = scaledGrid(wf->gMega, wf->occupancy);
 		wf->gMegaNrc = scaledGrid(wf->gMegaNrc, wf->occupancy);
@@ -1887,11 +1899,19 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 					const int *cp	 = (const int *) &cnt->rayCount[cur];
 					if (wf->nrcOn) {
 						const unsigned gn = (unsigned) wf->gMegaNrc;
-						if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
-						else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+						if (plain) {
+							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+							else hipLaunchKernelGGL((k_megakernel<false, false, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+						} else {
+							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+							else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+						}
 					} else if (media) {
 						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
 						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+					} else if (plain) {
+						if (count) hipLaunchKernelGGL((k_megakernel<false, true, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						else hipLaunchKernelGGL((k_megakernel<false, false, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
 					} else {
 						if (count) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
 						else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
@@ -1962,19 +1982,21 @@ namespace {
 
 // The persistent kernel variants of the multi-view frame and of the
 // single-view frame's camera samples (autotuner trials).
-template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS> struct Mega {
-	static int grid(int tb, size_t lds, int maxBlocks) { return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS>, tb, lds, maxBlocks); }
+template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS, bool PLAIN> struct Mega {
+	static int grid(int tb, size_t lds, int maxBlocks) { return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks); }
 	static void launch(unsigned g, int tb, size_t lds, hipStream_t st, const WfParams &P, const DevScene &ds, const TraceBvh &bvh,
 					   const RayQueueDev &q, const int *countPtr, const PixelStateDev &px, int *ovf, Counters *cnt, const NrcDev &nrc,
 					   const ViewsDev &V) {
-		hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, 0, px, ovf, cnt, nrc, V);
+		hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, 0, px, ovf, cnt, nrc, V);
 	}
 };
 
-template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, F &&f) {
-	if (nrc) return count ? f(Mega<false, true, true, VIEWS>{}) : f(Mega<false, false, true, VIEWS>{});
-	if (media) return count ? f(Mega<true, true, false, VIEWS>{}) : f(Mega<true, false, false, VIEWS>{});
-	return count ? f(Mega<false, true, false, VIEWS>{}) : f(Mega<false, false, false, VIEWS>{});
+template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, bool plain, F &&f) {
+	if (nrc && plain) return count ? f(Mega<false, true, true, VIEWS, true>{}) : f(Mega<false, false, true, VIEWS, true>{});
+	if (nrc) return count ? f(Mega<false, true, true, VIEWS, false>{}) : f(Mega<false, false, true, VIEWS, false>{});
+	if (media) return count ? f(Mega<true, true, false, VIEWS, false>{}) : f(Mega<true, false, false, VIEWS, false>{});
+	if (plain) return count ? f(Mega<false, true, false, VIEWS, true>{}) : f(Mega<false, false, false, VIEWS, true>{});
+	return count ? f(Mega<false, true, false, VIEWS, false>{}) : f(Mega<false, false, false, VIEWS, false>{});
 }
 
 ViewsDev viewsDev(krr_wf *wf) { return ViewsDev{wf->viewBuf.as<ViewDev>(), wf->viewStart.as<int>(), (int) wf->views.size(), wf->frameBuf.as<FrameDev>()}; }
@@ -1997,11 +2019,11 @@ int viewsLaunch(krr_wf *wf, int N, bool nrc, ViewsLaunch &L) {
 	L.media			   = wf->enableMedium && wf->sceneHasMedia;
 	L.nrc			   = nrc;
 	const uint64_t key = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) wf->accel.stackSize << 50) ^
-						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^
+						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^ (wf->sceneTextured ? 0 : 8) ^
 						 ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (key != wf->viewsGridKey) {
 		const int maxBlocks = std::max(1, (N + tb - 1) / tb);
-		const int g = withMega<true>(L.media, L.count, nrc, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
+		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
 		wf->gMegaViews		= scaledGrid(g, wf->occupancy);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) wf->gMegaViews * tb)) return rc;
 		wf->viewsGridKey = key;
@@ -2051,7 +2073,7 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 	for (int s = 0; s < wf->spp; s++) {
 		hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
 		timed(4, [&] {
-			withMega<true>(L.media, L.count, L.nrc, [&](auto k) {
+			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, [&](auto k) {
 				decltype(k)::launch(L.grid, tb, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, px, ovf, cnt, nrc, V);
 				return 0;
 			});
@@ -2232,7 +2254,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), viewsPx(wf));
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<true>(L.media, false, false, [&](auto k) {
+				withMega<true>(L.media, false, false, !wf->sceneTextured, [&](auto k) {
 					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, viewsPx(wf),
 										wf->ovf.as<int>(), wf->counters.as<Counters>(), NrcDev{}, viewsDev(wf));
 					return 0;
@@ -2243,7 +2265,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				const TraceBvh bvh = wf->accel.dev();
 				const size_t lds   = stackLds(wf->accel, t.tb);
 				const int maxBlocks = std::max(1, (N + t.tb - 1) / t.tb);
-				int g = withMega<false>(media, false, false, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
+				int g = withMega<false>(media, false, false, !wf->sceneTextured, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
 				g	  = scaledGrid(g, t.occ);
 				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
 				const PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
@@ -2254,7 +2276,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_camera, dim3(gN), dim3(256), 0, st, P, cameraDev(wf->cam), px, rayQueue(wf, 0), cnt);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<false>(media, false, false, [&](auto k) {
+				withMega<false>(media, false, false, !wf->sceneTextured, [&](auto k) {
 					decltype(k)::launch((unsigned) g, t.tb, lds, st, P, wf->ds, bvh, rayQueue(wf, 0), (const int *) &cnt->rayCount[0], px,
 										wf->ovf.as<int>(), cnt, NrcDev{}, ViewsDev{});
 					return 0;
