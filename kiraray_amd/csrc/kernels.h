@@ -39,6 +39,8 @@ struct PixelStateDev {
 	float4 *pixel;	// sum over samples
 	uint64_t *rng;	// PCG32 state
 	int *channel;	// hero channel of the medium code (SampledWavelengths::mainIndex)
+	const int *order; // fetch order of the camera paths (null: pixel order)
+	uint8_t *missed;  // 1 where the pixel's last camera ray left the scene (null: not recorded)
 };
 
 struct Counters {

@@ -803,6 +803,77 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 	}
 }
 
+// Camera paths in the order the persistent launch takes them up: first the
+// pixels whose camera ray hit geometry in the previous sample, in pixel
+// order, then those whose ray left the scene (one-vertex paths), so that the
+// last paths a launch starts are short and the frame's tail (waves kept
+// alive by their longest path once no new path is left) is shorter.  The
+// images do not depend on the order.  A stable partition over 256-pixel
+// blocks: counts, one scan, scatter.
+__global__ void __launch_bounds__(256) k_order_count(const uint8_t *missed, int N, int *blocks) {
+	const int i		 = blockIdx.x * 256 + threadIdx.x;
+	const uint64_t b = __ballot(i < N && !missed[i]);
+	__shared__ int w[4];
+	if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = __popcll(b);
+	__syncthreads();
+	if (threadIdx.x == 0) blocks[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+}
+
+// Exclusive scan of the n block counts in place, the total at blocks[n]
+// (one block of 1024 threads).
+__global__ void __launch_bounds__(1024) k_order_scan(int *blocks, int n) {
+	__shared__ int part[1024];
+	const int t = threadIdx.x, per = (n + 1023) / 1024;
+	const int b0 = min(n, t * per), b1 = min(n, b0 + per);
+	int sum = 0;
+	for (int k = b0; k < b1; k++) sum += blocks[k];
+	part[t] = sum;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		const int v = t >= off ? part[t - off] : 0;
+		__syncthreads();
+		part[t] += v;
+		__syncthreads();
+	}
+	int run = t == 0 ? 0 : part[t - 1];
+	for (int k = b0; k < b1; k++) {
+		const int c = blocks[k];
+		blocks[k]	= run;
+		run += c;
+	}
+	if (t == 1023) blocks[n] = part[1023];
+}
+
+__global__ void __launch_bounds__(256) k_order_scatter(const uint8_t *missed, int N, const int *blocks, int nBlocks, int *order) {
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	const bool valid = i < N, hit = valid && !missed[i];
+	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+	const uint64_t bh = __ballot(hit), bm = __ballot(valid && !hit);
+	__shared__ int wh[4], wm[4];
+	if (lane == 0) {
+		wh[wv] = __popcll(bh);
+		wm[wv] = __popcll(bm);
+	}
+	__syncthreads();
+	if (!valid) return;
+	int ph = 0, pm = 0;
+	for (int k = 0; k < wv; k++) {
+		ph += wh[k];
+		pm += wm[k];
+	}
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	ph += __popcll(bh & below);
+	pm += __popcll(bm & below);
+	const int hitsBefore = blocks[blockIdx.x];				  // in earlier (full) blocks
+	const int missBefore = blockIdx.x * 256 - hitsBefore;
+	order[hit ? hitsBefore + ph : blocks[nBlocks] + missBefore + pm] = i;
+}
+
+__global__ void k_order_identity(int N, int *order) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < N) order[i] = i;
+}
+
 __global__ void k_mega_begin(Counters *cnt) {
 	cnt->work[2]	= 0;
 	cnt->nrcQueries = 0;
@@ -844,7 +915,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				r		  = VIEWS ? viewCameraRay(P, V, px, i) : loadRayItem(q, i);
+				r		  = VIEWS ? viewCameraRay(P, V, px, i) : loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
 				depth	  = startDepth;
 				active	  = true;
 				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
@@ -855,6 +926,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		nClosest++;
 		const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
 		const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
+		if (!VIEWS && px.missed && depth == 0) px.missed[r.pixel] = h.prim < 0 ? 1 : 0;
 		bool wantNext = false, wantShadow = false;
 		RayItem nx;
 		ShadowOut so;
@@ -1197,6 +1269,7 @@ struct krr_wf {
 	// frame
 	int W = 0, H = 0;
 	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters, ovf;
+	DevBuf missed, order, orderBlocks; // camera-path order of the persistent launch (k_order_*)
 	krr_camera cam{};
 	uint64_t frameIndex = 0;
 	uint64_t gridKey	= ~0ull;
@@ -1579,6 +1652,13 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(wf->pixel.alloc(N * sizeof(float4)));
 	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
 	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
+	KRR_HIP(wf->missed.alloc(N));
+	KRR_HIP(hipMemset(wf->missed.p, 0, N));
+	KRR_HIP(wf->order.alloc(N * sizeof(int)));
+	KRR_HIP(wf->orderBlocks.alloc(((N + 255) / 256 + 1) * sizeof(int)));
+	hipLaunchKernelGGL(k_order_identity, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, nullptr, (int) N, wf->order.as<int>());
+	KRR_HIP(hipGetLastError());
+	KRR_HIP(hipDeviceSynchronize());
 	if (!wf->counters.p) {
 		KRR_HIP(wf->counters.alloc(sizeof(Counters)));
 		KRR_HIP(hipMemset(wf->counters.p, 0, sizeof(Counters)));
@@ -1809,7 +1889,8 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const WfParams P   = frameParams(wf, wf->W, wf->H, N, wf->frameIndex);
 	const CameraDev cam = cameraDev(wf->cam);
 
-	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
+	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>(), wf->order.as<int>(),
+					 wf->missed.as<uint8_t>()};
 	ShadowQueueDev sq{wf->shadowRay.as<float4>(), wf->shadowC.as<float4>(), wf->shadowExt.as<float4>()};
 	Counters *cnt	   = wf->counters.as<Counters>();
 	wf->accel.stackLds = wf->stackLdsEntries; // before dev(): the kernels' LDS stack depth must match lds below
@@ -1894,6 +1975,13 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 			const RayQueueDev cq = rayQueue(wf, cur);
 			if (depth >= wf->wavefrontDepths) { // the rest of every path in one launch
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
+				if (depth == 0) { // camera paths: the order from the last sample's misses
+					const unsigned nb = (unsigned) ((N + 255) / 256);
+					hipLaunchKernelGGL(k_order_count, dim3(nb), dim3(256), 0, st, wf->missed.as<uint8_t>(), N, wf->orderBlocks.as<int>());
+					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, wf->orderBlocks.as<int>(), (int) nb);
+					hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, wf->missed.as<uint8_t>(), N, wf->orderBlocks.as<int>(), (int) nb,
+									   wf->order.as<int>());
+				}
 				timed(4, [&] {
 					const unsigned g = (unsigned) wf->gMega;
 					const int *cp	 = (const int *) &cnt->rayCount[cur];
@@ -2268,6 +2356,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				int g = withMega<false>(media, false, false, !wf->sceneTextured, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
 				g	  = scaledGrid(g, t.occ);
 				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
+				// trials may cover fewer paths than pixels: pixel order, no miss record
 				const PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>()};
 				Counters *cnt = wf->counters.as<Counters>();
 				const unsigned gN = (unsigned) ((N + 255) / 256);

@@ -186,3 +186,30 @@ def test_homogeneous_medium_parity(krr, oracle, gpu, albedo):
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
     assert o[..., :3].max() > 0
+
+
+def test_camera_path_order_does_not_change_images(krr, oracle, gpu):
+    """From the second sample on, the persistent launch takes up the camera
+    paths that hit geometry in the previous sample first and the others
+    last.  A frame rendered after four others (reordered) equals the same
+    frame rendered first (pixel order), and the checker's frame, with the
+    same ray counts."""
+    print("case: camera-path order, icosphere field 160x90 frames 1..5", flush=True)
+    s = krr.Scene(icosphere_scene(count=256, level=2, res=(160, 90)))
+    params = {"spp": 2, "max_depth": 10, "rr_min_depth": 3}
+    fresh, fs = gpu_render(krr, gpu, s, 160, 90, params, frame_index=5)
+    wf = krr.WavefrontPathTracer(params)
+    wf.set_scene(s)
+    wf.resize(160, 90)
+    film = gpu.DeviceBuffer(160 * 90 * 16)
+    for f in range(1, 6):
+        wf.begin_frame(s.camera(0), f)
+        wf.render(film.ptr, 0)
+    st = wf.stats()
+    after = film.to_numpy(np.float32, (90, 160, 4))
+    wf.close()
+    assert np.array_equal(fresh, after)
+    assert st["closest_rays"] == fs["closest_rays"] and st["shadow_rays"] == fs["shadow_rays"]
+    o, os_ = oracle.render(s.desc, s.camera(0), 160, 90, 5, spp=2, max_depth=10, rr_min_depth=3)
+    assert os_["closest_rays"] == st["closest_rays"]
+    assert_images_match(after, o)
