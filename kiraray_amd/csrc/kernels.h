@@ -40,7 +40,7 @@ struct PixelStateDev {
 	uint64_t *rng;	// PCG32 state
 	int *channel;	// hero channel of the medium code (SampledWavelengths::mainIndex)
 	const int *order; // fetch order of the camera paths (null: pixel order)
-	uint8_t *missed;  // 1 where the pixel's last camera ray left the scene (null: not recorded)
+	uint8_t *plen;	  // length (vertices - 1) of the pixel's last path, capped at 255 (null: not recorded)
 };
 
 struct Counters {

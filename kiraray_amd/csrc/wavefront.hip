@@ -803,30 +803,48 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 	}
 }
 
-// Camera paths in the order the persistent launch takes them up: first the
-// pixels whose camera ray hit geometry in the previous sample, in pixel
-// order, then those whose ray left the scene (one-vertex paths), so that the
-// last paths a launch starts are short and the frame's tail (waves kept
-// alive by their longest path once no new path is left) is shorter.  The
-// images do not depend on the order.  A stable partition over 256-pixel
-// blocks: counts, one scan, scatter.
-__global__ void __launch_bounds__(256) k_order_count(const uint8_t *missed, int N, int *blocks) {
-	const int i		 = blockIdx.x * 256 + threadIdx.x;
-	const uint64_t b = __ballot(i < N && !missed[i]);
-	__shared__ int w[4];
-	if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = __popcll(b);
-	__syncthreads();
-	if (threadIdx.x == 0) blocks[blockIdx.x] = w[0] + w[1] + w[2] + w[3];
+// Camera paths in the order the persistent launch takes them up: 8 x 8-pixel
+// tiles by the mean length of their pixels' paths in the previous sample,
+// longest first, pixels row by row inside a tile.  Long paths then start
+// early and the last paths a launch takes up are short (many leave the
+// scene at once), so the frame's tail - waves kept alive by their longest
+// path once no new path is left - is shorter, while the 64 camera rays of a
+// wave stay coherent.  The images do not depend on the order.  A stable
+// counting sort of the tiles on 256 keys (4 x the mean length), then the
+// pixel positions by a scan of the tiles' pixel counts.
+__global__ void __launch_bounds__(64) k_tile_keys(const uint8_t *plen, int W, int H, int tilesX, uint8_t *key, int *cnt) {
+	const int t = blockIdx.x, x = (t % tilesX) * 8 + (threadIdx.x & 7), y = (t / tilesX) * 8 + (threadIdx.x >> 3);
+	const bool v = x < W && y < H;
+	int s = v ? plen[(size_t) y * W + x] : 0;
+	for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+	const int c = __popcll(__ballot(v)); // >= 1: every tile holds a pixel
+	if (threadIdx.x == 0) {
+		key[t] = (uint8_t) min(255, (4 * s) / c);
+		cnt[t] = c;
+	}
 }
 
-// Exclusive scan of the n block counts in place, the total at blocks[n]
-// (one block of 1024 threads).
-__global__ void __launch_bounds__(1024) k_order_scan(int *blocks, int n) {
+// Per 256-tile block, the count of each key, laid out key-major from the
+// largest key down so that one exclusive scan gives every (key, block) its
+// first slot.
+__global__ void __launch_bounds__(256) k_tile_hist(const uint8_t *key, int nTiles, int nBlk, int *hist) {
+	__shared__ int h[256];
+	h[threadIdx.x] = 0;
+	__syncthreads();
+	const int t = blockIdx.x * 256 + threadIdx.x;
+	if (t < nTiles) atomicAdd(&h[key[t]], 1);
+	__syncthreads();
+	hist[(255 - threadIdx.x) * nBlk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of n counts in place, the total at a[n] (one block of 1024
+// threads).
+__global__ void __launch_bounds__(1024) k_order_scan(int *a, int n) {
 	__shared__ int part[1024];
 	const int t = threadIdx.x, per = (n + 1023) / 1024;
 	const int b0 = min(n, t * per), b1 = min(n, b0 + per);
 	int sum = 0;
-	for (int k = b0; k < b1; k++) sum += blocks[k];
+	for (int k = b0; k < b1; k++) sum += a[k];
 	part[t] = sum;
 	__syncthreads();
 	for (int off = 1; off < 1024; off <<= 1) {
@@ -837,36 +855,38 @@ __global__ void __launch_bounds__(1024) k_order_scan(int *blocks, int n) {
 	}
 	int run = t == 0 ? 0 : part[t - 1];
 	for (int k = b0; k < b1; k++) {
-		const int c = blocks[k];
-		blocks[k]	= run;
+		const int c = a[k];
+		a[k]		= run;
 		run += c;
 	}
-	if (t == 1023) blocks[n] = part[1023];
+	if (t == 1023) a[n] = part[1023];
 }
 
-__global__ void __launch_bounds__(256) k_order_scatter(const uint8_t *missed, int N, const int *blocks, int nBlocks, int *order) {
-	const int i = blockIdx.x * 256 + threadIdx.x;
-	const bool valid = i < N, hit = valid && !missed[i];
-	const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-	const uint64_t bh = __ballot(hit), bm = __ballot(valid && !hit);
-	__shared__ int wh[4], wm[4];
-	if (lane == 0) {
-		wh[wv] = __popcll(bh);
-		wm[wv] = __popcll(bm);
-	}
+// Sorted slot of each tile (stable: equal keys keep tile order), and its
+// pixel count at that slot.
+__global__ void __launch_bounds__(256) k_tile_rank(const uint8_t *key, const int *cnt, int nTiles, int nBlk, const int *hist, int *sorted,
+												   int *sortedCnt) {
+	__shared__ uint8_t k[256];
+	const int t		= blockIdx.x * 256 + threadIdx.x;
+	k[threadIdx.x] = t < nTiles ? key[t] : 0;
 	__syncthreads();
-	if (!valid) return;
-	int ph = 0, pm = 0;
-	for (int k = 0; k < wv; k++) {
-		ph += wh[k];
-		pm += wm[k];
-	}
-	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-	ph += __popcll(bh & below);
-	pm += __popcll(bm & below);
-	const int hitsBefore = blocks[blockIdx.x];				  // in earlier (full) blocks
-	const int missBefore = blockIdx.x * 256 - hitsBefore;
-	order[hit ? hitsBefore + ph : blocks[nBlocks] + missBefore + pm] = i;
+	if (t >= nTiles) return;
+	const uint8_t my = k[threadIdx.x];
+	int r			 = 0;
+	for (int j = 0; j < (int) threadIdx.x; j++) r += k[j] == my;
+	const int slot	= hist[(255 - my) * nBlk + blockIdx.x] + r;
+	sorted[slot]	= t;
+	sortedCnt[slot] = cnt[t];
+}
+
+__global__ void __launch_bounds__(64) k_tile_pixels(const int *sorted, const int *offset, int W, int H, int tilesX, int *order) {
+	const int s = blockIdx.x, t = sorted[s];
+	const int x = (t % tilesX) * 8 + (threadIdx.x & 7), y = (t / tilesX) * 8 + (threadIdx.x >> 3);
+	const bool v	  = x < W && y < H;
+	const uint64_t b  = __ballot(v);
+	const int lane	  = threadIdx.x;
+	const uint64_t lo = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	if (v) order[offset[s] + __popcll(b & lo)] = y * W + x;
 }
 
 __global__ void k_order_identity(int N, int *order) {
@@ -926,7 +946,6 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		nClosest++;
 		const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
 		const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
-		if (!VIEWS && px.missed && depth == 0) px.missed[r.pixel] = h.prim < 0 ? 1 : 0;
 		bool wantNext = false, wantShadow = false;
 		RayItem nx;
 		ShadowOut so;
@@ -948,6 +967,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 			r = nx;
 			depth++;
 		} else {
+			if (!VIEWS && px.plen) px.plen[r.pixel] = (uint8_t) min(depth, 255);
 			active = false;
 		}
 	}
@@ -1269,7 +1289,7 @@ struct krr_wf {
 	// frame
 	int W = 0, H = 0;
 	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters, ovf;
-	DevBuf missed, order, orderBlocks; // camera-path order of the persistent launch (k_order_*)
+	DevBuf plen, order, tileKey, tileCnt, tileHist, tileSorted, tileOff; // camera-path order of the persistent launch (k_tile_*)
 	krr_camera cam{};
 	uint64_t frameIndex = 0;
 	uint64_t gridKey	= ~0ull;
@@ -1652,10 +1672,17 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(wf->pixel.alloc(N * sizeof(float4)));
 	KRR_HIP(wf->rng.alloc(N * sizeof(uint64_t)));
 	KRR_HIP(wf->channel.alloc(N * sizeof(int)));
-	KRR_HIP(wf->missed.alloc(N));
-	KRR_HIP(hipMemset(wf->missed.p, 0, N));
+	KRR_HIP(wf->plen.alloc(N));
+	KRR_HIP(hipMemset(wf->plen.p, 0, N));
 	KRR_HIP(wf->order.alloc(N * sizeof(int)));
-	KRR_HIP(wf->orderBlocks.alloc(((N + 255) / 256 + 1) * sizeof(int)));
+	{
+		const size_t nTiles = (size_t) ((wf->W + 7) / 8) * ((wf->H + 7) / 8), nBlk = (nTiles + 255) / 256;
+		KRR_HIP(wf->tileKey.alloc(nTiles));
+		KRR_HIP(wf->tileCnt.alloc(nTiles * sizeof(int)));
+		KRR_HIP(wf->tileHist.alloc((256 * nBlk + 1) * sizeof(int)));
+		KRR_HIP(wf->tileSorted.alloc(nTiles * sizeof(int)));
+		KRR_HIP(wf->tileOff.alloc((nTiles + 1) * sizeof(int)));
+	}
 	hipLaunchKernelGGL(k_order_identity, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, nullptr, (int) N, wf->order.as<int>());
 	KRR_HIP(hipGetLastError());
 	KRR_HIP(hipDeviceSynchronize());
@@ -1890,7 +1917,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const CameraDev cam = cameraDev(wf->cam);
 
 	PixelStateDev px{wf->L.as<float4>(), wf->pixel.as<float4>(), wf->rng.as<uint64_t>(), wf->channel.as<int>(), wf->order.as<int>(),
-					 wf->missed.as<uint8_t>()};
+					 wf->plen.as<uint8_t>()};
 	ShadowQueueDev sq{wf->shadowRay.as<float4>(), wf->shadowC.as<float4>(), wf->shadowExt.as<float4>()};
 	Counters *cnt	   = wf->counters.as<Counters>();
 	wf->accel.stackLds = wf->stackLdsEntries; // before dev(): the kernels' LDS stack depth must match lds below
@@ -1975,12 +2002,16 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 			const RayQueueDev cq = rayQueue(wf, cur);
 			if (depth >= wf->wavefrontDepths) { // the rest of every path in one launch
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
-				if (depth == 0) { // camera paths: the order from the last sample's misses
-					const unsigned nb = (unsigned) ((N + 255) / 256);
-					hipLaunchKernelGGL(k_order_count, dim3(nb), dim3(256), 0, st, wf->missed.as<uint8_t>(), N, wf->orderBlocks.as<int>());
-					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, wf->orderBlocks.as<int>(), (int) nb);
-					hipLaunchKernelGGL(k_order_scatter, dim3(nb), dim3(256), 0, st, wf->missed.as<uint8_t>(), N, wf->orderBlocks.as<int>(), (int) nb,
-									   wf->order.as<int>());
+				if (depth == 0) { // camera paths: the order from the last sample's path lengths
+					const int tilesX = (wf->W + 7) / 8, nTiles = tilesX * ((wf->H + 7) / 8), nBlk = (nTiles + 255) / 256;
+					uint8_t *key = wf->tileKey.as<uint8_t>();
+					int *tc = wf->tileCnt.as<int>(), *hist = wf->tileHist.as<int>(), *sorted = wf->tileSorted.as<int>(), *off = wf->tileOff.as<int>();
+					hipLaunchKernelGGL(k_tile_keys, dim3(nTiles), dim3(64), 0, st, wf->plen.as<uint8_t>(), wf->W, wf->H, tilesX, key, tc);
+					hipLaunchKernelGGL(k_tile_hist, dim3(nBlk), dim3(256), 0, st, key, nTiles, nBlk, hist);
+					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, hist, 256 * nBlk);
+					hipLaunchKernelGGL(k_tile_rank, dim3(nBlk), dim3(256), 0, st, key, tc, nTiles, nBlk, hist, sorted, off);
+					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, off, nTiles);
+					hipLaunchKernelGGL(k_tile_pixels, dim3(nTiles), dim3(64), 0, st, sorted, off, wf->W, wf->H, tilesX, wf->order.as<int>());
 				}
 				timed(4, [&] {
 					const unsigned g = (unsigned) wf->gMega;
