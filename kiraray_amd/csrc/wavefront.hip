@@ -803,6 +803,8 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 	}
 }
 
+constexpr int kTileKeys = 64, kTileBlock = 1024; // sort keys (4 x mean path length), tiles per histogram block
+
 // Camera paths in the order the persistent launch takes them up: 8 x 8-pixel
 // tiles by the mean length of their pixels' paths in the previous sample,
 // longest first, pixels row by row inside a tile.  Long paths then start
@@ -810,7 +812,7 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 // scene at once), so the frame's tail - waves kept alive by their longest
 // path once no new path is left - is shorter, while the 64 camera rays of a
 // wave stay coherent.  The images do not depend on the order.  A stable
-// counting sort of the tiles on 256 keys (4 x the mean length), then the
+// counting sort of the tiles on 64 keys (4 x the mean length), then the
 // pixel positions by a scan of the tiles' pixel counts.
 __global__ void __launch_bounds__(64) k_tile_keys(const uint8_t *plen, int W, int H, int tilesX, uint8_t *key, int *cnt) {
 	const int t = blockIdx.x, x = (t % tilesX) * 8 + (threadIdx.x & 7), y = (t / tilesX) * 8 + (threadIdx.x >> 3);
@@ -819,62 +821,72 @@ __global__ void __launch_bounds__(64) k_tile_keys(const uint8_t *plen, int W, in
 	for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
 	const int c = __popcll(__ballot(v)); // >= 1: every tile holds a pixel
 	if (threadIdx.x == 0) {
-		key[t] = (uint8_t) min(255, (4 * s) / c);
+		key[t] = (uint8_t) min(kTileKeys - 1, (4 * s) / c);
 		cnt[t] = c;
 	}
 }
 
-// Per 256-tile block, the count of each key, laid out key-major from the
-// largest key down so that one exclusive scan gives every (key, block) its
-// first slot.
-__global__ void __launch_bounds__(256) k_tile_hist(const uint8_t *key, int nTiles, int nBlk, int *hist) {
-	__shared__ int h[256];
-	h[threadIdx.x] = 0;
+// Per block of kTileBlock tiles, the count of each key, laid out key-major
+// from the largest key down so that one exclusive scan gives every (key,
+// block) its first slot.
+__global__ void __launch_bounds__(kTileBlock) k_tile_hist(const uint8_t *key, int nTiles, int nBlk, int *hist) {
+	__shared__ int h[kTileKeys];
+	if (threadIdx.x < kTileKeys) h[threadIdx.x] = 0;
 	__syncthreads();
-	const int t = blockIdx.x * 256 + threadIdx.x;
+	const int t = blockIdx.x * kTileBlock + threadIdx.x;
 	if (t < nTiles) atomicAdd(&h[key[t]], 1);
 	__syncthreads();
-	hist[(255 - threadIdx.x) * nBlk + blockIdx.x] = h[threadIdx.x];
+	if (threadIdx.x < kTileKeys) hist[(kTileKeys - 1 - threadIdx.x) * nBlk + blockIdx.x] = h[threadIdx.x];
 }
 
 // Exclusive scan of n counts in place, the total at a[n] (one block of 1024
-// threads).
+// threads).  The array is taken in chunks of 4096: each thread takes four
+// consecutive counts, wave sums by shuffles, a carry between chunks.
 __global__ void __launch_bounds__(1024) k_order_scan(int *a, int n) {
-	__shared__ int part[1024];
-	const int t = threadIdx.x, per = (n + 1023) / 1024;
-	const int b0 = min(n, t * per), b1 = min(n, b0 + per);
-	int sum = 0;
-	for (int k = b0; k < b1; k++) sum += a[k];
-	part[t] = sum;
+	__shared__ int wsum[16];
+	__shared__ int carry;
+	const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+	if (t == 0) carry = 0;
 	__syncthreads();
-	for (int off = 1; off < 1024; off <<= 1) {
-		const int v = t >= off ? part[t - off] : 0;
+	for (int base = 0; base < n; base += 4096) {
+		const int i0 = base + 4 * t;
+		int v[4];
+		for (int k = 0; k < 4; k++) v[k] = i0 + k < n ? a[i0 + k] : 0;
+		const int tsum = v[0] + v[1] + v[2] + v[3];
+		int incl = tsum; // inclusive scan of the thread sums inside the wave
+		for (int o = 1; o < 64; o <<= 1) {
+			const int u = __shfl_up(incl, o);
+			if (lane >= o) incl += u;
+		}
+		if (lane == 63) wsum[w] = incl;
 		__syncthreads();
-		part[t] += v;
+		int before = carry;
+		for (int k = 0; k < w; k++) before += wsum[k];
+		int run = before + incl - tsum;
+		for (int k = 0; k < 4; k++) {
+			if (i0 + k < n) a[i0 + k] = run;
+			run += v[k];
+		}
+		__syncthreads();
+		if (t == 1023) carry = run; // run of the last thread = carry + chunk total
 		__syncthreads();
 	}
-	int run = t == 0 ? 0 : part[t - 1];
-	for (int k = b0; k < b1; k++) {
-		const int c = a[k];
-		a[k]		= run;
-		run += c;
-	}
-	if (t == 1023) a[n] = part[1023];
+	if (t == 0) a[n] = carry;
 }
 
 // Sorted slot of each tile (stable: equal keys keep tile order), and its
 // pixel count at that slot.
-__global__ void __launch_bounds__(256) k_tile_rank(const uint8_t *key, const int *cnt, int nTiles, int nBlk, const int *hist, int *sorted,
-												   int *sortedCnt) {
-	__shared__ uint8_t k[256];
-	const int t		= blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(kTileBlock) k_tile_rank(const uint8_t *key, const int *cnt, int nTiles, int nBlk, const int *hist, int *sorted,
+														  int *sortedCnt) {
+	__shared__ uint8_t k[kTileBlock];
+	const int t		= blockIdx.x * kTileBlock + threadIdx.x;
 	k[threadIdx.x] = t < nTiles ? key[t] : 0;
 	__syncthreads();
 	if (t >= nTiles) return;
 	const uint8_t my = k[threadIdx.x];
 	int r			 = 0;
 	for (int j = 0; j < (int) threadIdx.x; j++) r += k[j] == my;
-	const int slot	= hist[(255 - my) * nBlk + blockIdx.x] + r;
+	const int slot	= hist[(kTileKeys - 1 - my) * nBlk + blockIdx.x] + r;
 	sorted[slot]	= t;
 	sortedCnt[slot] = cnt[t];
 }
@@ -916,7 +928,8 @@ __global__ void k_mega_begin(Counters *cnt) {
 // frame's PCG increment and training phase come from V.frame.
 template <bool MEDIA, bool COUNT, bool NRC = false, bool VIEWS = false, bool PLAIN = false>
 __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth, PixelStateDev px,
-							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{}) {
+							 int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{}, CameraDev cam = CameraDev{},
+							 int camFetch = 0) {
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	WfParams P			  = P0;
@@ -925,7 +938,9 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		if (NRC) nrc.phase = V.frame->phase;
 	}
 	if (NRC) nrc.keepProb = *nrc.keep;
-	const int count = VIEWS ? P.N : *countPtr;
+	// camFetch: the camera rays of this sample are generated here, at fetch
+	// (generateCameraRays, integrator.cpp:271-284), instead of read from q
+	const int count = (VIEWS || camFetch) ? P.N : *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
 	bool active = false, exhausted = false, trainPath = false;
 	int depth	= startDepth;
@@ -935,7 +950,15 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				r		  = VIEWS ? viewCameraRay(P, V, px, i) : loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
+				if (VIEWS) r = viewCameraRay(P, V, px, i);
+				else if (camFetch) {
+					const int pix = px.order ? px.order[i] : i;
+					if (!P.legacySpp) px.L[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
+					Pcg cs{px.rng[pix], P.inc};
+					r			 = cameraRay(cam, P.W, P.H, pix % P.W, pix / P.W, cs);
+					r.pixel		 = pix;
+					px.rng[pix] = cs.state;
+				} else r = loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
 				depth	  = startDepth;
 				active	  = true;
 				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
@@ -1676,10 +1699,10 @@ int allocFrame(krr_wf *wf) {
 	KRR_HIP(hipMemset(wf->plen.p, 0, N));
 	KRR_HIP(wf->order.alloc(N * sizeof(int)));
 	{
-		const size_t nTiles = (size_t) ((wf->W + 7) / 8) * ((wf->H + 7) / 8), nBlk = (nTiles + 255) / 256;
+		const size_t nTiles = (size_t) ((wf->W + 7) / 8) * ((wf->H + 7) / 8), nBlk = (nTiles + kTileBlock - 1) / kTileBlock;
 		KRR_HIP(wf->tileKey.alloc(nTiles));
 		KRR_HIP(wf->tileCnt.alloc(nTiles * sizeof(int)));
-		KRR_HIP(wf->tileHist.alloc((256 * nBlk + 1) * sizeof(int)));
+		KRR_HIP(wf->tileHist.alloc((kTileKeys * nBlk + 1) * sizeof(int)));
 		KRR_HIP(wf->tileSorted.alloc(nTiles * sizeof(int)));
 		KRR_HIP(wf->tileOff.alloc((nTiles + 1) * sizeof(int)));
 	}
@@ -1996,44 +2019,46 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 		nrc.phase	   = (int) (wf->frameIndex % (uint64_t) wf->nrcTrainMod);
 	}
 	for (int s = 0; s < wf->spp; s++) {
-		hipLaunchKernelGGL(k_camera, dim3(gN), dim3(bs), 0, st, P, cam, px, rayQueue(wf, 0), cnt);
+		// with every depth in the persistent launch, it makes the camera rays itself
+		if (wf->wavefrontDepths > 0) hipLaunchKernelGGL(k_camera, dim3(gN), dim3(bs), 0, st, P, cam, px, rayQueue(wf, 0), cnt);
 		for (int depth = 0; depth <= wf->maxDepth; depth++) {
 			const int cur		 = depth & 1;
 			const RayQueueDev cq = rayQueue(wf, cur);
 			if (depth >= wf->wavefrontDepths) { // the rest of every path in one launch
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
 				if (depth == 0) { // camera paths: the order from the last sample's path lengths
-					const int tilesX = (wf->W + 7) / 8, nTiles = tilesX * ((wf->H + 7) / 8), nBlk = (nTiles + 255) / 256;
+					const int tilesX = (wf->W + 7) / 8, nTiles = tilesX * ((wf->H + 7) / 8), nBlk = (nTiles + kTileBlock - 1) / kTileBlock;
 					uint8_t *key = wf->tileKey.as<uint8_t>();
 					int *tc = wf->tileCnt.as<int>(), *hist = wf->tileHist.as<int>(), *sorted = wf->tileSorted.as<int>(), *off = wf->tileOff.as<int>();
 					hipLaunchKernelGGL(k_tile_keys, dim3(nTiles), dim3(64), 0, st, wf->plen.as<uint8_t>(), wf->W, wf->H, tilesX, key, tc);
-					hipLaunchKernelGGL(k_tile_hist, dim3(nBlk), dim3(256), 0, st, key, nTiles, nBlk, hist);
-					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, hist, 256 * nBlk);
-					hipLaunchKernelGGL(k_tile_rank, dim3(nBlk), dim3(256), 0, st, key, tc, nTiles, nBlk, hist, sorted, off);
+					hipLaunchKernelGGL(k_tile_hist, dim3(nBlk), dim3(kTileBlock), 0, st, key, nTiles, nBlk, hist);
+					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, hist, kTileKeys * nBlk);
+					hipLaunchKernelGGL(k_tile_rank, dim3(nBlk), dim3(kTileBlock), 0, st, key, tc, nTiles, nBlk, hist, sorted, off);
 					hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, off, nTiles);
 					hipLaunchKernelGGL(k_tile_pixels, dim3(nTiles), dim3(64), 0, st, sorted, off, wf->W, wf->H, tilesX, wf->order.as<int>());
 				}
 				timed(4, [&] {
 					const unsigned g = (unsigned) wf->gMega;
+					const int camFetch = depth == 0 ? 1 : 0; // k_camera skipped: the kernel makes the camera rays
 					const int *cp	 = (const int *) &cnt->rayCount[cur];
 					if (wf->nrcOn) {
 						const unsigned gn = (unsigned) wf->gMegaNrc;
 						if (plain) {
-							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
-							else hipLaunchKernelGGL((k_megakernel<false, false, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
+							else hipLaunchKernelGGL((k_megakernel<false, false, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
 						} else {
-							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
-							else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc);
+							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
+							else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
 						}
 					} else if (media) {
-						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
-						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
+						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
 					} else if (plain) {
-						if (count) hipLaunchKernelGGL((k_megakernel<false, true, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
-						else hipLaunchKernelGGL((k_megakernel<false, false, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						if (count) hipLaunchKernelGGL((k_megakernel<false, true, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
+						else hipLaunchKernelGGL((k_megakernel<false, false, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
 					} else {
-						if (count) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
-						else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt);
+						if (count) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
+						else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
 					}
 				});
 				break;
