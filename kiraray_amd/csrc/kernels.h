@@ -138,6 +138,7 @@ struct ViewsDev {
 	const int *start; // n + 1 entries: first global pixel of each view, total last
 	int n;
 	const FrameDev *frame;
+	const int *tileStart; // n + 1 entries: first global 8x8 tile of each view, total last
 };
 
 KRR_DEV int laneId() { return __lane_id(); }

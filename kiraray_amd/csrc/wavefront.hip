@@ -901,6 +901,43 @@ __global__ void __launch_bounds__(64) k_tile_pixels(const int *sorted, const int
 	if (v) order[offset[s] + __popcll(b & lo)] = y * W + x;
 }
 
+// The same for many views: tiles of each view numbered after those of the
+// previous views (ViewsDev::tileStart), pixels by global number.
+KRR_DEV int findTileView(const ViewsDev &V, int t) {
+	int lo = 0, hi = V.n - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (V.tileStart[mid] <= t) lo = mid;
+		else hi = mid - 1;
+	}
+	return lo;
+}
+
+__global__ void __launch_bounds__(64) k_tile_keys_views(const uint8_t *plen, ViewsDev V, uint8_t *key, int *cnt) {
+	const int t = blockIdx.x, v = findTileView(V, t), lt = t - V.tileStart[v];
+	const int W = V.view[v].W, H = V.view[v].H, tilesX = (W + 7) / 8;
+	const int x = (lt % tilesX) * 8 + (threadIdx.x & 7), y = (lt / tilesX) * 8 + (threadIdx.x >> 3);
+	const bool ok = x < W && y < H;
+	int s		  = ok ? plen[V.start[v] + y * W + x] : 0;
+	for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+	const int c = __popcll(__ballot(ok));
+	if (threadIdx.x == 0) {
+		key[t] = (uint8_t) min(kTileKeys - 1, (4 * s) / c);
+		cnt[t] = c;
+	}
+}
+
+__global__ void __launch_bounds__(64) k_tile_pixels_views(const int *sorted, const int *offset, ViewsDev V, int *order) {
+	const int s = blockIdx.x, t = sorted[s], v = findTileView(V, t), lt = t - V.tileStart[v];
+	const int W = V.view[v].W, H = V.view[v].H, tilesX = (W + 7) / 8;
+	const int x = (lt % tilesX) * 8 + (threadIdx.x & 7), y = (lt / tilesX) * 8 + (threadIdx.x >> 3);
+	const bool ok	  = x < W && y < H;
+	const uint64_t b  = __ballot(ok);
+	const int lane	  = threadIdx.x;
+	const uint64_t lo = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	if (ok) order[offset[s] + __popcll(b & lo)] = V.start[v] + y * W + x;
+}
+
 __global__ void k_order_identity(int N, int *order) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < N) order[i] = i;
@@ -950,7 +987,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				if (VIEWS) r = viewCameraRay(P, V, px, i);
+				if (VIEWS) r = viewCameraRay(P, V, px, px.order ? px.order[i] : i);
 				else if (camFetch) {
 					const int pix = px.order ? px.order[i] : i;
 					if (!P.legacySpp) px.L[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -990,7 +1027,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 			r = nx;
 			depth++;
 		} else {
-			if (!VIEWS && px.plen) px.plen[r.pixel] = (uint8_t) min(depth, 255);
+			if (px.plen) px.plen[r.pixel] = (uint8_t) min(depth, 255);
 			active = false;
 		}
 	}
@@ -1313,6 +1350,8 @@ struct krr_wf {
 	int W = 0, H = 0;
 	DevBuf rayQ[2][2], hits, shadowRay, shadowC, shadowExt, L, pixel, rng, channel, counters, ovf;
 	DevBuf plen, order, tileKey, tileCnt, tileHist, tileSorted, tileOff; // camera-path order of the persistent launch (k_tile_*)
+	DevBuf vPlen, vOrder, vTileStart, vTileKey, vTileCnt, vTileHist, vTileSorted, vTileOff; // the same over all views
+	int vTiles = 0;
 	krr_camera cam{};
 	uint64_t frameIndex = 0;
 	uint64_t gridKey	= ~0ull;
@@ -2143,8 +2182,13 @@ template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, b
 	return count ? f(Mega<false, true, false, VIEWS, false>{}) : f(Mega<false, false, false, VIEWS, false>{});
 }
 
-ViewsDev viewsDev(krr_wf *wf) { return ViewsDev{wf->viewBuf.as<ViewDev>(), wf->viewStart.as<int>(), (int) wf->views.size(), wf->frameBuf.as<FrameDev>()}; }
-PixelStateDev viewsPx(krr_wf *wf) { return PixelStateDev{wf->vL.as<float4>(), wf->vPixel.as<float4>(), wf->vRng.as<uint64_t>(), wf->vChannel.as<int>()}; }
+ViewsDev viewsDev(krr_wf *wf) {
+	return ViewsDev{wf->viewBuf.as<ViewDev>(), wf->viewStart.as<int>(), (int) wf->views.size(), wf->frameBuf.as<FrameDev>(), wf->vTileStart.as<int>()};
+}
+PixelStateDev viewsPx(krr_wf *wf) {
+	return PixelStateDev{wf->vL.as<float4>(), wf->vPixel.as<float4>(), wf->vRng.as<uint64_t>(), wf->vChannel.as<int>(), wf->vOrder.as<int>(),
+						 wf->vPlen.as<uint8_t>()};
+}
 
 // Launch shape of the multi-view persistent kernel over N paths.
 struct ViewsLaunch {
@@ -2216,6 +2260,17 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 	int rc = KRR_OK;
 	for (int s = 0; s < wf->spp; s++) {
 		hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
+		{ // the order of this sample's camera paths, from the last sample's path lengths (k_tile_keys)
+			const int nT = wf->vTiles, nBlk = (nT + kTileBlock - 1) / kTileBlock;
+			uint8_t *key = wf->vTileKey.as<uint8_t>();
+			int *tc = wf->vTileCnt.as<int>(), *hist = wf->vTileHist.as<int>(), *sorted = wf->vTileSorted.as<int>(), *off = wf->vTileOff.as<int>();
+			hipLaunchKernelGGL(k_tile_keys_views, dim3(nT), dim3(64), 0, st, wf->vPlen.as<uint8_t>(), V, key, tc);
+			hipLaunchKernelGGL(k_tile_hist, dim3(nBlk), dim3(kTileBlock), 0, st, key, nT, nBlk, hist);
+			hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, hist, kTileKeys * nBlk);
+			hipLaunchKernelGGL(k_tile_rank, dim3(nBlk), dim3(kTileBlock), 0, st, key, tc, nT, nBlk, hist, sorted, off);
+			hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(1024), 0, st, off, nT);
+			hipLaunchKernelGGL(k_tile_pixels_views, dim3(nT), dim3(64), 0, st, sorted, off, V, wf->vOrder.as<int>());
+		}
 		timed(4, [&] {
 			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, [&](auto k) {
 				decltype(k)::launch(L.grid, tb, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, px, ovf, cnt, nrc, V);
@@ -2259,8 +2314,8 @@ int krr_wf_set_views(krr_wf *wf, const krr_view *views, int32_t n_views) {
 	if (!wf || !views || n_views < 1) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_views: need a handle and at least one view");
 	if (n_views > 1 << 20) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_set_views: at most 2^20 views");
 	std::vector<ViewDev> host((size_t) n_views);
-	std::vector<int> start((size_t) n_views + 1);
-	int64_t total = 0;
+	std::vector<int> start((size_t) n_views + 1), tileStart((size_t) n_views + 1);
+	int64_t total = 0, tiles = 0;
 	for (int v = 0; v < n_views; v++) {
 		const krr_view &kv = views[v];
 		if (kv.width < 1 || kv.height < 1 || kv.width > 65536 || kv.height > 65536)
@@ -2268,19 +2323,38 @@ int krr_wf_set_views(krr_wf *wf, const krr_view *views, int32_t n_views) {
 		if (!kv.film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_views: view %d has no film", v);
 		start[v]	 = (int) total;
 		total		+= (int64_t) kv.width * kv.height;
+		tileStart[v] = (int) tiles;
+		tiles		+= (int64_t) ((kv.width + 7) / 8) * ((kv.height + 7) / 8);
 		if (total > (int64_t) 1 << 30) return krr::fail(KRR_ERR_CAPACITY, "krr_wf_set_views: at most 2^30 pixels over all views");
 		host[v].cam	 = cameraDev(kv.camera);
 		host[v].W	 = kv.width;
 		host[v].H	 = kv.height;
 		host[v].film = (float4 *) kv.film_dev;
 	}
-	start[n_views] = (int) total;
+	start[n_views]	   = (int) total;
+	tileStart[n_views] = (int) tiles;
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_set_views: no HIP device");
 	KRR_HIP(hipDeviceSynchronize());
 	wf->dropGraph();
 	const size_t N = (size_t) total;
 	KRR_HIP(wf->viewBuf.upload(host.data(), host.size() * sizeof(ViewDev)));
 	KRR_HIP(wf->viewStart.upload(start.data(), start.size() * sizeof(int)));
+	KRR_HIP(wf->vTileStart.upload(tileStart.data(), tileStart.size() * sizeof(int)));
+	{ // camera-path order over all views: identity until the first sample's path lengths are in
+		const size_t nT = (size_t) tiles, nBlk = (nT + kTileBlock - 1) / kTileBlock;
+		KRR_HIP(wf->vPlen.alloc(N));
+		KRR_HIP(hipMemset(wf->vPlen.p, 0, N));
+		KRR_HIP(wf->vOrder.alloc(N * sizeof(int)));
+		KRR_HIP(wf->vTileKey.alloc(nT));
+		KRR_HIP(wf->vTileCnt.alloc(nT * sizeof(int)));
+		KRR_HIP(wf->vTileHist.alloc((kTileKeys * nBlk + 1) * sizeof(int)));
+		KRR_HIP(wf->vTileSorted.alloc(nT * sizeof(int)));
+		KRR_HIP(wf->vTileOff.alloc((nT + 1) * sizeof(int)));
+		hipLaunchKernelGGL(k_order_identity, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, nullptr, (int) N, wf->vOrder.as<int>());
+		KRR_HIP(hipGetLastError());
+		KRR_HIP(hipDeviceSynchronize());
+		wf->vTiles = (int) nT;
+	}
 	if (!wf->frameBuf.p) KRR_HIP(wf->frameBuf.alloc(sizeof(FrameDev)));
 	if (wf->vL.sz < N * sizeof(float4)) {
 		KRR_HIP(wf->vL.alloc(N * sizeof(float4)));
@@ -2395,11 +2469,15 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_set_frame, dim3(1), dim3(1), 0, st, wf->frameBuf.as<FrameDev>(), frameDev(wf, 1));
 				// begin + one persistent launch, timed from its start
 				hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
-				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), viewsPx(wf));
+				// trials may cover fewer paths than pixels: pixel order, no path-length record
+				PixelStateDev tpx = viewsPx(wf);
+				tpx.order		  = nullptr;
+				tpx.plen		  = nullptr;
+				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), tpx);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
 				KRR_HIP(hipEventRecord(e0, st));
 				withMega<true>(L.media, false, false, !wf->sceneTextured, [&](auto k) {
-					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, viewsPx(wf),
+					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, tpx,
 										wf->ovf.as<int>(), wf->counters.as<Counters>(), NrcDev{}, viewsDev(wf));
 					return 0;
 				});
