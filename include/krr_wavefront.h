@@ -51,7 +51,11 @@ enum krr_material_type {
  * (prepareSurfaceInteraction, src/render/shading.h:187-226) replace the
  * parameter they stand for at the hit's texture coordinates: the diffuse
  * texture's rgb the diffuse colour, the specular texture's alpha the
- * glossiness (roughness = 1 - a). */
+ * glossiness (roughness = 1 - a).  The transmission texture (or, without
+ * one, a non-zero transmission constant) gives the material the any-hit
+ * alpha test of shading.h:89-113: a hit is ignored with probability
+ * 1 - alpha, alpha = 1 - luminance(transmission), decided by a hash of the
+ * ray (pbrt.cpp:145-156 stores 1 - opacity there). */
 typedef struct krr_material {
 	int32_t type;			   /* krr_material_type */
 	float diffuse[3];		   /* "diffuse", default (1,1,1) */
@@ -61,6 +65,8 @@ typedef struct krr_material {
 	float anisotropic;		   /* "anisotropic", default 0 */
 	int32_t diffuse_texture;   /* 0 = none, else 1 + index into krr_scene_desc.textures */
 	int32_t specular_texture;  /* 0 = none, else 1 + index into krr_scene_desc.textures */
+	int32_t transmission_texture; /* 0 = none, else 1 + index into krr_scene_desc.textures */
+	float transmission[3];		  /* constant transmission when there is no texture; 0 = opaque */
 } krr_material;
 
 /* An RGBA float32 image used as a normalised-coordinate texture: height rows

@@ -201,7 +201,7 @@ struct Collapser {
 				t.v2[a] = pos[3 * (size_t) idx[3 * (size_t) p + 2] + a];
 			}
 			t.prim	= p;
-			t.flags = (nullFlag && nullFlag[p]) ? 1 : 0;
+			t.flags = nullFlag ? (int32_t) (nullFlag[p] & (kLeafNull | kLeafAlpha)) : 0;
 			out.tris.push_back(t);
 		}
 		out.nLeaves++;

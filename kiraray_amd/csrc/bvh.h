@@ -21,7 +21,9 @@ struct alignas(128) Bvh4Node {
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node must be 128 bytes");
 
 // Leaf triangle: three 16-byte records {v0, prim}, {v1, flags}, {v2, 0};
-// flags bit 0 = null material (skipped by shadow rays).
+// flags: kLeafNull = null material (skipped by shadow rays), kLeafAlpha =
+// the material has an alpha test (transmission texture or constant).
+constexpr int32_t kLeafNull = 1, kLeafAlpha = 2;
 struct LeafTri {
 	float v0[3];
 	int32_t prim;
@@ -53,7 +55,8 @@ struct BvhBuildOptions {
 	int threads		= 0;			// 0 = hardware concurrency (capped at 16)
 };
 
-// pos: 3 floats per vertex; idx: 3 ints per triangle; nullFlag may be null.
+// pos: 3 floats per vertex; idx: 3 ints per triangle; nullFlag: the
+// LeafTri flags of each triangle (kLeafNull | kLeafAlpha), may be null.
 void buildBvh4(const float *pos, const int32_t *idx, int64_t nTris, const uint8_t *nullFlag, const BvhBuildOptions &opt,
 			   Bvh4Host &out);
 

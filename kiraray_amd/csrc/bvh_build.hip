@@ -711,7 +711,7 @@ __global__ void k_leaf_tris(const float *pos, const int *idx, const unsigned cha
 		t.v2[a] = pos[3 * (size_t) idx[3 * (size_t) p + 2] + a];
 	}
 	t.prim	= p;
-	t.flags = nullFlag && nullFlag[p] ? 1 : 0;
+	t.flags = nullFlag ? (int32_t) (nullFlag[p] & (kLeafNull | kLeafAlpha)) : 0;
 	t.pad	= 0;
 	tris[i] = t;
 }

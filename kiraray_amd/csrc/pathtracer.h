@@ -20,6 +20,7 @@ struct PtArgs {
 	int *ovf;			  // traversal stack overflow: (stack size - LDS entries) * grid * tb ints
 	unsigned long long *rays; // [0] closest rays, [1] shadow rays (added to)
 	int tilesX, nTiles;		  // 8 x 8-pixel tiles, one per wave at a time
+	bool textured;			  // textures, image-lit infinite lights or alpha tests: the kernel with those lookups
 };
 
 // One persistent launch: grid blocks of tb threads (a multiple of 64), lds
@@ -27,6 +28,6 @@ struct PtArgs {
 hipError_t launchPathTracer(const PtArgs &a, int grid, int tb, size_t lds, hipStream_t st);
 
 // Resident blocks per CU of the kernel for this block size and LDS (>= 1).
-int pathTracerBlocksPerCU(int tb, size_t lds);
+int pathTracerBlocksPerCU(int tb, size_t lds, bool textured);
 
 } // namespace krr

@@ -22,7 +22,9 @@ KRR_DEV float misPower(float p0, float p1) {
 
 KRR_DEV bool finiteWeight(float w) { return !(__builtin_isnan(w) || __builtin_isinf(w)); }
 
-// The samples of pixel (x, y); returns their sum.
+// The samples of pixel (x, y); returns their sum.  TEX = false: the scene
+// has no textures, image-lit infinite lights or alpha tests.
+template <bool TEX>
 KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y, uint32_t &nClosest, uint32_t &nShadow) {
 	Pcg rng;
 	rng.setSeed((uint64_t) interleave32((uint32_t) x, (uint32_t) y), (uint64_t) A.sampleIndex);
@@ -40,7 +42,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 		f3 ctxP;
 		for (int depth = 0;; depth++) {
 			nClosest++;
-			const TraceResult h = traverse<false, false>(A.bvh, o, d, __builtin_inff(), stack, nv, nt);
+			const TraceResult h = traverse<false, false, TEX>(A.bvh, o, d, __builtin_inff(), stack, nv, nt);
 			if (h.prim < 0) { // handleMiss (device.cu:67-79)
 				for (int li = 0; li < nL; li++) {
 					const DevLight Lr = sc.lights[li];
@@ -50,7 +52,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 						w = misPower(pdf, kInv4Pi * selPdf);
 						if (!finiteWeight(w)) w = 1.f;
 					}
-					L = L + thp * w * infiniteLi(Lr, d);
+					L = L + thp * w * infiniteLi<TEX>(Lr, d);
 				}
 				break;
 			}
@@ -69,7 +71,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 			}
 			if (depth == A.maxDepth || (A.rr < 1.f && rng.get1D() > A.rr)) break;
 			thp = thp / A.rr;
-			const DevMaterial mat = materialAt(sc, intr);
+			const DevMaterial mat = materialAt<TEX>(sc, intr);
 			BSDF bsdf;
 			bsdf.setup(mat);
 			const f3 woL = intr.toLocal(intr.wo);
@@ -77,7 +79,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 				const float ul		 = rng.get1D();
 				const int li		 = min((int) (uint32_t) (ul * (float) nL), nL - 1);
 				const float u0		 = rng.get1D(), u1 = rng.get1D();
-				const LightSample ls = sampleLi(sc, sc.lights[li], u0, u1, intr.p);
+				const LightSample ls = sampleLi<TEX>(sc, sc.lights[li], u0, u1, intr.p);
 				const f3 wiL		 = intr.toLocal(normalize(ls.p - intr.p));
 				const float lightPdf = selPdf * ls.pdf;
 				if (lightPdf != 0.f) {
@@ -87,7 +89,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 					if (finiteWeight(w) && bv.any()) {
 						const f3 to	  = offsetRayOrigin(ls.p, ls.n, intr.p - ls.p);
 						const f3 p_o  = offsetRayOrigin(intr.p, intr.n, to - intr.p);
-						const TraceResult sh = traverse<true, false>(A.bvh, p_o, to - p_o, 1.f, stack, nv, nt);
+						const TraceResult sh = traverse<true, false, TEX>(A.bvh, p_o, to - p_o, 1.f, stack, nv, nt);
 						nShadow++;
 						if (sh.prim < 0) L = L + thp * bv * w / lightPdf * ls.L;
 					}
@@ -110,7 +112,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 	return color;
 }
 
-__global__ void __launch_bounds__(256) k_path_tracer(PtArgs A) {
+template <bool TEX> __global__ void __launch_bounds__(256) k_path_tracer(PtArgs A) {
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(A.bvh, ldsStack, A.ovf);
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
@@ -120,7 +122,7 @@ __global__ void __launch_bounds__(256) k_path_tracer(PtArgs A) {
 		const int x = (tile % A.tilesX) * 8 + (lane & 7);
 		const int y = (tile / A.tilesX) * 8 + (lane >> 3);
 		if (x < A.W && y < A.H) {
-			const f3 c		 = pathTracePixel(A, stack, x, y, nClosest, nShadow);
+			const f3 c		 = pathTracePixel<TEX>(A, stack, x, y, nClosest, nShadow);
 			const float inv	 = (float) A.spp;
 			A.film[(size_t) (A.H - 1 - y) * A.W + x] = make_float4(c.x / inv, c.y / inv, c.z / inv, 1.f);
 		}
@@ -135,13 +137,16 @@ __global__ void k_path_tracer_clear(unsigned long long *rays) {
 
 hipError_t launchPathTracer(const PtArgs &a, int grid, int tb, size_t lds, hipStream_t st) {
 	hipLaunchKernelGGL(k_path_tracer_clear, dim3(1), dim3(64), 0, st, a.rays);
-	hipLaunchKernelGGL(k_path_tracer, dim3(grid), dim3(tb), lds, st, a);
+	if (a.textured) hipLaunchKernelGGL(k_path_tracer<true>, dim3(grid), dim3(tb), lds, st, a);
+	else hipLaunchKernelGGL(k_path_tracer<false>, dim3(grid), dim3(tb), lds, st, a);
 	return hipGetLastError();
 }
 
-int pathTracerBlocksPerCU(int tb, size_t lds) {
+int pathTracerBlocksPerCU(int tb, size_t lds, bool textured) {
 	int nb = 0;
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_path_tracer, tb, lds) != hipSuccess || nb < 1) nb = 1;
+	const hipError_t e = textured ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_path_tracer<true>, tb, lds)
+								  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_path_tracer<false>, tb, lds);
+	if (e != hipSuccess || nb < 1) nb = 1;
 	return nb;
 }
 

@@ -1074,8 +1074,12 @@ void buildScene(krr_scene &s, const Json &j) {
 			krr_material mm = jsonMaterial(m);
 			// this build's keys: image files standing for the diffuse colour and the glossiness
 			const Json &p = m.contains("params") ? m.at("params") : m;
-			for (const char *k : {"diffuse_texture", "specular_texture"})
-				if (p.contains(k)) (k[0] == 'd' ? mm.diffuse_texture : mm.specular_texture) = textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
+			for (const char *k : {"diffuse_texture", "specular_texture", "transmission_texture"})
+				if (p.contains(k))
+					(k[0] == 'd' ? mm.diffuse_texture : k[0] == 's' ? mm.specular_texture : mm.transmission_texture) =
+						textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
+			// the alpha test's transmission = 1 - opacity (pbrt.cpp:152-156)
+			p.vec<3>("transmission", mm.transmission);
 			s.materials.push_back(mm);
 			if (m.contains("name")) overrides[m.value("name", "")] = mm;
 		}

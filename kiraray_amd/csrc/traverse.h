@@ -18,16 +18,81 @@
 #pragma once
 
 #include "bvh.h"
+#include "texture.h"
 #include "vecmath.h"
 
 namespace krr {
+
+// Alpha test data (alphaKilled, src/render/shading.h:89-113), present only
+// for scenes with a transmission texture or constant: per triangle the
+// texture coordinates of its vertices (zero for meshes without them) and its
+// material; per material the transmission image, or the constant when img is
+// null.
+struct AlphaTri {
+	float u0, v0, u1, v1, u2, v2;
+	int material, pad;
+};
+struct AlphaSrc {
+	const float4 *img;
+	int w, h;
+	float T[3];
+	int pad;
+};
 
 struct TraceBvh {
 	const Bvh4Node *nodes;
 	const LeafTri *tris;
 	int stackLds; // traversal stack entries held in LDS per lane
 	int nNodes;	  // node count: a traversal visits each node at most once
+	const AlphaTri *alphaTri = nullptr; // leaves flagged kLeafAlpha index these
+	const AlphaSrc *alphaSrc = nullptr;
 };
+
+// HashFloat(o, d) of src/util/hash.h:130-150: MurmurHash64A (Austin Appleby,
+// 64-bit MurmurHash2 for x64; hash.h:37-76) with seed 0 over the 24 bytes of
+// the ray origin and direction, low 32 bits scaled by 2^-32.
+KRR_DEV float hashRayFloat(const f3 &o, const f3 &d) {
+	const uint64_t m = 0xc6a4a7935bd1e995ull;
+	const uint64_t w[3] = {(uint64_t) __float_as_uint(o.x) | (uint64_t) __float_as_uint(o.y) << 32,
+						   (uint64_t) __float_as_uint(o.z) | (uint64_t) __float_as_uint(d.x) << 32,
+						   (uint64_t) __float_as_uint(d.y) | (uint64_t) __float_as_uint(d.z) << 32};
+	uint64_t h = 24ull * m;
+#pragma unroll
+	for (int i = 0; i < 3; i++) {
+		uint64_t k = w[i] * m;
+		k ^= k >> 47;
+		k *= m;
+		h ^= k;
+		h *= m;
+	}
+	h ^= h >> 47;
+	h *= m;
+	h ^= h >> 47;
+	return (float) (uint32_t) h * 0x1p-32f;
+}
+
+// alphaKilled (shading.h:89-113): alpha = 1 - luminance(transmission at the
+// hit's texture coordinates), luminance weights (0.299, 0.587, 0.114)
+// (spectrum.h:545-547); a hit is ignored when alpha <= 0, or when alpha < 1
+// and HashFloat(o, d) > alpha.  Null-material triangles are never flagged.
+KRR_DEV bool alphaKilled(const TraceBvh &bvh, int prim, float b1, float b2, const f3 &o, const f3 &d) {
+	const AlphaTri a = bvh.alphaTri[prim];
+	const AlphaSrc s = bvh.alphaSrc[a.material];
+	const float b0	 = 1.f - b1 - b2;
+	float r = s.T[0], g = s.T[1], bl = s.T[2];
+	if (s.img) {
+		const float u  = b0 * a.u0 + b1 * a.u1 + b2 * a.u2;
+		const float v  = b0 * a.v0 + b1 * a.v1 + b2 * a.v2;
+		const float4 c = texBilinear4(s.img, s.w, s.h, u, v);
+		r			   = c.x;
+		g			   = c.y;
+		bl			   = c.z;
+	}
+	const float alpha = 1.f - (r * 0.299f + g * 0.587f + bl * 0.114f);
+	if (alpha >= 1.f) return false;
+	if (alpha <= 0.f) return true;
+	return hashRayFloat(o, d) > alpha;
+}
 
 // Per-ray constants of the watertight test: origin, the axis permutation
 // (kx, ky, kz) as the two bits of each index, and the shear.
@@ -149,8 +214,11 @@ struct LaneStack {
 };
 
 // Traverses one ray.  ANY: stop at the first hit of a non-null-material
-// triangle (shadow rays, device.cu:83-100).
-template <bool ANY, bool COUNT>
+// triangle (shadow rays, device.cu:83-100).  ALPHA: hits on triangles flagged
+// kLeafAlpha pass the any-hit alpha test first (device.cu:65-67,86-87,111-113;
+// megakernel/device.cu:134-142); the result does not depend on the order of
+// the candidates, since the test is a function of the ray and the hit.
+template <bool ANY, bool COUNT, bool ALPHA = false>
 KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, const LaneStack &stack,
 							 uint32_t &nodeVisits, uint32_t &triTests) {
 	TraceResult res{tmax, -1, 0.f, 0.f};
@@ -203,12 +271,15 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 			for (int k = 0; k < cnt; k++) {
 				const float4 *tp = reinterpret_cast<const float4 *>(bvh.tris + first + k);
 				const float4 a = tp[0], b = tp[1], e = tp[2];
-				if (ANY && (__float_as_int(b.w) & 1)) continue;
+				if (ANY && (__float_as_int(b.w) & kLeafNull)) continue;
 				if (COUNT) triTests++;
 				float t, b1, b2;
 				if (!intersectTri(rp, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(e.x, e.y, e.z), t, b1, b2)) continue;
 				const int prim = __float_as_int(a.w);
 				if (t < res.t || (t == res.t && res.prim >= 0 && prim < res.prim)) {
+					if constexpr (ALPHA) {
+						if ((__float_as_int(b.w) & kLeafAlpha) && alphaKilled(bvh, prim, b1, b2, o, d)) continue;
+					}
 					res.t	 = t;
 					res.prim = prim;
 					res.b1	 = b1;

@@ -180,7 +180,7 @@ template <bool COUNT>
 KRR_DEV void traceClosestItem(const TraceBvh &bvh, const float4 *__restrict__ rays, float4 *__restrict__ hits, int i,
 							  const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
 	const float4 a = rays[2 * i], b = rays[2 * i + 1];
-	TraceResult r  = traverse<false, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
+	TraceResult r  = traverse<false, COUNT, true>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
 	hits[i]		   = make_float4(r.prim >= 0 ? r.t : __builtin_inff(), __int_as_float(r.prim), r.b1, r.b2);
 }
 
@@ -242,7 +242,7 @@ __global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *
 		const int i = base + laneId();
 		if (i < count) {
 			const float4 a = rays[2 * i], b = rays[2 * i + 1];
-			TraceResult r  = traverse<true, COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
+			TraceResult r  = traverse<true, COUNT, true>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt);
 			visible[i]	   = r.prim < 0 ? 1 : 0;
 		}
 	}
@@ -683,7 +683,7 @@ constexpr int kMaxNullCrossings = 1024;
 
 // traceTransmittance of one shadow ray (tMax = 1 to the offset light point):
 // ratio tracking through media, stepping across null-material surfaces.
-template <bool COUNT>
+template <bool COUNT, bool ALPHA = true>
 KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const TraceBvh &bvh, f3 o, f3 d, const float tMax, const int pixel,
 							   const int origMedium, const f3 &Ld, const f3 &spu, const f3 &spl, const PixelStateDev &px,
 							   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
@@ -698,7 +698,7 @@ KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const Trac
 	// the cap only bounds pathological stacks of coplanar null surfaces.
 	for (int crossing = 0; crossing < kMaxNullCrossings && !(fabsf(d.x) <= zeroEps && fabsf(d.y) <= zeroEps && fabsf(d.z) <= zeroEps);
 		 crossing++) {
-		const TraceResult h = traverse<false, COUNT>(bvh, o, d, tMax, stack, nv, nt);
+		const TraceResult h = traverse<false, COUNT, ALPHA>(bvh, o, d, tMax, stack, nv, nt);
 		const bool visible	= h.prim < 0;
 		if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
 		if (!visible && intr.material >= 0) {
@@ -743,10 +743,10 @@ KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const Trac
 }
 
 // Non-media shadow test of one shadow ray: add Ld / mean(pu + pl) if visible.
-template <bool COUNT>
+template <bool COUNT, bool ALPHA = true>
 KRR_DEV void occlusionItem(const TraceBvh &bvh, const f3 &o, const f3 &d, float tMax, int pixel, const f3 &c, const PixelStateDev &px,
 						   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
-	const TraceResult r = traverse<true, COUNT>(bvh, o, d, tMax, stack, nv, nt);
+	const TraceResult r = traverse<true, COUNT, ALPHA>(bvh, o, d, tMax, stack, nv, nt);
 	if (r.prim < 0) {
 		float4 L	= px.L[pixel];
 		L.x			= L.x + c.x;
@@ -1004,7 +1004,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		if (__ballot(active) == 0) break;
 		if (!active) continue;
 		nClosest++;
-		const TraceResult h = traverse<false, COUNT>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
+		const TraceResult h = traverse<false, COUNT, !PLAIN>(bvh, r.o, r.d, __builtin_inff(), stack, nv, nt);
 		const float4 hv		= make_float4(h.prim >= 0 ? h.t : __builtin_inff(), __int_as_float(h.prim), h.b1, h.b2);
 		bool wantNext = false, wantShadow = false;
 		RayItem nx;
@@ -1015,9 +1015,9 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		if (wantShadow) {
 			nShadow++;
 			if (MEDIA && P.enableMedium)
-				transmittanceItem<COUNT>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
+				transmittanceItem<COUNT, !PLAIN>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
 			else
-				occlusionItem<COUNT>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+				occlusionItem<COUNT, !PLAIN>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
 		}
 		if (NRC && trainSlot >= 0) { // radiance gathered up to and including this vertex's NEE
 			const float4 Lv = px.L[r.pixel];
@@ -1196,6 +1196,7 @@ struct DevBuf {
 
 struct AccelData {
 	DevBuf nodes, tris;
+	DevBuf alphaTri, alphaSrc, alphaImg; // alpha test data; empty when no material has one
 	int stackSize = 0;
 	int stackLds  = kShortStack; // stack entries per lane kept in LDS (the rest spill to global memory)
 	Bvh4Host host; // kept for statistics (nodes/tris arrays cleared after upload)
@@ -1204,8 +1205,69 @@ struct AccelData {
 	double buildSeconds = 0;
 	bool onDevice		= false; // built by the GPU builder
 	int ldsEntries() const { return std::min(stackSize, stackLds); }
-	TraceBvh dev() const { return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes}; }
+	TraceBvh dev() const {
+		return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes, alphaTri.as<AlphaTri>(), alphaSrc.as<AlphaSrc>()};
+	}
 };
+
+// A material has the any-hit alpha test when it has a transmission texture
+// or a non-zero transmission constant (rt::TextureData::isValid of the
+// Transmission slot, shading.h:92-95; pbrt.cpp:145-156 sets it only then).
+bool materialAlphaTested(const krr_material &m) {
+	return m.transmission_texture > 0 || m.transmission[0] != 0.f || m.transmission[1] != 0.f || m.transmission[2] != 0.f;
+}
+
+// Per-triangle texture coordinates and material, per-material transmission
+// source (images copied into one buffer), for the triangles flagged
+// kLeafAlpha.  Releases the buffers when no triangle is flagged.
+int uploadAlpha(const krr_scene_desc *s, const std::vector<uint8_t> &flags, AccelData &a) {
+	a.alphaTri.release();
+	a.alphaSrc.release();
+	a.alphaImg.release();
+	bool any = false;
+	for (uint8_t f : flags) any |= (f & kLeafAlpha) != 0;
+	if (!any) return KRR_OK;
+	std::vector<AlphaTri> tri((size_t) s->n_triangles);
+	for (int m = 0; m < s->n_meshes; m++) {
+		const krr_mesh &mm = s->meshes[m];
+		for (int t = 0; t < mm.n_triangles; t++) {
+			const int64_t p = (int64_t) mm.first_triangle + t;
+			AlphaTri &at	= tri[(size_t) p];
+			at.material		= std::max(mm.material, 0);
+			if (mm.has_texcoords && s->texcoords) { // shading.h:100-102: zero without texture coordinates
+				const int32_t *v = s->indices + 3 * p;
+				at.u0 = s->texcoords[2 * v[0]], at.v0 = s->texcoords[2 * v[0] + 1];
+				at.u1 = s->texcoords[2 * v[1]], at.v1 = s->texcoords[2 * v[1] + 1];
+				at.u2 = s->texcoords[2 * v[2]], at.v2 = s->texcoords[2 * v[2] + 1];
+			}
+		}
+	}
+	std::vector<AlphaSrc> src((size_t) std::max(s->n_materials, 1));
+	std::vector<size_t> off((size_t) std::max(s->n_materials, 1), 0);
+	size_t texels = 0;
+	for (int i = 0; i < s->n_materials; i++) {
+		const krr_material &m = s->materials[i];
+		for (int c = 0; c < 3; c++) src[i].T[c] = m.transmission[c];
+		if (m.transmission_texture > 0) {
+			const krr_texture &t = s->textures[m.transmission_texture - 1];
+			src[i].w = t.width;
+			src[i].h = t.height;
+			off[i]	 = texels;
+			texels	+= (size_t) t.width * t.height;
+		}
+	}
+	KRR_HIP(a.alphaImg.alloc(texels * sizeof(float4)));
+	for (int i = 0; i < s->n_materials; i++) {
+		const krr_material &m = s->materials[i];
+		if (m.transmission_texture <= 0) continue;
+		const krr_texture &t = s->textures[m.transmission_texture - 1];
+		src[i].img			 = a.alphaImg.as<float4>() + off[i];
+		KRR_HIP(hipMemcpy((void *) src[i].img, t.rgba, (size_t) t.width * t.height * sizeof(float4), hipMemcpyHostToDevice));
+	}
+	KRR_HIP(a.alphaTri.upload(tri.data(), tri.size() * sizeof(AlphaTri)));
+	KRR_HIP(a.alphaSrc.upload(src.data(), src.size() * sizeof(AlphaSrc)));
+	return KRR_OK;
+}
 
 int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCost, bool onDevice) {
 	if (!s || s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
@@ -1219,8 +1281,17 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCo
 		if (mm.first_triangle < 0 || mm.n_triangles < 0 || (int64_t) mm.first_triangle + mm.n_triangles > s->n_triangles)
 			return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d triangle range out of bounds", m);
 		if (mm.material < 0)
-			for (int t = 0; t < mm.n_triangles; t++) nullFlag[mm.first_triangle + t] = 1;
+			for (int t = 0; t < mm.n_triangles; t++) nullFlag[mm.first_triangle + t] = kLeafNull;
+		else if (mm.material < s->n_materials && s->materials && materialAlphaTested(s->materials[mm.material]))
+			for (int t = 0; t < mm.n_triangles; t++) nullFlag[mm.first_triangle + t] = kLeafAlpha;
 	}
+	for (int i = 0; i < s->n_materials && s->materials; i++) {
+		const int tt = s->materials[i].transmission_texture;
+		const bool ok = tt == 0 || (tt > 0 && tt <= s->n_textures && s->textures && s->textures[tt - 1].rgba && s->textures[tt - 1].width > 0 &&
+									s->textures[tt - 1].height > 0);
+		if (!ok) return krr::fail(KRR_ERR_INVALID_ARG, "scene: material %d transmission_texture %d is not a texture", i, tt);
+	}
+	if (int rc = uploadAlpha(s, nullFlag, a)) return rc;
 	BvhBuildOptions opt;
 	opt.maxLeafSize = leafSize;
 	opt.nodeCost	= nodeCost;
@@ -1670,7 +1741,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		ds.binv[a] = hi[a] > lo[a] ? 1.f / (hi[a] - lo[a]) : 1.f;
 	}
 	wf->sceneHasMedia = s->n_media > 0;
-	wf->sceneTextured = s->n_textures > 0;
+	wf->sceneTextured = s->n_textures > 0 || wf->accel.alphaTri.p != nullptr;
 	for (int i = 0; i < s->n_infinite_lights; i++) wf->sceneTextured |= s->infinite_lights[i].image_rgba != nullptr;
 	wf->haveScene	  = true;
 	return KRR_OK;
@@ -2789,12 +2860,12 @@ int krr_mk_render(krr_mk *mk, float *film_dev, void *hip_stream) {
 	const int nTiles = tilesX * tilesY;
 	const int needed = (nTiles + tb / 64 - 1) / (tb / 64);
 	const uint64_t key = ((uint64_t) nTiles << 20) ^ ((uint64_t) tb << 8) ^ ((uint64_t) wf->accel.stackSize << 44) ^
-						 ((uint64_t) wf->stackLdsEntries << 36) ^ ((uint64_t) (mk->occupancy * 1024.f) << 52);
+						 ((uint64_t) wf->stackLdsEntries << 36) ^ ((uint64_t) (mk->occupancy * 1024.f) << 52) ^ (wf->sceneTextured ? 1ull : 0ull);
 	if (key != mk->gridKey) {
 		int dev = 0, cus = 0;
 		KRR_HIP(hipGetDevice(&dev));
 		KRR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-		const int resident = scaledGrid(pathTracerBlocksPerCU(tb, lds) * std::max(cus, 1), mk->occupancy);
+		const int resident = scaledGrid(pathTracerBlocksPerCU(tb, lds, wf->sceneTextured) * std::max(cus, 1), mk->occupancy);
 		mk->grid		   = std::max(1, std::min(needed, resident));
 		if (int rc = reserveOverflow(mk->ovf, wf->accel, (size_t) mk->grid * tb)) return rc;
 		if (!mk->rays.p) KRR_HIP(mk->rays.alloc(2 * sizeof(unsigned long long)));
@@ -2817,6 +2888,7 @@ int krr_mk_render(krr_mk *mk, float *film_dev, void *hip_stream) {
 	a.rays		  = mk->rays.as<unsigned long long>();
 	a.tilesX	  = tilesX;
 	a.nTiles	  = nTiles;
+	a.textured	  = wf->sceneTextured;
 	mk->timed	  = mk->timeKernels;
 	if (mk->timed) {
 		for (hipEvent_t &e : mk->ev)

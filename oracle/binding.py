@@ -40,6 +40,7 @@ _PROTO = {
     "oracle_pcg32_advance": (None, [C.c_uint64, C.c_uint64, C.c_int64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "oracle_pixel_samples": (None, [C.c_int32, C.c_int32, C.c_uint32, C.c_int64, C.c_int32, C.c_void_p]),
     "oracle_interleave_32bit": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+    "oracle_murmur64a": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "oracle_trace_closest": (C.c_int, [C.POINTER(krr_scene_desc), C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "oracle_trace_any": (C.c_int, [C.POINTER(krr_scene_desc), C.c_void_p, C.c_int64, C.c_void_p, C.c_int32]),
     "oracle_camera_ray": (None, [C.POINTER(krr_camera), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
@@ -205,6 +206,11 @@ def bsdf_sample(m, wo, u):
     pdf = C.c_float()
     flags = load().oracle_bsdf_sample(C.byref(m), _ptr(wo), _ptr(u), _ptr(f), _ptr(wi), C.byref(pdf))
     return f, wi, pdf.value, flags
+
+
+def murmur64a(data, seed=0):
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    return load().oracle_murmur64a(_ptr(buf), len(buf), seed)
 
 
 def error_metric(metric, img, ref):

@@ -29,6 +29,35 @@
 
 namespace oracle {
 
+// MurmurHash64A (Austin Appleby's 64-bit MurmurHash2 for x64), the hash of
+// src/util/hash.h:37-76 behind HashFloat (hash.h:130-150): 8-byte blocks read
+// little-endian, then the 1..7 tail bytes, then the final avalanche.
+inline uint64_t murmurHash64A(const unsigned char *key, size_t len, uint64_t seed) {
+	const uint64_t m = 0xc6a4a7935bd1e995ull;
+	uint64_t h		 = seed ^ (len * m);
+	const size_t nb	 = len / 8;
+	for (size_t i = 0; i < nb; i++) {
+		uint64_t k;
+		std::memcpy(&k, key + 8 * i, 8);
+		k *= m;
+		k ^= k >> 47;
+		k *= m;
+		h ^= k;
+		h *= m;
+	}
+	const size_t rest = len & 7;
+	if (rest) {
+		uint64_t t = 0;
+		for (size_t i = 0; i < rest; i++) t |= (uint64_t) key[8 * nb + i] << (8 * i);
+		h ^= t;
+		h *= m;
+	}
+	h ^= h >> 47;
+	h *= m;
+	h ^= h >> 47;
+	return h;
+}
+
 // ----------------------------------------------------------------------------
 // Scalar math: binary32 routes through detmath, binary64 through libm.
 // ----------------------------------------------------------------------------

@@ -645,6 +645,7 @@ void oracle_pixel_samples(int32_t x, int32_t y, uint32_t sample_index, int64_t a
 }
 
 uint32_t oracle_interleave_32bit(uint32_t x, uint32_t y) { return interleave32(x, y); }
+uint64_t oracle_murmur64a(const void *key, uint64_t len, uint64_t seed) { return murmurHash64A((const unsigned char *) key, (size_t) len, seed); }
 
 static int traceImpl(const krr_scene_desc *scene, const float *rays, int64_t n, krr_hit *hits, int32_t *vis, int32_t use_bvh) {
 	if (!scene || !rays || n < 0) return KRR_ERR_INVALID_ARG;

@@ -31,6 +31,30 @@ struct Hit {
 	float b1 = 0, b2 = 0;
 };
 
+// Bilinear lookup with wrap-around of a normalised-coordinate RGBA texture
+// (texture.cpp:236-243).
+template <class R> void texBilinear4(const float *img, int w, int h, R u, R v, R out[4]) {
+	R x = u * R((float) w) - R(0.5f), y = v * R((float) h) - R(0.5f);
+	if (!(x > R(-1e6f) && x < R(1e6f))) x = R(0);
+	if (!(y > R(-1e6f) && y < R(1e6f))) y = R(0);
+	const R x0 = m_floor(x), y0 = m_floor(y);
+	const R fx = x - x0, fy = y - y0;
+	int i0 = (int) x0 % w, j0 = (int) y0 % h;
+	if (i0 < 0) i0 += w;
+	if (j0 < 0) j0 += h;
+	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
+	const float *a = img + 4 * ((size_t) j0 * w + i0), *b = img + 4 * ((size_t) j0 * w + i1);
+	const float *c = img + 4 * ((size_t) j1 * w + i0), *d = img + 4 * ((size_t) j1 * w + i1);
+	const R gx = R(1) - fx, gy = R(1) - fy;
+	for (int k = 0; k < 4; k++) out[k] = gy * (gx * R(a[k]) + fx * R(b[k])) + fy * (gx * R(c[k]) + fx * R(d[k]));
+}
+
+template <class R> V3<R> texBilinear(const float *img, int w, int h, R u, R v) {
+	R c[4];
+	texBilinear4<R>(img, w, h, u, v, c);
+	return V3<R>(c[0], c[1], c[2]);
+}
+
 // ----------------------------------------------------------------------------
 // Ray casting over the scene (closest: device.cu:76-81 with tmin 0; any:
 // device.cu:83-100 where the any-hit program skips null-material triangles).
@@ -47,6 +71,41 @@ struct Tracer {
 	}
 	bool isNull(int prim) const { return sv->d->meshes[sv->triMesh[prim]].material < 0; }
 
+	// alphaKilled (src/render/shading.h:89-113): the any-hit programs of the
+	// closest, shadow and transmittance rays (device.cu:65-67,86-87,111-113)
+	// ignore a hit on a material with a transmission texture or constant
+	// when alpha = 1 - luminance(transmission at the hit's uv) is <= 0, or is
+	// < 1 and HashFloat(origin, direction) > alpha.
+	bool alphaKilled(int prim, float b1, float b2, const float o[3], const float dir[3]) const {
+		const krr_scene_desc *d = sv->d;
+		const krr_mesh &m		= d->meshes[sv->triMesh[prim]];
+		if (m.material < 0 || m.material >= d->n_materials) return false;
+		const krr_material &mat = d->materials[m.material];
+		float T[3] = {mat.transmission[0], mat.transmission[1], mat.transmission[2]};
+		if (mat.transmission_texture <= 0 && T[0] == 0.f && T[1] == 0.f && T[2] == 0.f) return false;
+		if (mat.transmission_texture > 0) {
+			float u = 0.f, v = 0.f; // shading.h:98-102: zero without texture coordinates
+			if (m.has_texcoords && d->texcoords) {
+				const int32_t *vi = d->indices + 3 * (size_t) prim;
+				const float *t	  = d->texcoords;
+				const float b0	  = 1.f - b1 - b2;
+				u				  = b0 * t[2 * vi[0]] + b1 * t[2 * vi[1]] + b2 * t[2 * vi[2]];
+				v				  = b0 * t[2 * vi[0] + 1] + b1 * t[2 * vi[1] + 1] + b2 * t[2 * vi[2] + 1];
+			}
+			const krr_texture &tx = d->textures[mat.transmission_texture - 1];
+			float c[4];
+			texBilinear4<float>(tx.rgba, tx.width, tx.height, u, v, c);
+			T[0] = c[0], T[1] = c[1], T[2] = c[2];
+		}
+		const float alpha = 1.f - (T[0] * 0.299f + T[1] * 0.587f + T[2] * 0.114f); // luminance, spectrum.h:545-547
+		if (alpha >= 1.f) return false;
+		if (alpha <= 0.f) return true;
+		unsigned char buf[24];
+		std::memcpy(buf, o, 12);
+		std::memcpy(buf + 12, dir, 12);
+		return (float) (uint32_t) murmurHash64A(buf, sizeof(buf), 0) * 0x1p-32f > alpha;
+	}
+
 	template <bool ANY> Hit trace(const float o[3], const float dir[3], float tmax) const {
 		Hit h;
 		h.t			= tmax;
@@ -58,6 +117,7 @@ struct Tracer {
 			float t, b1, b2;
 			if (!intersectTri(rp, v0, v1, v2, t, b1, b2)) return false;
 			if (t < h.t || (t == h.t && h.prim >= 0 && prim < h.prim)) {
+				if (alphaKilled(prim, b1, b2, o, dir)) return false;
 				h.t	   = t;
 				h.prim = prim;
 				h.b1   = b1;
@@ -141,30 +201,6 @@ template <class R> struct SurfInt {
 		return spawnRayTo(to);
 	}
 };
-
-// Bilinear lookup with wrap-around of a normalised-coordinate RGBA texture
-// (texture.cpp:236-243).
-template <class R> void texBilinear4(const float *img, int w, int h, R u, R v, R out[4]) {
-	R x = u * R((float) w) - R(0.5f), y = v * R((float) h) - R(0.5f);
-	if (!(x > R(-1e6f) && x < R(1e6f))) x = R(0);
-	if (!(y > R(-1e6f) && y < R(1e6f))) y = R(0);
-	const R x0 = m_floor(x), y0 = m_floor(y);
-	const R fx = x - x0, fy = y - y0;
-	int i0 = (int) x0 % w, j0 = (int) y0 % h;
-	if (i0 < 0) i0 += w;
-	if (j0 < 0) j0 += h;
-	const int i1 = i0 + 1 == w ? 0 : i0 + 1, j1 = j0 + 1 == h ? 0 : j0 + 1;
-	const float *a = img + 4 * ((size_t) j0 * w + i0), *b = img + 4 * ((size_t) j0 * w + i1);
-	const float *c = img + 4 * ((size_t) j1 * w + i0), *d = img + 4 * ((size_t) j1 * w + i1);
-	const R gx = R(1) - fx, gy = R(1) - fy;
-	for (int k = 0; k < 4; k++) out[k] = gy * (gx * R(a[k]) + fx * R(b[k])) + fy * (gx * R(c[k]) + fx * R(d[k]));
-}
-
-template <class R> V3<R> texBilinear(const float *img, int w, int h, R u, R v) {
-	R c[4];
-	texBilinear4<R>(img, w, h, u, v, c);
-	return V3<R>(c[0], c[1], c[2]);
-}
 
 // getHitInfo + prepareSurfaceInteraction (src/render/shading.h:78-87,115-226)
 // for an identity instance transform (p unchanged, frame vectors normalised
