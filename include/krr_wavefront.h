@@ -67,6 +67,8 @@ typedef struct krr_material {
 	int32_t specular_texture;  /* 0 = none, else 1 + index into krr_scene_desc.textures */
 	int32_t transmission_texture; /* 0 = none, else 1 + index into krr_scene_desc.textures */
 	float transmission[3];		  /* constant transmission when there is no texture; 0 = opaque */
+	int32_t emissive_texture;	  /* 0 = none, else 1 + index into krr_scene_desc.textures: an emissive
+									 mesh of this material emits max(emission) * rgb(uv) (light.h:188-193) */
 } krr_material;
 
 /* An RGBA float32 image used as a normalised-coordinate texture: height rows

@@ -30,7 +30,8 @@ class krr_material(C.Structure):
     _fields_ = [("type", C.c_int32), ("diffuse", C.c_float * 3), ("specular", C.c_float * 4),
                 ("ior", C.c_float), ("specular_transmission", C.c_float), ("anisotropic", C.c_float),
                 ("diffuse_texture", C.c_int32), ("specular_texture", C.c_int32),
-                ("transmission_texture", C.c_int32), ("transmission", C.c_float * 3)]
+                ("transmission_texture", C.c_int32), ("transmission", C.c_float * 3),
+                ("emissive_texture", C.c_int32)]
 
 
 class krr_texture(C.Structure):

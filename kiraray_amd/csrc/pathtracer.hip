@@ -60,7 +60,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 			prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, -1, intr);
 			if (intr.light >= 0) { // handleHit (device.cu:52-65)
 				const DevLight Lr = sc.lights[intr.light];
-				const f3 Le		  = areaLightL(Lr, intr.n, intr.wo);
+				const f3 Le		  = areaLightL<TEX>(Lr, intr.n, intr.wo, intr.u, intr.v);
 				float w			  = 1.f;
 				if (A.nee && depth > 0) {
 					const float lightPdf = lightPdfLi(sc, Lr, intr.p, intr.n, ctxP) * selPdf;

@@ -485,7 +485,7 @@ struct ObjMaterial {
 	V kd{1, 1, 1}, ks{0, 0, 0}, ke{0, 0, 0}, tf{1, 1, 1};
 	float ns = -1.f, ni = -1.f, d = 1.f;
 	bool hasTf = false;
-	std::string mapKd, mapKs; // texture files (map_Kd, map_Ks), full paths
+	std::string mapKd, mapKs, mapKe; // texture files (map_Kd, map_Ks, map_Ke), full paths
 };
 
 std::string joinPath(const std::string &dir, const std::string &p) {
@@ -530,10 +530,10 @@ void readMtl(const std::string &path, std::map<std::string, ObjMaterial> &out) {
 			float tr = 0.f;
 			ss >> tr;
 			m.d = 1.f - tr;
-		} else if (k == "map_Kd" || k == "map_Ks") { // options first, the file name last
+		} else if (k == "map_Kd" || k == "map_Ks" || k == "map_Ke") { // options first, the file name last
 			std::string tok, file;
 			while (ss >> tok) file = tok;
-			if (!file.empty()) (k == "map_Kd" ? m.mapKd : m.mapKs) = joinPath(dirOf(path), file);
+			if (!file.empty()) (k == "map_Kd" ? m.mapKd : k == "map_Ks" ? m.mapKs : m.mapKe) = joinPath(dirOf(path), file);
 		}
 	}
 }
@@ -741,17 +741,24 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			krr_material fm	  = objMaterial(om);
 			fm.diffuse_texture	= om.mapKd.empty() ? 0 : textureFor(b.s, om.mapKd);
 			fm.specular_texture = om.mapKs.empty() ? 0 : textureFor(b.s, om.mapKs);
+			fm.emissive_texture = om.mapKe.empty() ? 0 : textureFor(b.s, om.mapKe);
 			krr_material mm = fm;
 			if (ov != overrides.end()) { // the JSON entry sets the params; the file's textures stay (krrscene.cpp:199-205)
 				mm = ov->second;
 				if (!mm.diffuse_texture) mm.diffuse_texture = fm.diffuse_texture;
 				if (!mm.specular_texture) mm.specular_texture = fm.specular_texture;
+				if (!mm.emissive_texture) mm.emissive_texture = fm.emissive_texture;
 			}
 			mi				= b.addMaterial(mm);
 			matIndex[r.mat] = mi;
 		}
 		const auto ot	 = mtl.find(r.mat);
-		const V emission = ot != mtl.end() ? ot->second.ke : V{0, 0, 0};
+		// Ke becomes a constant Emissive texture (assimp.cpp:168-180); the mesh
+		// light takes scale = max(Ke) and evaluates that texture, so it emits
+		// scale * Ke, or scale * map_Ke(uv) when the file has one
+		// (mesh.cpp:39-59, light.h:188-193)
+		V emission = ot != mtl.end() ? ot->second.ke : V{0, 0, 0};
+		if (ot != mtl.end() && ot->second.mapKe.empty()) emission = mul(emission, std::max(std::max(emission.x, emission.y), emission.z));
 		// vertices of the run: one per (position, normal) corner pair
 		std::map<std::tuple<int, int, int>, int> vid;
 		std::vector<V> verts, nrms;
@@ -1074,9 +1081,9 @@ void buildScene(krr_scene &s, const Json &j) {
 			krr_material mm = jsonMaterial(m);
 			// this build's keys: image files standing for the diffuse colour and the glossiness
 			const Json &p = m.contains("params") ? m.at("params") : m;
-			for (const char *k : {"diffuse_texture", "specular_texture", "transmission_texture"})
+			for (const char *k : {"diffuse_texture", "specular_texture", "transmission_texture", "emissive_texture"})
 				if (p.contains(k))
-					(k[0] == 'd' ? mm.diffuse_texture : k[0] == 's' ? mm.specular_texture : mm.transmission_texture) =
+					(k[0] == 'd' ? mm.diffuse_texture : k[0] == 's' ? mm.specular_texture : k[0] == 't' ? mm.transmission_texture : mm.emissive_texture) =
 						textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
 			// the alpha test's transmission = 1 - opacity (pbrt.cpp:152-156)
 			p.vec<3>("transmission", mm.transmission);

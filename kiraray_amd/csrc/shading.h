@@ -69,6 +69,7 @@ struct DevLight {
 	const float4 *img;
 	int iw, ih;
 	float R[9];
+	int hasUV; // area light with an emissive texture: its mesh has texture coordinates
 };
 
 struct DevMedium {
@@ -677,8 +678,12 @@ KRR_DEV float triArea(const DevScene &sc, int prim) {
 	return 0.5f * length(cross(p1 - p0, p2 - p0));
 }
 
-KRR_DEV f3 areaLightL(const DevLight &L, const f3 &n, const f3 &w) {
+// DiffuseAreaLight::L (light.h:188-193): one-sided; scale * the emissive
+// texture's rgb at (u, v) when the light has one, else scale * Le.
+// TEX = false: no light of the scene has a texture.
+template <bool TEX = true> KRR_DEV f3 areaLightL(const DevLight &L, const f3 &n, const f3 &w, float u, float v) {
 	if (dot(n, w) < 0.f) return f3(); // one-sided
+	if (TEX && L.img) return L.scale * texBilinear(L.img, L.iw, L.ih, u, v);
 	return L.scale * f3(L.Le[0], L.Le[1], L.Le[2]);
 }
 
@@ -754,7 +759,12 @@ template <bool TEX = true> KRR_DEV LightSample sampleLi(const DevScene &sc, cons
 	ls.p   = p;
 	ls.n   = n;
 	ls.pdf = pdf;
-	ls.L   = areaLightL(L, n, normalize(ctxP - p));
+	float su = 0.f, sv = 0.f; // Triangle::sample's uv (shape.h:73-78)
+	if (TEX && L.img && L.hasUV) {
+		su = b0 * sc.uv[2 * i0] + b1 * sc.uv[2 * i1] + b2 * sc.uv[2 * i2];
+		sv = b0 * sc.uv[2 * i0 + 1] + b1 * sc.uv[2 * i1 + 1] + b2 * sc.uv[2 * i2 + 1];
+	}
+	ls.L = areaLightL<TEX>(L, n, normalize(ctxP - p), su, sv);
 	return ls;
 }
 

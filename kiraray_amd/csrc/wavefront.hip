@@ -460,7 +460,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 	const float lightSelectPdf = 1.f / (float) sc.nLights;
 	if (hasHL) { // handleHit
 		const DevLight Lr = sc.lights[intr.light];
-		f3 Le			  = areaLightL(Lr, intr.n, intr.wo) * thp;
+		f3 Le			  = areaLightL<!PLAIN>(Lr, intr.n, intr.wo, intr.u, intr.v) * thp;
 		if (P.nee && r.depth && !(r.bsdfType & BSDF_DELTA)) {
 			const float lightPdf = lightPdfLi(sc, Lr, intr.p, intr.n, r.cp) * lightSelectPdf;
 			Le					 = Le / (pl * lightPdf + pu).mean();
@@ -1654,6 +1654,21 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		KRR_HIP(buf->upload(t.rgba, (size_t) t.width * t.height * 4 * sizeof(float)));
 		texs[i] = DevTexture{buf->as<float4>(), t.width, t.height};
 		wf->textureBufs.push_back(std::move(buf));
+	}
+	// emissive textures of mesh lights (DiffuseAreaLight's texture, mesh.cpp:39-59)
+	for (int mi = 0; mi < s->n_meshes; mi++) {
+		const krr_mesh &m = s->meshes[mi];
+		if (m.material < 0) continue;
+		const int et = s->materials[m.material].emissive_texture;
+		if (et < 0 || et > s->n_textures) return krr::fail(KRR_ERR_INVALID_ARG, "material %d: emissive_texture out of range", m.material);
+		if (et == 0 || meshes[mi].lightBase < 0) continue;
+		for (int t = 0; t < m.n_triangles; t++) {
+			DevLight &L = lights[(size_t) meshes[mi].lightBase + t];
+			L.img		= texs[et - 1].img;
+			L.iw		= texs[et - 1].w;
+			L.ih		= texs[et - 1].h;
+			L.hasUV		= m.has_texcoords ? 1 : 0;
+		}
 	}
 	// media
 	wf->mediumBufs.clear();

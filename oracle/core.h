@@ -730,6 +730,7 @@ struct LightRec {
 	const float *img = nullptr; // infinite light image: RGBA rows from v = 0 (null: constant)
 	int iw = 0, ih = 0;
 	float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}; // row-major, world from light
+	bool hasUV = false;	// area light with an emissive texture (img): its mesh has texture coordinates
 };
 
 struct SceneView {
@@ -788,6 +789,13 @@ static void buildSceneView(const krr_scene_desc *d, SceneView &sv) {
 			if (emissive) {
 				sv.triLight[prim] = (int) sv.lights.size();
 				LightRec L{0, prim, {m.emission[0] / scale, m.emission[1] / scale, m.emission[2] / scale}, scale};
+				const int et = m.material >= 0 ? d->materials[m.material].emissive_texture : 0;
+				if (et > 0) { // the material's emissive texture (mesh.cpp:39-59)
+					L.img	= d->textures[et - 1].rgba;
+					L.iw	= d->textures[et - 1].width;
+					L.ih	= d->textures[et - 1].height;
+					L.hasUV = m.has_texcoords && d->texcoords;
+				}
 				sv.lights.push_back(L);
 			}
 		}

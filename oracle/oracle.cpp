@@ -67,7 +67,7 @@ template <class R> struct Integrator {
 	// handleHit (integrator.cpp:183-195).
 	void handleHit(const HitLightItem<R> &w, V3<R> &L) const {
 		const LightRec &Lr = sv.lights[w.light];
-		V3<R> Le		   = areaL<R>(Lr, w.n, w.wo) * w.thp;
+		V3<R> Le		   = areaL<R>(Lr, w.n, w.wo, w.u, w.v) * w.thp;
 		if (P.nee && w.depth && !(w.bsdfType & BSDF_DELTA)) {
 			R lightPdf = areaPdfLi<R>(sv, Lr, w.p, w.n, w.ctx) * lightSelectPdf();
 			Le /= (w.pl * lightPdf + w.pu).mean();
@@ -136,7 +136,7 @@ template <class R> struct Integrator {
 					} else {
 						if (intr.light >= 0) {
 							hasHitLight = true;
-							hitLight	= {intr.light, cur.ctx, intr.p, intr.wo, intr.n, cur.thp, cur.pu, cur.pl, cur.bsdfType, cur.depth};
+							hitLight	= {intr.light, cur.ctx, intr.p, intr.wo, intr.n, cur.thp, cur.pu, cur.pl, cur.bsdfType, cur.depth, intr.u, intr.v};
 						}
 						if (cur.thp.any()) {
 							hasScatter = true;
@@ -211,7 +211,7 @@ template <class R> struct Integrator {
 						} else {
 							if (w.intr.light >= 0) {
 								hasHitLight = true;
-								hitLight	= {w.intr.light, w.ctx, w.intr.p, w.intr.wo, w.intr.n, thp, pu, pl, w.bsdfType, w.depth};
+								hitLight	= {w.intr.light, w.ctx, w.intr.p, w.intr.wo, w.intr.n, thp, pu, pl, w.bsdfType, w.depth, w.intr.u, w.intr.v};
 							}
 							hasScatter = true;
 							scatter	   = {thp, pu, w.intr, w.depth};
@@ -434,7 +434,7 @@ template <class R> struct MkIntegrator {
 				const SurfInt<R> intr = prepareSurfaceInteraction<R>(sv, h, ray);
 				if (intr.light >= 0) { // handleHit (device.cu:52-65)
 					const LightRec &Lr = sv.lights[intr.light];
-					const V3<R> Le	   = areaL<R>(Lr, intr.n, intr.wo);
+					const V3<R> Le	   = areaL<R>(Lr, intr.n, intr.wo, intr.u, intr.v);
 					R w				   = R(1);
 					if (P.nee && depth > 0) {
 						const R lightPdf = areaPdfLi<R>(sv, Lr, intr.p, intr.n, ctx) * selPdf;

@@ -277,8 +277,11 @@ template <class R> R triArea(const krr_scene_desc *d, int prim) {
 	return R(0.5f) * norm(cross(p1 - p0, p2 - p0));
 }
 
-template <class R> V3<R> areaL(const LightRec &L, const V3<R> &n, const V3<R> &w) {
+// DiffuseAreaLight::L (light.h:188-193): scale * the emissive texture's rgb
+// at (u, v) when the light has one (rt::TextureData::evaluate), else scale * Le.
+template <class R> V3<R> areaL(const LightRec &L, const V3<R> &n, const V3<R> &w, R u, R v) {
 	if (dot(n, w) < R(0)) return V3<R>(); // one-sided (twoSided = false for mesh lights)
+	if (L.img) return R(L.scale) * texBilinear<R>(L.img, L.iw, L.ih, u, v);
 	return R(L.scale) * V3<R>(R(L.Le[0]), R(L.Le[1]), R(L.Le[2]));
 }
 
@@ -325,7 +328,13 @@ template <class R> LightSample<R> sampleLi(const SceneView &sv, const LightRec &
 	ls.intr.n  = n;
 	ls.intr.wo = normalize(ctx.p - p);
 	ls.pdf	   = pdf;
-	ls.L	   = areaL<R>(L, n, ls.intr.wo);
+	R su = R(0), sv_ = R(0); // Triangle::sample's uv (shape.h:73-78)
+	if (L.img && L.hasUV) {
+		const float *t = d->texcoords;
+		su			   = b0 * R(t[2 * i0]) + b1 * R(t[2 * i1]) + b2 * R(t[2 * i2]);
+		sv_			   = b0 * R(t[2 * i0 + 1]) + b1 * R(t[2 * i1 + 1]) + b2 * R(t[2 * i2 + 1]);
+	}
+	ls.L = areaL<R>(L, n, ls.intr.wo, su, sv_);
 	return ls;
 }
 
@@ -591,6 +600,7 @@ template <class R> struct HitLightItem {
 	V3<R> p, wo, n;
 	V3<R> thp, pu, pl;
 	int bsdfType, depth;
+	R u = R(0), v = R(0); // texture coordinates of the hit (emissive textures)
 };
 
 } // namespace oracle

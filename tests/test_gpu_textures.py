@@ -44,3 +44,29 @@ def test_megakernel_texture_parity(krr, oracle, gpu, tmp_path):
     o, os_ = oracle.render_megakernel(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
+
+
+def _emissive_scene(krr, tmp_path):
+    rng = np.random.default_rng(13)
+    np.save(tmp_path / "ke.npy", rng.random((10, 14, 3), dtype=np.float32))
+    lamp = {"type": "mesh", "material": 0, "indices": [0, 2, 1, 0, 3, 2], "emission": [3.0, 2.0, 1.0],
+            "positions": [-0.5, 1.6, -0.6, 0.4, 1.6, -0.6, 0.4, 1.6, 0.2, -0.5, 1.6, 0.2], "normals": [0, -1, 0] * 4,
+            "texcoords": [0, 0, 1.25, 0, 1.25, 1.5, 0, 1.5]}
+    cfg = dict(CORNELL)
+    cfg["materials"] = [{"bsdf": "diffuse", "params": {"emissive_texture": str(tmp_path / "ke.npy")}}]
+    cfg["model"] = list(CORNELL["model"]) + [lamp]
+    return krr.Scene(cfg)
+
+
+@pytest.mark.parametrize("integrator", ["wavefront", "megakernel"])
+def test_emissive_texture_parity(krr, oracle, gpu, tmp_path, integrator):
+    print(f"case: emissive-textured lamp in the Cornell box, {integrator} 96x96", flush=True)
+    s = _emissive_scene(krr, tmp_path)
+    if integrator == "wavefront":
+        g, gs = gpu_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=3)
+        o, os_ = oracle.render(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
+    else:
+        g, gs = mk_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=3)
+        o, os_ = oracle.render_megakernel(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)

@@ -65,7 +65,9 @@ f 5 6 7
     assert s.n_triangles == 3 and len(meshes) == 2
     lamp, glass = meshes
     assert lamp.n_triangles == 2 and glass.n_triangles == 1
-    assert list(lamp.emission) == [4, 3, 2] and list(glass.emission) == [0, 0, 0]
+    # Ke 4 3 2: the mesh light evaluates the constant Emissive texture (Ke) times scale = max(Ke)
+    # (assimp.cpp:168-180, mesh.cpp:39-59, light.h:188-193)
+    assert list(lamp.emission) == [16, 12, 8] and list(glass.emission) == [0, 0, 0]
     t0 = pos[idx[0]]
     assert np.allclose(t0, [[1, 2, 3], [3, 2, 3], [3, 4, 3]])  # fan (0, 1, 2), scaled 2, translated
     m_lamp, m_glass = mats[lamp.material], mats[glass.material]
@@ -240,7 +242,8 @@ def test_reference_example_cbox_scene(krr):
     s = krr.Scene(cfg)
     _, _, nrm, meshes, mats = _desc_arrays(krr, s)
     lights = [m for m in meshes if any(e > 0 for e in m.emission)]
-    assert len(lights) == 1 and list(lights[0].emission) == [17, 12, 4]
+    # Ke 17 12 4 emits max(Ke) * Ke (mesh.cpp:39-59, light.h:188-193)
+    assert len(lights) == 1 and list(lights[0].emission) == [289, 204, 68]
     assert s.n_triangles == 36  # 18 quads: 5 walls, two 6-sided boxes, the light
     assert np.allclose(np.linalg.norm(nrm, axis=1), 1, atol=1e-5)
     left = [mats[m.material] for m in meshes if abs(mats[m.material].diffuse[0] - 0.63) < 1e-6]
@@ -300,6 +303,9 @@ def _export_obj(cfg, path):
             lines.append("vn " + " ".join(repr(float(x)) for x in n))
         mat = cfg["materials"][m["material"]]["params"]["diffuse"]
         mtl.append(f"newmtl mesh{mi}\nKd {mat[0]!r} {mat[1]!r} {mat[2]!r}\nKe " + " ".join(repr(e) for e in m["emission"]) + "\n")
+        # the file's light emits max(Ke) * Ke (mesh.cpp:39-59, light.h:188-193): the JSON side follows
+        ke = np.array(m["emission"], np.float32)
+        m["emission"] = [float(x) for x in ke * ke.max()]
         lines.append(f"usemtl mesh{mi}")
         ix = np.array(m["indices"]).reshape(-1, 3) + nv + 1
         for a, b, c in ix:
