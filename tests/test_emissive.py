@@ -64,3 +64,19 @@ def test_light_samples_read_the_texture(oracle, krr, tmp_path, integrator):
     assert np.allclose(ba, bb, rtol=0.08), (ba, bb)
     left, right = ba[:, :2].mean(), ba[:, 2:].mean()
     assert left > 1.5 * right  # the bright half lights the -x side
+
+
+def test_obj_map_ke_and_ke(krr, tmp_path):
+    """MTL Ke alone emits max(Ke) * Ke; with map_Ke the mesh keeps Ke (its
+    maximum is the light's scale) and the material takes the image as its
+    emissive texture (assimp.cpp:48-58,168-180; mesh.cpp:39-59)."""
+    from test_scene_import import _desc_arrays
+    np.save(tmp_path / "e.npy", np.ones((2, 2, 3), np.float32))
+    (tmp_path / "m.mtl").write_text("newmtl lamp\nKd 0 0 0\nKe 2 1 0.5\nmap_Ke e.npy\nnewmtl glow\nKd 0 0 0\nKe 2 1 0.5\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 1 1\nvn 0 0 1\n"
+                                    "usemtl lamp\nf 1/1/1 2/2/1 3/3/1\nusemtl glow\nf 1/1/1 3/3/1 4/2/1\n")
+    s = krr.Scene({"model": [{"model": str(tmp_path / "m.obj")}]})
+    _, _, _, meshes, mats = _desc_arrays(krr, s)
+    lamp, glow = meshes
+    assert list(lamp.emission) == [2, 1, 0.5] and mats[lamp.material].emissive_texture > 0
+    assert list(glow.emission) == [4, 2, 1] and mats[glow.material].emissive_texture == 0
