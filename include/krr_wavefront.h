@@ -88,7 +88,8 @@ typedef struct krr_mesh {
 	int32_t medium_inside;  /* index into media or -1 */
 	int32_t medium_outside; /* index into media or -1 */
 	int32_t has_normals;	   /* per-vertex normals in krr_scene_desc.normals are valid */
-	float emission[3];	   /* Le; any component > 0 makes every triangle a DiffuseAreaLight */
+	float emission[3];	   /* Le; any component > 0 makes every triangle a DiffuseAreaLight (scale = max(Le));
+							  with the material's emissive_texture the light emits scale * rgb(uv) */
 	int32_t has_texcoords;  /* per-vertex texture coordinates in krr_scene_desc.texcoords are valid (else uv = 0) */
 } krr_mesh;
 
