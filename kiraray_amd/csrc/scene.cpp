@@ -12,7 +12,10 @@
 //    "material": i, "emission": [r,g,b]}
 // Media: {"type": "value_noise_grid", "res", "octaves", "seed",
 //         "sigma_t_max", "albedo", "g", "threshold", "bounds": [[lo],[hi]]}
+//         (density = max(0, noise - threshold) scaled to max 1; threshold 0)
 //        {"type": "homogeneous", "sigma_t", "albedo", "g", "Le"}
+//        {"type": "dense_grid", "res", "density", "bounds", "sigma_t",
+//         "albedo", "g", "scale"}
 // Camera: {"eye", "target", "up", "focalLength", "focalDistance",
 //          "lensRadius", "filmSize", "medium"} with the aspect ratio taken
 // from "resolution" as the renderer does (src/main/renderer.cpp:28).
@@ -370,7 +373,7 @@ void valueNoiseMedium(krr_scene &s, const Json &j) {
 	float smax	  = (float) j.value("sigma_t_max", 20.0);
 	float albedo  = (float) j.value("albedo", 0.8);
 	float g		  = (float) j.value("g", 0.3);
-	float thresh  = (float) j.value("threshold", 0.45);
+	float thresh  = (float) j.value("threshold", 0.0);
 	float bounds[2][3] = {{0, 0, 0}, {1, 1, 1}};
 	if (j.contains("bounds")) {
 		const auto &B = j.at("bounds").items();
@@ -415,6 +418,55 @@ void valueNoiseMedium(krr_scene &s, const Json &j) {
 	}
 	m.g		  = g;
 	m.scale	  = 1.f;
+	m.density = grid->data();
+	s.densities.push_back(std::move(grid));
+	s.media.push_back(m);
+}
+
+// Dense grid medium given voxel by voxel: {"type": "dense_grid", "res":
+// [x, y, z], "density": [x-fastest values], "bounds": [[lo], [hi]],
+// "sigma_t": [3], "albedo": [3], "g", "scale"} (defaults: unit cube, sigma_t
+// 1, albedo 0.5, g 0, scale 1).
+void denseGridMedium(krr_scene &s, const Json &j) {
+	if (!j.contains("res") || !j.contains("density")) throw std::runtime_error("dense_grid: needs res and density");
+	const auto &R = j.at("res").items();
+	if (R.size() != 3) throw std::runtime_error("dense_grid: res must be [x, y, z]");
+	int res[3];
+	for (int a = 0; a < 3; a++) {
+		res[a] = (int) R[a].asNumber();
+		if (res[a] < 1 || res[a] > 1024) throw std::runtime_error("dense_grid: res out of range");
+	}
+	const auto &D = j.at("density").items();
+	const size_t n = (size_t) res[0] * res[1] * res[2];
+	if (D.size() != n) throw std::runtime_error("dense_grid: density needs res[0]*res[1]*res[2] values");
+	auto grid = std::make_unique<std::vector<float>>(n);
+	for (size_t i = 0; i < n; i++) {
+		(*grid)[i] = (float) D[i].asNumber();
+		if (!((*grid)[i] >= 0.f)) throw std::runtime_error("dense_grid: densities must be >= 0");
+	}
+	krr_medium m;
+	std::memset(&m, 0, sizeof(m));
+	m.type = KRR_MEDIUM_GRID;
+	float st[3] = {1, 1, 1}, al[3] = {0.5f, 0.5f, 0.5f};
+	j.vec<3>("sigma_t", st);
+	j.vec<3>("albedo", al);
+	float bounds[2][3] = {{0, 0, 0}, {1, 1, 1}};
+	if (j.contains("bounds")) {
+		const auto &B = j.at("bounds").items();
+		if (B.size() != 2) throw std::runtime_error("bounds must be [[lo],[hi]]");
+		for (int k = 0; k < 2; k++)
+			for (int a = 0; a < 3; a++) bounds[k][a] = (float) B[k][a].asNumber();
+	}
+	for (int a = 0; a < 3; a++) {
+		m.sigma_t[a]	= st[a];
+		m.albedo[a]		= al[a];
+		m.res[a]		= res[a];
+		m.bounds_min[a] = bounds[0][a];
+		m.bounds_max[a] = bounds[1][a];
+		if (!(bounds[1][a] > bounds[0][a])) throw std::runtime_error("dense_grid: empty bounds");
+	}
+	m.g		  = (float) j.value("g", 0.0);
+	m.scale	  = (float) j.value("scale", 1.0);
 	m.density = grid->data();
 	s.densities.push_back(std::move(grid));
 	s.media.push_back(m);
@@ -1005,6 +1057,7 @@ struct Importer {
 		else if (t == "heterogeneous" && p.contains("file"))
 			throw std::runtime_error("heterogeneous medium \"" + p.value("file", "") + "\": OpenVDB volumes are not read by this build");
 		else if (t == "heterogeneous" || t == "value_noise_grid") valueNoiseMedium(s, p);
+		else if (t == "dense_grid") denseGridMedium(s, p);
 		else throw std::runtime_error("unsupported medium type \"" + t + "\"");
 		const int mi = (int) s.media.size() - 1;
 		if (p.contains("bound")) {

@@ -48,7 +48,7 @@ struct TraceBvh {
 	const AlphaSrc *alphaSrc = nullptr;
 };
 
-// HashFloat(o, d) of src/util/hash.h:130-150: MurmurHash64A (Austin Appleby,
+// HashFloat(o, d) of src/util/hash.h:130-134: MurmurHash64A (Austin Appleby,
 // 64-bit MurmurHash2 for x64; hash.h:37-76) with seed 0 over the 24 bytes of
 // the ray origin and direction, low 32 bits scaled by 2^-32.
 KRR_DEV float hashRayFloat(const f3 &o, const f3 &d) {

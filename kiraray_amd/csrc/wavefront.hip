@@ -62,7 +62,7 @@ __global__ void k_reset_totals(Counters *cnt) {
 	cnt->rayCount[0] = cnt->rayCount[1] = cnt->shadowCount = 0;
 }
 
-// beginFrame (integrator.cpp:310-326): reset L and pixel, seed the sampler,
+// beginFrame (integrator.cpp:205-221): reset L and pixel, seed the sampler,
 // draw the wavelength sample whose main index picks the hero channel.
 __global__ void k_begin_frame(WfParams P, PixelStateDev px) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,7 +80,7 @@ __global__ void k_begin_frame(WfParams P, PixelStateDev px) {
 	px.rng[i]			= s.state;
 }
 
-// generateCameraRays (integrator.cpp:271-284).  Camera rays go to slot =
+// generateCameraRays (integrator.cpp:166-179).  Camera rays go to slot =
 // pixelId, so the first trace sees image-coherent rays.
 __global__ void k_camera(WfParams P, CameraDev cam, PixelStateDev px, RayQueueDev q, Counters *cnt) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,7 +257,7 @@ __global__ void k_trace_any(TraceBvh bvh, const float4 *__restrict__ rays, int *
 template <bool COUNT>
 __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, int *ovf, Counters *cnt);
 
-// sampleT_maj (wavefront.h:163-207) on the medium of the ray.  callback(p,
+// sampleT_maj (wavefront.h:34-78) on the medium of the ray.  callback(p,
 // mp, sigma_maj, T_maj) returns false to stop; the result is the remaining
 // majorant transmittance (or 1 when stopped by the callback).
 template <class F>
@@ -300,9 +300,9 @@ KRR_DEV f3 sampleTmaj(const DevScene &sc, int medium, f3 o, f3 d, float tMax, Pc
 // The fused shading stage for the rays of the current queue.  Per ray, in the
 // reference's launch order: closest-hit / miss routing (device.cu:43-74),
 // sampleMediumInteraction (medium.cpp:13-103), handleHit
-// (integrator.cpp:183-195), handleMiss (:197-213), the depth == maxDepth
+// (integrator.cpp:78-90), handleMiss (:197-213), the depth == maxDepth
 // break, sampleMediumScattering (medium.cpp:105-153) and
-// generateScatterRays (integrator.cpp:215-269).
+// generateScatterRays (integrator.cpp:110-164).
 struct ShadowOut {
 	f3 o, d, Ld, pu, pl;
 	int medium;
@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(256) k_shade(WfParams P, DevScene sc, RayQueue
 }
 
 // traceShadow with media: ShadowTr raygen + traceTransmittance
-// (device.cu:102-128, wavefront.h:209-268).  Ratio tracking through media,
+// (device.cu:102-128, wavefront.h:80-139).  Ratio tracking through media,
 // stepping across null-material surfaces.
 constexpr int kMaxNullCrossings = 1024;
 
@@ -757,7 +757,7 @@ KRR_DEV void occlusionItem(const TraceBvh &bvh, const f3 &o, const f3 &d, float 
 }
 
 // traceShadow with media: ShadowTr raygen + traceTransmittance
-// (device.cu:102-128, wavefront.h:209-268).
+// (device.cu:102-128, wavefront.h:80-139).
 template <bool COUNT>
 __global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px, int *ovf, Counters *cnt) {
 	extern __shared__ int ldsStack[];
@@ -976,7 +976,7 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 	}
 	if (NRC) nrc.keepProb = *nrc.keep;
 	// camFetch: the camera rays of this sample are generated here, at fetch
-	// (generateCameraRays, integrator.cpp:271-284), instead of read from q
+	// (generateCameraRays, integrator.cpp:166-179), instead of read from q
 	const int count = (VIEWS || camFetch) ? P.N : *countPtr;
 	uint32_t nv = 0, nt = 0, snv = 0, snt = 0, nClosest = 0, nShadow = 0;
 	bool active = false, exhausted = false, trainPath = false;
@@ -1073,7 +1073,7 @@ __global__ void k_nrc_mark_done(Counters *cnt, int capTrain, int capQuery) {
 	cnt->nrcQueriesTotal += (unsigned long long) min(cnt->nrcQueries, capQuery);
 }
 
-// pixel += L (integrator.cpp:362-365).
+// pixel += L (integrator.cpp:257-260).
 __global__ void k_accumulate(int N, PixelStateDev px) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= N) return;
@@ -1085,7 +1085,7 @@ __global__ void k_accumulate(int N, PixelStateDev px) {
 	px.pixel[i]	   = p;
 }
 
-// Film write (integrator.cpp:367-371, CudaRenderTarget::write,
+// Film write (integrator.cpp:262-266, CudaRenderTarget::write,
 // src/core/device/cuda.h:33-45): RGBA(pixel / spp, 1) at row H-1-y.
 __global__ void k_film(WfParams P, PixelStateDev px, float4 *film) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1557,6 +1557,22 @@ void buildMajorant(const krr_medium &m, const float invVoxel[3], std::vector<flo
 			}
 }
 
+// Host-side checks of a scene description that need no device: array
+// pointers against their counts and every mesh's triangle range, before any
+// per-triangle table is written.
+int validateSceneDesc(const krr_scene_desc *s) {
+	if (s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
+		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing triangle arrays");
+	if (s->n_vertices < 0 || (s->n_vertices > 0 && !s->positions)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing positions");
+	if (s->n_meshes < 0 || (s->n_meshes > 0 && !s->meshes)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing meshes");
+	for (int mi = 0; mi < s->n_meshes; mi++) {
+		const krr_mesh &m = s->meshes[mi];
+		if (m.first_triangle < 0 || m.n_triangles < 0 || (int64_t) m.first_triangle + m.n_triangles > s->n_triangles)
+			return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d triangle range out of bounds", mi);
+	}
+	return KRR_OK;
+}
+
 int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	if (s->n_vertices < 0 || (s->n_vertices > 0 && !s->positions)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing positions");
 	if (s->n_meshes < 0 || (s->n_meshes > 0 && !s->meshes)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing meshes");
@@ -1564,6 +1580,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	if (s->n_media < 0 || (s->n_media > 0 && !s->media)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing media");
 	if (s->n_infinite_lights < 0 || (s->n_infinite_lights > 0 && !s->infinite_lights))
 		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing infinite lights");
+	if (int rc = validateSceneDesc(s)) return rc;
 	// triangle -> mesh, mesh table with light bases (mesh lights in mesh order,
 	// then infinite lights: RTScene::uploadSceneLightData, device/scene.cpp:80-135)
 	std::vector<int> triMesh((size_t) s->n_triangles, -1);
@@ -1762,7 +1779,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	return KRR_OK;
 }
 
-// Kernel parameters of one frame (integrator.cpp:310-372 values).
+// Kernel parameters of one frame (integrator.cpp:205-267 values).
 WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex) {
 	WfParams P;
 	P.W			   = W;
@@ -1885,6 +1902,7 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 		if (stackLdsEntries < 0 || stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 		if (!(nodeCost >= 0.f && nodeCost <= 64.f)) return krr::fail(KRR_ERR_BAD_CONFIG, "sah_node_cost must be in [0, 64]");
 	}
+	if (int rc = validateSceneDesc(scene)) return rc;
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_accel_build: no HIP device");
 	for (int m = 0; m < scene->n_meshes; m++)
 		if (scene->meshes[m].material >= scene->n_materials) return krr::fail(KRR_ERR_INVALID_ARG, "mesh %d material out of range", m);
@@ -2016,6 +2034,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 int krr_wf_set_scene(krr_wf *wf, const krr_scene_desc *scene) {
 	krr::clearError();
 	if (!wf || !scene) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_set_scene: null argument");
+	if (int rc = validateSceneDesc(scene)) return rc;
 	if (!deviceAvailable()) return krr::fail(KRR_ERR_NO_DEVICE, "krr_wf_set_scene: no HIP device");
 	KRR_HIP(hipDeviceSynchronize());
 	wf->haveScene = false;
@@ -2043,7 +2062,7 @@ int krr_wf_resize(krr_wf *wf, int32_t width, int32_t height) {
 int krr_wf_begin_frame(krr_wf *wf, const krr_camera *camera, uint64_t frame_index) {
 	krr::clearError();
 	if (!wf || !camera) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_begin_frame: null argument");
-	if (!wf->haveScene || wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:311 early return
+	if (!wf->haveScene || wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:206 early return
 	if (camera->medium >= wf->ds.nMedia) return krr::fail(KRR_ERR_INVALID_ARG, "camera medium out of range");
 	wf->cam			 = *camera;
 	wf->frameIndex	 = frame_index;
@@ -2056,7 +2075,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	krr::clearError();
 	if (!wf) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: null handle");
 	if (!wf->haveScene) return krr::fail(KRR_ERR_NO_SCENE, "krr_wf_render: no scene set");
-	if (wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:329 early return
+	if (wf->W == 0 || wf->H == 0) return KRR_OK; // integrator.cpp:224 early return
 	if (!film_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_wf_render: null film");
 	hipStream_t st = (hipStream_t) hip_stream;
 	wf->lastStream = st;
@@ -2603,12 +2622,13 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 	const int tbs[]		= {64, 128, 256};
 	const int ldss[]	= {8, 16, 24};
 	const float occs[]	= {1.f, 0.75f, 0.5f};
-	for (int tb : tbs)
-		for (int l : ldss)
-			for (float o : occs) {
-				Trial t{tb, l, o, 0.f};
-				if ((rc = measure(t, 3))) break;
-				trials.push_back(t);
+	// the first failing candidate ends the search (its error is returned)
+	for (int ti = 0; ti < 3 && rc == KRR_OK; ti++)
+		for (int li = 0; li < 3 && rc == KRR_OK; li++)
+			for (int oi = 0; oi < 3 && rc == KRR_OK; oi++) {
+				Trial t{tbs[ti], ldss[li], occs[oi], 0.f};
+				rc = measure(t, 3);
+				if (rc == KRR_OK) trials.push_back(t);
 			}
 	wf->countTraversal = saveCount;
 	if (rc) {
@@ -2830,7 +2850,7 @@ int krr_mk_get_params(const krr_mk *mk, char *buf, int32_t cap) {
 int krr_mk_set_scene(krr_mk *mk, const krr_scene_desc *scene) {
 	krr::clearError();
 	if (!mk || !scene) return krr::fail(KRR_ERR_INVALID_ARG, "krr_mk_set_scene: null argument");
-	if (scene->n_meshes > 0 && !scene->meshes) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing meshes");
+	if (int rc = validateSceneDesc(scene)) return rc;
 	for (int i = 0; i < scene->n_meshes; i++)
 		if (scene->meshes[i].material < 0)
 			return krr::fail(KRR_ERR_UNSUPPORTED, "krr_mk_set_scene: mesh %d has no material (the megakernel path tracer renders surfaces only)", i);

@@ -67,6 +67,11 @@ _PROTO = {
     "oracle_dm_tan": (C.c_float, [C.c_float]),
     "oracle_dm_atan2": (C.c_float, [C.c_float, C.c_float]),
     "oracle_error_metric": (C.c_double, [C.c_int32, C.c_void_p, C.c_void_p, C.c_int64]),
+    "oracle_majorant_segments": (C.c_int32, [C.POINTER(krr_medium), C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
+                                             C.c_int32, C.c_void_p]),
+    "oracle_transmittance": (C.c_int, [C.POINTER(krr_scene_desc), C.c_void_p, C.c_void_p, C.c_float, C.c_int32,
+                                       C.c_int32, C.c_float, C.c_float, C.c_uint64, C.c_int64, C.c_int32, C.c_void_p,
+                                       C.c_void_p]),
 }
 
 _lib = None
@@ -78,27 +83,39 @@ def build():
     return LIB_PATH
 
 
+def _bind(path):
+    lib = C.CDLL(path)
+    for n, (res, args) in _PROTO.items():
+        f = getattr(lib, n)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
 def load():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        lib = C.CDLL(LIB_PATH)
-        for n, (res, args) in _PROTO.items():
-            f = getattr(lib, n)
-            f.restype = res
-            f.argtypes = args
-        _lib = lib
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+def load_planted(k):
+    """The checker built with planted mistake k (ORACLE_PLANT in core.h):
+    a separate library, for tests that show a pin fails on that mistake."""
+    path = os.path.join(HERE, "build", f"liboracle_plant{int(k)}.so")
+    subprocess.run(["make", "-s", "-C", HERE, "plant", f"PLANT={int(k)}"], check=True)
+    return _bind(path)
 
 
 def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def params(**kw):
+def params(lib=None, **kw):
     p = oracle_params()
-    load().oracle_default_params(C.byref(p))
+    (lib or load()).oracle_default_params(C.byref(p))
     for k, v in kw.items():
         if not hasattr(p, k):
             raise KeyError(k)
@@ -106,13 +123,14 @@ def params(**kw):
     return p
 
 
-def render(desc, cam, width, height, frame_index=1, film=None, **kw):
-    """One frame (beginFrame + render) -> (film[h, w, 4] float32, stats dict)."""
-    lib = load()
+def render(desc, cam, width, height, frame_index=1, film=None, lib=None, **kw):
+    """One frame (beginFrame + render) -> (film[h, w, 4] float32, stats dict).
+    lib: another build of the checker (load_planted), default the checker."""
+    lib = lib or load()
     if film is None:
         film = np.zeros((height, width, 4), np.float32)
     st = oracle_stats()
-    p = params(**kw)
+    p = params(lib=lib, **kw)
     rc = lib.oracle_render(C.byref(desc), C.byref(cam), C.byref(p), frame_index, width, height, _ptr(film), C.byref(st))
     if rc != 0:
         raise RuntimeError(f"oracle_render failed: {rc}")
@@ -217,3 +235,28 @@ def error_metric(metric, img, ref):
     img = np.ascontiguousarray(img, np.float32)
     ref = np.ascontiguousarray(ref, np.float32)
     return load().oracle_error_metric(metric, _ptr(img), _ptr(ref), img.size // 4)
+
+
+def majorant_segments(medium, o, d, tmax, precision=64, cap=4096, lib=None):
+    """sampleRay + MajorantIterator walk of medium (a krr_medium) along
+    o + t d, t in [0, tmax] -> array (n, 5): tMin, tMax, sigma_maj rgb."""
+    seg = np.zeros((cap, 5), np.float64)
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    n = (lib or load()).oracle_majorant_segments(C.byref(medium), _ptr(o), _ptr(d), float(tmax), precision, cap, _ptr(seg))
+    assert 0 <= n <= cap, n
+    return seg[:n]
+
+
+def transmittance(desc, o, d, tmax, medium, channel, n, seed=1, sr_pu=0.0, sr_pl=1.0, precision=64, lib=None):
+    """Mean and standard error (per channel) of traceTransmittance's
+    contribution for the shadow ray {o, d, tmax, medium}, Ld = 1, over n runs."""
+    mean = np.zeros(3, np.float64)
+    sq = np.zeros(3, np.float64)
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    rc = (lib or load()).oracle_transmittance(C.byref(desc), _ptr(o), _ptr(d), float(tmax), medium, channel, sr_pu, sr_pl,
+                                              seed, n, precision, _ptr(mean), _ptr(sq))
+    assert rc == 0, rc
+    se = np.sqrt(np.maximum(sq - mean ** 2, 0.0) / n)
+    return mean, se

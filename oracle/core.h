@@ -27,10 +27,18 @@
 
 #include "detmath.h"
 
+// Planted mistakes for the pin tests: built with -DORACLE_PLANT=k, the
+// checker carries one deliberate error (a dropped term, a wrong index or a
+// wrong constant) in the medium code, and the pin that covers that code must
+// then fail.  0, the default, is the restatement itself.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
+
 namespace oracle {
 
 // MurmurHash64A (Austin Appleby's 64-bit MurmurHash2 for x64), the hash of
-// src/util/hash.h:37-76 behind HashFloat (hash.h:130-150): 8-byte blocks read
+// src/util/hash.h:37-76 behind HashFloat (hash.h:130-134): 8-byte blocks read
 // little-endian, then the 1..7 tail bytes, then the final avalanche.
 inline uint64_t murmurHash64A(const unsigned char *key, size_t len, uint64_t seed) {
 	const uint64_t m = 0xc6a4a7935bd1e995ull;
@@ -316,7 +324,7 @@ template <class R> inline V3<R> ToSameHemisphere(const V3<R> &w, const V3<R> &wp
 	return {w.x, w.y, w.z * wp.z > 0 ? w.z : -w.z};
 }
 
-// Refract with absolute eta (matutils.h:333-347).
+// Refract with absolute eta (matutils.h:85-99).
 template <class R> bool Refract(const V3<R> &wi, const V3<R> &n, R eta, R *etap, V3<R> *wt) {
 	R cosThetaI = dot(n, wi);
 	if (wi.z < 0) eta = R(1) / eta;

@@ -6,7 +6,7 @@
 // the product never does.
 //
 // One pixel's path runs through the reference's per-depth stage sequence
-// (src/render/wavefront/integrator.cpp:328-372).  Each stage sees at most one
+// (src/render/wavefront/integrator.cpp:223-267).  Each stage sees at most one
 // work item of the pixel, as in the queues, so a sequential per-pixel loop
 // reproduces the wavefront's RNG consumption and radiance accumulation order.
 #include "stages.h"
@@ -64,7 +64,7 @@ template <class R> struct Integrator {
 		return r;
 	}
 
-	// handleHit (integrator.cpp:183-195).
+	// handleHit (integrator.cpp:78-90).
 	void handleHit(const HitLightItem<R> &w, V3<R> &L) const {
 		const LightRec &Lr = sv.lights[w.light];
 		V3<R> Le		   = areaL<R>(Lr, w.n, w.wo, w.u, w.v) * w.thp;
@@ -75,7 +75,7 @@ template <class R> struct Integrator {
 		L += Le;
 	}
 
-	// handleMiss (integrator.cpp:197-213).
+	// handleMiss (integrator.cpp:92-108).
 	void handleMiss(const RayItem<R> &w, V3<R> &L) const {
 		V3<R> Lm;
 		for (const LightRec &Lr : sv.lights) {
@@ -89,7 +89,7 @@ template <class R> struct Integrator {
 		L += w.thp * Lm;
 	}
 
-	// One pixel: beginFrame (integrator.cpp:310-326) + render (328-372).
+	// One pixel: beginFrame (integrator.cpp:205-221) + render (223-267).
 	void renderPixel(int px, int py, int W, int H, uint32_t sampleIndex, const krr_camera &cam, V3<R> &pixel) {
 		const int pixelId = py * W + px;
 		Pcg sampler;
@@ -104,7 +104,7 @@ template <class R> struct Integrator {
 		pixel = V3<R>();
 		for (int s = 0; s < P.spp; s++) {
 			if (!P.legacy_spp_accumulate) L = V3<R>();
-			// generateCameraRays (integrator.cpp:271-284): {get2D(), get2D(), get1D()}
+			// generateCameraRays (integrator.cpp:166-179): {get2D(), get2D(), get1D()}
 			float cs[5];
 			for (int i = 0; i < 5; i++) cs[i] = sampler.get1D();
 			RayItem<R> cur;
@@ -165,7 +165,11 @@ template <class R> struct Integrator {
 							if (w.depth < P.max_depth && mp.Le.any()) {
 								R pr	 = sigma_maj[channel] * Tm[channel];
 								V3<R> pe = pu * sigma_maj * Tm / pr;
+#if ORACLE_PLANT == 4
+								if (pe.any()) Lm += thp * mp.sigma_s * Tm * mp.Le / (pr * pe.mean()); // planted: sigma_s for sigma_a
+#else
 								if (pe.any()) Lm += thp * mp.sigma_a * Tm * mp.Le / (pr * pe.mean());
+#endif
 							}
 							R pAbsorb  = mp.sigma_a[channel] / sigma_maj[channel];
 							R pScatter = mp.sigma_s[channel] / sigma_maj[channel];
@@ -191,7 +195,11 @@ template <class R> struct Integrator {
 								thp *= Tm * sigma_n / pr;
 								if (pr == 0) thp = V3<R>();
 								pu *= Tm * sigma_n / pr;
+#if ORACLE_PLANT == 6
+								pl *= Tm * sigma_n / pr; // planted: light-path ratio takes sigma_n
+#else
 								pl *= Tm * sigma_maj / pr;
+#endif
 								return thp.any() && pu.any();
 							}
 						});
@@ -243,7 +251,11 @@ template <class R> struct Integrator {
 						V3<R> Ld	  = thp * ls.L;
 						if (Ld.any() && lightPdf > 0) {
 							hasShadow = true;
+#if ORACLE_PLANT == 5
+							shadow	  = {shadowRay, R(1), Ld, w.pu * lightPdf, w.pu * lightPdf}; // planted: phase pdf dropped
+#else
 							shadow	  = {shadowRay, R(1), Ld, w.pu * phasePdf, w.pu * lightPdf};
+#endif
 						}
 					}
 					R u0 = R(sampler.get1D()), u1 = R(sampler.get1D());
@@ -263,7 +275,7 @@ template <class R> struct Integrator {
 					}
 				}
 
-				// generateScatterRays (integrator.cpp:215-269).
+				// generateScatterRays (integrator.cpp:110-164).
 				if (hasScatter) {
 					ScatterItem<R> &w = scatter;
 					bool alive		  = true;
@@ -330,7 +342,7 @@ template <class R> struct Integrator {
 		}
 	}
 
-	// traceTransmittance (wavefront.h:209-268) with the ShadowTr closest-hit
+	// traceTransmittance (wavefront.h:80-139) with the ShadowTr closest-hit
 	// program (device.cu:102-128): the hit interaction takes the medium of the
 	// original shadow ray.
 	void traceTransmittance(const ShadowItem<R> &sr, Pcg &sampler, int channel, V3<R> &L) {
@@ -359,13 +371,21 @@ template <class R> struct Integrator {
 					[&](const V3<R> &, const MediumProps<R> &mp, const V3<R> &sigma_maj, const V3<R> &Tm) -> bool {
 						V3<R> sigma_n = vmaxs(sigma_maj - mp.sigma_a - mp.sigma_s, R(0));
 						R pr		  = Tm[channel] * sigma_maj[channel];
+#if ORACLE_PLANT == 1
+						T_ray *= Tm * sigma_maj / pr; // planted: null-collision ratio dropped
+#else
 						T_ray *= Tm * sigma_n / pr;
+#endif
 						pl *= Tm * sigma_maj / pr;
 						pu *= Tm * sigma_n / pr;
 						V3<R> Tr = T_ray / (pu + pl).mean();
 						if (Tr.maxCoeff() < R(0.05f)) {
 							if (sampler.get1D() < 0.75f) T_ray = V3<R>();
+#if ORACLE_PLANT == 3
+							else T_ray /= R(0.75f); // planted: wrong roulette compensation
+#else
 							else T_ray /= R(0.25f);
+#endif
 						}
 						return T_ray.any();
 					});
@@ -773,6 +793,77 @@ int oracle_majorant_grid(const krr_medium *m, float *out) {
 	md.m = m;
 	buildMajorant(md);
 	std::memcpy(out, md.majorant.voxels.data(), sizeof(float) * 64 * 64 * 64);
+	return KRR_OK;
+}
+
+int32_t oracle_majorant_segments(const krr_medium *m, const float o[3], const float d[3], float tmax, int32_t precision,
+								 int32_t cap, double *seg) {
+	if (!m || !o || !d || cap < 0 || (cap > 0 && !seg)) return KRR_ERR_INVALID_ARG;
+	Medium md;
+	md.m = m;
+	if (m->type == KRR_MEDIUM_GRID) buildMajorant(md);
+	auto walk = [&](auto zero) -> int32_t {
+		using R = decltype(zero);
+		Ray<R> ray;
+		ray.o = V3<R>(R(o[0]), R(o[1]), R(o[2]));
+		ray.d = V3<R>(R(d[0]), R(d[1]), R(d[2]));
+		MajorantIter<R> it = sampleRay(md, ray, R(tmax));
+		MajorantSeg<R> s;
+		int32_t n = 0;
+		while (it.next(s)) {
+			if (n < cap) {
+				double *q = seg + 5 * (size_t) n;
+				q[0] = (double) s.tMin, q[1] = (double) s.tMax;
+				for (int c = 0; c < 3; c++) q[2 + c] = (double) s.sigma_maj[c];
+			}
+			n++;
+		}
+		return n;
+	};
+	return precision == 64 ? walk(0.0) : walk(0.f);
+}
+
+int oracle_transmittance(const krr_scene_desc *scene, const float o[3], const float d[3], float tmax, int32_t medium,
+						 int32_t channel, float sr_pu, float sr_pl, uint64_t seed, int64_t n, int32_t precision, double out[3],
+						 double out_sq[3]) {
+	if (!scene || !o || !d || !out || n <= 0 || channel < 0 || channel > 2 || medium < -1 || medium >= scene->n_media)
+		return KRR_ERR_INVALID_ARG;
+	SceneView sv;
+	buildSceneView(scene, sv);
+	sceneRadius(scene, sv);
+	Tracer tr;
+	tr.sv = &sv;
+	oracle_params p;
+	oracle_default_params(&p);
+	auto run = [&](auto zero) {
+		using R = decltype(zero);
+		Integrator<R> integ(sv, tr, p);
+		ShadowItem<R> sr;
+		sr.ray.o	  = V3<R>(R(o[0]), R(o[1]), R(o[2]));
+		sr.ray.d	  = V3<R>(R(d[0]), R(d[1]), R(d[2]));
+		sr.ray.medium = medium;
+		sr.tMax		  = R(tmax);
+		sr.Ld		  = V3<R>(R(1));
+		sr.pu		  = V3<R>(R(sr_pu));
+		sr.pl		  = V3<R>(R(sr_pl));
+		Pcg sampler;
+		sampler.setSeed(seed, 1);
+		double sum[3] = {0, 0, 0}, sq[3] = {0, 0, 0};
+		for (int64_t i = 0; i < n; i++) {
+			V3<R> L;
+			integ.traceTransmittance(sr, sampler, channel, L);
+			for (int c = 0; c < 3; c++) {
+				sum[c] += (double) L[c];
+				sq[c] += (double) L[c] * (double) L[c];
+			}
+		}
+		for (int c = 0; c < 3; c++) {
+			out[c] = sum[c] / (double) n;
+			if (out_sq) out_sq[c] = sq[c] / (double) n;
+		}
+	};
+	if (precision == 64) run(0.0);
+	else run(0.f);
 	return KRR_OK;
 }
 

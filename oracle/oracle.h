@@ -6,8 +6,8 @@
  * baseline may load it.  The product never links or calls it.
  *
  * It restates, stage by stage and with the same RNG consumption order, the
- * reference integrator (src/render/wavefront/integrator.cpp:183-372,
- * device.cu:43-128, medium.cpp:13-153, wavefront.h:163-268) and the shading
+ * reference integrator (src/render/wavefront/integrator.cpp:78-267,
+ * device.cu:43-128, medium.cpp:13-153, wavefront.h:34-139) and the shading
  * code it calls, in FP32 (parity mode: decisions identical to the GPU) or
  * FP64 (reference-image mode).  Every pixel runs its path sequentially; the
  * stage order inside one loop depth is the reference's launch order, which is
@@ -116,6 +116,18 @@ int32_t oracle_sample_discrete3(const float w[3], float u, float *pmf);
 /* Majorant grid (64^3 cells) and trilinear density of medium m. */
 int oracle_majorant_grid(const krr_medium *m, float *out /* 64^3 */);
 float oracle_density(const krr_medium *m, const float p[3]);
+/* Majorant segments of the ray o + t d, t in [0, tmax] (sampleRay +
+ * MajorantIterator::next, media.h:41-106): returns their number and writes
+ * at most cap of them as {tMin, tMax, sigma_maj r, g, b} into seg. */
+int32_t oracle_majorant_segments(const krr_medium *m, const float o[3], const float d[3], float tmax, int32_t precision,
+								 int32_t cap, double *seg);
+/* traceTransmittance (wavefront.h:80-139) of one shadow ray {o, d, tmax,
+ * medium} with Ld = 1 and MIS ratios (sr_pu, sr_pl), run n times from one
+ * PCG stream (seed): the mean (and mean square) of the radiance it adds,
+ * per channel, for the given hero channel. */
+int oracle_transmittance(const krr_scene_desc *scene, const float o[3], const float d[3], float tmax, int32_t medium,
+						 int32_t channel, float sr_pu, float sr_pl, uint64_t seed, int64_t n, int32_t precision, double out[3],
+						 double out_sq[3]);
 
 /* Deterministic elementary functions used on the shading path (checked
  * against libm double in the tests). */

@@ -5,12 +5,12 @@
 // the product never does.
 //
 // One pixel's path runs through the reference's per-depth stage sequence
-// (src/render/wavefront/integrator.cpp:328-372): closest trace with the
+// (src/render/wavefront/integrator.cpp:223-267): closest trace with the
 // closest-hit/miss routing of device.cu:43-81, medium sampling
-// (medium.cpp:13-103), handleHit (integrator.cpp:183-195), handleMiss
+// (medium.cpp:13-103), handleHit (integrator.cpp:78-90), handleMiss
 // (:197-213), the maxDepth break, medium scattering (medium.cpp:105-153),
-// generateScatterRays (integrator.cpp:215-269) and the shadow trace
-// (device.cu:83-128, wavefront.h:209-268).  Each stage sees at most one work
+// generateScatterRays (integrator.cpp:110-164) and the shadow trace
+// (device.cu:83-128, wavefront.h:80-139).  Each stage sees at most one work
 // item of the pixel, exactly as in the queues.
 #pragma once
 #include "core.h"
@@ -163,7 +163,7 @@ struct Tracer {
 };
 
 // ----------------------------------------------------------------------------
-// Interactions (src/core/raytracing.h:106-180, src/render/shared.h:303-324).
+// Interactions (src/core/raytracing.h:106-180, src/render/shared.h:76-97).
 // ----------------------------------------------------------------------------
 template <class R> V3<R> offsetRayOrigin(const V3<R> &p, const V3<R> &n, const V3<R> &w) {
 	V3<R> offset = n * R(kRayEps);
@@ -349,7 +349,7 @@ template <class R> R areaPdfLi(const SceneView &sv, const LightRec &L, const V3<
 
 // ----------------------------------------------------------------------------
 // Media (src/render/media.h:41-209, src/render/media.cpp:83-111,
-// src/render/wavefront/wavefront.h:163-207).
+// src/render/wavefront/wavefront.h:34-78).
 // ----------------------------------------------------------------------------
 template <class R> struct MediumProps {
 	V3<R> sigma_a, sigma_s, Le;
@@ -416,7 +416,11 @@ template <class R> struct MajorantIter {
 		if (nextCrossingT[stepAxis] > tMax) tMin = tMax;
 		voxel[stepAxis] += step[stepAxis];
 		if (voxel[stepAxis] == voxelLimit[stepAxis]) tMin = tMax;
+#if ORACLE_PLANT == 2
+		nextCrossingT[stepAxis] += deltaT[(stepAxis + 1) % 3]; // planted: wrong axis
+#else
 		nextCrossingT[stepAxis] += deltaT[stepAxis];
+#endif
 		return true;
 	}
 };
@@ -518,7 +522,7 @@ template <class R> V3<R> hgSample(float gv, const V3<R> &wo, R u0, R u1, R *pdf)
 	return wi;
 }
 
-// sampleT_maj (wavefront.h:163-207).  callback(p, mp, sigma_maj, T_maj) -> bool.
+// sampleT_maj (wavefront.h:34-78).  callback(p, mp, sigma_maj, T_maj) -> bool.
 template <class R, class F> V3<R> sampleT_maj(const SceneView &sv, Ray<R> ray, R tMax, Pcg &sampler, int channel, F callback) {
 	tMax *= norm(ray.d);
 	ray.d			 = normalize(ray.d);

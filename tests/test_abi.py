@@ -181,3 +181,30 @@ def test_megakernel_rejects_medium_interfaces(krr):
     with pytest.raises(krr.KrrError) as e:
         mk.set_scene(s)
     assert e.value.code == -7  # KRR_ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("first,count", [(-1, 2), (34, 3), (0, -1), (36, 1)])
+def test_bad_mesh_range_is_rejected_before_upload(krr, first, count):
+    """A mesh whose triangle range leaves the scene's triangle array is an
+    invalid argument for every entry point that takes a scene, reported
+    before any per-triangle table is written (and before the device check,
+    so it holds without a GPU too)."""
+    s = krr.Scene(CORNELL)
+    d = s.desc
+    meshes = (krr._abi.krr_mesh * d.n_meshes)()
+    for i in range(d.n_meshes):
+        meshes[i] = d.meshes[i]
+    meshes[0].first_triangle, meshes[0].n_triangles = first, count
+    bad = krr._abi.krr_scene_desc()
+    C.pointer(bad)[0] = d
+    bad.meshes = C.cast(meshes, C.POINTER(krr._abi.krr_mesh))
+    lib = krr.load_library()
+    wf = krr.WavefrontPathTracer(None)
+    assert lib.krr_wf_set_scene(wf._h, C.byref(bad)) == krr._abi.KRR_ERR_INVALID_ARG
+    assert b"triangle range" in lib.krr_last_error()
+    wf.close()
+    mk = krr.MegakernelPathTracer(None)
+    assert lib.krr_mk_set_scene(mk._h, C.byref(bad)) == krr._abi.KRR_ERR_INVALID_ARG
+    mk.close()
+    h = C.c_void_p()
+    assert lib.krr_accel_build(C.byref(bad), None, C.byref(h)) == krr._abi.KRR_ERR_INVALID_ARG

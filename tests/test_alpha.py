@@ -2,7 +2,7 @@
 checker: a material with a transmission texture or constant T (pbrt.cpp:
 145-156 stores 1 - opacity there) ignores a hit when alpha = 1 -
 luminance(T) is <= 0, or is < 1 and HashFloat(origin, direction) > alpha
-(hash.h:130-150, MurmurHash64A over the 24 bytes of the ray).
+(hash.h:130-134, MurmurHash64A over the 24 bytes of the ray).
 
 Pins:
 * MurmurHash64A reproduces SMHasher's published verification code for it
