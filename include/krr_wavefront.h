@@ -250,6 +250,19 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
  * overflow area: they must not overlap (same stream, or synchronized). */
 int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev,
 							void *hip_stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): krr_accel_trace_closest with the same arguments and the
+ * same hits, but no hipMalloc, hipFree or allocating library call.  Its work
+ * counter and traversal-stack overflow live in a fixed device array of
+ * krr_accel_noalloc_scratch_bytes() (16 MiB) that each GPU holds until the
+ * process ends; the resident grid is sized to fit it.  KRR_ERR_CAPACITY when
+ * one 128-thread block's stack overflow does not fit (raise
+ * stack_lds_entries).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (any accel, any stream) race. */
+int krr_accel_trace_closest_noalloc(krr_accel *accel, const float *rays_dev, int64_t n,
+									krr_hit *hits_dev, void *hip_stream);
+size_t krr_accel_noalloc_scratch_bytes(void);
 /* visible_dev[i] = 1 if no triangle of a non-null material is hit in (0, tmax). */
 int krr_accel_trace_any(krr_accel *accel, const float *rays_dev, int64_t n, int32_t *visible_dev,
 						void *hip_stream);

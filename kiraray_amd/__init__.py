@@ -110,6 +110,12 @@ class Accel:
         _check(self._lib.krr_accel_trace_closest(self._h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
                                                  C.c_void_p(stream)))
 
+    def trace_closest_noalloc(self, rays_ptr, n, hits_ptr, stream=0):
+        """trace_closest without device allocations (fixed per-GPU scratch;
+        calls that overlap on one GPU race)."""
+        _check(self._lib.krr_accel_trace_closest_noalloc(self._h, C.c_void_p(rays_ptr), n, C.c_void_p(hits_ptr),
+                                                         C.c_void_p(stream)))
+
     def trace_any(self, rays_ptr, n, visible_ptr, stream=0):
         _check(self._lib.krr_accel_trace_any(self._h, C.c_void_p(rays_ptr), n, C.c_void_p(visible_ptr),
                                              C.c_void_p(stream)))

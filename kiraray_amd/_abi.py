@@ -106,6 +106,8 @@ PROTOTYPES = {
     "krr_accel_build": (C.c_int, [C.POINTER(krr_scene_desc), C.c_char_p, C.POINTER(C.c_void_p)]),
     "krr_accel_trace_closest": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "krr_accel_trace_any": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "krr_accel_trace_closest_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "krr_accel_noalloc_scratch_bytes": (C.c_size_t, []),
     "krr_accel_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "krr_accel_destroy": (C.c_int, [C.c_void_p]),

@@ -1939,6 +1939,80 @@ int krr_accel_trace_closest(krr_accel *accel, const float *rays_dev, int64_t n, 
 	return KRR_OK;
 }
 
+} // extern "C"
+
+namespace {
+
+// Device memory of krr_accel_trace_closest_noalloc: a Counters record whose
+// work[0] is the persistent grid's ray counter, then the traversal-stack
+// overflow of the grid.  Every GPU the library runs on holds its own copy
+// until the process ends.  In host code `scratch` names a host-side stand-in:
+// only the address hipGetSymbolAddress returns for the current device may
+// reach a kernel or a copy.
+constexpr size_t kScratchBytes		  = size_t(16) << 20;
+constexpr size_t kScratchCounterBytes = (sizeof(Counters) + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratchMu;
+void *g_scratchDev[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratchAddress(void **out) {
+	int dev = 0;
+	KRR_HIP(hipGetDevice(&dev));
+	if (dev < 0 || dev >= kMaxScratchDevices) return krr::fail(KRR_ERR_CAPACITY, "scratch: device %d beyond %d devices", dev, kMaxScratchDevices);
+	std::lock_guard<std::mutex> lock(g_scratchMu);
+	if (!g_scratchDev[dev]) {
+		void *p		 = nullptr;
+		hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !p) return krr::fail(KRR_ERR_HIP, "scratch: hipGetSymbolAddress: %s", hipGetErrorString(e));
+		g_scratchDev[dev] = p;
+	}
+	*out = g_scratchDev[dev];
+	return KRR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+size_t krr_accel_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int krr_accel_trace_closest_noalloc(krr_accel *accel, const float *rays_dev, int64_t n, krr_hit *hits_dev, void *hip_stream) {
+	krr::clearError();
+	if (!accel || n < 0 || (n > 0 && (!rays_dev || !hits_dev)))
+		return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_closest_noalloc: bad argument");
+	if (n > (1 << 30)) return krr::fail(KRR_ERR_CAPACITY, "krr_accel_trace_closest_noalloc: at most 2^30 rays per call");
+	if (n == 0) return KRR_OK;
+	const int tb	 = kAccelBlock;
+	const size_t lds = stackLds(accel->a, tb);
+	// The resident grid is capped by what the scratch holds: each thread needs
+	// its stack entries beyond the LDS part.
+	const size_t perBlock = stackOverflowInts(accel->a) * tb * sizeof(int);
+	const size_t room	  = kScratchBytes - kScratchCounterBytes;
+	if (perBlock > room)
+		return krr::fail(KRR_ERR_CAPACITY,
+						 "krr_accel_trace_closest_noalloc: a %d-thread block needs %zu B of stack overflow, the scratch holds %zu B "
+						 "(raise stack_lds_entries or use krr_accel_trace_closest)",
+						 tb, perBlock, room);
+	void *base = nullptr;
+	if (int rc = scratchAddress(&base)) return rc;
+	if (!accel->gClosest) accel->gClosest = persistentGrid(k_trace_closest<false>, tb, lds, 1 << 20);
+	int grid = std::min(accel->gClosest, (int) ((n + tb - 1) / tb));
+	if (perBlock > 0) grid = std::min(grid, (int) (room / perBlock));
+	Counters *cnt  = reinterpret_cast<Counters *>(base);
+	int *ovf	   = reinterpret_cast<int *>(reinterpret_cast<unsigned char *>(base) + kScratchCounterBytes);
+	hipStream_t st = (hipStream_t) hip_stream;
+	// the ray counter starts at zero on every call (a kernel, not a memset)
+	hipLaunchKernelGGL(k_accel_begin, dim3(1), dim3(1), 0, st, cnt);
+	hipLaunchKernelGGL(k_trace_closest<false>, dim3(grid), dim3(tb), lds, st, accel->a.dev(), (const float4 *) rays_dev, (float4 *) hits_dev,
+					   (int) n, ovf, cnt);
+	KRR_HIP(hipGetLastError());
+	return KRR_OK;
+}
+
 int krr_accel_trace_any(krr_accel *accel, const float *rays_dev, int64_t n, int32_t *visible_dev, void *hip_stream) {
 	krr::clearError();
 	if (!accel || n < 0 || (n > 0 && (!rays_dev || !visible_dev))) return krr::fail(KRR_ERR_INVALID_ARG, "krr_accel_trace_any: bad argument");

@@ -88,6 +88,12 @@ class DeviceBuffer:
         check(runtime().hipMemcpy(b._p, a.ctypes.data_as(C.c_void_p), a.nbytes, hipMemcpyHostToDevice))
         return b
 
+    def copy_from_numpy(self, a):
+        """Uploads a into the start of the buffer."""
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(runtime().hipMemcpy(self._p, a.ctypes.data_as(C.c_void_p), a.nbytes, hipMemcpyHostToDevice))
+
     def to_numpy(self, dtype, shape):
         out = np.empty(shape, dtype)
         assert out.nbytes <= self.nbytes

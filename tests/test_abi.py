@@ -208,3 +208,15 @@ def test_bad_mesh_range_is_rejected_before_upload(krr, first, count):
     mk.close()
     h = C.c_void_p()
     assert lib.krr_accel_build(C.byref(bad), None, C.byref(h)) == krr._abi.KRR_ERR_INVALID_ARG
+
+
+def test_noalloc_trace_argument_errors(krr):
+    """The no-allocation trace checks its arguments before touching a device:
+    a null accel or a negative count is KRR_ERR_INVALID_ARG with a message;
+    its scratch array is 16 MiB."""
+    lib = krr.load_library()
+    assert lib.krr_accel_noalloc_scratch_bytes() == 16 << 20
+    rc = lib.krr_accel_trace_closest_noalloc(None, None, 0, None, None)
+    assert rc == -1 and b"noalloc" in lib.krr_last_error()
+    rc = lib.krr_accel_trace_closest_noalloc(None, None, -5, None, None)
+    assert rc == -1
