@@ -203,15 +203,27 @@ KRR_DEV float nextFloatDown(float v) {
 	return __builtin_bit_cast(float, ui);
 }
 
+// (x, y, z) by an axis given as a0 = (axis == 0), a1 = (axis == 1), over
+// values: a conditional over array elements or members would be a load
+// through a selected address.
+KRR_DEV float sel3(float x, float y, float z, bool a0, bool a1) { return a0 ? x : (a1 ? y : z); }
+KRR_DEV int sel3(int x, int y, int z, bool a0, bool a1) { return a0 ? x : (a1 ? y : z); }
+
+// The scan over the three weights is written out with constant indices: a
+// loop indexing the array by its running offset keeps it in scratch memory.
 KRR_DEV int sampleDiscrete3(const float w[3], float u) {
+	const float w0 = w[0], w1 = w[1], w2 = w[2];
 	float sumWeights = 0.f;
-	for (int i = 0; i < 3; i++) sumWeights += w[i];
+	sumWeights += w0;
+	sumWeights += w1;
+	sumWeights += w2;
 	float up = u * sumWeights;
 	if (up == sumWeights) up = nextFloatDown(up);
-	int offset = 0;
-	float sum  = 0.f;
-	while (offset < 2 && sum + w[offset] <= up) sum += w[offset++];
-	return offset;
+	float sum = 0.f;
+	if (!(sum + w0 <= up)) return 0;
+	sum += w0;
+	if (!(sum + w1 <= up)) return 1;
+	return 2;
 }
 
 // getPerpendicular (src/util/math_utils.h:120-130).
@@ -416,26 +428,32 @@ struct BSDF {
 	float etaS; // dielectric eta
 	GGX dist;
 
-	// setup() of DiffuseBrdf / ConductorBsdf / DielectricBsdf.
+	// setup() of DiffuseBrdf / ConductorBsdf / DielectricBsdf.  Every field
+	// is written once, after the branches: field stores that differ per
+	// branch get merged into one store through a selected address, which
+	// keeps the whole object in scratch memory instead of registers.
 	KRR_DEV void setup(const DevMaterial &m) {
-		type = m.type;
-		f3 d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-		if (type == MAT_DIFFUSE) {
-			color = d;
-		} else if (type == MAT_CONDUCTOR) {
-			color		 = vmins(d, 0.9999f);
+		const f3 d(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+		f3 c = d, e, kk;
+		float eS = 1.f, ax = 1.f, ay = 1.f;
+		if (m.type == MAT_CONDUCTOR) {
+			c			 = vmins(d, 0.9999f);
 			float aspect = sqrtf(1.f - m.anisotropic * .9f);
-			float axv	 = maxf(.001f, pow2(m.roughness) / aspect);
-			float ayv	 = maxf(.001f, pow2(m.roughness) * aspect);
-			dist.init(axv, ayv);
-			eta = f3(m.ior);
-			k	= (2.f * vsqrt(color)) / vsqrt(vmaxs(f3(1.f) - color, 0.f));
-		} else if (type == MAT_DIELECTRIC) {
-			color	= d * m.specTrans;
-			etaS	= m.ior;
-			float a = pow2(m.roughness);
-			dist.init(a, a);
+			ax			 = maxf(.001f, pow2(m.roughness) / aspect);
+			ay			 = maxf(.001f, pow2(m.roughness) * aspect);
+			e			 = f3(m.ior);
+			kk			 = (2.f * vsqrt(c)) / vsqrt(vmaxs(f3(1.f) - c, 0.f));
+		} else if (m.type == MAT_DIELECTRIC) {
+			c  = d * m.specTrans;
+			eS = m.ior;
+			ax = ay = pow2(m.roughness);
 		}
+		type  = m.type;
+		color = c;
+		eta	  = e;
+		k	  = kk;
+		etaS  = eS;
+		dist.init(ax, ay);
 	}
 
 	KRR_DEV f3 f(const f3 &wo, const f3 &wi) const {
@@ -628,9 +646,13 @@ struct SurfaceInteraction {
 
 	KRR_DEV f3 toLocal(const f3 &v) const { return {dot(tangent, v), dot(bitangent, v), dot(n, v)}; }
 	KRR_DEV f3 toWorld(const f3 &v) const { return tangent * v.x + bitangent * v.y + n * v.z; }
+	// Values, not a conditional over the members: a conditional over
+	// lvalues is a load through a selected address, which keeps the whole
+	// interaction in scratch memory.
 	KRR_DEV int getMedium(const f3 &w) const {
-		if (interface_) return dot(w, n) > 0.f ? mOut : mIn;
-		return medium;
+		const int in = mIn, out = mOut, own = medium;
+		if (interface_) return dot(w, n) > 0.f ? out : in;
+		return own;
 	}
 };
 
@@ -887,13 +909,24 @@ struct MajorantIter {
 		int bits = ((nextCrossingT[0] < nextCrossingT[1]) << 2) + ((nextCrossingT[0] < nextCrossingT[2]) << 1) +
 				   ((nextCrossingT[1] < nextCrossingT[2]));
 		const int stepAxis = (0x00221212 >> (4 * bits)) & 3; // cmpToAxis = {2,1,2,1,2,2,0,0}
-		float tVoxelExit   = fminf(tMax, nextCrossingT[stepAxis]);
-		seg				   = {tMin, tVoxelExit, sigma_t * grid[voxel[0] + 64 * (voxel[1] + 64 * voxel[2])]};
-		tMin			   = tVoxelExit;
-		if (nextCrossingT[stepAxis] > tMax) tMin = tMax;
-		voxel[stepAxis] += step[stepAxis];
-		if (voxel[stepAxis] == voxelLimit[stepAxis]) tMin = tMax;
-		nextCrossingT[stepAxis] += deltaT[stepAxis];
+		// Per-axis state is read and written through selects on the axis with
+		// constant indices: indexing the arrays by stepAxis would keep the
+		// iterator in scratch memory.
+		const bool a0 = stepAxis == 0, a1 = stepAxis == 1, a2 = stepAxis == 2;
+		const float nct	 = sel3(nextCrossingT[0], nextCrossingT[1], nextCrossingT[2], a0, a1);
+		float tVoxelExit = fminf(tMax, nct);
+		seg				 = {tMin, tVoxelExit, sigma_t * grid[voxel[0] + 64 * (voxel[1] + 64 * voxel[2])]};
+		tMin			 = tVoxelExit;
+		if (nct > tMax) tMin = tMax;
+		const int v = sel3(voxel[0], voxel[1], voxel[2], a0, a1) + sel3(step[0], step[1], step[2], a0, a1);
+		if (a0) voxel[0] = v;
+		if (a1) voxel[1] = v;
+		if (a2) voxel[2] = v;
+		if (v == sel3(voxelLimit[0], voxelLimit[1], voxelLimit[2], a0, a1)) tMin = tMax;
+		const float nctNext = nct + sel3(deltaT[0], deltaT[1], deltaT[2], a0, a1);
+		if (a0) nextCrossingT[0] = nctNext;
+		if (a1) nextCrossingT[1] = nctNext;
+		if (a2) nextCrossingT[2] = nctNext;
 		return true;
 	}
 };

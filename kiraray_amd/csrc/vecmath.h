@@ -21,7 +21,12 @@ struct f3 {
 	KRR_HD f3() : x(0), y(0), z(0) {}
 	KRR_HD f3(float a, float b, float c) : x(a), y(b), z(c) {}
 	KRR_HD explicit f3(float a) : x(a), y(a), z(a) {}
-	KRR_HD float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+	// over values: a conditional over the members is a load through a
+	// selected address, which keeps the vector in scratch memory
+	KRR_HD float operator[](int i) const {
+		const float a = x, b = y, c = z;
+		return i == 0 ? a : (i == 1 ? b : c);
+	}
 	KRR_HD f3 operator+(const f3 &b) const { return {x + b.x, y + b.y, z + b.z}; }
 	KRR_HD f3 operator-(const f3 &b) const { return {x - b.x, y - b.y, z - b.z}; }
 	KRR_HD f3 operator*(const f3 &b) const { return {x * b.x, y * b.y, z * b.z}; }

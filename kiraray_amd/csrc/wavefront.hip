@@ -135,16 +135,40 @@ __global__ void k_begin_views(WfParams P, ViewsDev V, PixelStateDev px) {
 	px.rng[i]			= s.state;
 }
 
-// Camera ray of global pixel i for the current sample (k_camera per view).
-KRR_DEV RayItem viewCameraRay(const WfParams &P, const ViewsDev &V, const PixelStateDev &px, int i) {
+// The pixel state a path updates (PixelStateBuffer's L, sampler and hero
+// channel, workqueue.h:15-26): the queue kernels read it from memory and
+// write it back per stage; the persistent kernel keeps it in registers from
+// the path's first vertex to its last.  The radiance additions are the same
+// in number and order either way.
+struct PathRegs {
+	f3 L;
+	uint64_t rng;
+	int channel;
+};
+
+KRR_DEV PathRegs loadPathRegs(const PixelStateDev &px, int pixel, bool media) {
+	const float4 L = px.L[pixel];
+	return PathRegs{f3(L.x, L.y, L.z), px.rng[pixel], media ? px.channel[pixel] : 0};
+}
+
+KRR_DEV void storePathRegs(const PixelStateDev &px, int pixel, const PathRegs &pr) {
+	px.L[pixel]	  = make_float4(pr.L.x, pr.L.y, pr.L.z, 0.f);
+	px.rng[pixel] = pr.rng;
+}
+
+// Camera ray of global pixel i for the current sample (k_camera per view);
+// the pixel's radiance (zero unless the reference's spp loop keeps it) and
+// sampler state after the camera draws go to pr.
+KRR_DEV RayItem viewCameraRay(const WfParams &P, const ViewsDev &V, const PixelStateDev &px, int i, PathRegs &pr, bool media) {
 	const int v		= findView(V, i);
 	const int local = i - V.start[v];
 	const ViewDev &vw = V.view[v];
-	if (!P.legacySpp) px.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-	Pcg s{px.rng[i], P.inc};
+	pr				  = loadPathRegs(px, i, media);
+	if (!P.legacySpp) pr.L = f3();
+	Pcg s{pr.rng, P.inc};
 	RayItem r = cameraRay(vw.cam, vw.W, vw.H, local % vw.W, local / vw.W, s);
 	r.pixel	  = i;
-	px.rng[i] = s.state;
+	pr.rng	  = s.state;
 	return r;
 }
 
@@ -308,8 +332,9 @@ struct ShadowOut {
 	int medium;
 };
 
+
 // One ray of the queue through the stages above; new work is returned in
-// (wantNext, nx) and (wantShadow, so), radiance and RNG state go to memory.
+// (wantNext, nx) and (wantShadow, so), radiance and RNG state in pr.
 // Spherical angles of a unit vector, mapped to [0,1]: theta / pi, phi / (2 pi) + 1/2.
 KRR_DEV void unitAngles(const f3 &w, float &th, float &ph) {
 	const float z = fminf(1.f, fmaxf(-1.f, w.z));
@@ -354,17 +379,16 @@ template <bool TEX> KRR_DEV void nrcVertex(const DevScene &sc, const SurfaceInte
 // PLAIN: no textures and no image-lit infinite lights in the scene; the
 // texture lookups are compiled out.
 template <bool MEDIA, bool NRC = false, bool PLAIN = false>
-KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, const PixelStateDev &px, int loopDepth,
+KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, const float4 h, PathRegs &pr, int loopDepth,
 					   bool &wantNext, RayItem &nx, bool &wantShadow, ShadowOut &so, const NrcDev *nrc = nullptr, bool trainPath = false,
 					   int *trainSlot = nullptr) {
 	const int pixel = r.pixel;
 	f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
 	int &sMedium = so.medium;
 	const int prim	= __float_as_int(h.y);
-	Pcg rng{px.rng[pixel], P.inc};
-	const int channel = MEDIA ? px.channel[pixel] : 0;
-	const float4 Lv	  = px.L[pixel];
-	f3 L(Lv.x, Lv.y, Lv.z);
+	Pcg rng{pr.rng, P.inc};
+	const int channel = MEDIA ? pr.channel : 0;
+	f3 L			  = pr.L;
 	f3 thp = r.thp, pu = r.pu, pl = r.pl;
 	bool hasHL = false, hasScatter = false, hasMiss = false, hasMS = false, hasMSc = false;
 	SurfaceInteraction intr;
@@ -626,8 +650,8 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 			}
 		}
 	}
-	px.L[pixel]	  = make_float4(L.x, L.y, L.z, 0.f);
-	px.rng[pixel] = rng.state;
+	pr.L   = L;
+	pr.rng = rng.state;
 }
 
 template <bool MEDIA>
@@ -642,7 +666,9 @@ KRR_DEV void shadeItem(const WfParams &P, const DevScene &sc, const RayQueueDev 
 	if (i < count) {
 		const RayItem r = loadRayItem(cur, i);
 		pixel			= r.pixel;
-		shadeCore<MEDIA>(P, sc, r, hits[i], px, loopDepth, wantNext, nx, wantShadow, so);
+		PathRegs pr		= loadPathRegs(px, pixel, MEDIA);
+		shadeCore<MEDIA>(P, sc, r, hits[i], pr, loopDepth, wantNext, nx, wantShadow, so);
+		storePathRegs(px, pixel, pr);
 	}
 	const f3 &sO = so.o, &sD = so.d, &sLd = so.Ld, &sPu = so.pu, &sPl = so.pl;
 	const int sMedium = so.medium;
@@ -684,11 +710,11 @@ constexpr int kMaxNullCrossings = 1024;
 // traceTransmittance of one shadow ray (tMax = 1 to the offset light point):
 // ratio tracking through media, stepping across null-material surfaces.
 template <bool COUNT, bool ALPHA = true>
-KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const TraceBvh &bvh, f3 o, f3 d, const float tMax, const int pixel,
-							   const int origMedium, const f3 &Ld, const f3 &spu, const f3 &spl, const PixelStateDev &px,
-							   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
-	Pcg rng{px.rng[pixel], P.inc};
-	const int channel = px.channel[pixel];
+KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const TraceBvh &bvh, f3 o, f3 d, const float tMax,
+							   const int origMedium, const f3 &Ld, const f3 &spu, const f3 &spl, PathRegs &pr, const LaneStack &stack,
+							   uint32_t &nv, uint32_t &nt) {
+	Pcg rng{pr.rng, P.inc};
+	const int channel = pr.channel;
 	int medium		  = origMedium;
 	const f3 pLight	  = o + d * tMax;
 	f3 T_ray(1.f), pu(1.f), pl(1.f);
@@ -732,28 +758,17 @@ KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const Trac
 		medium		 = intr.getMedium(d);
 	}
 	if (T_ray.any()) {
-		const f3 c	= Ld * T_ray / (spu * pu + spl * pl).mean();
-		float4 L	= px.L[pixel];
-		L.x			= L.x + c.x;
-		L.y			= L.y + c.y;
-		L.z			= L.z + c.z;
-		px.L[pixel] = L;
+		const f3 c = Ld * T_ray / (spu * pu + spl * pl).mean();
+		pr.L	   = pr.L + c;
 	}
-	px.rng[pixel] = rng.state;
+	pr.rng = rng.state;
 }
 
-// Non-media shadow test of one shadow ray: add Ld / mean(pu + pl) if visible.
+// Non-media shadow test of one shadow ray: true when nothing occludes it
+// (the caller then adds Ld / mean(pu + pl)).
 template <bool COUNT, bool ALPHA = true>
-KRR_DEV void occlusionItem(const TraceBvh &bvh, const f3 &o, const f3 &d, float tMax, int pixel, const f3 &c, const PixelStateDev &px,
-						   const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
-	const TraceResult r = traverse<true, COUNT, ALPHA>(bvh, o, d, tMax, stack, nv, nt);
-	if (r.prim < 0) {
-		float4 L	= px.L[pixel];
-		L.x			= L.x + c.x;
-		L.y			= L.y + c.y;
-		L.z			= L.z + c.z;
-		px.L[pixel] = L;
-	}
+KRR_DEV bool occluded(const TraceBvh &bvh, const f3 &o, const f3 &d, float tMax, const LaneStack &stack, uint32_t &nv, uint32_t &nt) {
+	return traverse<true, COUNT, ALPHA>(bvh, o, d, tMax, stack, nv, nt).prim >= 0;
 }
 
 // traceShadow with media: ShadowTr raygen + traceTransmittance
@@ -772,8 +787,11 @@ __global__ void k_shadow_tr(WfParams P, DevScene sc, TraceBvh bvh, ShadowQueueDe
 		if (i >= count) continue;
 		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1];
 		const float4 e0 = sq.ext[3 * i], e1 = sq.ext[3 * i + 1], e2 = sq.ext[3 * i + 2];
-		transmittanceItem<COUNT>(P, sc, bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, __float_as_int(b.w), __float_as_int(sq.c[i].w) - 1,
-								 f3(e0.x, e0.y, e0.z), f3(e1.x, e1.y, e1.z), f3(e2.x, e2.y, e2.z), px, stack, nv, nt);
+		const int pixel = __float_as_int(b.w);
+		PathRegs pr		= loadPathRegs(px, pixel, true);
+		transmittanceItem<COUNT>(P, sc, bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, __float_as_int(sq.c[i].w) - 1, f3(e0.x, e0.y, e0.z),
+								 f3(e1.x, e1.y, e1.z), f3(e2.x, e2.y, e2.z), pr, stack, nv, nt);
+		storePathRegs(px, pixel, pr);
 	}
 	if (COUNT) {
 		waveAddU64(&cnt->shadowNodeVisits, nv);
@@ -794,8 +812,14 @@ __global__ void k_trace_shadow(TraceBvh bvh, ShadowQueueDev sq, PixelStateDev px
 		const int i = base + laneId();
 		if (i >= count) continue;
 		const float4 a = sq.ray[2 * i], b = sq.ray[2 * i + 1], c = sq.c[i];
-		occlusionItem<COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, __float_as_int(b.w), f3(c.x, c.y, c.z), px, stack, nv,
-							 nt);
+		if (!occluded<COUNT>(bvh, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), a.w, stack, nv, nt)) {
+			const int pixel = __float_as_int(b.w);
+			float4 L		= px.L[pixel];
+			L.x				= L.x + c.x;
+			L.y				= L.y + c.y;
+			L.z				= L.z + c.z;
+			px.L[pixel]		= L;
+		}
 	}
 	if (COUNT) {
 		waveAddU64(&cnt->shadowNodeVisits, nv);
@@ -982,20 +1006,25 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 	bool active = false, exhausted = false, trainPath = false;
 	int depth	= startDepth;
 	RayItem r;
+	PathRegs pr; // the path's pixel state, in registers until the path ends
 	while (true) {
 		if (!exhausted) {
 			const int i = wavePush(&cnt->work[2], !active);
 			if (__ballot(!active && i >= count)) exhausted = true;
 			if (!active && i < count) {
-				if (VIEWS) r = viewCameraRay(P, V, px, px.order ? px.order[i] : i);
+				if (VIEWS) r = viewCameraRay(P, V, px, px.order ? px.order[i] : i, pr, MEDIA);
 				else if (camFetch) {
 					const int pix = px.order ? px.order[i] : i;
-					if (!P.legacySpp) px.L[pix] = make_float4(0.f, 0.f, 0.f, 0.f);
-					Pcg cs{px.rng[pix], P.inc};
-					r			 = cameraRay(cam, P.W, P.H, pix % P.W, pix / P.W, cs);
-					r.pixel		 = pix;
-					px.rng[pix] = cs.state;
-				} else r = loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
+					pr			  = loadPathRegs(px, pix, MEDIA);
+					if (!P.legacySpp) pr.L = f3();
+					Pcg cs{pr.rng, P.inc};
+					r		= cameraRay(cam, P.W, P.H, pix % P.W, pix / P.W, cs);
+					r.pixel = pix;
+					pr.rng	= cs.state;
+				} else {
+					r  = loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
+					pr = loadPathRegs(px, r.pixel, MEDIA);
+				}
 				depth	  = startDepth;
 				active	  = true;
 				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
@@ -1011,22 +1040,21 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 		ShadowOut so;
 		so.medium = -1;
 		int trainSlot = -1;
-		shadeCore<MEDIA, NRC, PLAIN>(P, sc, r, hv, px, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
+		shadeCore<MEDIA, NRC, PLAIN>(P, sc, r, hv, pr, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
 		if (wantShadow) {
 			nShadow++;
 			if (MEDIA && P.enableMedium)
-				transmittanceItem<COUNT, !PLAIN>(P, sc, bvh, so.o, so.d, 1.f, r.pixel, so.medium, so.Ld, so.pu, so.pl, px, stack, snv, snt);
-			else
-				occlusionItem<COUNT, !PLAIN>(bvh, so.o, so.d, 1.f, r.pixel, so.Ld / (so.pl + so.pu).mean(), px, stack, snv, snt);
+				transmittanceItem<COUNT, !PLAIN>(P, sc, bvh, so.o, so.d, 1.f, so.medium, so.Ld, so.pu, so.pl, pr, stack, snv, snt);
+			else if (!occluded<COUNT, !PLAIN>(bvh, so.o, so.d, 1.f, stack, snv, snt))
+				pr.L = pr.L + so.Ld / (so.pl + so.pu).mean();
 		}
-		if (NRC && trainSlot >= 0) { // radiance gathered up to and including this vertex's NEE
-			const float4 Lv = px.L[r.pixel];
-			nrc.train[(size_t) trainSlot * 6 + 5] = make_float4(Lv.x, Lv.y, Lv.z, 0.f);
-		}
+		if (NRC && trainSlot >= 0) // radiance gathered up to and including this vertex's NEE
+			nrc.train[(size_t) trainSlot * 6 + 5] = make_float4(pr.L.x, pr.L.y, pr.L.z, 0.f);
 		if (wantNext && depth < P.maxDepth) {
 			r = nx;
 			depth++;
 		} else {
+			storePathRegs(px, r.pixel, pr);
 			if (px.plen) px.plen[r.pixel] = (uint8_t) min(depth, 255);
 			active = false;
 		}
