@@ -2,10 +2,10 @@
 // gfx950 (replaces optixTrace, src/render/wavefront/device.cu:12-29).
 //
 // Each lane traverses its own ray through the 4-wide BVH (bvh.h): one node
-// fetch = eight 16-byte loads of one 128-byte line, four slab tests, leaves
-// intersected on the spot, inner children sorted front to back with the far
-// ones pushed on a per-lane stack in LDS (stride = block size, so the 64
-// lanes of a wave hit 64 different banks).
+// fetch = seven 16-byte loads of one 128-byte line, four slab tests, hit
+// children (inner nodes and leaves) sorted front to back with the far ones
+// pushed on a per-lane stack in LDS (stride = block size, so the 64 lanes of
+// a wave hit 64 different banks); a leaf is one more step of the loop.
 //
 // Hit rule (shared with the CPU checker): the Woop-Benthin-Wald watertight
 // test (JCGT 2(1), 2013), hit iff t > 0 and t < tbest, or t == tbest with a
@@ -43,7 +43,7 @@ struct TraceBvh {
 	const Bvh4Node *nodes;
 	const LeafTri *tris;
 	int stackLds; // traversal stack entries held in LDS per lane
-	int nNodes;	  // node count: a traversal visits each node at most once
+	int steps;	  // node count + leaf triangle count: a traversal takes each node and leaf at most once
 	const AlphaTri *alphaTri = nullptr; // leaves flagged kLeafAlpha index these
 	const AlphaSrc *alphaSrc = nullptr;
 };
@@ -218,6 +218,15 @@ struct LaneStack {
 // kLeafAlpha pass the any-hit alpha test first (device.cu:65-67,86-87,111-113;
 // megakernel/device.cu:134-142); the result does not depend on the order of
 // the candidates, since the test is a function of the ray and the hit.
+//
+// Each loop iteration takes one entry: an inner node (slab test of its four
+// children, hit children - inner and leaf alike - sorted near to far, the
+// nearest taken next, the others pushed) or a leaf (its triangles tested).
+// A lane therefore runs at most one leaf's tests per iteration, and the wave
+// runs the leaf code once per iteration for all lanes that hold a leaf:
+// testing the leaves of a node on the spot, slot by slot, made the wave run
+// up to four leaf blocks per node visit, each for the few lanes whose child in
+// that slot was a hit leaf.  The closest hit does not depend on the order.
 template <bool ANY, bool COUNT, bool ALPHA = false>
 KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, const LaneStack &stack,
 							 uint32_t &nodeVisits, uint32_t &triTests) {
@@ -232,41 +241,14 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 	const float kWiden = 1.00000036f;					// 1 + 2*gamma(3)
 	const float kTie   = 1.00000095367431640625f;		// 1 + 2^-20
 	int sp			   = 0;
-	int node		   = 0;
-	// Visits beyond the node count can only come from a corrupt stack or
-	// tree; the budget turns that into a finished (wrong) traversal instead
-	// of a wave that never ends.
-	int budget = bvh.nNodes;
+	int entry		   = 0; // >= 0: inner node; < 0: leaf code ~(first * 8 + count - 1)
+	// Each node and leaf is taken at most once; more steps can only come from
+	// a corrupt stack or tree, and the budget turns that into a finished
+	// (wrong) traversal instead of a wave that never ends.
+	int budget = bvh.steps;
 	while (--budget >= 0) {
-		if (COUNT) nodeVisits++;
-		const uint32_t nb = (uint32_t) node * 128u; // node array < 4 GiB (checked at build)
-#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
-		const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
-		const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
-#undef KRR_PLANE
-		const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
-		const float tlim = res.t * kTie;
-		float tn[4];
-		bool hit[4];
-		const int cc[4] = {ch.x, ch.y, ch.z, ch.w};
-#define KRR_SLAB(c, comp)                                                                                            \
-	{                                                                                                                \
-		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
-		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
-		t1		 = fminf(t1, tlim);                                                                                  \
-		tn[c]	 = t0;                                                                                               \
-		hit[c]	 = t0 <= t1;                                                                                         \
-	}
-		KRR_SLAB(0, x)
-		KRR_SLAB(1, y)
-		KRR_SLAB(2, z)
-		KRR_SLAB(3, w)
-#undef KRR_SLAB
-		// Leaves first, on the spot.
-#pragma unroll
-		for (int c = 0; c < 4; c++) {
-			if (!hit[c] || cc[c] >= 0) continue;
-			const int code	= ~cc[c];
+		if (entry < 0) {
+			const int code	= ~entry;
 			const int first = code >> 3, cnt = (code & 7) + 1;
 			for (int k = 0; k < cnt; k++) {
 				const float4 *tp = reinterpret_cast<const float4 *>(bvh.tris + first + k);
@@ -287,13 +269,33 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 					if (ANY) return res;
 				}
 			}
+			if (sp == 0) break;
+			entry = stack.get(--sp);
+			continue;
 		}
-		// Inner children, nearest first.
-		float k0 = (hit[0] && cc[0] >= 0) ? tn[0] : __builtin_inff();
-		float k1 = (hit[1] && cc[1] >= 0) ? tn[1] : __builtin_inff();
-		float k2 = (hit[2] && cc[2] >= 0) ? tn[2] : __builtin_inff();
-		float k3 = (hit[3] && cc[3] >= 0) ? tn[3] : __builtin_inff();
-		int c0 = cc[0], c1 = cc[1], c2 = cc[2], c3 = cc[3];
+		if (COUNT) nodeVisits++;
+		const uint32_t nb = (uint32_t) entry * 128u; // node array < 4 GiB (checked at build)
+#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
+		const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
+		const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
+#undef KRR_PLANE
+		const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
+		const float tlim = res.t * kTie;
+		float k0, k1, k2, k3;
+#define KRR_SLAB(kc, comp)                                                                                           \
+	{                                                                                                                \
+		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
+		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
+		t1		 = fminf(t1, tlim);                                                                                  \
+		kc		 = t0 <= t1 ? t0 : __builtin_inff();                                                                 \
+	}
+		KRR_SLAB(k0, x)
+		KRR_SLAB(k1, y)
+		KRR_SLAB(k2, z)
+		KRR_SLAB(k3, w)
+#undef KRR_SLAB
+		// Hit children, nearest first.
+		int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
 		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
 #define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
 	if (kb < ka) {                                                                                                   \
@@ -312,13 +314,13 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 #undef KRR_CSWAP
 		if (n == 0) {
 			if (sp == 0) break;
-			node = stack.get(--sp);
+			entry = stack.get(--sp);
 			continue;
 		}
 		if (n >= 4) stack.put(sp++, c3);
 		if (n >= 3) stack.put(sp++, c2);
 		if (n >= 2) stack.put(sp++, c1);
-		node = c0;
+		entry = c0;
 	}
 	return res;
 }

@@ -1234,7 +1234,8 @@ struct AccelData {
 	bool onDevice		= false; // built by the GPU builder
 	int ldsEntries() const { return std::min(stackSize, stackLds); }
 	TraceBvh dev() const {
-		return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) nNodes, alphaTri.as<AlphaTri>(), alphaSrc.as<AlphaSrc>()};
+		return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) std::min<int64_t>(INT32_MAX, nNodes + nLeafTris),
+						alphaTri.as<AlphaTri>(), alphaSrc.as<AlphaSrc>()};
 	}
 };
 
