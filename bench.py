@@ -32,8 +32,10 @@ CONFIGS = {
                         "camera": {"eye": [0.5, 0.75, 2.3], "target": [0.5, 0.3, 0.45]}},
                  params={"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3},
                  desc="icosphere_field_1.31Mtri_1920x1080_1spp_depth16_rr3",
-                 # FP64 reference of the same view at 1/8 resolution (a 1080p one would take hours of CPU time)
-                 reference_lowres=("tests/golden/icosphere_field_240x135_fp64_4096spp.npz", 240, 135)),
+                 # FP64 reference of the same view at 1/8 resolution, 4096 spp
+                 reference_lowres=("tests/golden/icosphere_field_240x135_fp64_4096spp.npz", 240, 135),
+                 # and at the measured resolution, 2048 spp
+                 reference_fp64="tests/golden/icosphere_field_1920x1080_fp64_2048spp.npz"),
     "cornell_box_256": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [256, 256],
                         "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
                  params={"spp": 16, "max_depth": 8, "rr": 0.8},
@@ -42,7 +44,8 @@ CONFIGS = {
     "cornell_box_1024": dict(scene={"model": [{"type": "cornell_box"}], "resolution": [1024, 1024],
                                     "camera": {"eye": [0, 1, 2.8], "target": [0, 1, 0]}},
                              params={"spp": 1, "max_depth": 8, "rr": 0.8},
-                             desc="cornell_box_36tri_1024x1024_1spp_depth8"),
+                             desc="cornell_box_36tri_1024x1024_1spp_depth8",
+                             reference_fp64="tests/golden/cornell_1024_fp64_1024spp.npz"),
     "volume_720p": dict(scene={"model": [{"type": "icosphere_field", "count": 1024, "level": 3, "seed": 7}],
                         "media": [{"type": "value_noise_grid", "res": 256, "octaves": 5, "seed": 11,
                                    "sigma_t_max": 20.0, "albedo": 0.8, "g": 0.3}],
@@ -194,6 +197,9 @@ def gpu_bench(cfg, steps, warmup, traffic):
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     k = kernels[dom]
     tr = (traffic or {}).get(k["kernel"])
+    # The HBM roofline is the ceiling the traversal's bytes are priced
+    # against; what limits the kernel is read from its own counters (profiles/):
+    # measured HBM bytes over the launch time, L2 hit rate, VALU lane use.
     roofline = {"bound": "hbm", "achieved": k["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(k["achieved_gbs"] / HBM_PEAK_GBS, 4),
                 "traffic": tr.get("bytes_per_launch") if tr else None,
@@ -202,6 +208,13 @@ def gpu_bench(cfg, steps, warmup, traffic):
                 "bytes_model": k["bytes_model"], "per_ray": k["per_ray"]}
     if tr:
         roofline["traffic_source"] = tr.get("source")
+        if k["avg_launch_ms"] > 0:
+            measured_gbs = tr["bytes_per_launch"] / (k["avg_launch_ms"] * 1e-3) / 1e9
+            roofline["measured_gbs"] = round(measured_gbs, 1)
+            roofline["measured_frac"] = round(measured_gbs / HBM_PEAK_GBS, 4)
+        for key in ("l2_hit_rate", "valu_lane_utilisation", "limiter"):
+            if key in tr:
+                roofline[key] = tr[key]
     out = {
         "value": round(rays / secs / 1e6, 2),
         "ms_per_step": round(secs * 1e3 / steps, 3),
@@ -224,9 +237,11 @@ def gpu_bench(cfg, steps, warmup, traffic):
 
 
 # Algorithmic bytes: what each kernel must move at least once per item, with
-# no re-reads - a BVH4 node is one 128-byte line, a leaf triangle 48 bytes,
-# a queued ray 32 bytes {o, tmax, d, meta} and its hit record 16 bytes.
-NODE_B, TRI_B, RAY_B, HIT_B = 128, 48, 32, 16
+# no re-reads - a BVH4 node is one 128-byte line as floats (the default
+# node_format) or 64 bytes quantised, a leaf triangle 48 bytes, a queued ray 32 bytes
+# {o, tmax, d, meta} and its hit record 16 bytes.
+TRI_B, RAY_B, HIT_B = 48, 32, 16
+NODE_BYTES = {"quantized": 64, "float": 128}
 
 
 def kernel_rooflines(cfg, prm, steps, tot, cnt):
@@ -237,6 +252,7 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     (count_traversal, a second untimed pass)."""
     spp = prm["spp"]
     max_depth = prm["max_depth"]
+    NODE_B = NODE_BYTES[prm.get("node_format", "float")]
     wdepths = prm["wavefront_depths"]
     mega_launches = steps * spp if wdepths <= max_depth else 0
     wf_closest = cnt["closest_rays"] - cnt["megakernel_closest_rays"]
@@ -254,7 +270,7 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     wf_launches = tot["launches"] - mega_launches
     entry("closest_trace", "k_trace_closest_queue", tot["trace_ms"], wf_launches,
           wf_closest * (RAY_B * 2 + HIT_B) + cnt["node_visits"] * NODE_B + cnt["tri_tests"] * TRI_B,
-          "64 B ray + 16 B hit per ray + 128 B per node visit + 48 B per triangle test",
+          f"64 B ray + 16 B hit per ray + {NODE_B} B per node visit + 48 B per triangle test",
           wf_closest, cnt["node_visits"], cnt["tri_tests"])
     shadow_launches = 0
     if prm["nee"]:  # no shadow rays are traced at the last depth
@@ -262,12 +278,12 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
     media = bool(cfg["scene"].get("media")) and prm["enable_medium"]
     entry("shadow_trace", "k_shadow_tr" if media else "k_trace_shadow", tot["shadow_ms"], shadow_launches,
           wf_shadow * (RAY_B * 2 + 16) + cnt["shadow_node_visits"] * NODE_B + cnt["shadow_tri_tests"] * TRI_B,
-          "64 B ray + 16 B contribution per ray + 128 B per node visit + 48 B per triangle test",
+          f"64 B ray + 16 B contribution per ray + {NODE_B} B per node visit + 48 B per triangle test",
           wf_shadow, cnt["shadow_node_visits"], cnt["shadow_tri_tests"])
     mrays = cnt["megakernel_closest_rays"] + cnt["megakernel_shadow_rays"]
     entry("megakernel", "k_megakernel", tot["mega_ms"], mega_launches,
           cnt["megakernel_node_visits"] * NODE_B + cnt["megakernel_tri_tests"] * TRI_B,
-          "traversal only (lower bound): 128 B per node visit + 48 B per triangle test, "
+          f"traversal only (lower bound): {NODE_B} B per node visit + 48 B per triangle test, "
           "closest and shadow rays",
           mrays, cnt["megakernel_node_visits"], cnt["megakernel_tri_tests"])
     return out
@@ -276,16 +292,37 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
 NRC_PARAMS = {"train_mod": 16, "batch": 65536, "steps_per_frame": 4, "lr": 1e-3}
 
 
-def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc_params=None):
+def load_reference_image(path, w, h):
+    """FP64 checker image (film layout, row H-1-y first) as float32 RGBA, or
+    None when the file is absent."""
+    import numpy as np
+    full = os.path.join(ROOT, path)
+    if not os.path.exists(full):
+        return None, None
+    with np.load(full, allow_pickle=False) as z:
+        img = z["image"].astype(np.float32)
+        spp = int(z["spp"])
+    assert img.shape == (h, w, 3), (img.shape, w, h)
+    rgba = np.ones((h, w, 4), np.float32)
+    rgba[..., :3] = img
+    return rgba, spp
+
+
+def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc_params=None, tail_frames=96):
     """Radiance cache against the plain tracer on the same workload.
 
-    Reference: the plain tracer with ref_frames x ref_spp samples per pixel
-    (frame indices 100001.., independent of the compared frames).  Plain:
-    frames 1..frames.  Cache: warm_frames of online training, then frames
-    more (training continues every frame).  Each frame is timed on its own
-    (host wall clock around begin_frame + render + sync, training included
-    for the cache).  Reports the relMSE of single frames, the per-frame time,
-    and the relMSE of the running average after equal wall time."""
+    Error reference: the FP64 checker image of the same view
+    (cfg["reference_fp64"]); when that file is absent, the plain tracer with
+    ref_frames x ref_spp samples per pixel (frame indices 100001..) stands
+    in, and the output says so.  Plain: frames 1..frames.  Cache: warm_frames
+    of online training, then frames more (training continues every frame).
+    Each frame is timed on its own (host wall clock around begin_frame +
+    render + sync, training included for the cache); the warm-up frames are
+    timed too, so the wall-clock figures are given with and without them.
+    Reports the relMSE of single frames, of the running average after equal
+    wall time, the time each needs to reach plain's final relMSE, and the
+    cache's bias: after tail_frames more frames, the relMSE floor of its
+    running average and the relative error of its mean image brightness."""
     import numpy as np
     import kiraray_amd as krr
     from kiraray_amd import hip
@@ -299,23 +336,27 @@ def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc
     ref = hip.DeviceBuffer(n * 16)
     stream = hip.Stream()
 
-    def run(params, first, count, accumulate_into=None, warm=0):
+    def run(params, first, count, accumulate_into=None, warm=0, tail=0):
         wf = krr.WavefrontPathTracer(params)
         wf.set_scene(scene)
         wf.resize(w, h)
+        stream.synchronize()
+        t0 = time.perf_counter()
         for i in range(warm):
             wf.begin_frame(cam, first - warm + i)
             wf.render(film.ptr, stream.handle)
         stream.synchronize()
+        t_warm = time.perf_counter() - t0
         times, single, running = [], [], []
-        for i in range(count):
+        for i in range(count + tail):
             t0 = time.perf_counter()
             wf.begin_frame(cam, first + i)
             wf.render(film.ptr, stream.handle)
             stream.synchronize()
             times.append(time.perf_counter() - t0)
             if accumulate_into is None:
-                single.append(krr.error_metric(3, film.ptr, ref.ptr, n, stream.handle))
+                if i < count:
+                    single.append(krr.error_metric(3, film.ptr, ref.ptr, n, stream.handle))
                 krr.accumulate(acc.ptr, film.ptr, n, i, 0, 0, 64, stream.handle)
                 stream.synchronize()
                 running.append(krr.error_metric(3, film.ptr, ref.ptr, n, stream.handle))
@@ -323,41 +364,64 @@ def nrc_equal_time(cfg, warm_frames=96, frames=24, ref_frames=96, ref_spp=4, nrc
                 krr.accumulate(accumulate_into.ptr, film.ptr, n, i, 0, 0, 64, stream.handle)
         stream.synchronize()
         st = wf.stats()
+        mean_img = film.to_numpy(np.float32, (h, w, 4))[..., :3] if accumulate_into is None else None
         wf.close()
-        return times, single, running, st
+        return times, single, running, st, t_warm, mean_img
 
     base = dict(cfg["params"])
-    log(f"nrc: reference {ref_frames} x {ref_spp} spp")
-    run(dict(base, spp=ref_spp), 100001, ref_frames, accumulate_into=acc)
-    hip.check(hip.runtime().hipMemcpy(C_void(ref.ptr), C_void(film.ptr), n * 16, 3))
+    ref_img, ref_spp_fp64 = load_reference_image(cfg["reference_fp64"], w, h) if cfg.get("reference_fp64") else (None, None)
+    if ref_img is not None:
+        log(f"nrc: FP64 checker reference {cfg['reference_fp64']} ({ref_spp_fp64} spp)")
+        ref.copy_from_numpy(ref_img)
+        ref_desc = f"FP64 CPU oracle, {ref_spp_fp64} spp ({cfg['reference_fp64']})"
+    else:
+        log(f"nrc: no FP64 reference image; plain tracer {ref_frames} x {ref_spp} spp instead")
+        run(dict(base, spp=ref_spp), 100001, ref_frames, accumulate_into=acc)
+        hip.check(hip.runtime().hipMemcpy(C_void(ref.ptr), C_void(film.ptr), n * 16, 3))
+        ref_desc = f"plain tracer, {ref_frames * ref_spp} spp (no FP64 oracle image for this workload)"
+    ref_mean = float(ref.to_numpy(np.float32, (h, w, 4))[..., :3].astype(np.float64).mean())
     log("nrc: plain tracer")
-    tp, sp, rp, _ = run(base, 1, frames)
+    tp, sp, rp, _, _, plain_img = run(base, 1, frames, tail=tail_frames)
     log("nrc: radiance cache")
     nrc_params = dict(NRC_PARAMS, **(nrc_params or {}))
-    tn, sn, rn, stn = run(dict(base, nrc=nrc_params, time_kernels=True), warm_frames + 1, frames, warm=warm_frames)
-    ms_p, ms_n = 1e3 * float(np.median(tp)), 1e3 * float(np.median(tn))
+    tn, sn, rn, stn, t_warm, nrc_img = run(dict(base, nrc=nrc_params, time_kernels=True), warm_frames + 1, frames, warm=warm_frames,
+                                           tail=tail_frames)
+    ms_p, ms_n = 1e3 * float(np.median(tp[:frames])), 1e3 * float(np.median(tn[:frames]))
+    cum_p, cum_n = np.cumsum(tp[:frames]), np.cumsum(tn[:frames])
+    rp_c, rn_c = rp[:frames], rn[:frames]
     # running average at equal wall time: the cache's curve at the time plain needs for all its frames
-    t_plain = float(np.sum(tp))
-    k = int(np.searchsorted(np.cumsum(tn), t_plain, side="right"))
-    rel_n_equal = rn[min(max(k, 1), len(rn)) - 1]
+    t_plain = float(cum_p[-1])
+    k = int(np.searchsorted(cum_n, t_plain, side="right"))
+    rel_n_equal = rn_c[min(max(k, 1), len(rn_c)) - 1]
     # wall time each needs to bring its running average to plain's final relMSE
-    target = float(rp[-1])
-    hit = [i for i, x in enumerate(rn) if x <= target]
-    t_nrc_reach = float(np.cumsum(tn)[hit[0]]) if hit else None
-    return {"reference": f"plain tracer, {ref_frames * ref_spp} spp", "warm_frames": warm_frames, "frames": frames,
+    target = float(rp_c[-1])
+    hit = [i for i, x in enumerate(rn_c) if x <= target]
+    t_nrc_reach = float(cum_n[hit[0]]) if hit else None
+    # with the warm-up frames on the cache's clock
+    t_nrc_reach_w = t_nrc_reach + t_warm if t_nrc_reach is not None else None
+
+    def bias(img):
+        return round((float(img.astype(np.float64).mean()) - ref_mean) / ref_mean, 5)
+
+    return {"reference": ref_desc, "warm_frames": warm_frames, "frames": frames,
             "nrc_params": nrc_params,
             "plain": {"ms_per_frame": round(ms_p, 3), "relmse_single_frame": round(float(np.median(sp)), 5),
-                      "relmse_running": [round(float(x), 5) for x in rp]},
+                      "relmse_running": [round(float(x), 5) for x in rp_c]},
             "nrc": {"ms_per_frame": round(ms_n, 3), "relmse_single_frame": round(float(np.median(sn)), 5),
-                    "relmse_running": [round(float(x), 5) for x in rn], "loss": round(stn["nrc_loss"], 5),
+                    "relmse_running": [round(float(x), 5) for x in rn_c], "loss": round(stn["nrc_loss"], 5),
                     "queries_per_frame": int(stn["nrc_queries"]), "train_vertices_per_frame": int(stn["nrc_train_vertices"]),
-                    "kernel_ms_per_frame": round(stn["nrc_ms"], 3)},
-            "equal_time": {"wall_s": round(t_plain, 4), "plain_relmse": round(float(rp[-1]), 5),
-                           "nrc_frames": min(max(k, 1), len(rn)), "nrc_relmse": round(float(rel_n_equal), 5)},
+                    "kernel_ms_per_frame": round(stn["nrc_ms"], 3), "warm_up_s": round(t_warm, 4)},
+            "equal_time": {"wall_s": round(t_plain, 4), "plain_relmse": round(float(rp_c[-1]), 5),
+                           "nrc_frames": min(max(k, 1), len(rn_c)), "nrc_relmse": round(float(rel_n_equal), 5)},
             "time_to_plain_relmse": {"relmse": round(target, 5), "plain_s": round(t_plain, 4),
                                      "nrc_s": round(t_nrc_reach, 4) if t_nrc_reach else None,
                                      "nrc_frames": hit[0] + 1 if hit else None,
-                                     "speedup": round(t_plain / t_nrc_reach, 3) if t_nrc_reach else None},
+                                     "speedup": round(t_plain / t_nrc_reach, 3) if t_nrc_reach else None,
+                                     "nrc_s_with_warm_up": round(t_nrc_reach_w, 4) if t_nrc_reach_w else None,
+                                     "speedup_with_warm_up": round(t_plain / t_nrc_reach_w, 3) if t_nrc_reach_w else None},
+            "converged": {"frames": frames + tail_frames,
+                          "plain_relmse": round(float(rp[-1]), 5), "nrc_relmse": round(float(rn[-1]), 5),
+                          "plain_mean_rel_error": bias(plain_img), "nrc_mean_rel_error": bias(nrc_img)},
             "speed_ratio_per_frame": round(ms_p / ms_n, 3),
             "single_frame_relmse_ratio": round(float(np.median(sp)) / float(np.median(sn)), 3)}
 
@@ -468,6 +532,58 @@ def multiview_bench(frames=4, warmup=1, ref_frames=16, nrc_warm=8, nrc_params=No
             "launch": {k: prm[k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")},
             "reference": f"plain tracer, {ref_frames * base['spp']} spp per pixel",
             "plain": plain, "nrc": nrc}
+
+
+def node_format_ab(cfg, frames=10, warmup=3):
+    """The benchmark workload with each BVH node format - 128-byte float
+    nodes (the default) and 64-byte quantised nodes - in one process: ms per
+    frame and Mrays/s (HIP events on the render stream), node visits and
+    node bytes per ray (count_traversal frames).  Each side gets its own
+    autotuned launch shape.  The images are the same (the closest hit does
+    not depend on the tree's boxes)."""
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    w, h = cfg["scene"]["resolution"]
+    scene = krr.Scene(cfg["scene"])
+    cam = scene.camera(0)
+    film = hip.DeviceBuffer(w * h * 16)
+    stream = hip.Stream()
+    out = {"workload": cfg["desc"], "frames": frames}
+    for fmt in ("quantized", "float"):
+        wf = krr.WavefrontPathTracer(dict(cfg["params"], node_format=fmt))
+        wf.set_scene(scene)
+        wf.resize(w, h)
+        wf.begin_frame(cam, 1)
+        wf.autotune(stream=stream.handle)
+        for i in range(warmup):
+            wf.begin_frame(cam, 500 + i)
+            wf.render(film.ptr, stream.handle)
+        ev0, ev1 = hip.Event(), hip.Event()
+        rays = 0
+        ev0.record(stream)
+        for i in range(frames):
+            wf.begin_frame(cam, 1 + i)
+            wf.render(film.ptr, stream.handle)
+            st = wf.stats()
+            rays += st["closest_rays"] + st["shadow_rays"]
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_ms(ev1) / frames
+        wf.close()
+        wc = krr.WavefrontPathTracer(dict(cfg["params"], node_format=fmt, count_traversal=True))
+        wc.set_scene(scene)
+        wc.resize(w, h)
+        wc.begin_frame(cam, 1)
+        wc.render(film.ptr, stream.handle)
+        st = wc.stats()
+        wc.close()
+        nr = st["megakernel_closest_rays"] + st["megakernel_shadow_rays"]
+        visits = st["megakernel_node_visits"] / max(1, nr)
+        out[fmt] = {"ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / frames / ms / 1e3, 1),
+                    "node_visits_per_ray": round(visits, 2), "node_bytes_per_ray": round(visits * NODE_BYTES[fmt], 1)}
+        log(f"node format {fmt}: {out[fmt]}")
+    return out
 
 
 def megakernel_ab(cfg, frames=5, warmup=2):
@@ -599,10 +715,14 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel path tracer A/B")
-    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel"],
+    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel", "nodes"],
                     help="nrc / multiview: run only that comparison and print its JSON (profiling runs)")
     ap.add_argument("--nrc-warm", type=int, default=96, help="training frames before the NRC comparison")
     ap.add_argument("--nrc-frames", type=int, default=24, help="compared frames of the NRC comparison")
+    ap.add_argument("--nrc-tail", type=int, default=96,
+                    help="further frames of the NRC comparison for the converged relMSE and bias (0 for profiling runs)")
+    ap.add_argument("--wavefront-depths", type=int, default=None,
+                    help="run the first N depths of each path as per-depth queue kernels (the pass param wavefront_depths)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -615,14 +735,18 @@ def main():
               "and not launched (use --gpus 1)", file=sys.stderr)
         raise SystemExit(2)
     cfg = CONFIGS[args.config]
+    if args.wavefront_depths is not None:
+        cfg = dict(cfg, params=dict(cfg["params"], wavefront_depths=args.wavefront_depths))
     if args.section != "all":
         from kiraray_amd import hip
         hip.check(hip.runtime().hipSetDevice(0))
         if args.section == "nrc":
             out = {"icosphere_field_1080p": nrc_equal_time(CONFIGS["icosphere_field_1080p"], warm_frames=args.nrc_warm,
-                                                           frames=args.nrc_frames)}
+                                                           frames=args.nrc_frames, tail_frames=args.nrc_tail)}
         elif args.section == "megakernel":
             out = megakernel_ab(CONFIGS["icosphere_field_1080p"])
+        elif args.section == "nodes":
+            out = node_format_ab(cfg)
         else:
             out = multiview_bench()
         print(json.dumps(out), flush=True)
@@ -650,13 +774,16 @@ def main():
         line["cpu_baseline"] = None
     if args.config == "icosphere_field_1080p" and not args.no_nrc:
         # the open workload, and a closed scene where paths are long
-        line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg, warm_frames=args.nrc_warm, frames=args.nrc_frames),
+        line["nrc_vs_plain"] = {"icosphere_field_1080p": nrc_equal_time(cfg, warm_frames=args.nrc_warm, frames=args.nrc_frames,
+                                                                        tail_frames=args.nrc_tail),
                                 "cornell_box_1024": nrc_equal_time(CONFIGS["cornell_box_1024"],
-                                                                   nrc_params={"steps_per_frame": 2, "batch": 32768})}
+                                                                   nrc_params={"steps_per_frame": 2, "batch": 32768},
+                                                                   tail_frames=args.nrc_tail)}
     if args.config == "icosphere_field_1080p" and not args.no_multiview:
         line["multiview"] = multiview_bench()
     if args.config == "icosphere_field_1080p" and not args.no_megakernel:
         line["megakernel_ab"] = megakernel_ab(cfg)
+        line["node_format_ab"] = node_format_ab(cfg)
     print(json.dumps(line), flush=True)
 
 

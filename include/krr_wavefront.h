@@ -242,7 +242,9 @@ typedef struct krr_accel krr_accel;
 /* json_opts (NULL or "" = defaults): leaf_size (4) max triangles per leaf,
  * sah_node_cost (0.25) SAH cost of a node visit in triangle tests,
  * stack_lds_entries (16) traversal stack entries per lane kept in LDS, the
- * deeper ones in a global overflow area. */
+ * deeper ones in a global overflow area, node_format ("float") the nodes the
+ * traversal reads: "float" 128-byte nodes or "quantized" 64-byte nodes with
+ * 8-bit child boxes (same hits; half the node bytes). */
 int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_accel **out);
 /* rays_dev: n records of 8 floats {ox,oy,oz,tmax,dx,dy,dz,0} in device memory.
  * Closest hit in (0, tmax); ties in t go to the lowest primitive index.
@@ -292,7 +294,8 @@ typedef struct krr_wf krr_wf;
  *                            depths of every path run in one persistent
  *                            launch with the same per-path stage order, so
  *                            the image does not depend on this value
- *   leaf_size (4), sah_node_cost (0.25), stack_lds_entries (16)  as for krr_accel_build
+ *   leaf_size (4), sah_node_cost (0.25), stack_lds_entries (16),
+ *   node_format ("float")    as for krr_accel_build
  *   nrc (false)              neural radiance cache (include/krr_nrc.h): true,
  *                            or {"train_mod": 16, "batch": 65536,
  *                            "steps_per_frame": 4, and the krr_nrc_create
@@ -360,7 +363,7 @@ typedef struct krr_mk krr_mk;
  *   block_size (256)         threads per block, a multiple of 64
  *   occupancy (1.0)          fraction of the resident blocks launched
  *   time_kernels (false)     time the launch with HIP events
- *   leaf_size, sah_node_cost, stack_lds_entries, bvh_builder  as for krr_wf_create
+ *   leaf_size, sah_node_cost, stack_lds_entries, bvh_builder, node_format  as for krr_wf_create
  * NULL or "" means all defaults.  Replaces RenderPassFactory::deserialize<
  * MegakernelPathTracer> + from_json (pathtracer.h:45-49). */
 int krr_mk_create(const char *json_params, krr_mk **out);

@@ -20,6 +20,24 @@ struct alignas(128) Bvh4Node {
 };
 static_assert(sizeof(Bvh4Node) == 128, "Bvh4Node must be 128 bytes");
 
+// Quantised node: the same tree in 64 bytes (two nodes per 128-byte line).
+// Per axis a, the child boxes are 8-bit steps on a grid of spacing 2^e[a]
+// from org[a], a multiple of 2^e[a] small enough that every plane
+// org[a] + q * 2^e[a] (q < 256) is a float exactly: the traversal decodes a
+// plane with one exact fma and runs the same robust slab test as on the
+// float node.  Each decoded box contains the float box it stands for (the
+// planes are rounded outward), so a traversal over quantised nodes visits a
+// superset of the boxes and finds the same closest hit.
+struct alignas(64) Bvh4QNode {
+	float org[3];
+	int8_t e[3];
+	uint8_t valid;	  // bit c set: child slot c is used
+	uint32_t q[6];	  // planes lox, hix, loy, hiy, loz, hiz; byte c = child c
+	int32_t child[4]; // as Bvh4Node::child
+	int32_t pad[2];
+};
+static_assert(sizeof(Bvh4QNode) == 64, "Bvh4QNode must be 64 bytes");
+
 // Leaf triangle: three 16-byte records {v0, prim}, {v1, flags}, {v2, 0};
 // flags: kLeafNull = null material (skipped by shadow rays), kLeafAlpha =
 // the material has an alpha test (transmission texture or constant).

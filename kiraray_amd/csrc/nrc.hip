@@ -292,7 +292,10 @@ __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int
 // tile's loss sum.  Samples i >= n contribute nothing.
 __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const uint16_t *wAll, const float *vertices, const float *targets,
 													int64_t nHost, const int *nDev, int64_t capTotal, int step, int steps, int64_t nTiles,
-													int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT, float *tileLoss) {
+													int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT, float *tileLoss, int *adamStep) {
+	// a training step (not a bare gradient) counts Adam's step here: the
+	// kernels after this one in the stream read the new count
+	if (adamStep && blockIdx.x == 0 && threadIdx.x == 0) *adamStep += 1;
 	// samples: nHost, or with a device count, the step-th of `steps` equal
 	// slices of the first min(*nDev, capTotal) samples
 	int64_t n = nHost;
@@ -445,22 +448,26 @@ __global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xTAll, cons
 	}
 }
 
-// grad[p] = sum over chunks (fixed order) of the layer's partials, for every
-// weight p (layer-major like the master weights).
-__global__ void k_reduce_chunks(const float *partialAll, int chunks, float *grad) {
-	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= KRR_NRC_PARAMS) return;
+// Sum over chunks (fixed order) of the layer's partials of weight p
+// (layer-major like the master weights).
+__device__ __forceinline__ float chunkSum(const float *partialAll, int chunks, int p) {
 	const int layer = min(p / 4096, kHidden);
 	const int q		= p - layer * 4096;
 	const int count = layer < kHidden ? 4096 : 64 * 3;
 	const float *src = partialAll + (int64_t) layer * chunks * 4096 + q;
 	float s			 = 0.f;
 	for (int c = 0; c < chunks; c++) s += src[(int64_t) c * count];
-	grad[p] = s;
+	return s;
 }
 
-// Deterministic loss total over tiles (one block).
-__global__ void k_reduce_loss(const float *tileLoss, int64_t nTiles, float *out) {
+__global__ void k_reduce_chunks(const float *partialAll, int chunks, float *grad) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= KRR_NRC_PARAMS) return;
+	grad[p] = chunkSum(partialAll, chunks, p);
+}
+
+// Deterministic loss total over tiles (one block of 256 threads).
+__device__ void reduceLoss(const float *tileLoss, int64_t nTiles, float *out) {
 	__shared__ float part[256];
 	float s = 0.f;
 	for (int64_t t = threadIdx.x; t < nTiles; t += blockDim.x) s += tileLoss[t];
@@ -472,6 +479,8 @@ __global__ void k_reduce_loss(const float *tileLoss, int64_t nTiles, float *out)
 	}
 	if (threadIdx.x == 0) *out = part[0];
 }
+
+__global__ void k_reduce_loss(const float *tileLoss, int64_t nTiles, float *out) { reduceLoss(tileLoss, nTiles, out); }
 
 // Rewrites both bf16 copies from the fp32 master weights.
 __device__ void writeCopies(const float *w, uint16_t *wt, uint16_t *wb, int p) {
@@ -495,21 +504,32 @@ __global__ void k_write_copies(const float *w, uint16_t *wt, uint16_t *wb) {
 // Adam (Kingma & Ba 2015) with bias corrections c1 = 1 - beta1^t, c2 = 1 - beta2^t.
 // Adam's step count lives in device memory, so that training steps recorded
 // in a graph count up on every replay.
-__global__ void k_adam_count(int *t) { *t += 1; }
-
-__global__ void k_adam(float *w, float *mo, float *ve, const float *grad, float lr, float b1, float b2, float eps, const int *tDev,
-					   uint16_t *wt, uint16_t *wb) {
-	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= KRR_NRC_PARAMS) return;
-	const float t  = (float) *tDev;
+__device__ __forceinline__ void adamUpdate(float *w, float *mo, float *ve, float gr, float lr, float b1, float b2, float eps, float t,
+										   uint16_t *wt, uint16_t *wb, int p) {
 	const float c1 = 1.f - powf(b1, t), c2 = 1.f - powf(b2, t); // bias corrections
-	const float gr = grad[p];
 	const float m  = b1 * mo[p] + (1.f - b1) * gr;
 	const float v  = b2 * ve[p] + (1.f - b2) * gr * gr;
 	mo[p]		   = m;
 	ve[p]		   = v;
 	w[p] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
 	writeCopies(w, wt, wb, p);
+}
+
+// One training step's tail in one launch: per weight the fixed-order chunk
+// sum (the gradient, also stored), the Adam update and the bf16 copies; the
+// last block also totals the loss.  Same arithmetic, in the same order, as
+// k_reduce_chunks + k_reduce_loss followed by the update.  The step count was
+// raised by k_train_tile.
+__global__ void __launch_bounds__(256) k_reduce_adam(const float *partialAll, int chunks, float *grad, float *w, float *mo, float *ve,
+													 float lr, float b1, float b2, float eps, const int *tDev, uint16_t *wt, uint16_t *wb,
+													 const float *tileLoss, int64_t nTiles, float *loss) {
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p < KRR_NRC_PARAMS) {
+		const float gr = chunkSum(partialAll, chunks, p);
+		grad[p]		   = gr;
+		adamUpdate(w, mo, ve, gr, lr, b1, b2, eps, (float) *tDev, wt, wb, p);
+	}
+	if (blockIdx.x == gridDim.x - 1) reduceLoss(tileLoss, nTiles, loss);
 }
 
 } // namespace nrc
@@ -598,7 +618,9 @@ std::vector<float> initialWeights(uint64_t seed) {
 
 bool validN(int64_t n) { return n >= 0 && n <= ((int64_t) 1 << 31); }
 
-int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st, const int *nDev = nullptr,
+// Gradient of one batch; with adam, also the Adam step (3 launches: tiles,
+// split-K weight gradients, reduction + update).
+int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, int64_t n, hipStream_t st, bool adam, const int *nDev = nullptr,
 				 int64_t capTotal = 0, int step = 0, int steps = 1) {
 	const int64_t chunks = (n + kChunk - 1) / kChunk;
 	const int64_t nTiles = chunks * (kChunk / 32);
@@ -606,12 +628,20 @@ int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, i
 	const int blocks	 = (int) std::min<int64_t>(1024, (nTiles + 3) / 4);
 	hipLaunchKernelGGL(k_train_tile, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), (const float *) vert,
 					   targets, n, nDev, capTotal, step, steps, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(),
-					   h->tileLoss.as<float>());
+					   h->tileLoss.as<float>(), adam ? h->step.as<int>() : nullptr);
 	NRC_HIP(hipGetLastError());
 	const int wchunks = (int) ((nTiles * 32 + kWChunk - 1) / kWChunk);
 	hipLaunchKernelGGL(k_weight_grad, dim3(1, (unsigned) wchunks, kHidden + 1), dim3(256), 0, st, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(),
 					   h->ld, kWChunk, wchunks, h->partial.as<float>());
 	NRC_HIP(hipGetLastError());
+	if (adam) {
+		hipLaunchKernelGGL(k_reduce_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), wchunks,
+						   h->grad.as<float>(), h->w.as<float>(), h->m.as<float>(), h->v.as<float>(), h->lr, h->beta1, h->beta2, h->eps,
+						   (const int *) h->step.as<int>(), h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), h->tileLoss.as<float>(), nTiles,
+						   h->loss.as<float>());
+		NRC_HIP(hipGetLastError());
+		return KRR_OK;
+	}
 	hipLaunchKernelGGL(k_reduce_chunks, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), wchunks,
 					   h->grad.as<float>());
 	NRC_HIP(hipGetLastError());
@@ -723,7 +753,7 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 	if (!nrc || n < 1 || !vertices_dev || !targets_dev || !grad_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_gradient: bad argument");
 	if (n > nrc->maxBatch) return krr::fail(KRR_ERR_CAPACITY, "krr_nrc_gradient: batch %lld exceeds max_batch %lld", (long long) n, (long long) nrc->maxBatch);
 	hipStream_t st = (hipStream_t) hip_stream;
-	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st)) return rc;
+	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st, false)) return rc;
 	NRC_HIP(hipMemcpyAsync(grad_dev, nrc->grad.p, sizeof(float) * KRR_NRC_PARAMS, hipMemcpyDeviceToDevice, st));
 	if (loss_out) {
 		NRC_HIP(hipMemcpyAsync(loss_out, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
@@ -733,19 +763,6 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 }
 
 } // extern "C"
-
-namespace {
-
-int adamStep(krr_nrc *nrc, hipStream_t st) {
-	hipLaunchKernelGGL(k_adam_count, dim3(1), dim3(1), 0, st, nrc->step.as<int>());
-	hipLaunchKernelGGL(k_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, nrc->w.as<float>(), nrc->m.as<float>(), nrc->v.as<float>(),
-					   (const float *) nrc->grad.as<float>(), nrc->lr, nrc->beta1, nrc->beta2, nrc->eps, (const int *) nrc->step.as<int>(),
-					   nrc->wt.as<uint16_t>(), nrc->wb.as<uint16_t>());
-	NRC_HIP(hipGetLastError());
-	return KRR_OK;
-}
-
-} // namespace
 
 int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const int *count, int capacity, float4 *L, hipStream_t st) {
 	if (capacity <= 0) return KRR_OK;
@@ -760,10 +777,8 @@ int nrcTrainDevice(krr_nrc *nrc, const float *vertices, const float *targets, co
 	const int64_t cap = std::min<int64_t>(capacity, nrc->maxBatch);
 	if (cap <= 0 || steps <= 0) return KRR_OK;
 	const int64_t m = (cap + steps - 1) / steps; // largest mini-batch
-	for (int b = 0; b < steps; b++) {
-		if (int rc = gradientImpl(nrc, (const krr_nrc_vertex *) vertices, targets, m, st, count, cap, b, steps)) return rc;
-		if (int rc = adamStep(nrc, st)) return rc;
-	}
+	for (int b = 0; b < steps; b++)
+		if (int rc = gradientImpl(nrc, (const krr_nrc_vertex *) vertices, targets, m, st, true, count, cap, b, steps)) return rc;
 	return KRR_OK;
 }
 
@@ -782,8 +797,7 @@ int krr_nrc_train_step(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const f
 	if (!nrc || n < 1 || !vertices_dev || !targets_dev) return krr::fail(KRR_ERR_INVALID_ARG, "krr_nrc_train_step: bad argument");
 	if (n > nrc->maxBatch) return krr::fail(KRR_ERR_CAPACITY, "krr_nrc_train_step: batch %lld exceeds max_batch %lld", (long long) n, (long long) nrc->maxBatch);
 	hipStream_t st = (hipStream_t) hip_stream;
-	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st)) return rc;
-	if (int rc = adamStep(nrc, st)) return rc;
+	if (int rc = gradientImpl(nrc, vertices_dev, targets_dev, n, st, true)) return rc;
 	if (loss_out) {
 		NRC_HIP(hipMemcpyAsync(loss_out, nrc->loss.p, sizeof(float), hipMemcpyDeviceToHost, st));
 		NRC_HIP(hipStreamSynchronize(st));

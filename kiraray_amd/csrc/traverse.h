@@ -46,6 +46,7 @@ struct TraceBvh {
 	int steps;	  // node count + leaf triangle count: a traversal takes each node and leaf at most once
 	const AlphaTri *alphaTri = nullptr; // leaves flagged kLeafAlpha index these
 	const AlphaSrc *alphaSrc = nullptr;
+	const Bvh4QNode *qnodes	 = nullptr; // the same tree quantised; when set ("node_format": "quantized"), the traversal reads these
 };
 
 // HashFloat(o, d) of src/util/hash.h:130-134: MurmurHash64A (Austin Appleby,
@@ -274,14 +275,51 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 			continue;
 		}
 		if (COUNT) nodeVisits++;
-		const uint32_t nb = (uint32_t) entry * 128u; // node array < 4 GiB (checked at build)
-#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
-		const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
-		const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
-#undef KRR_PLANE
-		const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
 		const float tlim = res.t * kTie;
 		float k0, k1, k2, k3;
+		int c0, c1, c2, c3;
+		if (bvh.qnodes) {
+			// One 64-byte quantised node: {org, e, valid}, planes lox hix loy
+			// hiy, planes loz hiz + children 0 1, children 2 3.  Plane c of a
+			// dword is byte c; org + q 2^e is exact (one fma, no rounding).
+			const uint4 *np = reinterpret_cast<const uint4 *>(bvh.qnodes + entry);
+			const uint4 A = np[0], B = np[1], C = np[2], D = np[3];
+			const float sx = __int_as_float(((int) (int8_t) (A.w & 0xffu) + 127) << 23);
+			const float sy = __int_as_float(((int) (int8_t) ((A.w >> 8) & 0xffu) + 127) << 23);
+			const float sz = __int_as_float(((int) (int8_t) ((A.w >> 16) & 0xffu) + 127) << 23);
+			const float gx = __uint_as_float(A.x), gy = __uint_as_float(A.y), gz = __uint_as_float(A.z);
+			const uint32_t valid = A.w >> 24;
+			const uint32_t qnx = nX ? B.y : B.x, qfx = nX ? B.x : B.y;
+			const uint32_t qny = nY == 48u ? B.w : B.z, qfy = nY == 48u ? B.z : B.w;
+			const uint32_t qnz = nZ == 80u ? C.y : C.x, qfz = nZ == 80u ? C.x : C.y;
+#define KRR_DEQ(w, c, sc, g) fmaf((float) (((w) >> (8 * (c))) & 0xffu), sc, g)
+#define KRR_QSLAB(kc, c)                                                                                             \
+	{                                                                                                                \
+		float t0 = fmaxf(fmaxf((KRR_DEQ(qnx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qny, c, sy, gy) - o.y) * iy),           \
+						 fmaxf((KRR_DEQ(qnz, c, sz, gz) - o.z) * iz, 0.0f));                                         \
+		float t1 = fminf(fminf((KRR_DEQ(qfx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qfy, c, sy, gy) - o.y) * iy),           \
+						 (KRR_DEQ(qfz, c, sz, gz) - o.z) * iz) *                                                     \
+				   kWiden;                                                                                           \
+		t1 = fminf(t1, tlim);                                                                                        \
+		kc = (t0 <= t1 && ((valid >> (c)) & 1u)) ? t0 : __builtin_inff();                                            \
+	}
+			KRR_QSLAB(k0, 0)
+			KRR_QSLAB(k1, 1)
+			KRR_QSLAB(k2, 2)
+			KRR_QSLAB(k3, 3)
+#undef KRR_QSLAB
+#undef KRR_DEQ
+			c0 = (int) C.z;
+			c1 = (int) C.w;
+			c2 = (int) D.x;
+			c3 = (int) D.y;
+		} else {
+			const uint32_t nb = (uint32_t) entry * 128u; // node array < 4 GiB (checked at build)
+#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
+			const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
+			const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
+#undef KRR_PLANE
+			const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
 #define KRR_SLAB(kc, comp)                                                                                           \
 	{                                                                                                                \
 		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
@@ -289,13 +327,17 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 		t1		 = fminf(t1, tlim);                                                                                  \
 		kc		 = t0 <= t1 ? t0 : __builtin_inff();                                                                 \
 	}
-		KRR_SLAB(k0, x)
-		KRR_SLAB(k1, y)
-		KRR_SLAB(k2, z)
-		KRR_SLAB(k3, w)
+			KRR_SLAB(k0, x)
+			KRR_SLAB(k1, y)
+			KRR_SLAB(k2, z)
+			KRR_SLAB(k3, w)
 #undef KRR_SLAB
+			c0 = ch.x;
+			c1 = ch.y;
+			c2 = ch.z;
+			c3 = ch.w;
+		}
 		// Hit children, nearest first.
-		int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
 		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
 #define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
 	if (kb < ka) {                                                                                                   \

@@ -1224,6 +1224,8 @@ struct DevBuf {
 
 struct AccelData {
 	DevBuf nodes, tris;
+	DevBuf qnodes;			 // the same tree as Bvh4QNode (64 B per node)
+	bool quantized = false; // traversal reads qnodes ("node_format": "quantized") or nodes ("float", the default)
 	DevBuf alphaTri, alphaSrc, alphaImg; // alpha test data; empty when no material has one
 	int stackSize = 0;
 	int stackLds  = kShortStack; // stack entries per lane kept in LDS (the rest spill to global memory)
@@ -1235,7 +1237,7 @@ struct AccelData {
 	int ldsEntries() const { return std::min(stackSize, stackLds); }
 	TraceBvh dev() const {
 		return TraceBvh{nodes.as<Bvh4Node>(), tris.as<LeafTri>(), ldsEntries(), (int) std::min<int64_t>(INT32_MAX, nNodes + nLeafTris),
-						alphaTri.as<AlphaTri>(), alphaSrc.as<AlphaSrc>()};
+						alphaTri.as<AlphaTri>(), alphaSrc.as<AlphaSrc>(), quantized ? qnodes.as<Bvh4QNode>() : nullptr};
 	}
 };
 
@@ -1298,6 +1300,78 @@ int uploadAlpha(const krr_scene_desc *s, const std::vector<uint8_t> &flags, Acce
 	return KRR_OK;
 }
 
+} // namespace
+
+// Quantised copy of a float node (Bvh4QNode): per axis, the grid spacing
+// 2^e is the smallest that spans the children's extent in 254 steps and keeps
+// every plane org + q 2^e (q < 256, org a multiple of 2^e) exact in binary32;
+// planes are rounded outward and then checked, so each decoded child box
+// contains its float box.  The arithmetic is on doubles, where these values
+// and the differences taken are exact.
+__device__ void quantizeNode(const Bvh4Node &nd, Bvh4QNode &q) {
+	const float *lo[3] = {nd.lox, nd.loy, nd.loz}, *hi[3] = {nd.hix, nd.hiy, nd.hiz};
+	uint32_t valid = 0;
+	for (int c = 0; c < 4; c++)
+		if (nd.lox[c] <= nd.hix[c] && nd.loy[c] <= nd.hiy[c] && nd.loz[c] <= nd.hiz[c]) valid |= 1u << c;
+	for (int a = 0; a < 3; a++) {
+		double L = 0.0, H = 0.0;
+		bool any = false;
+		for (int c = 0; c < 4; c++)
+			if ((valid >> c) & 1u) {
+				L	= any ? fmin(L, (double) lo[a][c]) : (double) lo[a][c];
+				H	= any ? fmax(H, (double) hi[a][c]) : (double) hi[a][c];
+				any = true;
+			}
+		const double m = fmax(fabs(L), fabs(H));
+		int e		   = -126;
+		if (m > 0.0) e = max(e, ilogb(m) - 22);
+		if (H > L) e = max(e, (int) ceil(log2((H - L) / 254.0)));
+		double sc = ldexp(1.0, e), org = floor(L / sc) * sc;
+		while ((H - org) / sc > 255.0 || (double) (float) org != org) { // spacing too fine for the extent or the origin
+			e++;
+			sc	= ldexp(1.0, e);
+			org = floor(L / sc) * sc;
+		}
+		q.org[a] = (float) org;
+		q.e[a]	 = (int8_t) e;
+		uint32_t wl = 0, wh = 0;
+		for (int c = 0; c < 4; c++) {
+			int ql = 255, qh = 0; // unused slot: an inverted box, also masked by `valid`
+			if ((valid >> c) & 1u) {
+				ql = (int) fmin(255.0, fmax(0.0, floor(((double) lo[a][c] - org) / sc)));
+				qh = (int) fmin(255.0, fmax(0.0, ceil(((double) hi[a][c] - org) / sc)));
+				while (ql > 0 && org + ql * sc > (double) lo[a][c]) ql--;
+				while (qh < 255 && org + qh * sc < (double) hi[a][c]) qh++;
+			}
+			wl |= (uint32_t) ql << (8 * c);
+			wh |= (uint32_t) qh << (8 * c);
+		}
+		q.q[2 * a]	   = wl;
+		q.q[2 * a + 1] = wh;
+	}
+	q.valid = (uint8_t) valid;
+	for (int c = 0; c < 4; c++) q.child[c] = nd.child[c];
+	q.pad[0] = q.pad[1] = 0;
+}
+
+__global__ void k_quantize_nodes(const Bvh4Node *in, int64_t n, Bvh4QNode *out) {
+	const int64_t i = blockIdx.x * (int64_t) blockDim.x + threadIdx.x;
+	if (i < n) quantizeNode(in[i], out[i]);
+}
+
+namespace {
+
+int quantizeAccel(AccelData &a) {
+	KRR_HIP(a.qnodes.alloc((size_t) std::max<int64_t>(a.nNodes, 1) * sizeof(Bvh4QNode)));
+	if (a.nNodes > 0) {
+		hipLaunchKernelGGL(k_quantize_nodes, dim3((unsigned) ((a.nNodes + 255) / 256)), dim3(256), 0, 0, a.nodes.as<Bvh4Node>(), a.nNodes,
+						   a.qnodes.as<Bvh4QNode>());
+		KRR_HIP(hipGetLastError());
+		KRR_HIP(hipDeviceSynchronize());
+	}
+	return KRR_OK;
+}
+
 int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCost, bool onDevice) {
 	if (!s || s->n_triangles < 0 || (s->n_triangles > 0 && (!s->positions || !s->indices)))
 		return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing triangle arrays");
@@ -1343,7 +1417,7 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCo
 		a.nLeafTris	   = d.nTris;
 		a.buildSeconds = d.buildSeconds;
 		if (a.stackSize > 96) return krr::fail(KRR_ERR_CAPACITY, "BVH depth %d needs a %d-entry stack (max 96)", a.maxDepth, a.stackSize);
-		return KRR_OK;
+		return quantizeAccel(a);
 	}
 	buildBvh4(s->positions, s->indices, s->n_triangles, nullFlag.data(), opt, a.host);
 	// traversal addresses nodes with 32-bit byte offsets
@@ -1361,7 +1435,7 @@ int buildAccel(const krr_scene_desc *s, AccelData &a, int leafSize, float nodeCo
 	a.host.tris.clear();
 	a.host.tris.shrink_to_fit();
 	if (a.stackSize > 96) return krr::fail(KRR_ERR_CAPACITY, "BVH depth %d needs a %d-entry stack (max 96)", a.maxDepth, a.stackSize);
-	return KRR_OK;
+	return quantizeAccel(a);
 }
 
 // Blocks that fit on the GPU at once for this kernel and launch shape
@@ -1405,6 +1479,7 @@ struct krr_wf {
 	int stackLdsEntries = kShortStack;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
+	bool quantizedNodes = false; // "node_format": "float" (default) | "quantized"
 	// neural radiance cache ("nrc" params)
 	bool nrcOn = false;
 	int nrcTrainMod = 16, nrcBatch = 65536, nrcSteps = 4;
@@ -1538,6 +1613,9 @@ int parseParams(krr_wf *wf, const char *json) {
 		const std::string bb = j.value("bvh_builder", "device");
 		if (bb != "device" && bb != "host") throw std::runtime_error("bvh_builder must be \"device\" or \"host\"");
 		wf->deviceBuilder = bb == "device";
+		const std::string nf = j.value("node_format", "float");
+		if (nf != "quantized" && nf != "float") throw std::runtime_error("node_format must be \"float\" or \"quantized\"");
+		wf->quantizedNodes = nf == "quantized";
 		wf->useGraph		= j.value("graph", true);
 	} catch (const std::exception &e) {
 		return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
@@ -1782,6 +1860,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	if (!texs.empty()) KRR_HIP(wf->textures.upload(texs.data(), texs.size() * sizeof(DevTexture)));
 	else wf->textures.release();
 	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost, wf->deviceBuilder);
+	wf->accel.quantized = wf->quantizedNodes;
 	if (rc) return rc;
 	DevScene &ds	= wf->ds;
 	ds.pos			= wf->pos.as<float>();
@@ -1914,7 +1993,7 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	*out = nullptr;
 	int leafSize = 4, stackLdsEntries = kShortStack;
 	float nodeCost = kSahNodeCost;
-	bool deviceBuilder = true;
+	bool deviceBuilder = true, quantized = false;
 	if (json_opts && *json_opts) {
 		try {
 			Json j			= Json::parse(json_opts);
@@ -1922,6 +2001,9 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 			const std::string b = j.value("builder", "device");
 			if (b != "device" && b != "host") throw std::runtime_error("builder must be \"device\" or \"host\"");
 			deviceBuilder = b == "device";
+			const std::string nf = j.value("node_format", "float");
+			if (nf != "quantized" && nf != "float") throw std::runtime_error("node_format must be \"float\" or \"quantized\"");
+			quantized		= nf == "quantized";
 			stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 			nodeCost		= (float) j.value("sah_node_cost", (double) kSahNodeCost);
 		} catch (const std::exception &e) {
@@ -1938,7 +2020,8 @@ int krr_accel_build(const krr_scene_desc *scene, const char *json_opts, krr_acce
 	auto a = std::make_unique<krr_accel>();
 	int rc = buildAccel(scene, a->a, leafSize, nodeCost, deviceBuilder);
 	if (rc) return rc;
-	a->a.stackLds = stackLdsEntries;
+	a->a.stackLds  = stackLdsEntries;
+	a->a.quantized = quantized;
 	KRR_HIP(a->counters.alloc(sizeof(Counters)));
 	KRR_HIP(hipMemset(a->counters.p, 0, sizeof(Counters)));
 	*out = a.release();
@@ -2116,6 +2199,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	j.set("occupancy", Json((double) wf->occupancy));
 	j.set("bvh_builder", Json(std::string(wf->deviceBuilder ? "device" : "host")));
+	j.set("node_format", Json(std::string(wf->quantizedNodes ? "quantized" : "float")));
 	j.set("graph", Json(wf->useGraph));
 	if (wf->nrcOn) {
 		Json n = Json::object();
@@ -2896,7 +2980,7 @@ int parseMkParams(krr_mk *mk, const char *json, std::string &inner) {
 			mk->blockSize	= (int) j.value("block_size", 256.0);
 			mk->occupancy	= (float) j.value("occupancy", 1.0);
 			mk->timeKernels = j.value("time_kernels", false);
-			for (const char *k : {"leaf_size", "sah_node_cost", "stack_lds_entries", "bvh_builder"})
+			for (const char *k : {"leaf_size", "sah_node_cost", "stack_lds_entries", "bvh_builder", "node_format"})
 				if (j.contains(k)) in.set(k, j.at(k));
 		} catch (const std::exception &e) {
 			return krr::fail(KRR_ERR_BAD_CONFIG, "pass params: %s", e.what());
@@ -2941,6 +3025,7 @@ int krr_mk_get_params(const krr_mk *mk, char *buf, int32_t cap) {
 	j.set("sah_node_cost", Json((double) mk->wf->sahNodeCost));
 	j.set("stack_lds_entries", Json((double) mk->wf->stackLdsEntries));
 	j.set("bvh_builder", Json(std::string(mk->wf->deviceBuilder ? "device" : "host")));
+	j.set("node_format", Json(std::string(mk->wf->quantizedNodes ? "quantized" : "float")));
 	std::string s = j.dump();
 	if (buf && cap > 0) {
 		size_t n = std::min((size_t) cap - 1, s.size());

@@ -161,3 +161,37 @@ def test_repeated_calls_of_varying_size(krr, oracle, gpu):
         assert np.array_equal(h[:n, 0][o["prim"][:n] >= 0], o["t"][:n][o["prim"][:n] >= 0]), n
         assert not h[n:].any(), f"call with n={n} wrote past its rays"
     acc.close()
+
+
+@pytest.mark.parametrize("node_format", ["quantized", "float"])
+def test_node_formats_on_hard_rays(krr, oracle, gpu, node_format):
+    """Both node formats (128-byte float nodes, the default, and 64-byte
+    quantised nodes) give the brute-force hits on rays that stress the box test:
+    axis-aligned directions (zero direction components), origins 10^4 away,
+    and origins on the triangles' own planes (t near 0)."""
+    print(f"case: hard rays, node_format={node_format}", flush=True)
+    s = krr.Scene(icosphere_scene(count=64, level=2, res=(16, 16)))
+    pos, idx, _ = s.arrays()
+    rng = np.random.default_rng(7)
+    n = 6000
+    rays = np.zeros((3 * n, 8), np.float32)
+    # axis-aligned: one of +-x, +-y, +-z, from points inside the scene box
+    axis = rng.integers(0, 3, n)
+    sign = rng.choice([-1.0, 1.0], n)
+    rays[:n, 0:3] = rng.random((n, 3)) * np.array([1.0, 0.3, 1.0]) + np.array([0.0, 0.0, 0.0])
+    rays[np.arange(n), 4 + axis] = sign
+    # far origins aimed at triangle points
+    far = _rays_towards_triangles(pos, idx, n, rng, np.array([-1e4, -1e4, -1e4]), np.array([1e4, 1e4, 1e4]), jitter=0.0)
+    rays[n:2 * n] = far
+    # origins on triangle points, leaving in random directions
+    tri = rng.integers(0, len(idx), n)
+    b = rng.random((n, 3)).astype(np.float32)
+    b /= b.sum(1, keepdims=True)
+    rays[2 * n:, 0:3] = pos[idx[tri, 0]] * b[:, :1] + pos[idx[tri, 1]] * b[:, 1:2] + pos[idx[tri, 2]] * b[:, 2:3]
+    rays[2 * n:, 4:7] = rng.standard_normal((n, 3))
+    rays[:, 3] = np.inf
+    g, vis, _ = _gpu_closest(krr, gpu, s, rays, {"node_format": node_format})
+    o = oracle.trace_closest(s.desc, rays)
+    _assert_same_hits(g, o)
+    assert np.array_equal(vis, oracle.trace_any(s.desc, rays))
+    assert (o["prim"] >= 0).mean() > 0.2
