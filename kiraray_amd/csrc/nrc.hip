@@ -129,9 +129,9 @@ __device__ __forceinline__ float16v zero16() {
 
 __device__ __forceinline__ float16v mfma(short4v a, short4v b, float16v c) { return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0); }
 
-// A operand: 4 consecutive bf16 at row `row` (64 bf16 per row), column 8s + 4g.
-__device__ __forceinline__ short4v ldsA(const uint16_t *m, int row, int s, int g, int rowLen = 64) {
-	return *reinterpret_cast<const short4v *>(m + row * rowLen + 8 * s + 4 * g);
+// A operand: 4 consecutive bf16 at row `row` (rows `stride` bf16 apart), column 8s + 4g.
+__device__ __forceinline__ short4v ldsA(const uint16_t *m, int row, int s, int g, int stride = kRowH) {
+	return *reinterpret_cast<const short4v *>(m + row * stride + 8 * s + 4 * g);
 }
 
 // One hidden layer, forward: out (B layout, relu, bf16) = W^T x.
@@ -161,11 +161,22 @@ __device__ __forceinline__ float16v outputForward(const uint16_t *wt5, int lane,
 	return acc;
 }
 
-// Copies the bf16 weights (n elements, multiple of 8) into LDS.
-__device__ void loadWeights(uint16_t *lds, const uint16_t *src, int n) {
-	const uint4 *s = reinterpret_cast<const uint4 *>(src);
-	uint4 *d	   = reinterpret_cast<uint4 *>(lds);
-	for (int i = threadIdx.x; i < n / 8; i += blockDim.x) d[i] = s[i];
+// Copies `rows` dense rows of rowLen bf16 into LDS rows of rowLen + kPad.
+__device__ void loadRows(uint16_t *lds, const uint16_t *src, int rows, int rowLen) {
+	const int per = rowLen / 4; // 8-byte pieces per row
+	for (int i = threadIdx.x; i < rows * per; i += blockDim.x) {
+		const int r = i / per, c = i - r * per;
+		*reinterpret_cast<uint2 *>(lds + r * (rowLen + kPad) + 4 * c) = *reinterpret_cast<const uint2 *>(src + r * rowLen + 4 * c);
+	}
+}
+
+// Forward weights (W^T of every layer, output rows padded to 32) into LDS.
+__device__ void loadWeights(uint16_t *lds, const uint16_t *wtAll) { loadRows(lds, wtAll, kHidden * 64 + 32, 64); }
+
+// Backward weights (W of every layer, output columns padded to 32) into LDS.
+__device__ void loadWeightsBwd(uint16_t *lds, const uint16_t *wbAll) {
+	loadRows(lds, wbAll, kHidden * 64, 64);
+	loadRows(lds + kHidden * 64 * kRowH, wbAll + kHidden * 4096, 64, 32);
 }
 
 // ----------------------------------------------------------------- kernels
@@ -182,7 +193,7 @@ __global__ void __launch_bounds__(256) k_encode(const float *vertices, int64_t n
 // Inference: each wave evaluates tiles of 32 samples.
 __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const float *vertices, int64_t n, float *radiance) {
 	extern __shared__ uint16_t ldsW[];
-	loadWeights(ldsW, wtAll, kFwdElems);
+	loadWeights(ldsW, wtAll);
 	__syncthreads();
 	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
 	const int wavesPerBlock = blockDim.x >> 6;
@@ -210,11 +221,11 @@ __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const floa
 		// not unrolled: an unrolled loop hoists every layer's weight loads
 #pragma unroll 1
 		for (int l = 0; l < kHidden; l++) {
-			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
+			hiddenForward(ldsW + l * 64 * kRowH, lane, g, x, y, a0, a1);
 #pragma unroll
 			for (int s = 0; s < 8; s++) x[s] = y[s];
 		}
-		const float16v o = outputForward(ldsW + kHidden * 4096, lane, g, x);
+		const float16v o = outputForward(ldsW + kHidden * 64 * kRowH, lane, g, x);
 		if (g == 0 && i < n) {
 			radiance[3 * i]		= o[0];
 			radiance[3 * i + 1] = o[1];
@@ -228,7 +239,7 @@ __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const floa
 __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, const float4 *queries, int stride, const int *countPtr, int capacity,
 													 float4 *L) {
 	extern __shared__ uint16_t ldsW[];
-	loadWeights(ldsW, wtAll, kFwdElems);
+	loadWeights(ldsW, wtAll);
 	__syncthreads();
 	const int n	   = min(max(*countPtr, 0), capacity);
 	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
@@ -259,11 +270,11 @@ __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, cons
 		// not unrolled: an unrolled loop hoists every layer's weight loads
 #pragma unroll 1
 		for (int l = 0; l < kHidden; l++) {
-			hiddenForward(ldsW + l * 4096, lane, g, x, y, a0, a1);
+			hiddenForward(ldsW + l * 64 * kRowH, lane, g, x, y, a0, a1);
 #pragma unroll
 			for (int s = 0; s < 8; s++) x[s] = y[s];
 		}
-		const float16v o = outputForward(ldsW + kHidden * 4096, lane, g, x);
+		const float16v o = outputForward(ldsW + kHidden * 64 * kRowH, lane, g, x);
 		if (g == 0 && i < n) {
 			const int pixel = __float_as_int(pay.w);
 			float4 acc		= L[pixel];
@@ -290,6 +301,7 @@ __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int
 // Forward + backward of one tile of 32 samples per wave.  Writes X_l^T and
 // dZ_l^T (bf16, feature-major, leading dimension ld = padded batch) and the
 // tile's loss sum.  Samples i >= n contribute nothing.
+template <bool BWD_LDS>
 __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const uint16_t *wAll, const float *vertices, const float *targets,
 													int64_t nHost, const int *nDev, int64_t capTotal, int step, int steps, int64_t nTiles,
 													int64_t ld, float lossEps, uint16_t *xT, uint16_t *dzT, float *tileLoss, int *adamStep) {
@@ -306,12 +318,18 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 		vertices += off * 16;
 		targets += off * 3;
 	}
-	// forward A operands from LDS; backward ones straight from global memory
-	// (45 KB, cache resident), which keeps the block within 64 KB of LDS
+	// A operands from LDS; with BWD_LDS false (a device whose blocks cannot
+	// hold both copies) the backward ones come straight from global memory
+	// (45 KB, cache resident), unpadded
 	extern __shared__ uint16_t lds[];
-	uint16_t *ldsWt		= lds;
+	uint16_t *ldsWt = lds;
+	loadWeights(ldsWt, wtAll);
 	const uint16_t *ldsW = wAll;
-	loadWeights(ldsWt, wtAll, kFwdElems);
+	constexpr int sH = BWD_LDS ? kRowH : 64, sO = BWD_LDS ? kRowO : 32;
+	if (BWD_LDS) {
+		loadWeightsBwd(lds + kFwdLdsElems, wAll);
+		ldsW = lds + kFwdLdsElems;
+	}
 	__syncthreads();
 	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
 	const int wavesPerBlock = blockDim.x >> 6;
@@ -344,7 +362,7 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 #pragma unroll 1
 		for (int l = 0; l < kHidden; l++) {
 			storeFeatureMajor(xT + (int64_t) l * 64 * ld, ld, i, g, x);
-			hiddenForward(ldsWt + l * 4096, lane, g, x, y, a0, a1);
+			hiddenForward(ldsWt + l * 64 * kRowH, lane, g, x, y, a0, a1);
 			uint32_t mk = 0;
 #pragma unroll
 			for (int s = 0; s < 8; s++) {
@@ -355,7 +373,7 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 			mask[l + 1] = mk;
 		}
 		storeFeatureMajor(xT + (int64_t) kHidden * 64 * ld, ld, i, g, x);
-		const float16v o = outputForward(ldsWt + kHidden * 4096, lane, g, x);
+		const float16v o = outputForward(ldsWt + kHidden * 64 * kRowH, lane, g, x);
 		// relative L2 loss and its gradient (lanes g == 0 hold the 3 outputs)
 		float d[3]	 = {0.f, 0.f, 0.f};
 		float lsum	 = 0.f;
@@ -380,13 +398,13 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 		// backward through the output layer: dX_5^T = W_5 dZ_5^T (K = 32 padded outputs)
 		const int r0 = lane & 31;
 		{
-			const uint16_t *w5 = ldsW + kHidden * 4096;
+			const uint16_t *w5 = ldsW + kHidden * 64 * sH;
 			a0 = zero16();
 			a1 = zero16();
 #pragma unroll
 			for (int s = 0; s < 4; s++) {
-				a0 = mfma(ldsA(w5, r0, s, g, 32), dz[s], a0);
-				a1 = mfma(ldsA(w5, 32 + r0, s, g, 32), dz[s], a1);
+				a0 = mfma(ldsA(w5, r0, s, g, sO), dz[s], a0);
+				a1 = mfma(ldsA(w5, 32 + r0, s, g, sO), dz[s], a1);
 			}
 		}
 #pragma unroll 1
@@ -402,13 +420,13 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 				}
 			storeFeatureMajor(dzT + (int64_t) l * 64 * ld, ld, i, g, dz);
 			if (l > 0) {
-				const uint16_t *w = ldsW + l * 4096;
+				const uint16_t *w = ldsW + l * 64 * sH;
 				a0				  = zero16();
 				a1				  = zero16();
 #pragma unroll
 				for (int s = 0; s < 8; s++) {
-					a0 = mfma(ldsA(w, r0, s, g), dz[s], a0);
-					a1 = mfma(ldsA(w, 32 + r0, s, g), dz[s], a1);
+					a0 = mfma(ldsA(w, r0, s, g, sH), dz[s], a0);
+					a1 = mfma(ldsA(w, 32 + r0, s, g, sH), dz[s], a1);
 				}
 			}
 		}
@@ -517,16 +535,35 @@ __device__ __forceinline__ void adamUpdate(float *w, float *mo, float *ve, float
 
 // One training step's tail in one launch: per weight the fixed-order chunk
 // sum (the gradient, also stored), the Adam update and the bf16 copies; the
-// last block also totals the loss.  Same arithmetic, in the same order, as
-// k_reduce_chunks + k_reduce_loss followed by the update.  The step count was
-// raised by k_train_tile.
+// last block also totals the loss.  The step count was raised by
+// k_train_tile.
+//
+// A block of 256 threads takes 32 weights: thread (group q, weight j) sums
+// the chunks of its eighth of the chunk range in order, and the eight group
+// sums are added in group order (a fixed order: deterministic).  One thread
+// per weight would leave most of the chip idle on a few hundred
+// latency-bound sequential loads.
+constexpr int kReduceWeights = 32, kReduceGroups = 8;
 __global__ void __launch_bounds__(256) k_reduce_adam(const float *partialAll, int chunks, float *grad, float *w, float *mo, float *ve,
 													 float lr, float b1, float b2, float eps, const int *tDev, uint16_t *wt, uint16_t *wb,
 													 const float *tileLoss, int64_t nTiles, float *loss) {
-	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	__shared__ float part[kReduceGroups][kReduceWeights];
+	const int j = threadIdx.x % kReduceWeights, q = threadIdx.x / kReduceWeights;
+	const int p = blockIdx.x * kReduceWeights + j;
+	float s		= 0.f;
 	if (p < KRR_NRC_PARAMS) {
-		const float gr = chunkSum(partialAll, chunks, p);
-		grad[p]		   = gr;
+		const int layer	 = min(p / 4096, kHidden);
+		const int count	 = layer < kHidden ? 4096 : 64 * 3;
+		const float *src = partialAll + (int64_t) layer * chunks * 4096 + (p - layer * 4096);
+		const int c0 = q * chunks / kReduceGroups, c1 = (q + 1) * chunks / kReduceGroups;
+		for (int c = c0; c < c1; c++) s += src[(int64_t) c * count];
+	}
+	part[q][j] = s;
+	__syncthreads();
+	if (q == 0 && p < KRR_NRC_PARAMS) {
+		float gr = part[0][j];
+		for (int k = 1; k < kReduceGroups; k++) gr += part[k][j];
+		grad[p] = gr;
 		adamUpdate(w, mo, ve, gr, lr, b1, b2, eps, (float) *tDev, wt, wb, p);
 	}
 	if (blockIdx.x == gridDim.x - 1) reduceLoss(tileLoss, nTiles, loss);
@@ -573,6 +610,7 @@ struct krr_nrc {
 	float lr = 1e-3f, beta1 = 0.9f, beta2 = 0.99f, eps = 1e-8f, lossEps = 0.01f;
 	uint64_t seed	= 1;
 	int64_t maxBatch = 65536, ld = 0;
+	bool bwdLds		 = false; // the device's blocks hold both padded weight copies in LDS
 	Buf w, m, v, grad, wt, wb, xT, dzT, partial, tileLoss, loss, step;
 };
 
@@ -624,9 +662,11 @@ int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, i
 				 int64_t capTotal = 0, int step = 0, int steps = 1) {
 	const int64_t chunks = (n + kChunk - 1) / kChunk;
 	const int64_t nTiles = chunks * (kChunk / 32);
-	const int lds		 = kFwdElems * (int) sizeof(uint16_t);
+	const bool bwdLds	 = h->bwdLds;
+	const int lds		 = (kFwdLdsElems + (bwdLds ? kBwdLdsElems : 0)) * (int) sizeof(uint16_t);
 	const int blocks	 = (int) std::min<int64_t>(1024, (nTiles + 3) / 4);
-	hipLaunchKernelGGL(k_train_tile, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), (const float *) vert,
+	hipLaunchKernelGGL(bwdLds ? k_train_tile<true> : k_train_tile<false>, dim3(blocks), dim3(256), lds, st, h->wt.as<uint16_t>(),
+					   h->wb.as<uint16_t>(), (const float *) vert,
 					   targets, n, nDev, capTotal, step, steps, nTiles, h->ld, h->lossEps, h->xT.as<uint16_t>(), h->dzT.as<uint16_t>(),
 					   h->tileLoss.as<float>(), adam ? h->step.as<int>() : nullptr);
 	NRC_HIP(hipGetLastError());
@@ -635,7 +675,8 @@ int gradientImpl(krr_nrc *h, const krr_nrc_vertex *vert, const float *targets, i
 					   h->ld, kWChunk, wchunks, h->partial.as<float>());
 	NRC_HIP(hipGetLastError());
 	if (adam) {
-		hipLaunchKernelGGL(k_reduce_adam, dim3((KRR_NRC_PARAMS + 255) / 256), dim3(256), 0, st, h->partial.as<float>(), wchunks,
+		hipLaunchKernelGGL(k_reduce_adam, dim3((KRR_NRC_PARAMS + kReduceWeights - 1) / kReduceWeights), dim3(kReduceWeights * kReduceGroups), 0,
+						   st, h->partial.as<float>(), wchunks,
 						   h->grad.as<float>(), h->w.as<float>(), h->m.as<float>(), h->v.as<float>(), h->lr, h->beta1, h->beta2, h->eps,
 						   (const int *) h->step.as<int>(), h->wt.as<uint16_t>(), h->wb.as<uint16_t>(), h->tileLoss.as<float>(), nTiles,
 						   h->loss.as<float>());
@@ -685,6 +726,12 @@ int krr_nrc_create(const char *json, krr_nrc **out) {
 	NRC_HIP(h->m.alloc(sizeof(float) * KRR_NRC_PARAMS));
 	NRC_HIP(h->v.alloc(sizeof(float) * KRR_NRC_PARAMS));
 	NRC_HIP(h->grad.alloc(sizeof(float) * KRR_NRC_PARAMS));
+	{
+		int dev = 0, maxLds = 0;
+		NRC_HIP(hipGetDevice(&dev));
+		NRC_HIP(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+		h->bwdLds = maxLds >= (kFwdLdsElems + kBwdLdsElems) * (int) sizeof(uint16_t);
+	}
 	NRC_HIP(h->wt.alloc(sizeof(uint16_t) * kFwdElems));
 	NRC_HIP(h->wb.alloc(sizeof(uint16_t) * kBwdElems));
 	NRC_HIP(h->xT.alloc(sizeof(uint16_t) * 64 * (kHidden + 1) * h->ld));
@@ -741,7 +788,7 @@ int krr_nrc_infer(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, int64_t n, f
 	if (n == 0) return KRR_OK;
 	const int64_t tiles = (n + 31) / 32;
 	const int blocks	= (int) std::min<int64_t>(2048, (tiles + 3) / 4);
-	hipLaunchKernelGGL(k_infer, dim3(blocks), dim3(256), kFwdElems * (int) sizeof(uint16_t), (hipStream_t) hip_stream, nrc->wt.as<uint16_t>(),
+	hipLaunchKernelGGL(k_infer, dim3(blocks), dim3(256), kFwdLdsElems * (int) sizeof(uint16_t), (hipStream_t) hip_stream, nrc->wt.as<uint16_t>(),
 					   (const float *) vertices_dev, n, radiance_dev);
 	NRC_HIP(hipGetLastError());
 	return KRR_OK;
@@ -767,7 +814,7 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const int *count, int capacity, float4 *L, hipStream_t st) {
 	if (capacity <= 0) return KRR_OK;
 	const int blocks = std::max(1, std::min(1024, (capacity + 127) / 128));
-	hipLaunchKernelGGL(k_infer_queue, dim3(blocks), dim3(256), kFwdElems * (int) sizeof(uint16_t), st, nrc->wt.as<uint16_t>(), queries, strideF4,
+	hipLaunchKernelGGL(k_infer_queue, dim3(blocks), dim3(256), kFwdLdsElems * (int) sizeof(uint16_t), st, nrc->wt.as<uint16_t>(), queries, strideF4,
 					   count, capacity, L);
 	NRC_HIP(hipGetLastError());
 	return KRR_OK;

@@ -15,6 +15,15 @@ constexpr int kHidden = KRR_NRC_HIDDEN_LAYERS; // hidden layers of width 64 (= m
 // columns)
 constexpr int kFwdElems = kHidden * 64 * 64 + 32 * 64;
 constexpr int kBwdElems = kHidden * 64 * 64 + 64 * 32;
+// The copies in LDS pad every row by 4 bf16 (8 bytes): the 32 lanes that read
+// an A operand read 32 consecutive rows, and unpadded 128-byte rows would put
+// all of them on one bank.  With the padding consecutive rows start 2 banks
+// apart, so a wave's 8-byte reads spread over every bank.
+constexpr int kPad			= 4;
+constexpr int kRowH			= 64 + kPad; // LDS row of a 64-wide matrix
+constexpr int kRowO			= 32 + kPad; // LDS row of the backward output matrix (32 padded columns)
+constexpr int kFwdLdsElems = (kHidden * 64 + 32) * kRowH;
+constexpr int kBwdLdsElems = kHidden * 64 * kRowH + 64 * kRowO;
 
 typedef short short4v __attribute__((ext_vector_type(4)));	// 4 x bf16 (bit patterns)
 typedef float float16v __attribute__((ext_vector_type(16))); // 32x32 accumulator slice of one lane
