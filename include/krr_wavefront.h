@@ -22,7 +22,7 @@
 extern "C" {
 #endif
 
-#define KRR_ABI_VERSION 1
+#define KRR_ABI_VERSION 2
 
 enum krr_status {
 	KRR_OK				  = 0,
@@ -69,6 +69,9 @@ typedef struct krr_material {
 	float transmission[3];		  /* constant transmission when there is no texture; 0 = opaque */
 	int32_t emissive_texture;	  /* 0 = none, else 1 + index into krr_scene_desc.textures: an emissive
 									 mesh of this material emits max(emission) * rgb(uv) (light.h:188-193) */
+	int32_t normal_texture;		  /* 0 = none, else 1 + index into krr_scene_desc.textures: tangent-space
+									 normal map, n = normalize(T x + B y + N z) with (x, y, z) = 2 rgb(uv) - 1,
+									 on meshes with texture coordinates (shading.h:187-203) */
 } krr_material;
 
 /* An RGBA float32 image used as a normalised-coordinate texture: height rows
@@ -91,6 +94,9 @@ typedef struct krr_mesh {
 	float emission[3];	   /* Le; any component > 0 makes every triangle a DiffuseAreaLight (scale = max(Le));
 							  with the material's emissive_texture the light emits scale * rgb(uv) */
 	int32_t has_texcoords;  /* per-vertex texture coordinates in krr_scene_desc.texcoords are valid (else uv = 0) */
+	int32_t has_tangents;	/* per-vertex tangents in krr_scene_desc.tangents are valid: the shading frame's
+							   tangent is the interpolated one, made orthogonal to n (shading.h:140-145);
+							   else getPerpendicular(n) */
 } krr_mesh;
 
 enum krr_medium_type {
@@ -150,6 +156,7 @@ typedef struct krr_scene_desc {
 	const float *texcoords; /* 2 * n_vertices, may be NULL if no mesh has texture coordinates */
 	int32_t n_textures;
 	const krr_texture *textures;
+	const float *tangents; /* 3 * n_vertices, may be NULL if no mesh has tangents */
 } krr_scene_desc;
 
 /* rt::CameraData (src/core/camera.h:20-30).  transform is the camera-to-world

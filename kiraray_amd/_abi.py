@@ -31,7 +31,7 @@ class krr_material(C.Structure):
                 ("ior", C.c_float), ("specular_transmission", C.c_float), ("anisotropic", C.c_float),
                 ("diffuse_texture", C.c_int32), ("specular_texture", C.c_int32),
                 ("transmission_texture", C.c_int32), ("transmission", C.c_float * 3),
-                ("emissive_texture", C.c_int32)]
+                ("emissive_texture", C.c_int32), ("normal_texture", C.c_int32)]
 
 
 class krr_texture(C.Structure):
@@ -41,7 +41,7 @@ class krr_texture(C.Structure):
 class krr_mesh(C.Structure):
     _fields_ = [("first_triangle", C.c_int32), ("n_triangles", C.c_int32), ("material", C.c_int32),
                 ("medium_inside", C.c_int32), ("medium_outside", C.c_int32), ("has_normals", C.c_int32),
-                ("emission", C.c_float * 3), ("has_texcoords", C.c_int32)]
+                ("emission", C.c_float * 3), ("has_texcoords", C.c_int32), ("has_tangents", C.c_int32)]
 
 
 class krr_medium(C.Structure):
@@ -62,7 +62,8 @@ class krr_scene_desc(C.Structure):
                 ("n_materials", C.c_int32), ("materials", C.POINTER(krr_material)),
                 ("n_media", C.c_int32), ("media", C.POINTER(krr_medium)),
                 ("n_infinite_lights", C.c_int32), ("infinite_lights", C.POINTER(krr_infinite_light)),
-                ("texcoords", C.POINTER(C.c_float)), ("n_textures", C.c_int32), ("textures", C.POINTER(krr_texture))]
+                ("texcoords", C.POINTER(C.c_float)), ("n_textures", C.c_int32), ("textures", C.POINTER(krr_texture)),
+                ("tangents", C.POINTER(C.c_float))]
 
 
 class krr_camera(C.Structure):

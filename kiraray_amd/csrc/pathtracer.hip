@@ -57,7 +57,7 @@ KRR_DEV f3 pathTracePixel(const PtArgs &A, const LaneStack &stack, int x, int y,
 				break;
 			}
 			SurfaceInteraction intr;
-			prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, -1, intr);
+			prepareSurfaceInteraction<TEX>(sc, h.prim, h.b1, h.b2, d, -1, intr);
 			if (intr.light >= 0) { // handleHit (device.cu:52-65)
 				const DevLight Lr = sc.lights[intr.light];
 				const f3 Le		  = areaLightL<TEX>(Lr, intr.n, intr.wo, intr.u, intr.v);

@@ -56,7 +56,7 @@
 #include "krr_wavefront.h"
 
 struct krr_scene {
-	std::vector<float> positions, normals, texcoords;
+	std::vector<float> positions, normals, texcoords, tangents;
 	std::vector<int32_t> indices;
 	std::vector<krr_mesh> meshes;
 	std::vector<std::string> meshNames; // one per mesh ("" when unnamed)
@@ -69,7 +69,7 @@ struct krr_scene {
 	std::map<std::string, int32_t> textureByPath; // 1 + index into textures
 	std::vector<krr_camera> cameras;
 	std::vector<std::pair<int32_t, int32_t>> viewSizes;
-	bool anyNormals = false, anyTexcoords = false;
+	bool anyNormals = false, anyTexcoords = false, anyTangents = false;
 };
 
 namespace krr {
@@ -127,10 +127,10 @@ struct Builder {
 		s.materials.push_back(m);
 		return (int) s.materials.size() - 1;
 	}
-	// Appends a mesh; normals may be empty.
+	// Appends a mesh; normals, texture coordinates and tangents may be empty.
 	int addMesh(const std::vector<V> &verts, const std::vector<V> &nrms, const std::vector<int> &tris, int material,
 				V emission = {0, 0, 0}, int mIn = -1, int mOut = -1, const std::string &name = "",
-				const std::vector<float> &uvs = {}) {
+				const std::vector<float> &uvs = {}, const std::vector<V> &tans = {}) {
 		krr_mesh m;
 		std::memset(&m, 0, sizeof(m));
 		int32_t base	 = (int32_t) (s.positions.size() / 3);
@@ -141,6 +141,7 @@ struct Builder {
 		m.medium_outside = mOut;
 		m.has_normals	 = nrms.empty() ? 0 : 1;
 		m.has_texcoords	 = uvs.empty() ? 0 : 1;
+		m.has_tangents	 = tans.empty() ? 0 : 1;
 		m.emission[0]	 = emission.x;
 		m.emission[1]	 = emission.y;
 		m.emission[2]	 = emission.z;
@@ -154,9 +155,14 @@ struct Builder {
 			s.normals.push_back(n.z);
 			s.texcoords.push_back(uvs.empty() ? 0.f : uvs[2 * i]);
 			s.texcoords.push_back(uvs.empty() ? 0.f : uvs[2 * i + 1]);
+			V t = tans.empty() ? V{0, 0, 0} : tans[i];
+			s.tangents.push_back(t.x);
+			s.tangents.push_back(t.y);
+			s.tangents.push_back(t.z);
 		}
 		if (!nrms.empty()) s.anyNormals = true;
 		if (!uvs.empty()) s.anyTexcoords = true;
+		if (!tans.empty()) s.anyTangents = true;
 		for (int t : tris) s.indices.push_back(base + t);
 		s.meshes.push_back(m);
 		s.meshNames.push_back(name);
@@ -315,12 +321,18 @@ void explicitMesh(Builder &b, const Json &j) {
 		if (T.size() != 2 * verts.size()) throw std::runtime_error("mesh: texcoords must hold 2 values per vertex");
 		for (const Json &t : T) uvs.push_back((float) t.asNumber());
 	}
+	std::vector<V> tans;
+	if (j.contains("tangents")) {
+		const auto &T = j.at("tangents").items();
+		if (T.size() != P.size()) throw std::runtime_error("mesh: tangents must match positions");
+		for (size_t i = 0; i < T.size(); i += 3) tans.push_back({(float) T[i].asNumber(), (float) T[i + 1].asNumber(), (float) T[i + 2].asNumber()});
+	}
 	float em[3] = {0, 0, 0};
 	j.vec<3>("emission", em);
 	int mat	 = (int) j.value("material", 0.0);
 	int mIn	 = (int) j.value("medium_inside", -1.0);
 	int mOut = (int) j.value("medium_outside", -1.0);
-	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut, j.value("name", ""), uvs);
+	b.addMesh(verts, nrms, tris, mat, {em[0], em[1], em[2]}, mIn, mOut, j.value("name", ""), uvs, tans);
 }
 
 void quadModel(Builder &b, const Json &j) {
@@ -538,6 +550,10 @@ struct ObjMaterial {
 	float ns = -1.f, ni = -1.f, d = 1.f;
 	bool hasTf = false;
 	std::string mapKd, mapKs, mapKe; // texture files (map_Kd, map_Ks, map_Ke), full paths
+	// the normal map: "norm", else "map_Bump" / "bump", else "disp" (the
+	// reference takes assimp's NORMALS, HEIGHT and DISPLACEMENT slots, in that
+	// order, as its Normal texture: assimp.cpp:48-58,66-90)
+	std::string mapNorm, mapBump, mapDisp;
 };
 
 std::string joinPath(const std::string &dir, const std::string &p) {
@@ -586,6 +602,10 @@ void readMtl(const std::string &path, std::map<std::string, ObjMaterial> &out) {
 			std::string tok, file;
 			while (ss >> tok) file = tok;
 			if (!file.empty()) (k == "map_Kd" ? m.mapKd : k == "map_Ks" ? m.mapKs : m.mapKe) = joinPath(dirOf(path), file);
+		} else if (k == "norm" || k == "map_Bump" || k == "map_bump" || k == "bump" || k == "disp") {
+			std::string tok, file;
+			while (ss >> tok) file = tok;
+			if (!file.empty()) (k == "norm" ? m.mapNorm : k == "disp" ? m.mapDisp : m.mapBump) = joinPath(dirOf(path), file);
 		}
 	}
 }
@@ -683,6 +703,40 @@ NodeXform nodeXform(const Json &m) {
 		for (int j = 0; j < 3; j++) x.L[3 * i + j] = R[3 * i + j] * sc[j];
 	x.t = {tr[0], tr[1], tr[2]};
 	return x;
+}
+
+// Per-vertex tangents of a mesh with texture coordinates, standing for
+// assimp's aiProcess_CalcTangentSpace (assimp.cpp:249) as the reference
+// stores them (assimp.cpp:376-383: made orthogonal to the vertex normal and
+// normalised).  Per triangle the tangent is the direction of increasing u
+// (Lengyel's construction from the edge vectors and texture-coordinate
+// deltas); each vertex sums the tangents of its triangles.  A vertex whose
+// triangles have no usable texture-coordinate gradient gets a unit vector
+// perpendicular to n (n crossed with the axis of n's smallest component).
+std::vector<V> uvTangents(const std::vector<V> &verts, const std::vector<V> &nrms, const std::vector<float> &uvs, const std::vector<int> &tris) {
+	std::vector<V> acc(verts.size(), V{0, 0, 0});
+	for (size_t t = 0; t + 2 < tris.size(); t += 3) {
+		const int a = tris[t], b = tris[t + 1], c = tris[t + 2];
+		const V e1 = sub(verts[b], verts[a]), e2 = sub(verts[c], verts[a]);
+		const float du1 = uvs[2 * b] - uvs[2 * a], dv1 = uvs[2 * b + 1] - uvs[2 * a + 1];
+		const float du2 = uvs[2 * c] - uvs[2 * a], dv2 = uvs[2 * c + 1] - uvs[2 * a + 1];
+		const float det = du1 * dv2 - du2 * dv1;
+		if (det == 0.f || !std::isfinite(det)) continue;
+		const V tg = mul(sub(mul(e1, dv2), mul(e2, dv1)), 1.f / det);
+		for (int v : {a, b, c}) acc[v] = add(acc[v], tg);
+	}
+	std::vector<V> out(verts.size());
+	for (size_t i = 0; i < verts.size(); i++) {
+		const V n = nrms[i];
+		V t		  = sub(acc[i], mul(n, dt(n, acc[i])));
+		if (dt(t, t) <= 1e-20f * std::max(dt(acc[i], acc[i]), 1e-30f) || !std::isfinite(dt(t, t))) {
+			const float ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+			const V e = ax <= ay && ax <= az ? V{1, 0, 0} : ay <= az ? V{0, 1, 0} : V{0, 0, 1};
+			t		  = crs(n, e);
+		}
+		out[i] = nrmz(t);
+	}
+	return out;
 }
 
 // Wavefront OBJ: v, vn, f (v, v/t, v//n, v/t/n, negative indices, polygons
@@ -794,12 +848,15 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			fm.diffuse_texture	= om.mapKd.empty() ? 0 : textureFor(b.s, om.mapKd);
 			fm.specular_texture = om.mapKs.empty() ? 0 : textureFor(b.s, om.mapKs);
 			fm.emissive_texture = om.mapKe.empty() ? 0 : textureFor(b.s, om.mapKe);
+			const std::string &nm = !om.mapNorm.empty() ? om.mapNorm : !om.mapBump.empty() ? om.mapBump : om.mapDisp;
+			fm.normal_texture	  = nm.empty() ? 0 : textureFor(b.s, nm);
 			krr_material mm = fm;
 			if (ov != overrides.end()) { // the JSON entry sets the params; the file's textures stay (krrscene.cpp:199-205)
 				mm = ov->second;
 				if (!mm.diffuse_texture) mm.diffuse_texture = fm.diffuse_texture;
 				if (!mm.specular_texture) mm.specular_texture = fm.specular_texture;
 				if (!mm.emissive_texture) mm.emissive_texture = fm.emissive_texture;
+				if (!mm.normal_texture) mm.normal_texture = fm.normal_texture;
 			}
 			mi				= b.addMaterial(mm);
 			matIndex[r.mat] = mi;
@@ -836,7 +893,7 @@ void loadObj(Builder &b, const std::string &path, const NodeXform &xf, const std
 			}
 			tris.push_back(it->second);
 		}
-		b.addMesh(verts, nrms, tris, mi, emission, -1, -1, r.name, uvs);
+		b.addMesh(verts, nrms, tris, mi, emission, -1, -1, r.name, uvs, haveT ? uvTangents(verts, nrms, uvs, tris) : std::vector<V>{});
 	}
 }
 
@@ -1134,10 +1191,10 @@ void buildScene(krr_scene &s, const Json &j) {
 			krr_material mm = jsonMaterial(m);
 			// this build's keys: image files standing for the diffuse colour and the glossiness
 			const Json &p = m.contains("params") ? m.at("params") : m;
-			for (const char *k : {"diffuse_texture", "specular_texture", "transmission_texture", "emissive_texture"})
+			for (const char *k : {"diffuse_texture", "specular_texture", "transmission_texture", "emissive_texture", "normal_texture"})
 				if (p.contains(k))
-					(k[0] == 'd' ? mm.diffuse_texture : k[0] == 's' ? mm.specular_texture : k[0] == 't' ? mm.transmission_texture : mm.emissive_texture) =
-						textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
+					(k[0] == 'd' ? mm.diffuse_texture : k[0] == 's' ? mm.specular_texture : k[0] == 't' ? mm.transmission_texture
+					 : k[0] == 'e' ? mm.emissive_texture : mm.normal_texture) = textureFor(s, joinPath(j.value("base_dir", ""), p.value(k, "")));
 			// the alpha test's transmission = 1 - opacity (pbrt.cpp:152-156)
 			p.vec<3>("transmission", mm.transmission);
 			s.materials.push_back(mm);
@@ -1267,6 +1324,7 @@ int krr_scene_get_desc(const krr_scene *s, krr_scene_desc *d) {
 	d->texcoords		  = s->anyTexcoords ? s->texcoords.data() : nullptr;
 	d->n_textures		  = (int32_t) s->textures.size();
 	d->textures			  = s->textures.data();
+	d->tangents			  = s->anyTangents ? s->tangents.data() : nullptr;
 	return KRR_OK;
 }
 

@@ -43,6 +43,7 @@ struct DevMaterial {
 	float anisotropic;
 	float diffuse[3];
 	int texDiffuse, texSpecular; // textures replacing diffuse / specular.a (-1 none)
+	int texNormal;				 // tangent-space normal map (-1 none)
 };
 
 struct DevTexture {
@@ -57,6 +58,7 @@ struct DevMesh {
 	int mediumIn, mediumOut; // interface media (-1 none); transition iff different
 	int hasNormals;
 	int hasUV;
+	int hasTangents;
 };
 
 struct DevLight {
@@ -94,6 +96,7 @@ struct DevScene {
 	const DevMedium *media;
 	int nMedia;
 	const float *uv; // 2 per vertex (may be null)
+	const float *tan; // 3 per vertex (may be null)
 	const DevTexture *textures;
 	float sceneRadius;
 	float bmin[3], binv[3]; // scene box: p -> (p - bmin) * binv maps it to [0,1]^3
@@ -657,7 +660,13 @@ struct SurfaceInteraction {
 };
 
 // prepareSurfaceInteraction (src/render/shading.h:115-226) for an identity
-// instance transform; b1, b2 are the barycentrics of vertices 1 and 2.
+// instance transform; b1, b2 are the barycentrics of vertices 1 and 2.  The
+// tangent is the mesh's interpolated tangent made orthogonal to n when the
+// mesh has tangents (shading.h:140-145), else getPerpendicular(n); a normal
+// map (shading.h:187-203) then tilts the frame on meshes with texture
+// coordinates.  TEX = false: no mesh has tangents and no material a normal
+// map (the lookups are compiled out).
+template <bool TEX = true>
 KRR_DEV void prepareSurfaceInteraction(const DevScene &sc, int prim, float b1, float b2, const f3 &rayDir, int rayMedium,
 									   SurfaceInteraction &it) {
 	const DevMesh m = sc.meshes[sc.triMesh[prim]];
@@ -669,7 +678,10 @@ KRR_DEV void prepareSurfaceInteraction(const DevScene &sc, int prim, float b1, f
 	it.p = b0 * p0 + b1 * p1 + b2 * p2;
 	if (m.hasNormals) it.n = normalize(b0 * loadV(sc.nrm, v0) + b1 * loadV(sc.nrm, v1) + b2 * loadV(sc.nrm, v2));
 	else it.n = normalize(cross(p1 - p0, p2 - p0));
-	it.tangent	  = getPerpendicular(it.n);
+	if (TEX && m.hasTangents) {
+		it.tangent = normalize(b0 * loadV(sc.tan, v0) + b1 * loadV(sc.tan, v1) + b2 * loadV(sc.tan, v2));
+		it.tangent = normalize(it.tangent - it.n * dot(it.n, it.tangent));
+	} else it.tangent = getPerpendicular(it.n);
 	it.bitangent  = normalize(cross(it.n, it.tangent));
 	it.light	  = m.lightBase >= 0 ? m.lightBase + (prim - m.firstTri) : -1;
 	it.material	  = m.material;
@@ -683,6 +695,17 @@ KRR_DEV void prepareSurfaceInteraction(const DevScene &sc, int prim, float b1, f
 	if (m.hasUV) { // shading.h:147-149: b0 t0 + b1 t1 + b2 t2
 		it.u = b0 * sc.uv[2 * v0] + b1 * sc.uv[2 * v1] + b2 * sc.uv[2 * v2];
 		it.v = b0 * sc.uv[2 * v0 + 1] + b1 * sc.uv[2 * v1 + 1] + b2 * sc.uv[2 * v2 + 1];
+		if (TEX && it.material >= 0) {
+			const int tn = sc.materials[it.material].texNormal;
+			if (tn >= 0) { // rgbToNormal (shading.h:32): 2 rgb - 1, in the (T, B, N) frame
+				const DevTexture t = sc.textures[tn];
+				const float4 c	   = texBilinear4(t.img, t.w, t.h, it.u, it.v);
+				const float nx = 2.f * c.x - 1.f, ny = 2.f * c.y - 1.f, nz = 2.f * c.z - 1.f;
+				it.n		 = normalize(it.tangent * nx + it.bitangent * ny + it.n * nz);
+				it.tangent	 = normalize(it.tangent - it.n * dot(it.tangent, it.n));
+				it.bitangent = normalize(cross(it.n, it.tangent));
+			}
+		}
 	}
 }
 

@@ -397,7 +397,7 @@ KRR_DEV void shadeCore(const WfParams &P, const DevScene &sc, const RayItem &r, 
 	f3 mscP, mscThp, mscPu, mscWo;
 	float mscG = 0.f;
 	if (prim >= 0) {
-		prepareSurfaceInteraction(sc, prim, h.z, h.w, r.d, r.medium, intr);
+		prepareSurfaceInteraction<!PLAIN>(sc, prim, h.z, h.w, r.d, r.medium, intr);
 		if (MEDIA && P.enableMedium && r.medium >= 0) {
 			hasMS	  = true;
 			msTmax	  = h.x;
@@ -726,7 +726,8 @@ KRR_DEV void transmittanceItem(const WfParams &P, const DevScene &sc, const Trac
 		 crossing++) {
 		const TraceResult h = traverse<false, COUNT, ALPHA>(bvh, o, d, tMax, stack, nv, nt);
 		const bool visible	= h.prim < 0;
-		if (!visible) prepareSurfaceInteraction(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
+		// only a null-material surface lets the ray on, and it has no normal map
+		if (!visible) prepareSurfaceInteraction<false>(sc, h.prim, h.b1, h.b2, d, origMedium, intr);
 		if (!visible && intr.material >= 0) {
 			T_ray = f3();
 			break;
@@ -1517,7 +1518,7 @@ struct krr_wf {
 	AccelData accel;
 	DevBuf pos, nrm, idx, triMesh, meshes, materials, lights, media;
 	std::vector<std::unique_ptr<DevBuf>> mediumBufs, imageBufs, textureBufs;
-	DevBuf uv, textures;
+	DevBuf uv, tan, textures;
 	DevScene ds{};
 	bool sceneHasMedia = false;
 	bool sceneTextured = false; // textures or image-lit infinite lights: the kernels with texture lookups
@@ -1706,6 +1707,8 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		d.hasNormals = m.has_normals ? 1 : 0;
 		if (m.has_normals && !s->normals) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_normals but normals is NULL", mi);
 		d.hasUV = m.has_texcoords ? 1 : 0;
+		if (m.has_tangents && !s->tangents) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_tangents but tangents is NULL", mi);
+		d.hasTangents = m.has_tangents ? 1 : 0;
 		if (m.has_texcoords && !s->texcoords) return krr::fail(KRR_ERR_INVALID_ARG, "scene: mesh %d has_texcoords but texcoords is NULL", mi);
 		const bool emissive = m.emission[0] > 0 || m.emission[1] > 0 || m.emission[2] > 0;
 		d.lightBase			= -1;
@@ -1761,10 +1764,12 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 		mats[i].specTrans	= m.specular_transmission;
 		mats[i].anisotropic = m.anisotropic;
 		for (int a = 0; a < 3; a++) mats[i].diffuse[a] = m.diffuse[a];
-		if (m.diffuse_texture < 0 || m.diffuse_texture > s->n_textures || m.specular_texture < 0 || m.specular_texture > s->n_textures)
+		if (m.diffuse_texture < 0 || m.diffuse_texture > s->n_textures || m.specular_texture < 0 || m.specular_texture > s->n_textures ||
+			m.normal_texture < 0 || m.normal_texture > s->n_textures)
 			return krr::fail(KRR_ERR_INVALID_ARG, "material %d: texture index out of range", i);
 		mats[i].texDiffuse	= m.diffuse_texture - 1;
 		mats[i].texSpecular = m.specular_texture - 1;
+		mats[i].texNormal	= m.normal_texture - 1;
 	}
 	// textures
 	if (s->n_textures < 0 || (s->n_textures > 0 && !s->textures)) return krr::fail(KRR_ERR_INVALID_ARG, "scene: missing textures");
@@ -1857,6 +1862,8 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	KRR_HIP(wf->media.upload(media.data(), media.size() * sizeof(DevMedium)));
 	if (s->texcoords) KRR_HIP(wf->uv.upload(s->texcoords, (size_t) s->n_vertices * 2 * sizeof(float)));
 	else wf->uv.release();
+	if (s->tangents) KRR_HIP(wf->tan.upload(s->tangents, (size_t) s->n_vertices * 3 * sizeof(float)));
+	else wf->tan.release();
 	if (!texs.empty()) KRR_HIP(wf->textures.upload(texs.data(), texs.size() * sizeof(DevTexture)));
 	else wf->textures.release();
 	int rc = buildAccel(s, wf->accel, wf->leafSize, wf->sahNodeCost, wf->deviceBuilder);
@@ -1874,6 +1881,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	ds.media		= wf->media.as<DevMedium>();
 	ds.nMedia		= s->n_media;
 	ds.uv			= s->texcoords ? wf->uv.as<float>() : nullptr;
+	ds.tan			= s->tangents ? wf->tan.as<float>() : nullptr;
 	ds.textures		= texs.empty() ? nullptr : wf->textures.as<DevTexture>();
 	ds.sceneRadius	= sceneRadius;
 	for (int a = 0; a < 3; a++) {
@@ -1882,6 +1890,7 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	}
 	wf->sceneHasMedia = s->n_media > 0;
 	wf->sceneTextured = s->n_textures > 0 || wf->accel.alphaTri.p != nullptr;
+	for (int i = 0; i < s->n_meshes; i++) wf->sceneTextured |= s->meshes[i].has_tangents != 0;
 	for (int i = 0; i < s->n_infinite_lights; i++) wf->sceneTextured |= s->infinite_lights[i].image_rgba != nullptr;
 	wf->haveScene	  = true;
 	return KRR_OK;

@@ -221,7 +221,13 @@ template <class R> SurfInt<R> prepareSurfaceInteraction(const SceneView &sv, con
 		it.n = normalize(b0 * cvt<R>(nrm(d, v0)) + bb1 * cvt<R>(nrm(d, v1)) + bb2 * cvt<R>(nrm(d, v2)));
 	else
 		it.n = normalize(cross(p1 - p0, p2 - p0));
-	it.tangent	 = getPerpendicular(it.n);
+	if (m.has_tangents) { // shading.h:140-145: interpolated, then made orthogonal to n
+		const float *tg = d->tangents;
+		auto tv			= [&](int v) { return V3<R>(R(tg[3 * (size_t) v]), R(tg[3 * (size_t) v + 1]), R(tg[3 * (size_t) v + 2])); };
+		it.tangent		= normalize(b0 * tv(v0) + bb1 * tv(v1) + bb2 * tv(v2));
+		it.tangent		= normalize(it.tangent - it.n * dot(it.n, it.tangent));
+	} else
+		it.tangent = getPerpendicular(it.n);
 	it.bitangent = normalize(cross(it.n, it.tangent));
 	it.light	 = sv.triLight[prim];
 	it.material	 = m.material;
@@ -257,6 +263,17 @@ template <class R> SurfInt<R> prepareSurfaceInteraction(const SceneView &sv, con
 		R c[4];
 		texBilinear4<R>(t.rgba, t.width, t.height, it.u, it.v, c);
 		it.sd.roughness = R(1) - c[3];
+	}
+	// normal map (shading.h:197-203, rgbToNormal of shading.h:32), only with
+	// texture coordinates
+	if (mat.normal_texture > 0 && m.has_texcoords) {
+		const krr_texture &t = d->textures[mat.normal_texture - 1];
+		R c[4];
+		texBilinear4<R>(t.rgba, t.width, t.height, it.u, it.v, c);
+		const R nx = R(2) * c[0] - R(1), ny = R(2) * c[1] - R(1), nz = R(2) * c[2] - R(1);
+		it.n		 = normalize(it.tangent * nx + it.bitangent * ny + it.n * nz);
+		it.tangent	 = normalize(it.tangent - it.n * dot(it.tangent, it.n));
+		it.bitangent = normalize(cross(it.n, it.tangent));
 	}
 	return it;
 }

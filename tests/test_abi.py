@@ -35,7 +35,7 @@ def test_exports_every_declared_function(krr):
 
 def test_version_and_device_count(krr):
     lib = krr.load_library()
-    assert lib.krr_abi_version() == 1
+    assert lib.krr_abi_version() == 2
     assert lib.krr_device_count() >= 0
 
 

@@ -70,3 +70,39 @@ def test_emissive_texture_parity(krr, oracle, gpu, tmp_path, integrator):
         o, os_ = oracle.render_megakernel(s.desc, s.camera(0), 96, 96, 3, spp=4, max_depth=8)
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
+
+
+def _normal_mapped_scene(krr, tmp_path):
+    """The Cornell box plus a quad with per-vertex tangents, a random normal
+    map (tangent-space normals tilted by up to about 25 degrees) and a
+    diffuse texture."""
+    rng = np.random.default_rng(17)
+    nm = np.empty((12, 10, 3), np.float32)
+    nm[..., :2] = 0.5 + 0.35 * (rng.random((12, 10, 2), dtype=np.float32) - 0.5) * 2
+    nm[..., 2] = 0.85 + 0.15 * rng.random((12, 10), dtype=np.float32)
+    np.save(tmp_path / "nm.npy", nm)
+    np.save(tmp_path / "kd2.npy", rng.random((6, 6, 3), dtype=np.float32))
+    quad = {"type": "mesh", "material": 2, "indices": [0, 1, 2, 0, 2, 3],
+            "positions": [-0.6, 0.05, -0.3, 0.5, 0.05, -0.3, 0.5, 0.9, -0.8, -0.6, 0.9, -0.8],
+            "normals": [0, 0.51, 0.86] * 4, "tangents": [1, 0.2, 0] * 2 + [0.9, 0, 0.3] * 2,
+            "texcoords": [0, 0, 2, 0, 2, 1.5, 0, 1.5]}
+    cfg = dict(CORNELL)
+    cfg["materials"] = [{"bsdf": "diffuse", "params": {"diffuse_texture": str(tmp_path / "kd2.npy"),
+                                                       "normal_texture": str(tmp_path / "nm.npy")}}]
+    quad["material"] = 0
+    cfg["model"] = list(CORNELL["model"]) + [quad]
+    return krr.Scene(cfg)
+
+
+@pytest.mark.parametrize("integrator", ["wavefront", "megakernel"])
+def test_normal_map_parity(krr, oracle, gpu, tmp_path, integrator):
+    print(f"case: normal-mapped quad with tangents in the Cornell box, {integrator} 96x96", flush=True)
+    s = _normal_mapped_scene(krr, tmp_path)
+    if integrator == "wavefront":
+        g, gs = gpu_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=5)
+        o, os_ = oracle.render(s.desc, s.camera(0), 96, 96, 5, spp=4, max_depth=8)
+    else:
+        g, gs = mk_render(krr, gpu, s, 96, 96, {"spp": 4, "max_depth": 8}, frame_index=5)
+        o, os_ = oracle.render_megakernel(s.desc, s.camera(0), 96, 96, 5, spp=4, max_depth=8)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
