@@ -34,11 +34,14 @@
 // override the file's materials of that name (krrscene.cpp:160-238), and
 // "camera": {"mData": {...}} with "cameraController": {"mData": {"target",
 // "radius", "pitch", "yaw"}} places the camera as the orbit controller does
-// (camera.cpp:48-61).  What this build cannot render is an error: other
-// model formats, OpenVDB volumes, point / spot / directional lights and
-// textured environments.
+// (camera.cpp:48-61).  OpenVDB float grids (.vdb model files and
+// "heterogeneous" media with a "file") become dense-grid media inside a
+// null-material box (OpenVDBImporter::import, openvdb.cpp:8-67).  What this
+// build cannot render is an error: other model formats, temperature and
+// albedo grids, point / spot / directional lights.
 #include <algorithm>
 #include <cctype>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -1046,6 +1049,311 @@ void addInfinite(krr_scene &s, const float tint[3], float scale, const std::stri
 	s.infinite.push_back(il);
 }
 
+// ---------------------------------------------------------------------------
+// OpenVDB volumes.
+//
+// Reader of float grids in the .vdb files OpenVDB's io::File writes (file
+// format 222 and later), standing for openvdb::io::File::readGrid followed by
+// nanovdb::openToNanoVDB in the reference (util/volume.cpp:46-78).  A grid is
+// read in the order io::File writes it: its compression flags, metadata and
+// transform, the 5-4-3 tree topology (root tiles and children; internal nodes
+// with child mask, value mask and tile values; leaf value masks), then the
+// leaf buffers.  Read are Tree_float_5_4_3 grids (values saved as float or
+// half) with a (uniform) ScaleMap or ScaleTranslateMap and no compression or
+// active-mask compression; zip / blosc compressed grids, other value types and
+// transforms are errors.  The grid becomes a dense grid over the bounding box
+// of its active voxels and active tiles: voxel (i, j, k) of that box holds the
+// tree's value at index lo + (i, j, k), which is what the reference's
+// trilinear sampler reads there (inactive voxels keep their stored values;
+// outside the box every value is the zero background).
+// ---------------------------------------------------------------------------
+struct VdbGrid {
+	int lo[3], hi[3];			// index box of the active voxels, inclusive
+	double voxel[3], origin[3]; // index -> world (before node transforms): origin + voxel * index
+	std::vector<float> dense;	// (hi - lo + 1) voxels per axis, x fastest
+};
+
+struct VdbIn {
+	const std::vector<uint8_t> &d;
+	size_t pos;
+	const std::string &path;
+	void need(size_t n) const {
+		if (n > d.size() || pos > d.size() - n) throw std::runtime_error(path + ": truncated OpenVDB file");
+	}
+	template <class T> T get() {
+		need(sizeof(T));
+		T v;
+		std::memcpy(&v, d.data() + pos, sizeof(T));
+		pos += sizeof(T);
+		return v;
+	}
+	std::string str() {
+		const uint32_t n = get<uint32_t>();
+		need(n);
+		std::string s(reinterpret_cast<const char *>(d.data() + pos), n);
+		pos += n;
+		return s;
+	}
+	void skip(size_t n) {
+		need(n);
+		pos += n;
+	}
+	// MetaMap: count, then per entry the name, the type name and the value's bytes
+	void skipMeta() {
+		const uint32_t n = get<uint32_t>();
+		for (uint32_t i = 0; i < n; i++) {
+			str();
+			str();
+			skip(get<uint32_t>());
+		}
+	}
+	// NodeMask of `bits` bits stored as 64-bit words, bit i in word i / 64 at bit i % 64
+	std::vector<uint8_t> mask(int bits) {
+		need((size_t) bits / 8);
+		std::vector<uint8_t> m((size_t) bits);
+		for (int i = 0; i < bits; i++) m[(size_t) i] = (d[pos + (size_t) (i >> 3)] >> (i & 7)) & 1u;
+		pos += (size_t) bits / 8;
+		return m;
+	}
+};
+
+// IEEE binary16 -> binary32 (exact).
+float halfToFloat(uint16_t h) {
+	const uint32_t s = (uint32_t) (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, f = h & 0x3ffu;
+	uint32_t u;
+	if (e == 0 && f == 0) u = s;
+	else if (e == 0) { // subnormal: shift the fraction up to a leading 1
+		uint32_t m = f, ex = 127 - 15 + 1;
+		while (!(m & 0x400u)) {
+			m <<= 1;
+			ex--;
+		}
+		u = s | (ex << 23) | ((m & 0x3ffu) << 13);
+	} else if (e == 31) u = s | 0x7f800000u | (f << 13);
+	else u = s | ((e + 127 - 15) << 23) | (f << 13);
+	float r;
+	std::memcpy(&r, &u, 4);
+	return r;
+}
+
+// One node's values as io::readCompressedValues stores them: a metadata byte
+// naming how inactive values are kept (0: background, 1: -background, 2: one
+// stored value, 3-5: a selection mask picking between two of them, 6: all
+// values stored), the stored inactive values (full floats), the selection
+// mask, then the active values only (with active-mask compression) or all n,
+// as half or float.
+std::vector<float> vdbValues(VdbIn &in, int n, const std::vector<uint8_t> &valueMask, bool maskCompressed, bool half, float bg) {
+	const int8_t meta = in.get<int8_t>();
+	if (meta < 0 || meta > 6) throw std::runtime_error(in.path + ": bad OpenVDB value metadata");
+	float inactive0 = meta == 0 ? bg : -bg, inactive1 = bg;
+	if (meta == 2 || meta == 4 || meta == 5) {
+		inactive0 = in.get<float>();
+		if (meta == 5) inactive1 = in.get<float>();
+	}
+	std::vector<uint8_t> sel;
+	if (meta == 3 || meta == 4 || meta == 5) sel = in.mask(n);
+	int count = n;
+	if (maskCompressed && meta != 6) {
+		count = 0;
+		for (int i = 0; i < n; i++) count += valueMask[(size_t) i];
+	}
+	std::vector<float> vals((size_t) count);
+	for (int i = 0; i < count; i++) vals[(size_t) i] = half ? halfToFloat(in.get<uint16_t>()) : in.get<float>();
+	if (count == n) return vals;
+	std::vector<float> out((size_t) n);
+	for (int i = 0, k = 0; i < n; i++)
+		out[(size_t) i] = valueMask[(size_t) i] ? vals[(size_t) k++] : (!sel.empty() && sel[(size_t) i] ? inactive1 : inactive0);
+	return out;
+}
+
+struct VdbFile {
+	struct Desc {
+		std::string name, type, instanceParent;
+		int64_t gridPos, blockPos, endPos;
+	};
+	std::vector<uint8_t> bytes;
+	std::vector<Desc> grids;
+	std::string path;
+};
+
+// File header and grid descriptors (io::File::open).
+VdbFile openVdb(const std::string &path) {
+	VdbFile f;
+	f.path = path;
+	std::ifstream is(path, std::ios::binary);
+	if (!is) throw std::runtime_error("cannot open OpenVDB file " + path);
+	f.bytes.assign(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>());
+	VdbIn in{f.bytes, 0, f.path};
+	if (in.get<int64_t>() != 0x56444220) throw std::runtime_error(path + ": not an OpenVDB file");
+	if (in.get<uint32_t>() < 222) throw std::runtime_error(path + ": OpenVDB file format older than 222 is not read");
+	in.get<uint32_t>(); // library major, minor
+	in.get<uint32_t>();
+	if (!in.get<uint8_t>()) throw std::runtime_error(path + ": OpenVDB file without grid offsets is not read");
+	in.skip(36); // UUID
+	in.skipMeta();
+	// Each grid's descriptor sits in front of the grid's data; the next
+	// descriptor follows the grid's end.
+	const int32_t nGrids = in.get<int32_t>();
+	for (int32_t g = 0; g < nGrids; g++) {
+		VdbFile::Desc d;
+		d.name = in.str();
+		d.name = d.name.substr(0, d.name.find('\x1e')); // unique-name suffix of repeated names
+		d.type = in.str();
+		d.instanceParent = in.str();
+		d.gridPos		 = in.get<int64_t>();
+		d.blockPos		 = in.get<int64_t>();
+		d.endPos		 = in.get<int64_t>();
+		if (d.gridPos < (int64_t) in.pos || d.blockPos < d.gridPos || d.endPos < d.blockPos || (size_t) d.endPos > f.bytes.size())
+			throw std::runtime_error(path + ": bad grid offsets");
+		f.grids.push_back(d);
+		in.pos = (size_t) d.endPos;
+	}
+	return f;
+}
+
+bool vdbHasGrid(const VdbFile &f, const std::string &name) {
+	for (const auto &d : f.grids)
+		if (d.name == name) return true;
+	return false;
+}
+
+VdbGrid readVdbGrid(const VdbFile &f, const std::string &key) {
+	const VdbFile::Desc *desc = nullptr;
+	for (const auto &d : f.grids)
+		if (d.name == key) {
+			desc = &d;
+			break;
+		}
+	const std::string &path = f.path;
+	if (!desc) throw std::runtime_error(path + ": no grid named \"" + key + "\"");
+	const bool half = desc->type == "Tree_float_5_4_3_HalfFloat";
+	if (!half && desc->type != "Tree_float_5_4_3")
+		throw std::runtime_error(path + ": grid \"" + key + "\" of type " + desc->type + " is not read (float grids only)");
+	if (!desc->instanceParent.empty()) throw std::runtime_error(path + ": instanced grid \"" + key + "\" is not read");
+	VdbIn in{f.bytes, (size_t) desc->gridPos, path};
+	const uint32_t compression = in.get<uint32_t>();
+	if (compression & ~2u) throw std::runtime_error(path + ": zip or blosc compressed grids are not read");
+	const bool maskCompressed = (compression & 2u) != 0;
+	in.skipMeta();
+	VdbGrid g;
+	const std::string map = in.str();
+	if (map == "UniformScaleTranslateMap" || map == "ScaleTranslateMap") {
+		// translation, scale, voxel size, then three derived vectors
+		double rec[18];
+		for (double &x : rec) x = in.get<double>();
+		for (int a = 0; a < 3; a++) {
+			g.origin[a] = rec[a];
+			g.voxel[a]	= rec[3 + a];
+		}
+	} else if (map == "UniformScaleMap" || map == "ScaleMap") {
+		// scale, voxel size, then three derived vectors
+		double rec[15];
+		for (double &x : rec) x = in.get<double>();
+		for (int a = 0; a < 3; a++) {
+			g.origin[a] = 0.0;
+			g.voxel[a]	= rec[a];
+		}
+	} else throw std::runtime_error(path + ": transform " + map + " is not read (scale and scale-translate maps only)");
+	for (int a = 0; a < 3; a++)
+		if (!(g.voxel[a] > 0.0)) throw std::runtime_error(path + ": transforms with non-positive scales are not read");
+	if (in.get<int32_t>() != 1) throw std::runtime_error(path + ": trees with more than one buffer are not read");
+	const float bg = in.get<float>();
+	if (bg != 0.f) throw std::runtime_error(path + ": grids with a non-zero background are not read");
+	const uint32_t nTiles = in.get<uint32_t>(), nChildren = in.get<uint32_t>();
+	struct Block { // a leaf (span 8) or a tile (span 8, 128 or 4096)
+		int o[3], span;
+		float v;
+		bool active;
+	};
+	std::vector<Block> leaves, tiles;
+	for (uint32_t t = 0; t < nTiles; t++) {
+		Block b;
+		for (int &c : b.o) c = in.get<int32_t>();
+		b.v		 = in.get<float>();
+		b.active = in.get<uint8_t>() != 0;
+		b.span	 = 4096;
+		if (b.v != 0.f) throw std::runtime_error(path + ": root tiles with non-zero values are not read");
+		if (b.active) tiles.push_back(b);
+	}
+	// internal node of 2^(3 L) entries, each covering 2^(3 C) voxels (L = 5, C = 7 and L = 4, C = 3)
+	auto internal = [&](auto &&self, int L, const int *o) -> void {
+		const int n = 1 << (3 * L), C = L == 5 ? 7 : 3, m = (1 << L) - 1;
+		const std::vector<uint8_t> childMask = in.mask(n), valueMask = in.mask(n);
+		const std::vector<float> vals		 = vdbValues(in, n, valueMask, maskCompressed, half, bg);
+		for (int i = 0; i < n; i++) {
+			const int co[3] = {o[0] + ((i >> (2 * L)) << C), o[1] + (((i >> L) & m) << C), o[2] + ((i & m) << C)};
+			if (childMask[(size_t) i]) {
+				if (L == 5) self(self, 4, co);
+				else {
+					in.skip(64); // the leaf's value mask, read again with its buffer
+					leaves.push_back(Block{{co[0], co[1], co[2]}, 8, 0.f, true});
+				}
+			} else if (valueMask[(size_t) i] || vals[(size_t) i] != 0.f)
+				tiles.push_back(Block{{co[0], co[1], co[2]}, 1 << C, vals[(size_t) i], valueMask[(size_t) i] != 0});
+		}
+	};
+	for (uint32_t c = 0; c < nChildren; c++) {
+		int o[3];
+		for (int &x : o) x = in.get<int32_t>();
+		internal(internal, 5, o);
+	}
+	if ((int64_t) in.pos != desc->blockPos) throw std::runtime_error(path + ": tree topology does not end where the buffers start");
+	std::vector<std::vector<float>> leafVals(leaves.size());
+	std::vector<std::vector<uint8_t>> leafMask(leaves.size());
+	for (size_t l = 0; l < leaves.size(); l++) {
+		leafMask[l] = in.mask(512);
+		leafVals[l] = vdbValues(in, 512, leafMask[l], maskCompressed, half, bg);
+	}
+	if ((int64_t) in.pos != desc->endPos) throw std::runtime_error(path + ": leaf buffers do not end at the grid's end");
+	// bounding box of the active voxels and active tiles
+	for (int a = 0; a < 3; a++) {
+		g.lo[a] = INT32_MAX;
+		g.hi[a] = INT32_MIN;
+	}
+	auto grow = [&](const int *p, int span) {
+		for (int a = 0; a < 3; a++) {
+			g.lo[a] = std::min(g.lo[a], p[a]);
+			g.hi[a] = std::max(g.hi[a], p[a] + span - 1);
+		}
+	};
+	for (size_t l = 0; l < leaves.size(); l++)
+		for (int i = 0; i < 512; i++)
+			if (leafMask[l][(size_t) i]) {
+				const int p[3] = {leaves[l].o[0] + (i >> 6), leaves[l].o[1] + ((i >> 3) & 7), leaves[l].o[2] + (i & 7)};
+				grow(p, 1);
+			}
+	for (const Block &t : tiles)
+		if (t.active) grow(t.o, t.span);
+	if (g.lo[0] > g.hi[0]) throw std::runtime_error(path + ": grid \"" + key + "\" has no active voxels");
+	int64_t res[3];
+	for (int a = 0; a < 3; a++) {
+		res[a] = (int64_t) g.hi[a] - g.lo[a] + 1;
+		if (res[a] > 1024) throw std::runtime_error(path + ": active voxels span more than 1024 voxels along an axis");
+	}
+	g.dense.assign((size_t) (res[0] * res[1] * res[2]), 0.f);
+	auto at = [&](int x, int y, int z) -> float & {
+		return g.dense[(size_t) ((x - g.lo[0]) + res[0] * ((y - g.lo[1]) + res[1] * (int64_t) (z - g.lo[2])))];
+	};
+	for (const Block &t : tiles) {
+		int a0[3], a1[3]; // the tile clipped to the box
+		for (int a = 0; a < 3; a++) {
+			a0[a] = std::max(t.o[a], g.lo[a]);
+			a1[a] = std::min(t.o[a] + t.span - 1, g.hi[a]);
+		}
+		for (int z = a0[2]; z <= a1[2]; z++)
+			for (int y = a0[1]; y <= a1[1]; y++)
+				for (int x = a0[0]; x <= a1[0]; x++) at(x, y, z) = t.v;
+	}
+	for (size_t l = 0; l < leaves.size(); l++)
+		for (int i = 0; i < 512; i++) {
+			const int x = leaves[l].o[0] + (i >> 6), y = leaves[l].o[1] + ((i >> 3) & 7), z = leaves[l].o[2] + (i & 7);
+			if (x >= g.lo[0] && x <= g.hi[0] && y >= g.lo[1] && y <= g.hi[1] && z >= g.lo[2] && z <= g.hi[2]) at(x, y, z) = leafVals[l][(size_t) i];
+		}
+	for (float v : g.dense)
+		if (!(v >= 0.f)) throw std::runtime_error(path + ": negative or NaN densities are not read");
+	return g;
+}
+
 // The scene graph of KiRaRay's scene JSON (SceneImporter::importNode,
 // krrscene.cpp:305-343): "model" is an array, a model file path, or a node
 // {"name", "translate", "rotate", "scale", "params"} holding a nested
@@ -1067,26 +1375,31 @@ struct Importer {
 		return true;
 	}
 
-	void modelFile(const std::string &file, const NodeXform &xf) {
-		if (endsWith(file, ".vdb")) throw std::runtime_error("model file \"" + file + "\": OpenVDB volumes are not read by this build");
-		if (!endsWith(file, ".obj")) throw std::runtime_error("model file \"" + file + "\": only Wavefront .obj files are read");
+	// A model file with the params of the node that names it
+	// (SceneImporter::loadModel, krrscene.cpp:8-26).
+	void modelFile(const std::string &file, const NodeXform &xf, const Json &params) {
+		if (endsWith(file, ".vdb")) {
+			vdbVolume(joinPath(baseDir, file), params, xf);
+			return;
+		}
+		if (!endsWith(file, ".obj")) throw std::runtime_error("model file \"" + file + "\": only Wavefront .obj and OpenVDB .vdb files are read");
 		loadObj(b, joinPath(baseDir, file), xf, overrides);
 	}
 
-	void node(const Json &j, const NodeXform &parent) {
+	void node(const Json &j, const NodeXform &parent, const Json &parentParams = Json::object()) {
 		if (j.isArray()) {
-			for (const Json &c : j.items()) node(c, parent);
+			for (const Json &c : j.items()) node(c, parent, parentParams);
 			return;
 		}
 		if (j.isString()) {
-			modelFile(j.asString(), parent);
+			modelFile(j.asString(), parent, parentParams);
 			return;
 		}
 		if (!j.isObject()) throw std::runtime_error("model entries are arrays, file paths or objects, not " + j.dump());
 		const NodeXform xf = parent.then(nodeXform(j));
 		const Json params  = j.contains("params") ? j.at("params") : Json::object();
 		if (j.contains("model")) {
-			node(j.at("model"), xf);
+			node(j.at("model"), xf, params);
 			return;
 		}
 		const std::string t = j.value("type", "mesh");
@@ -1111,8 +1424,7 @@ struct Importer {
 	// it inside / outside the meshes of those names (krrscene.cpp:113-129).
 	int medium(const Json &p, const std::string &t, const NodeXform &xf) {
 		if (t == "homogeneous") homogeneousMedium(s, p);
-		else if (t == "heterogeneous" && p.contains("file"))
-			throw std::runtime_error("heterogeneous medium \"" + p.value("file", "") + "\": OpenVDB volumes are not read by this build");
+		else if (t == "heterogeneous" && p.contains("file")) return vdbVolume(joinPath(baseDir, p.at("file").asString()), p, xf);
 		else if (t == "heterogeneous" || t == "value_noise_grid") valueNoiseMedium(s, p);
 		else if (t == "dense_grid") denseGridMedium(s, p);
 		else throw std::runtime_error("unsupported medium type \"" + t + "\"");
@@ -1128,8 +1440,70 @@ struct Importer {
 		return mi;
 	}
 
+	// An OpenVDB density grid as a heterogeneous medium inside a box without
+	// material around its active voxels (OpenVDBImporter::import,
+	// openvdb.cpp:8-67): keys sigma_t (default 1), albedo (0.5), sigma_a /
+	// sigma_s (as for homogeneous media), g (0), scale (1, the density
+	// multiplier) and key_density ("density").  The box spans the grid's
+	// world bounding box, index lo to hi + 1 (NanoVDB's worldBBox), and the
+	// node transform places both; this build keeps media axis-aligned, so
+	// the transform may translate and scale but not rotate or mirror.
+	// Temperature (emission) and albedo grids are not rendered: a file that
+	// holds a grid under key_temperature ("temperature") or key_albedo
+	// ("albedo") is an error rather than a volume drawn without them.
+	int vdbVolume(const std::string &path, const Json &p, const NodeXform &xf) {
+		if (xf.L[1] != 0.f || xf.L[2] != 0.f || xf.L[3] != 0.f || xf.L[5] != 0.f || xf.L[6] != 0.f || xf.L[7] != 0.f ||
+			!(xf.L[0] > 0.f && xf.L[4] > 0.f && xf.L[8] > 0.f))
+			throw std::runtime_error(path + ": OpenVDB volumes may be translated and scaled (by positive factors), not rotated or mirrored");
+		const VdbFile f = openVdb(path);
+		for (const char *k : {"temperature", "albedo"}) {
+			const std::string key = p.value(std::string("key_") + k, k);
+			if (vdbHasGrid(f, key)) throw std::runtime_error(path + ": the " + k + " grid \"" + key + "\" is not rendered by this build");
+		}
+		VdbGrid g = readVdbGrid(f, p.value("key_density", "density"));
+		krr_medium m;
+		std::memset(&m, 0, sizeof(m));
+		m.type		= KRR_MEDIUM_GRID;
+		float st[3] = {1, 1, 1}, al[3] = {0.5f, 0.5f, 0.5f};
+		p.vec<3>("sigma_t", st);
+		p.vec<3>("albedo", al);
+		if (p.contains("sigma_a") || p.contains("sigma_s")) {
+			float sa[3] = {0.5f, 0.5f, 0.5f}, ss[3] = {0.5f, 0.5f, 0.5f};
+			p.vec<3>("sigma_a", sa);
+			p.vec<3>("sigma_s", ss);
+			for (int a = 0; a < 3; a++) {
+				st[a] = sa[a] + ss[a];
+				al[a] = ss[a] / st[a];
+			}
+		}
+		V lo, hi;
+		float *plo = &lo.x, *phi = &hi.x;
+		for (int a = 0; a < 3; a++) {
+			m.sigma_t[a] = st[a];
+			m.albedo[a]	 = al[a];
+			m.res[a]	 = g.hi[a] - g.lo[a] + 1;
+			plo[a]		 = (float) (g.origin[a] + g.voxel[a] * g.lo[a]);
+			phi[a]		 = (float) (g.origin[a] + g.voxel[a] * ((double) g.hi[a] + 1.0));
+		}
+		const V wlo = xf.point(lo), whi = xf.point(hi);
+		for (int a = 0; a < 3; a++) {
+			m.bounds_min[a] = (&wlo.x)[a];
+			m.bounds_max[a] = (&whi.x)[a];
+		}
+		m.g		= (float) p.value("g", 0.0);
+		m.scale = (float) p.value("scale", 1.0);
+		auto grid = std::make_unique<std::vector<float>>(std::move(g.dense));
+		m.density = grid->data();
+		s.densities.push_back(std::move(grid));
+		s.media.push_back(m);
+		const int mi = (int) s.media.size() - 1;
+		const size_t slash = path.find_last_of('/');
+		box(lo, hi, xf, mi, "Medium box-" + (slash == std::string::npos ? path : path.substr(slash + 1)));
+		return mi;
+	}
+
 	// Twelve triangles wound so that their normals face out of the box.
-	void box(V lo, V hi, const NodeXform &xf, int mi) {
+	void box(V lo, V hi, const NodeXform &xf, int mi, const std::string &name = "Medium box") {
 		const V c{0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 0.5f * (lo.z + hi.z)};
 		const V h{0.5f * (hi.x - lo.x), 0.5f * (hi.y - lo.y), 0.5f * (hi.z - lo.z)};
 		const V X{h.x, 0, 0}, Y{0, h.y, 0}, Z{0, 0, h.z};
@@ -1147,7 +1521,7 @@ struct Importer {
 			verts.push_back(xf.point(add(sub(o, f[1]), f[2])));
 			for (int k : {0, 1, 2, 0, 2, 3}) tris.push_back(base + k);
 		}
-		b.addMesh(verts, {}, tris, -1, {0, 0, 0}, mi, -1, "Medium box");
+		b.addMesh(verts, {}, tris, -1, {0, 0, 0}, mi, -1, name);
 	}
 
 	void bindMedia() {

@@ -128,10 +128,10 @@ def test_unsupported_entries_are_errors(krr, tmp_path):
     bad = [
         {"model": [{"model": str(tmp_path / "x.gltf")}]},
         {"model": [{"model": str(tmp_path / "missing.obj")}]},
-        {"model": [{"model": "smoke.vdb", "params": {"g": 0.5}}]},
+        {"model": [{"model": str(tmp_path / "missing.vdb"), "params": {"g": 0.5}}]},
         {"model": [quad, {"type": "light", "params": {"type": "point", "position": [0, 1, 0]}}]},
         {"model": [quad, {"type": "light", "params": {"type": "infinite", "texture": "sky.exr"}}]},
-        {"model": [quad, {"type": "medium", "params": {"type": "heterogeneous", "file": "smoke.vdb"}}]},
+        {"model": [quad, {"type": "medium", "params": {"type": "heterogeneous", "file": str(tmp_path / "missing.vdb")}}]},
         {"model": [quad, {"type": "medium", "params": {"type": "homogeneous", "meshes": ["nothing"]}}]},
         {"model": [dict(quad, translate=[1, 0, 0])]},
         {"model": [quad], "environment": "sky.exr"},
