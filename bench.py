@@ -586,6 +586,48 @@ def node_format_ab(cfg, frames=10, warmup=3):
     return out
 
 
+def refill_ab(cfg, values=(0, 8, 16, 24, 32, 48, 64), frames=10, warmup=3):
+    """The benchmark workload with the persistent launch's refill threshold
+    (pass param refill_lanes: a wave's traversal round ends once fewer lanes
+    than this are still traversing; 0 runs every ray to its end) in one
+    process: ms per frame and Mrays/s (HIP events on the render stream, no
+    host synchronisation inside the timed frames), each value with its own
+    autotuned launch shape.  The images do not depend on the value."""
+    import kiraray_amd as krr
+    from kiraray_amd import hip
+
+    w, h = cfg["scene"]["resolution"]
+    scene = krr.Scene(cfg["scene"])
+    cam = scene.camera(0)
+    film = hip.DeviceBuffer(w * h * 16)
+    stream = hip.Stream()
+    out = {"workload": cfg["desc"], "frames": frames}
+    for v in values:
+        wf = krr.WavefrontPathTracer(dict(cfg["params"], refill_lanes=v))
+        wf.set_scene(scene)
+        wf.resize(w, h)
+        wf.begin_frame(cam, 1)
+        tune = wf.autotune(stream=stream.handle)
+        for i in range(warmup):
+            wf.begin_frame(cam, 500 + i)
+            wf.render(film.ptr, stream.handle)
+        ev0, ev1 = hip.Event(), hip.Event()
+        ev0.record(stream)
+        for i in range(frames):
+            wf.begin_frame(cam, 1 + i)
+            wf.render(film.ptr, stream.handle)
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_ms(ev1) / frames
+        st = wf.stats()  # the last timed frame
+        rays = st["closest_rays"] + st["shadow_rays"]
+        wf.close()
+        out[str(v)] = {"ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / ms / 1e3, 1),
+                       "launch": {k: tune["best"][k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")}}
+        log(f"refill_lanes {v}: {out[str(v)]}")
+    return out
+
+
 def megakernel_ab(cfg, frames=5, warmup=2):
     """The megakernel path tracer (one lane per pixel, MegakernelPathTracer)
     against the wavefront pass in its default form - every depth of every
@@ -715,7 +757,7 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel path tracer A/B")
-    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel", "nodes"],
+    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel", "nodes", "refill"],
                     help="nrc / multiview: run only that comparison and print its JSON (profiling runs)")
     ap.add_argument("--nrc-warm", type=int, default=96, help="training frames before the NRC comparison")
     ap.add_argument("--nrc-frames", type=int, default=24, help="compared frames of the NRC comparison")
@@ -723,6 +765,8 @@ def main():
                     help="further frames of the NRC comparison for the converged relMSE and bias (0 for profiling runs)")
     ap.add_argument("--wavefront-depths", type=int, default=None,
                     help="run the first N depths of each path as per-depth queue kernels (the pass param wavefront_depths)")
+    ap.add_argument("--refill-lanes", type=int, default=None,
+                    help="the pass param refill_lanes of the benchmark workload (default: the pass default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -737,6 +781,8 @@ def main():
     cfg = CONFIGS[args.config]
     if args.wavefront_depths is not None:
         cfg = dict(cfg, params=dict(cfg["params"], wavefront_depths=args.wavefront_depths))
+    if args.refill_lanes is not None:
+        cfg = dict(cfg, params=dict(cfg["params"], refill_lanes=args.refill_lanes))
     if args.section != "all":
         from kiraray_amd import hip
         hip.check(hip.runtime().hipSetDevice(0))
@@ -747,6 +793,8 @@ def main():
             out = megakernel_ab(CONFIGS["icosphere_field_1080p"])
         elif args.section == "nodes":
             out = node_format_ab(cfg)
+        elif args.section == "refill":
+            out = refill_ab(cfg)
         else:
             out = multiview_bench()
         print(json.dumps(out), flush=True)

@@ -316,7 +316,14 @@ typedef struct krr_wf krr_wf;
  *                            Needs wavefront_depths = 0 and no media.
  *   occupancy (1.0)          fraction of the resident blocks a persistent
  *                            launch uses (in-flight path capacity)
- *   graph (true)             record the multi-view frame into a HIP graph
+ *   refill_lanes (0)         persistent launch without media: a wave's
+ *                            traversal round ends once fewer than this many
+ *                            of its 64 lanes are still traversing; the
+ *                            others shade or take their next ray, and the
+ *                            unfinished rays resume in the next round (0: each
+ *                            ray runs to its end; 1..64); images and ray
+ *                            counts do not depend on it
+ *   graph (true)            record the multi-view frame into a HIP graph
  *                            once and replay it (krr_wf_render_views)
  * NULL or "" means all defaults. */
 /* Replaces RenderPassFactory::deserialize<WavefrontPathTracer> + from_json

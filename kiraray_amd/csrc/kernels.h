@@ -116,6 +116,7 @@ struct WfParams {
 	int legacySpp;
 	uint64_t inc;		  // PCG32 increment of this frame
 	uint32_t sampleIndex; // frameIndex * spp (as uint)
+	int refillLanes;	  // persistent launch without media: end a traversal round below this many traversing lanes (1..64)
 };
 
 // Values that change from frame to frame, read from device memory by the

@@ -367,4 +367,172 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 	return res;
 }
 
+// A traversal that stops and resumes between steps: the entry to take next,
+// the stack depth (the entries stay in the lane's stack), the step budget and
+// the best hit so far.  A ray advanced by traverseSteps until it is done takes
+// exactly the steps of traverse() and ends with the same result.
+struct TravState {
+	int entry, sp, budget;
+	float t;
+	int prim;
+	float b1, b2;
+};
+
+KRR_DEV void travInit(TravState &s, const TraceBvh &bvh, float tmax) {
+	s.entry	 = 0;
+	s.sp	 = 0;
+	s.budget = bvh.steps;
+	s.t		 = tmax;
+	s.prim	 = -1;
+	s.b1	 = 0.f;
+	s.b2	 = 0.f;
+}
+
+// Advances the rays of the lanes with `going` set, one entry per lane per
+// iteration, until fewer than minLanes lanes of the wave are still going
+// (minLanes 1: until every ray is done).  A lane clears `going` when its ray
+// is done.  any: the lane's ray is a shadow ray (stop at the first hit of a
+// non-null-material triangle); otherwise the closest hit is kept.  All lanes
+// of the wave call this together; the loop exit is wave-uniform.
+template <bool COUNT, bool ALPHA>
+KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const bool any, bool &going, TravState &s,
+						   const LaneStack &stack, const int minLanes, uint32_t &nodeVisits, uint32_t &triTests) {
+	const RayPre rp = rayPre(o, d);
+	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
+	const char *nodeBase = reinterpret_cast<const char *>(bvh.nodes);
+	const float kWiden = 1.00000036f;			  // 1 + 2*gamma(3)
+	const float kTie   = 1.00000095367431640625f; // 1 + 2^-20
+	while (true) {
+		const uint64_t lanes = __ballot(going);
+		if (lanes == 0 || __popcll(lanes) < minLanes) break;
+		if (!going) continue;
+		if (--s.budget < 0) {
+			going = false;
+			continue;
+		}
+		if (s.entry < 0) {
+			const int code	= ~s.entry;
+			const int first = code >> 3, cnt = (code & 7) + 1;
+			bool stop = false;
+			for (int k = 0; k < cnt; k++) {
+				const float4 *tp = reinterpret_cast<const float4 *>(bvh.tris + first + k);
+				const float4 a = tp[0], b = tp[1], e = tp[2];
+				if (any && (__float_as_int(b.w) & kLeafNull)) continue;
+				if (COUNT) triTests++;
+				float t, b1, b2;
+				if (!intersectTri(rp, f3(a.x, a.y, a.z), f3(b.x, b.y, b.z), f3(e.x, e.y, e.z), t, b1, b2)) continue;
+				const int prim = __float_as_int(a.w);
+				if (t < s.t || (t == s.t && s.prim >= 0 && prim < s.prim)) {
+					if constexpr (ALPHA) {
+						if ((__float_as_int(b.w) & kLeafAlpha) && alphaKilled(bvh, prim, b1, b2, o, d)) continue;
+					}
+					s.t	   = t;
+					s.prim = prim;
+					s.b1   = b1;
+					s.b2   = b2;
+					if (any) {
+						stop = true;
+						break;
+					}
+				}
+			}
+			if (stop || s.sp == 0) {
+				going = false;
+				continue;
+			}
+			s.entry = stack.get(--s.sp);
+			continue;
+		}
+		if (COUNT) nodeVisits++;
+		const float tlim = s.t * kTie;
+		float k0, k1, k2, k3;
+		int c0, c1, c2, c3;
+		if (bvh.qnodes) {
+			const uint4 *np = reinterpret_cast<const uint4 *>(bvh.qnodes + s.entry);
+			const uint4 A = np[0], B = np[1], C = np[2], D = np[3];
+			const float sx = __int_as_float(((int) (int8_t) (A.w & 0xffu) + 127) << 23);
+			const float sy = __int_as_float(((int) (int8_t) ((A.w >> 8) & 0xffu) + 127) << 23);
+			const float sz = __int_as_float(((int) (int8_t) ((A.w >> 16) & 0xffu) + 127) << 23);
+			const float gx = __uint_as_float(A.x), gy = __uint_as_float(A.y), gz = __uint_as_float(A.z);
+			const uint32_t valid = A.w >> 24;
+			const uint32_t qnx = nX ? B.y : B.x, qfx = nX ? B.x : B.y;
+			const uint32_t qny = nY == 48u ? B.w : B.z, qfy = nY == 48u ? B.z : B.w;
+			const uint32_t qnz = nZ == 80u ? C.y : C.x, qfz = nZ == 80u ? C.x : C.y;
+#define KRR_DEQ(w, c, sc, g) fmaf((float) (((w) >> (8 * (c))) & 0xffu), sc, g)
+#define KRR_QSLAB(kc, c)                                                                                             \
+	{                                                                                                                \
+		float t0 = fmaxf(fmaxf((KRR_DEQ(qnx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qny, c, sy, gy) - o.y) * iy),           \
+						 fmaxf((KRR_DEQ(qnz, c, sz, gz) - o.z) * iz, 0.0f));                                         \
+		float t1 = fminf(fminf((KRR_DEQ(qfx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qfy, c, sy, gy) - o.y) * iy),           \
+						 (KRR_DEQ(qfz, c, sz, gz) - o.z) * iz) *                                                     \
+				   kWiden;                                                                                           \
+		t1 = fminf(t1, tlim);                                                                                        \
+		kc = (t0 <= t1 && ((valid >> (c)) & 1u)) ? t0 : __builtin_inff();                                            \
+	}
+			KRR_QSLAB(k0, 0)
+			KRR_QSLAB(k1, 1)
+			KRR_QSLAB(k2, 2)
+			KRR_QSLAB(k3, 3)
+#undef KRR_QSLAB
+#undef KRR_DEQ
+			c0 = (int) C.z;
+			c1 = (int) C.w;
+			c2 = (int) D.x;
+			c3 = (int) D.y;
+		} else {
+			const uint32_t nb = (uint32_t) s.entry * 128u; // node array < 4 GiB (checked at build)
+#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
+			const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
+			const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
+#undef KRR_PLANE
+			const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
+#define KRR_SLAB(kc, comp)                                                                                           \
+	{                                                                                                                \
+		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
+		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
+		t1		 = fminf(t1, tlim);                                                                                  \
+		kc		 = t0 <= t1 ? t0 : __builtin_inff();                                                                 \
+	}
+			KRR_SLAB(k0, x)
+			KRR_SLAB(k1, y)
+			KRR_SLAB(k2, z)
+			KRR_SLAB(k3, w)
+#undef KRR_SLAB
+			c0 = ch.x;
+			c1 = ch.y;
+			c2 = ch.z;
+			c3 = ch.w;
+		}
+		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
+#define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
+	if (kb < ka) {                                                                                                   \
+		float tk = ka;                                                                                               \
+		ka		 = kb;                                                                                               \
+		kb		 = tk;                                                                                               \
+		int tc	 = ca;                                                                                               \
+		ca		 = cb;                                                                                               \
+		cb		 = tc;                                                                                               \
+	}
+		KRR_CSWAP(k0, c0, k1, c1)
+		KRR_CSWAP(k2, c2, k3, c3)
+		KRR_CSWAP(k0, c0, k2, c2)
+		KRR_CSWAP(k1, c1, k3, c3)
+		KRR_CSWAP(k1, c1, k2, c2)
+#undef KRR_CSWAP
+		if (n == 0) {
+			if (s.sp == 0) {
+				going = false;
+				continue;
+			}
+			s.entry = stack.get(--s.sp);
+			continue;
+		}
+		if (n >= 4) stack.put(s.sp++, c3);
+		if (n >= 3) stack.put(s.sp++, c2);
+		if (n >= 2) stack.put(s.sp++, c1);
+		s.entry = c0;
+	}
+}
+
 } // namespace krr

@@ -1070,6 +1070,125 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 	}
 }
 
+// The persistent per-path launch of k_megakernel (scenes without media) with
+// dynamic ray refill: each lane has one ray in flight, its path's closest ray
+// or its shadow ray, and all lanes of a wave traverse together until fewer
+// than P.refillLanes of them are still traversing.  The lanes whose ray is
+// done then shade (closest) or add the light sample (shadow) and set up their
+// path's next ray, or take a new path; the others keep their traversal state
+// (next entry, stack, best hit) and go on in the next round.  A traversal
+// loop otherwise runs as long as its slowest lane, with the other lanes of the
+// wave idle.  Each path still runs closest trace, shading, shadow trace and
+// next depth in this order, and each ray takes the same traversal steps, so
+// the radiance additions, RNG draws and hits are those of k_megakernel.  Once
+// no path is left to take, the rounds run every traversal to its end.
+template <bool COUNT, bool NRC = false, bool VIEWS = false, bool PLAIN = false>
+__global__ void k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth,
+									PixelStateDev px, int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{},
+									CameraDev cam = CameraDev{}, int camFetch = 0) {
+	extern __shared__ int ldsStack[];
+	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
+	WfParams P			  = P0;
+	if (VIEWS) {
+		P.inc = V.frame->inc;
+		if (NRC) nrc.phase = V.frame->phase;
+	}
+	if (NRC) nrc.keepProb = *nrc.keep;
+	const int count = (VIEWS || camFetch) ? P.N : *countPtr;
+	uint32_t nv = 0, nt = 0, nClosest = 0, nShadow = 0;
+	bool active = false, exhausted = false, trainPath = false;
+	bool going = false, shadowRay = false, pendingNext = false;
+	int depth = startDepth, trainSlot = -1;
+	RayItem r; // the path's current vertex ray; while its shadow ray is in flight, the next one
+	PathRegs pr;
+	f3 to, td;	 // the ray in flight
+	f3 contrib;	 // the shadow ray's radiance when unoccluded
+	TravState ts;
+	while (true) {
+		if (!exhausted) {
+			const int i = wavePush(&cnt->work[2], !active);
+			if (__ballot(!active && i >= count)) exhausted = true;
+			if (!active && i < count) {
+				if (VIEWS) r = viewCameraRay(P, V, px, px.order ? px.order[i] : i, pr, false);
+				else if (camFetch) {
+					const int pix = px.order ? px.order[i] : i;
+					pr			  = loadPathRegs(px, pix, false);
+					if (!P.legacySpp) pr.L = f3();
+					Pcg cs{pr.rng, P.inc};
+					r		= cameraRay(cam, P.W, P.H, pix % P.W, pix / P.W, cs);
+					r.pixel = pix;
+					pr.rng	= cs.state;
+				} else {
+					r  = loadRayItem(q, (px.order && startDepth == 0) ? px.order[i] : i);
+					pr = loadPathRegs(px, r.pixel, false);
+				}
+				depth	  = startDepth;
+				active	  = true;
+				trainPath = NRC && trainHash((uint32_t) r.pixel) % (uint32_t) nrc.trainMod == (uint32_t) nrc.phase;
+				to		  = r.o;
+				td		  = r.d;
+				shadowRay = false;
+				going	  = true;
+				travInit(ts, bvh, __builtin_inff());
+				nClosest++;
+			}
+		}
+		if (__ballot(active) == 0) break;
+		traverseSteps<COUNT, !PLAIN>(bvh, to, td, shadowRay, going, ts, stack, exhausted ? 1 : P.refillLanes, nv, nt);
+		if (!active || going) continue;
+		bool next = false; // go on with the path's next closest ray
+		if (shadowRay) {
+			if (ts.prim < 0) pr.L = pr.L + contrib;
+			if (NRC && trainSlot >= 0) nrc.train[(size_t) trainSlot * 6 + 5] = make_float4(pr.L.x, pr.L.y, pr.L.z, 0.f);
+			next = pendingNext;
+		} else {
+			const float4 hv = make_float4(ts.prim >= 0 ? ts.t : __builtin_inff(), __int_as_float(ts.prim), ts.b1, ts.b2);
+			bool wantNext = false, wantShadow = false;
+			RayItem nx;
+			ShadowOut so;
+			so.medium = -1;
+			trainSlot = -1;
+			shadeCore<false, NRC, PLAIN>(P, sc, r, hv, pr, depth, wantNext, nx, wantShadow, so, &nrc, trainPath, &trainSlot);
+			const bool more = wantNext && depth < P.maxDepth;
+			if (more) r = nx;
+			if (wantShadow) {
+				nShadow++;
+				contrib		= so.Ld / (so.pl + so.pu).mean();
+				to			= so.o;
+				td			= so.d;
+				shadowRay	= true;
+				pendingNext = more;
+				going		= true;
+				travInit(ts, bvh, 1.f);
+				continue;
+			}
+			if (NRC && trainSlot >= 0) nrc.train[(size_t) trainSlot * 6 + 5] = make_float4(pr.L.x, pr.L.y, pr.L.z, 0.f);
+			next = more;
+		}
+		if (next) {
+			depth++;
+			to		  = r.o;
+			td		  = r.d;
+			shadowRay = false;
+			going	  = true;
+			travInit(ts, bvh, __builtin_inff());
+			nClosest++;
+		} else {
+			storePathRegs(px, r.pixel, pr);
+			if (px.plen) px.plen[r.pixel] = (uint8_t) min(depth, 255);
+			active = false;
+		}
+	}
+	waveAddU64(&cnt->closestTotal, nClosest);
+	waveAddU64(&cnt->shadowTotal, nShadow);
+	waveAddU64(&cnt->megaClosest, nClosest);
+	waveAddU64(&cnt->megaShadow, nShadow);
+	if (COUNT) {
+		waveAddU64(&cnt->megaNodeVisits, nv);
+		waveAddU64(&cnt->megaTriTests, nt);
+	}
+}
+
 // Training targets of the vertices recorded in this sample: the radiance the
 // path gathered after the vertex, divided by the throughput at the vertex.
 __global__ void k_nrc_targets(NrcDev nrc, const int *donePtr, const float4 *L, float4 *batchV, float *batchT) {
@@ -1465,6 +1584,40 @@ int reserveOverflow(DevBuf &buf, const AccelData &a, size_t threads) {
 	return KRR_OK;
 }
 
+// The persistent kernel variants of every frame kind.
+// REFILL: k_megakernel_refill (scenes without media).
+template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS, bool PLAIN, bool REFILL = false> struct Mega {
+	static int grid(int tb, size_t lds, int maxBlocks) {
+		if constexpr (REFILL) return persistentGrid(k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks);
+		else return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks);
+	}
+	static void launch(unsigned g, int tb, size_t lds, hipStream_t st, const WfParams &P, const DevScene &ds, const TraceBvh &bvh,
+					   const RayQueueDev &q, const int *countPtr, const PixelStateDev &px, int *ovf, Counters *cnt, const NrcDev &nrc,
+					   const ViewsDev &V, int startDepth = 0, const CameraDev &cam = CameraDev{}, int camFetch = 0) {
+		if constexpr (REFILL)
+			hipLaunchKernelGGL((k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, startDepth, px, ovf,
+							   cnt, nrc, V, cam, camFetch);
+		else
+			hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, startDepth, px, ovf,
+							   cnt, nrc, V, cam, camFetch);
+	}
+};
+
+template <bool VIEWS, bool REFILL, class F> auto withMegaR(bool media, bool count, bool nrc, bool plain, F &&f) {
+	if (nrc && plain) return count ? f(Mega<false, true, true, VIEWS, true, REFILL>{}) : f(Mega<false, false, true, VIEWS, true, REFILL>{});
+	if (nrc) return count ? f(Mega<false, true, true, VIEWS, false, REFILL>{}) : f(Mega<false, false, true, VIEWS, false, REFILL>{});
+	if (media) return count ? f(Mega<true, true, false, VIEWS, false>{}) : f(Mega<true, false, false, VIEWS, false>{});
+	if (plain) return count ? f(Mega<false, true, false, VIEWS, true, REFILL>{}) : f(Mega<false, false, false, VIEWS, true, REFILL>{});
+	return count ? f(Mega<false, true, false, VIEWS, false, REFILL>{}) : f(Mega<false, false, false, VIEWS, false, REFILL>{});
+}
+
+// The persistent kernel for these flags; the refill launch for scenes
+// without media when refill_lanes > 0.
+template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, bool plain, bool refill, F &&f) {
+	if (refill && !media) return withMegaR<VIEWS, true>(media, count, nrc, plain, f);
+	return withMegaR<VIEWS, false>(media, count, nrc, plain, f);
+}
+
 } // namespace
 
 struct krr_accel {
@@ -1478,6 +1631,9 @@ struct krr_wf {
 	bool nee = true, enableMedium = true, enableClamp = false, legacySpp = false, countTraversal = false, timeKernels = false;
 	int maxDepth = 10, spp = 1, rrMinDepth = 0, blockSize = 256, traceBlockSize = 128, leafSize = 4, wavefrontDepths = 0;
 	int stackLdsEntries = kShortStack;
+	// persistent launch without media: traversal rounds end when fewer than
+	// this many lanes of a wave are still traversing (0: each ray to its end)
+	int refillLanes = 0;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
 	bool quantizedNodes = false; // "node_format": "float" (default) | "quantized"
@@ -1590,6 +1746,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
+		wf->refillLanes		= (int) j.value("refill_lanes", 0.0);
 		if (j.contains("nrc")) {
 			const Json &n = j.at("nrc");
 			if (n.isBool()) wf->nrcOn = n.asBool();
@@ -1629,6 +1786,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
 	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
+	if (wf->refillLanes < 0 || wf->refillLanes > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "refill_lanes must be in [0, 64]");
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
@@ -1913,6 +2071,7 @@ WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex)
 	P.legacySpp	   = wf->legacySpp ? 1 : 0;
 	P.sampleIndex  = (uint32_t) (frameIndex * (uint64_t) wf->spp);
 	P.inc		   = ((uint64_t) P.sampleIndex << 1u) | 1u;
+	P.refillLanes  = wf->refillLanes;
 	return P;
 }
 
@@ -2205,6 +2364,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("leaf_size", Json((double) wf->leafSize));
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
+	j.set("refill_lanes", Json((double) wf->refillLanes));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	j.set("occupancy", Json((double) wf->occupancy));
 	j.set("bvh_builder", Json(std::string(wf->deviceBuilder ? "device" : "host")));
@@ -2294,25 +2454,19 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const bool count   = wf->countTraversal;
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
 	const bool plain   = !wf->sceneTextured;
+	const bool refill  = wf->refillLanes > 0;
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
-							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ (plain ? 8 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
+							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ (plain ? 8 : 0) ^ (refill ? 16 : 0) ^
+							  ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (shapeKey != wf->gridKey) {
 		// Counting variants use more registers: size each grid for the variant launched.
 		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
 		if (media) wf->gShadow = count ? persistentGrid(k_shadow_tr<true>, tb, lds, gTmax) : persistentGrid(k_shadow_tr<false>, tb, lds, gTmax);
 		else wf->gShadow = count ? persistentGrid(k_trace_shadow<true>, tb, lds, gTmax) : persistentGrid(k_trace_shadow<false>, tb, lds, gTmax);
 		wf->gShade = media ? persistentGrid(k_shade<true>, bs, 0, (int) gN) : persistentGrid(k_shade<false>, bs, 0, (int) gN);
-		if (media) wf->gMega = count ? persistentGrid(k_megakernel<true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<true, false>, tb, lds, gTmax);
-		else if (plain)
-			wf->gMega = count ? persistentGrid(k_megakernel<false, true, false, false, true>, tb, lds, gTmax)
-							  : persistentGrid(k_megakernel<false, false, false, false, true>, tb, lds, gTmax);
-		else wf->gMega = count ? persistentGrid(k_megakernel<false, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false>, tb, lds, gTmax);
-		if (wf->nrcOn && plain)
-			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true, false, true>, tb, lds, gTmax)
-								 : persistentGrid(k_megakernel<false, false, true, false, true>, tb, lds, gTmax);
-		else if (wf->nrcOn)
-			wf->gMegaNrc = count ? persistentGrid(k_megakernel<false, true, true>, tb, lds, gTmax) : persistentGrid(k_megakernel<false, false, true>, tb, lds, gTmax);
+		wf->gMega = withMega<false>(media, count, false, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax); });
+		if (wf->nrcOn) wf->gMegaNrc = withMega<false>(media, count, true, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax); });
 		wf->gMega	 = scaledGrid(wf->gMega, wf->occupancy);
 		wf->gMegaNrc = scaledGrid(wf->gMegaNrc, wf->occupancy);
 		const int gMax = std::max(std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega), wf->nrcOn ? wf->gMegaNrc : 0);
@@ -2381,25 +2535,11 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 					const unsigned g = (unsigned) wf->gMega;
 					const int camFetch = depth == 0 ? 1 : 0; // k_camera skipped: the kernel makes the camera rays
 					const int *cp	 = (const int *) &cnt->rayCount[cur];
-					if (wf->nrcOn) {
-						const unsigned gn = (unsigned) wf->gMegaNrc;
-						if (plain) {
-							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
-							else hipLaunchKernelGGL((k_megakernel<false, false, true, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
-						} else {
-							if (count) hipLaunchKernelGGL((k_megakernel<false, true, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
-							else hipLaunchKernelGGL((k_megakernel<false, false, true>), dim3(gn), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, nrc, ViewsDev{}, cam, camFetch);
-						}
-					} else if (media) {
-						if (count) hipLaunchKernelGGL((k_megakernel<true, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-						else hipLaunchKernelGGL((k_megakernel<true, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-					} else if (plain) {
-						if (count) hipLaunchKernelGGL((k_megakernel<false, true, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-						else hipLaunchKernelGGL((k_megakernel<false, false, false, false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-					} else {
-						if (count) hipLaunchKernelGGL((k_megakernel<false, true>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-						else hipLaunchKernelGGL((k_megakernel<false, false>), dim3(g), dim3(tb), lds, st, P, wf->ds, bvh, cq, cp, depth, px, ovf, cnt, NrcDev{}, ViewsDev{}, cam, camFetch);
-					}
+					withMega<false>(media, count, wf->nrcOn, plain, refill, [&](auto k) {
+						decltype(k)::launch(wf->nrcOn ? (unsigned) wf->gMegaNrc : g, tb, lds, st, P, wf->ds, bvh, cq, cp, px, ovf, cnt,
+											wf->nrcOn ? nrc : NrcDev{}, ViewsDev{}, depth, cam, camFetch);
+						return 0;
+					});
 				});
 				break;
 			}
@@ -2464,25 +2604,6 @@ int krr_wf_end_frame(krr_wf *wf) {
 
 namespace {
 
-// The persistent kernel variants of the multi-view frame and of the
-// single-view frame's camera samples (autotuner trials).
-template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS, bool PLAIN> struct Mega {
-	static int grid(int tb, size_t lds, int maxBlocks) { return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks); }
-	static void launch(unsigned g, int tb, size_t lds, hipStream_t st, const WfParams &P, const DevScene &ds, const TraceBvh &bvh,
-					   const RayQueueDev &q, const int *countPtr, const PixelStateDev &px, int *ovf, Counters *cnt, const NrcDev &nrc,
-					   const ViewsDev &V) {
-		hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, 0, px, ovf, cnt, nrc, V);
-	}
-};
-
-template <bool VIEWS, class F> auto withMega(bool media, bool count, bool nrc, bool plain, F &&f) {
-	if (nrc && plain) return count ? f(Mega<false, true, true, VIEWS, true>{}) : f(Mega<false, false, true, VIEWS, true>{});
-	if (nrc) return count ? f(Mega<false, true, true, VIEWS, false>{}) : f(Mega<false, false, true, VIEWS, false>{});
-	if (media) return count ? f(Mega<true, true, false, VIEWS, false>{}) : f(Mega<true, false, false, VIEWS, false>{});
-	if (plain) return count ? f(Mega<false, true, false, VIEWS, true>{}) : f(Mega<false, false, false, VIEWS, true>{});
-	return count ? f(Mega<false, true, false, VIEWS, false>{}) : f(Mega<false, false, false, VIEWS, false>{});
-}
-
 ViewsDev viewsDev(krr_wf *wf) {
 	return ViewsDev{wf->viewBuf.as<ViewDev>(), wf->viewStart.as<int>(), (int) wf->views.size(), wf->frameBuf.as<FrameDev>(), wf->vTileStart.as<int>()};
 }
@@ -2509,10 +2630,10 @@ int viewsLaunch(krr_wf *wf, int N, bool nrc, ViewsLaunch &L) {
 	L.nrc			   = nrc;
 	const uint64_t key = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) wf->accel.stackSize << 50) ^
 						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^ (wf->sceneTextured ? 0 : 8) ^
-						 ((uint64_t) (wf->occupancy * 1024.f) << 32);
+						 (wf->refillLanes > 0 ? 16 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (key != wf->viewsGridKey) {
 		const int maxBlocks = std::max(1, (N + tb - 1) / tb);
-		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
+		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
 		wf->gMegaViews		= scaledGrid(g, wf->occupancy);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) wf->gMegaViews * tb)) return rc;
 		wf->viewsGridKey = key;
@@ -2573,7 +2694,7 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 			hipLaunchKernelGGL(k_tile_pixels_views, dim3(nT), dim3(64), 0, st, sorted, off, V, wf->vOrder.as<int>());
 		}
 		timed(4, [&] {
-			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, [&](auto k) {
+			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
 				decltype(k)::launch(L.grid, tb, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, px, ovf, cnt, nrc, V);
 				return 0;
 			});
@@ -2777,7 +2898,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), tpx);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<true>(L.media, false, false, !wf->sceneTextured, [&](auto k) {
+				withMega<true>(L.media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
 					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, tpx,
 										wf->ovf.as<int>(), wf->counters.as<Counters>(), NrcDev{}, viewsDev(wf));
 					return 0;
@@ -2788,7 +2909,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				const TraceBvh bvh = wf->accel.dev();
 				const size_t lds   = stackLds(wf->accel, t.tb);
 				const int maxBlocks = std::max(1, (N + t.tb - 1) / t.tb);
-				int g = withMega<false>(media, false, false, !wf->sceneTextured, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
+				int g = withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
 				g	  = scaledGrid(g, t.occ);
 				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
 				// trials may cover fewer paths than pixels: pixel order, no miss record
@@ -2800,7 +2921,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_camera, dim3(gN), dim3(256), 0, st, P, cameraDev(wf->cam), px, rayQueue(wf, 0), cnt);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<false>(media, false, false, !wf->sceneTextured, [&](auto k) {
+				withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
 					decltype(k)::launch((unsigned) g, t.tb, lds, st, P, wf->ds, bvh, rayQueue(wf, 0), (const int *) &cnt->rayCount[0], px,
 										wf->ovf.as<int>(), cnt, NrcDev{}, ViewsDev{});
 					return 0;
