@@ -162,11 +162,31 @@ __device__ __forceinline__ float16v outputForward(const uint16_t *wt5, int lane,
 }
 
 // Copies `rows` dense rows of rowLen bf16 into LDS rows of rowLen + kPad.
+// Each thread issues eight 8-byte loads before it stores them: one load per
+// store would put a full memory round trip between consecutive loads, and
+// the training kernel runs one wave per SIMD with nothing to hide it.
 __device__ void loadRows(uint16_t *lds, const uint16_t *src, int rows, int rowLen) {
+	constexpr int kBatch = 8;
 	const int per = rowLen / 4; // 8-byte pieces per row
-	for (int i = threadIdx.x; i < rows * per; i += blockDim.x) {
-		const int r = i / per, c = i - r * per;
-		*reinterpret_cast<uint2 *>(lds + r * (rowLen + kPad) + 4 * c) = *reinterpret_cast<const uint2 *>(src + r * rowLen + 4 * c);
+	const int total = rows * per, stride = (int) blockDim.x;
+	for (int base = (int) threadIdx.x; base < total; base += kBatch * stride) {
+		uint2 v[kBatch];
+#pragma unroll
+		for (int k = 0; k < kBatch; k++) {
+			const int i = base + k * stride;
+			if (i < total) {
+				const int r = i / per, c = i - r * per;
+				v[k]		= *reinterpret_cast<const uint2 *>(src + r * rowLen + 4 * c);
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < kBatch; k++) {
+			const int i = base + k * stride;
+			if (i < total) {
+				const int r = i / per, c = i - r * per;
+				*reinterpret_cast<uint2 *>(lds + r * (rowLen + kPad) + 4 * c) = v[k];
+			}
+		}
 	}
 }
 
@@ -451,10 +471,18 @@ __global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xTAll, cons
 	float16v acc		 = zero16();
 	const uint16_t *arow = xT + (int64_t) (32 * kt + r0) * ld + m0;
 	const uint16_t *brow = dzT + (int64_t) (32 * nt + r0) * ld + m0;
-	for (int s = 0; s < chunk / 8; s++) {
-		const short4v a = *reinterpret_cast<const short4v *>(arow + 8 * s + 4 * g);
-		const short4v b = *reinterpret_cast<const short4v *>(brow + 8 * s + 4 * g);
-		acc				= mfma(a, b, acc);
+	// eight k-steps of loads in flight before their MFMAs (same order of
+	// accumulation); chunk is a multiple of 64 (checked by the caller)
+	constexpr int kU = 8;
+	for (int s0 = 0; s0 < chunk / 8; s0 += kU) {
+		short4v a[kU], b[kU];
+#pragma unroll
+		for (int u = 0; u < kU; u++) {
+			a[u] = *reinterpret_cast<const short4v *>(arow + 8 * (s0 + u) + 4 * g);
+			b[u] = *reinterpret_cast<const short4v *>(brow + 8 * (s0 + u) + 4 * g);
+		}
+#pragma unroll
+		for (int u = 0; u < kU; u++) acc = mfma(a[u], b[u], acc);
 	}
 	// D[i = k][j = n]: lane holds column n = r0, rows k = 8(r/4) + 4g + r%4
 	float *dst = partialAll + (int64_t) layer * chunks * 4096 + (int64_t) blockIdx.y * 64 * nOut;
@@ -603,6 +631,7 @@ struct Buf {
 
 constexpr int kChunk	= 1024; // batch granularity of the training buffers
 constexpr int kWChunk = 256;	// samples per split-K chunk of the weight gradient
+static_assert(kWChunk % 64 == 0, "k_weight_grad loads eight 8-sample k-steps at a time");
 
 } // namespace
 
