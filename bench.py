@@ -281,7 +281,8 @@ def kernel_rooflines(cfg, prm, steps, tot, cnt):
           f"64 B ray + 16 B contribution per ray + {NODE_B} B per node visit + 48 B per triangle test",
           wf_shadow, cnt["shadow_node_visits"], cnt["shadow_tri_tests"])
     mrays = cnt["megakernel_closest_rays"] + cnt["megakernel_shadow_rays"]
-    entry("megakernel", "k_megakernel", tot["mega_ms"], mega_launches,
+    refill = prm.get("refill_lanes", 0) > 0 and not media  # the persistent launch with dynamic ray refill
+    entry("megakernel", "k_megakernel_refill" if refill else "k_megakernel", tot["mega_ms"], mega_launches,
           cnt["megakernel_node_visits"] * NODE_B + cnt["megakernel_tri_tests"] * TRI_B,
           f"traversal only (lower bound): {NODE_B} B per node visit + 48 B per triangle test, "
           "closest and shadow rays",
@@ -586,7 +587,7 @@ def node_format_ab(cfg, frames=10, warmup=3):
     return out
 
 
-def refill_ab(cfg, values=(0, 8, 16, 24, 32, 48, 64), frames=10, warmup=3):
+def refill_ab(cfg, values=(0, 2, 4, 6, 8, 12, 16, 32), frames=10, warmup=3):
     """The benchmark workload with the persistent launch's refill threshold
     (pass param refill_lanes: a wave's traversal round ends once fewer lanes
     than this are still traversing; 0 runs every ray to its end) in one

@@ -316,7 +316,7 @@ typedef struct krr_wf krr_wf;
  *                            Needs wavefront_depths = 0 and no media.
  *   occupancy (1.0)          fraction of the resident blocks a persistent
  *                            launch uses (in-flight path capacity)
- *   refill_lanes (0)         persistent launch without media: a wave's
+ *   refill_lanes (8)         persistent launch without media: a wave's
  *                            traversal round ends once fewer than this many
  *                            of its 64 lanes are still traversing; the
  *                            others shade or take their next ray, and the

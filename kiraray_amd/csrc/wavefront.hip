@@ -1633,7 +1633,7 @@ struct krr_wf {
 	int stackLdsEntries = kShortStack;
 	// persistent launch without media: traversal rounds end when fewer than
 	// this many lanes of a wave are still traversing (0: each ray to its end)
-	int refillLanes = 0;
+	int refillLanes = kDefaultRefillLanes;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
 	bool quantizedNodes = false; // "node_format": "float" (default) | "quantized"
@@ -1746,7 +1746,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
-		wf->refillLanes		= (int) j.value("refill_lanes", 0.0);
+		wf->refillLanes		= (int) j.value("refill_lanes", (double) kDefaultRefillLanes);
 		if (j.contains("nrc")) {
 			const Json &n = j.at("nrc");
 			if (n.isBool()) wf->nrcOn = n.asBool();

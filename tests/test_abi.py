@@ -50,6 +50,7 @@ def test_pass_params_roundtrip(krr):
     assert abs(d["rr"] - 0.8) < 1e-7 and d["enable_clamp"] is False and d["clamp_max"] == 1000
     assert d["wavefront_depths"] == 0 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
     assert d["occupancy"] == 1 and d["graph"] is True and d["bvh_builder"] == "device"
+    assert d["refill_lanes"] == 8
 
 
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',

@@ -1,4 +1,5 @@
-"""The persistent launch with dynamic ray refill ("refill_lanes" > 0).
+"""The persistent launch with dynamic ray refill ("refill_lanes" > 0, the
+default) and without it ("refill_lanes": 0).
 
 A wave's traversal round ends once fewer than refill_lanes of its lanes are
 still traversing; the lanes whose ray is done shade or add their light
@@ -19,7 +20,7 @@ from test_gpu_render import assert_images_match, gpu_render
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("refill", [1, 16, 48, 64])
+@pytest.mark.parametrize("refill", [0, 1, 16, 48, 64])
 def test_refill_icosphere_parity(krr, oracle, gpu, refill):
     print(f"case: refill_lanes={refill} icosphere field 240x135 depth 16", flush=True)
     s = krr.Scene(icosphere_scene(res=(240, 135)))
