@@ -590,10 +590,16 @@ def node_format_ab(cfg, frames=10, warmup=3):
 def refill_ab(cfg, values=(0, 2, 4, 6, 8, 12, 16, 32), frames=10, warmup=3):
     """The benchmark workload with the persistent launch's refill threshold
     (pass param refill_lanes: a wave's traversal round ends once fewer lanes
-    than this are still traversing; 0 runs every ray to its end) in one
+    than this are still traversing; 0 runs every ray to its end)."""
+    return param_sweep(cfg, "refill_lanes", values, frames, warmup)
+
+
+def param_sweep(cfg, key, values, frames=10, warmup=3):
+    """The benchmark workload with each value of one pass param in one
     process: ms per frame and Mrays/s (HIP events on the render stream, no
     host synchronisation inside the timed frames), each value with its own
-    autotuned launch shape.  The images do not depend on the value."""
+    autotuned launch shape.  For launch params the images do not depend on
+    the value."""
     import kiraray_amd as krr
     from kiraray_amd import hip
 
@@ -602,9 +608,9 @@ def refill_ab(cfg, values=(0, 2, 4, 6, 8, 12, 16, 32), frames=10, warmup=3):
     cam = scene.camera(0)
     film = hip.DeviceBuffer(w * h * 16)
     stream = hip.Stream()
-    out = {"workload": cfg["desc"], "frames": frames}
+    out = {"workload": cfg["desc"], "frames": frames, "param": key}
     for v in values:
-        wf = krr.WavefrontPathTracer(dict(cfg["params"], refill_lanes=v))
+        wf = krr.WavefrontPathTracer(dict(cfg["params"], **{key: v}))
         wf.set_scene(scene)
         wf.resize(w, h)
         wf.begin_frame(cam, 1)
@@ -625,7 +631,7 @@ def refill_ab(cfg, values=(0, 2, 4, 6, 8, 12, 16, 32), frames=10, warmup=3):
         wf.close()
         out[str(v)] = {"ms_per_frame": round(ms, 3), "mrays_per_s": round(rays / ms / 1e3, 1),
                        "launch": {k: tune["best"][k] for k in ("trace_block_size", "stack_lds_entries", "occupancy")}}
-        log(f"refill_lanes {v}: {out[str(v)]}")
+        log(f"{key} {v}: {out[str(v)]}")
     return out
 
 
@@ -758,7 +764,7 @@ def main():
     ap.add_argument("--no-nrc", action="store_true", help="skip the radiance-cache equal-time comparison")
     ap.add_argument("--no-multiview", action="store_true", help="skip the 256-view batched frame")
     ap.add_argument("--no-megakernel", action="store_true", help="skip the megakernel path tracer A/B")
-    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel", "nodes", "refill"],
+    ap.add_argument("--section", default="all", choices=["all", "nrc", "multiview", "megakernel", "nodes", "refill", "sweep"],
                     help="nrc / multiview: run only that comparison and print its JSON (profiling runs)")
     ap.add_argument("--nrc-warm", type=int, default=96, help="training frames before the NRC comparison")
     ap.add_argument("--nrc-frames", type=int, default=24, help="compared frames of the NRC comparison")
@@ -766,6 +772,8 @@ def main():
                     help="further frames of the NRC comparison for the converged relMSE and bias (0 for profiling runs)")
     ap.add_argument("--wavefront-depths", type=int, default=None,
                     help="run the first N depths of each path as per-depth queue kernels (the pass param wavefront_depths)")
+    ap.add_argument("--sweep-key", default="lds_nodes", help="--section sweep: the pass param to vary")
+    ap.add_argument("--sweep-values", default="0,21,85,341", help="--section sweep: comma-separated integer values")
     ap.add_argument("--refill-lanes", type=int, default=None,
                     help="the pass param refill_lanes of the benchmark workload (default: the pass default)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -796,6 +804,8 @@ def main():
             out = node_format_ab(cfg)
         elif args.section == "refill":
             out = refill_ab(cfg)
+        elif args.section == "sweep":
+            out = param_sweep(cfg, args.sweep_key, [int(v) for v in args.sweep_values.split(",")])
         else:
             out = multiview_bench()
         print(json.dumps(out), flush=True)

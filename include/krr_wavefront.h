@@ -323,6 +323,9 @@ typedef struct krr_wf krr_wf;
  *                            unfinished rays resume in the next round (0: each
  *                            ray runs to its end; 1..64); images and ray
  *                            counts do not depend on it
+ *   lds_nodes (0)            refill launch with float nodes: the first
+ *                            lds_nodes nodes of the tree (its top levels) are
+ *                            read from a per-block LDS copy (0..1024)
  *   graph (true)            record the multi-view frame into a HIP graph
  *                            once and replay it (krr_wf_render_views)
  * NULL or "" means all defaults. */

@@ -372,6 +372,8 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 // 8 traversing lanes of 64 took 4.12 ms per frame against 4.57 ms for rays
 // run to their end, 4.23 ms at 16 and 4.49 ms at 32.
 constexpr int kDefaultRefillLanes = 8;
+// Largest per-block LDS copy of the top tree nodes (128 KB of 128-byte nodes).
+constexpr int kMaxLdsNodes = 1024;
 
 // A traversal that stops and resumes between steps: the entry to take next,
 // the stack depth (the entries stay in the lane's stack), the step budget and
@@ -400,9 +402,14 @@ KRR_DEV void travInit(TravState &s, const TraceBvh &bvh, float tmax) {
 // is done.  any: the lane's ray is a shadow ray (stop at the first hit of a
 // non-null-material triangle); otherwise the closest hit is kept.  All lanes
 // of the wave call this together; the loop exit is wave-uniform.
+//
+// nodeLds: the first nLdsNodes nodes of the tree (float nodes; the top levels
+// of a breadth-first numbered tree) copied into LDS by the block; those are
+// read from there.
 template <bool COUNT, bool ALPHA>
 KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const bool any, bool &going, TravState &s,
-						   const LaneStack &stack, const int minLanes, uint32_t &nodeVisits, uint32_t &triTests) {
+						   const LaneStack &stack, const int minLanes, uint32_t &nodeVisits, uint32_t &triTests,
+						   const float4 *nodeLds = nullptr, int nLdsNodes = 0) {
 	const RayPre rp = rayPre(o, d);
 	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
 	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
@@ -487,12 +494,33 @@ KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const 
 			c2 = (int) D.x;
 			c3 = (int) D.y;
 		} else {
-			const uint32_t nb = (uint32_t) s.entry * 128u; // node array < 4 GiB (checked at build)
+			float4 nx, fx, ny, fy, nz, fz;
+			int4 ch;
+			if (s.entry < nLdsNodes) {
+				const float4 *np = nodeLds + (uint32_t) s.entry * 8u;
+				nx				 = np[nX >> 4];
+				fx				 = np[(nX ^ 16u) >> 4];
+				ny				 = np[nY >> 4];
+				fy				 = np[(nY ^ 16u) >> 4];
+				nz				 = np[nZ >> 4];
+				fz				 = np[(nZ ^ 16u) >> 4];
+				const float4 c	 = np[6];
+				// keeps these LDS reads ds_reads: without the barrier the compiler
+				// may merge both branches into flat loads of a selected address
+				asm volatile("" : "+v"(nx.x));
+				ch = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
+			} else {
+				const uint32_t nb = (uint32_t) s.entry * 128u; // node array < 4 GiB (checked at build)
 #define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
-			const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
-			const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
+				nx = KRR_PLANE(nX);
+				fx = KRR_PLANE(nX ^ 16u);
+				ny = KRR_PLANE(nY);
+				fy = KRR_PLANE(nY ^ 16u);
+				nz = KRR_PLANE(nZ);
+				fz = KRR_PLANE(nZ ^ 16u);
 #undef KRR_PLANE
-			const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
+				ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
+			}
 #define KRR_SLAB(kc, comp)                                                                                           \
 	{                                                                                                                \
 		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \

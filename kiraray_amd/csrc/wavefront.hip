@@ -1089,6 +1089,13 @@ __global__ void k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQ
 	extern __shared__ int ldsStack[];
 	const LaneStack stack = laneStack(bvh, ldsStack, ovf);
 	WfParams P			  = P0;
+	// the top P.ldsNodes nodes of the tree, copied into LDS behind the stack
+	float4 *nodeLds = reinterpret_cast<float4 *>(ldsStack + (size_t) (bvh.stackLds + 1) * blockDim.x);
+	if (P.ldsNodes > 0) {
+		const float4 *src = reinterpret_cast<const float4 *>(bvh.nodes);
+		for (int i = (int) threadIdx.x; i < P.ldsNodes * 8; i += (int) blockDim.x) nodeLds[i] = src[i];
+		__syncthreads();
+	}
 	if (VIEWS) {
 		P.inc = V.frame->inc;
 		if (NRC) nrc.phase = V.frame->phase;
@@ -1134,7 +1141,7 @@ __global__ void k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQ
 			}
 		}
 		if (__ballot(active) == 0) break;
-		traverseSteps<COUNT, !PLAIN>(bvh, to, td, shadowRay, going, ts, stack, exhausted ? 1 : P.refillLanes, nv, nt);
+		traverseSteps<COUNT, !PLAIN>(bvh, to, td, shadowRay, going, ts, stack, exhausted ? 1 : P.refillLanes, nv, nt, nodeLds, P.ldsNodes);
 		if (!active || going) continue;
 		bool next = false; // go on with the path's next closest ray
 		if (shadowRay) {
@@ -1586,17 +1593,18 @@ int reserveOverflow(DevBuf &buf, const AccelData &a, size_t threads) {
 
 // The persistent kernel variants of every frame kind.
 // REFILL: k_megakernel_refill (scenes without media).
+// nodeBytes: the refill launch's LDS copy of the top tree nodes, behind the stack.
 template <bool MEDIA, bool COUNT, bool NRC, bool VIEWS, bool PLAIN, bool REFILL = false> struct Mega {
-	static int grid(int tb, size_t lds, int maxBlocks) {
-		if constexpr (REFILL) return persistentGrid(k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks);
+	static int grid(int tb, size_t lds, int maxBlocks, size_t nodeBytes = 0) {
+		if constexpr (REFILL) return persistentGrid(k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>, tb, lds + nodeBytes, maxBlocks);
 		else return persistentGrid(k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>, tb, lds, maxBlocks);
 	}
 	static void launch(unsigned g, int tb, size_t lds, hipStream_t st, const WfParams &P, const DevScene &ds, const TraceBvh &bvh,
 					   const RayQueueDev &q, const int *countPtr, const PixelStateDev &px, int *ovf, Counters *cnt, const NrcDev &nrc,
 					   const ViewsDev &V, int startDepth = 0, const CameraDev &cam = CameraDev{}, int camFetch = 0) {
 		if constexpr (REFILL)
-			hipLaunchKernelGGL((k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, startDepth, px, ovf,
-							   cnt, nrc, V, cam, camFetch);
+			hipLaunchKernelGGL((k_megakernel_refill<COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds + (size_t) P.ldsNodes * sizeof(Bvh4Node), st, P,
+							   ds, bvh, q, countPtr, startDepth, px, ovf, cnt, nrc, V, cam, camFetch);
 		else
 			hipLaunchKernelGGL((k_megakernel<MEDIA, COUNT, NRC, VIEWS, PLAIN>), dim3(g), dim3(tb), lds, st, P, ds, bvh, q, countPtr, startDepth, px, ovf,
 							   cnt, nrc, V, cam, camFetch);
@@ -1634,6 +1642,10 @@ struct krr_wf {
 	// persistent launch without media: traversal rounds end when fewer than
 	// this many lanes of a wave are still traversing (0: each ray to its end)
 	int refillLanes = kDefaultRefillLanes;
+	// refill launch with float nodes: the first ldsNodes nodes of the tree
+	// (its top levels: the device builder numbers nodes level by level) are
+	// read from a per-block copy in LDS
+	int ldsNodes = 0;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
 	bool quantizedNodes = false; // "node_format": "float" (default) | "quantized"
@@ -1747,6 +1759,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 		wf->refillLanes		= (int) j.value("refill_lanes", (double) kDefaultRefillLanes);
+		wf->ldsNodes		= (int) j.value("lds_nodes", 0.0);
 		if (j.contains("nrc")) {
 			const Json &n = j.at("nrc");
 			if (n.isBool()) wf->nrcOn = n.asBool();
@@ -1787,6 +1800,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 	if (wf->refillLanes < 0 || wf->refillLanes > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "refill_lanes must be in [0, 64]");
+	if (wf->ldsNodes < 0 || wf->ldsNodes > kMaxLdsNodes) return krr::fail(KRR_ERR_BAD_CONFIG, "lds_nodes must be in [0, %d]", kMaxLdsNodes);
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
@@ -2054,6 +2068,13 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	return KRR_OK;
 }
 
+// Nodes of the tree the refill launch reads from LDS: lds_nodes, at most the
+// tree's node count, none with quantised nodes or without the refill launch.
+int ldsNodeCount(const krr_wf *wf) {
+	if (wf->ldsNodes <= 0 || wf->refillLanes <= 0 || wf->quantizedNodes) return 0;
+	return (int) std::min<int64_t>(wf->ldsNodes, wf->accel.nNodes);
+}
+
 // Kernel parameters of one frame (integrator.cpp:205-267 values).
 WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex) {
 	WfParams P;
@@ -2072,6 +2093,7 @@ WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex)
 	P.sampleIndex  = (uint32_t) (frameIndex * (uint64_t) wf->spp);
 	P.inc		   = ((uint64_t) P.sampleIndex << 1u) | 1u;
 	P.refillLanes  = wf->refillLanes;
+	P.ldsNodes	   = ldsNodeCount(wf);
 	return P;
 }
 
@@ -2365,6 +2387,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("refill_lanes", Json((double) wf->refillLanes));
+	j.set("lds_nodes", Json((double) wf->ldsNodes));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	j.set("occupancy", Json((double) wf->occupancy));
 	j.set("bvh_builder", Json(std::string(wf->deviceBuilder ? "device" : "host")));
@@ -2458,15 +2481,17 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
 							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ (plain ? 8 : 0) ^ (refill ? 16 : 0) ^
-							  ((uint64_t) (wf->occupancy * 1024.f) << 32);
+							  ((uint64_t) ldsNodeCount(wf) << 5) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (shapeKey != wf->gridKey) {
 		// Counting variants use more registers: size each grid for the variant launched.
 		wf->gClosest = count ? persistentGrid(k_trace_closest_queue<true>, tb, lds, gTmax) : persistentGrid(k_trace_closest_queue<false>, tb, lds, gTmax);
 		if (media) wf->gShadow = count ? persistentGrid(k_shadow_tr<true>, tb, lds, gTmax) : persistentGrid(k_shadow_tr<false>, tb, lds, gTmax);
 		else wf->gShadow = count ? persistentGrid(k_trace_shadow<true>, tb, lds, gTmax) : persistentGrid(k_trace_shadow<false>, tb, lds, gTmax);
 		wf->gShade = media ? persistentGrid(k_shade<true>, bs, 0, (int) gN) : persistentGrid(k_shade<false>, bs, 0, (int) gN);
-		wf->gMega = withMega<false>(media, count, false, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax); });
-		if (wf->nrcOn) wf->gMegaNrc = withMega<false>(media, count, true, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax); });
+		const size_t nodeBytes = (size_t) ldsNodeCount(wf) * sizeof(Bvh4Node);
+		wf->gMega = withMega<false>(media, count, false, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax, nodeBytes); });
+		if (wf->nrcOn)
+			wf->gMegaNrc = withMega<false>(media, count, true, plain, refill, [&](auto k) { return decltype(k)::grid(tb, lds, gTmax, nodeBytes); });
 		wf->gMega	 = scaledGrid(wf->gMega, wf->occupancy);
 		wf->gMegaNrc = scaledGrid(wf->gMegaNrc, wf->occupancy);
 		const int gMax = std::max(std::max(std::max(wf->gClosest, wf->gShadow), wf->gMega), wf->nrcOn ? wf->gMegaNrc : 0);
@@ -2630,10 +2655,12 @@ int viewsLaunch(krr_wf *wf, int N, bool nrc, ViewsLaunch &L) {
 	L.nrc			   = nrc;
 	const uint64_t key = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) wf->accel.stackSize << 50) ^
 						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^ (wf->sceneTextured ? 0 : 8) ^
-						 (wf->refillLanes > 0 ? 16 : 0) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
+						 (wf->refillLanes > 0 ? 16 : 0) ^ ((uint64_t) ldsNodeCount(wf) << 5) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (key != wf->viewsGridKey) {
 		const int maxBlocks = std::max(1, (N + tb - 1) / tb);
-		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks); });
+		const size_t nodeBytes = (size_t) ldsNodeCount(wf) * sizeof(Bvh4Node);
+		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, wf->refillLanes > 0,
+									 [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks, nodeBytes); });
 		wf->gMegaViews		= scaledGrid(g, wf->occupancy);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) wf->gMegaViews * tb)) return rc;
 		wf->viewsGridKey = key;
@@ -2909,7 +2936,9 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				const TraceBvh bvh = wf->accel.dev();
 				const size_t lds   = stackLds(wf->accel, t.tb);
 				const int maxBlocks = std::max(1, (N + t.tb - 1) / t.tb);
-				int g = withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks); });
+				const size_t nodeBytes = (size_t) ldsNodeCount(wf) * sizeof(Bvh4Node);
+				int g = withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0,
+										[&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks, nodeBytes); });
 				g	  = scaledGrid(g, t.occ);
 				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
 				// trials may cover fewer paths than pixels: pixel order, no miss record

@@ -56,7 +56,8 @@ def test_pass_params_roundtrip(krr):
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',
                                  '{"stack_lds_entries": -1}', '{"stack_lds_entries": 97}', '{"wavefront_depths": -1}',
                                  '{"occupancy": 0}', '{"occupancy": 1.5}', '{"bvh_builder": "gpu"}',
-                                 '{"nrc": {"spread": -1}}', '{"refill_lanes": -1}', '{"refill_lanes": 65}'])
+                                 '{"nrc": {"spread": -1}}', '{"refill_lanes": -1}', '{"refill_lanes": 65}',
+                                 '{"lds_nodes": -1}', '{"lds_nodes": 1025}'])
 def test_bad_params_are_reported(krr, bad):
     with pytest.raises(krr.KrrError) as e:
         krr.WavefrontPathTracer(bad)

@@ -112,3 +112,15 @@ def test_refill_views_equal_plain_launch(krr, gpu):
     for x, y in zip(ia, ib):
         assert np.array_equal(x, y)
 
+
+@pytest.mark.parametrize("lds_nodes", [21, 341])
+def test_lds_node_copy_parity(krr, oracle, gpu, lds_nodes):
+    """The refill launch reading the top tree nodes from a per-block LDS copy
+    ("lds_nodes") traces the same rays with the same hits."""
+    print(f"case: lds_nodes={lds_nodes} icosphere field 240x135 depth 16", flush=True)
+    s = krr.Scene(icosphere_scene(res=(240, 135)))
+    params = {"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3, "lds_nodes": lds_nodes}
+    g, gs = gpu_render(krr, gpu, s, 240, 135, params)
+    o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
