@@ -28,6 +28,7 @@
 //   one-blob(4) of x in [0,1]: exp(-0.5 (4 (x - (i + 0.5) / 4))^2), i < 4.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -162,21 +163,23 @@ __device__ __forceinline__ float16v outputForward(const uint16_t *wt5, int lane,
 }
 
 // Copies `rows` dense rows of rowLen bf16 into LDS rows of rowLen + kPad.
-// Each thread issues eight 8-byte loads before it stores them: one load per
-// store would put a full memory round trip between consecutive loads, and
-// the training kernel runs one wave per SIMD with nothing to hide it.
+// Each thread issues up to twelve 16-byte loads before it stores them (two
+// 8-byte LDS writes each: padded rows are 8-byte aligned), so a block of 256
+// threads fetches one copy of all layers' weights (~48 KB) in one round trip.  One
+// load per store would put a memory round trip between consecutive loads,
+// and the training kernel runs one wave per SIMD with nothing to hide it.
 __device__ void loadRows(uint16_t *lds, const uint16_t *src, int rows, int rowLen) {
-	constexpr int kBatch = 8;
-	const int per = rowLen / 4; // 8-byte pieces per row
+	constexpr int kBatch = 12;
+	const int per = rowLen / 8; // 16-byte pieces per row
 	const int total = rows * per, stride = (int) blockDim.x;
 	for (int base = (int) threadIdx.x; base < total; base += kBatch * stride) {
-		uint2 v[kBatch];
+		uint4 v[kBatch];
 #pragma unroll
 		for (int k = 0; k < kBatch; k++) {
 			const int i = base + k * stride;
 			if (i < total) {
 				const int r = i / per, c = i - r * per;
-				v[k]		= *reinterpret_cast<const uint2 *>(src + r * rowLen + 4 * c);
+				v[k]		= *reinterpret_cast<const uint4 *>(src + r * rowLen + 8 * c);
 			}
 		}
 #pragma unroll
@@ -184,7 +187,9 @@ __device__ void loadRows(uint16_t *lds, const uint16_t *src, int rows, int rowLe
 			const int i = base + k * stride;
 			if (i < total) {
 				const int r = i / per, c = i - r * per;
-				*reinterpret_cast<uint2 *>(lds + r * (rowLen + kPad) + 4 * c) = v[k];
+				uint2 *d = reinterpret_cast<uint2 *>(lds + r * (rowLen + kPad) + 8 * c);
+				d[0]	 = make_uint2(v[k].x, v[k].y);
+				d[1]	 = make_uint2(v[k].z, v[k].w);
 			}
 		}
 	}
@@ -358,6 +363,11 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 		const int64_t i	  = t * 32 + m;
 		const bool valid = i < n;
 		float v[16];
+		float tg[3] = {0.f, 0.f, 0.f}; // loaded with the inputs: the loss needs them after the forward pass
+		if (valid && g == 0) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) tg[c] = targets[3 * i + c];
+		}
 		if (valid) {
 			const float4 *p = reinterpret_cast<const float4 *>(vertices + i * 16);
 #pragma unroll
@@ -400,7 +410,7 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 		if (g == 0 && valid) {
 #pragma unroll
 			for (int c = 0; c < 3; c++) {
-				const float yv = o[c], r = yv - targets[3 * i + c];
+				const float yv = o[c], r = yv - tg[c];
 				const float den = yv * yv + lossEps;
 				lsum += r * r / den;
 				d[c] = 2.0f * r / den * invCount;
@@ -640,6 +650,7 @@ struct krr_nrc {
 	uint64_t seed	= 1;
 	int64_t maxBatch = 65536, ld = 0;
 	bool bwdLds		 = false; // the device's blocks hold both padded weight copies in LDS
+	int inferBlocks	 = 1024;  // k_infer_queue blocks resident at once on the device (each loads the weights once)
 	Buf w, m, v, grad, wt, wb, xT, dzT, partial, tileLoss, loss, step;
 };
 
@@ -760,6 +771,10 @@ int krr_nrc_create(const char *json, krr_nrc **out) {
 		NRC_HIP(hipGetDevice(&dev));
 		NRC_HIP(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
 		h->bwdLds = maxLds >= (kFwdLdsElems + kBwdLdsElems) * (int) sizeof(uint16_t);
+		int cus = 0, perCu = 0;
+		NRC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+		NRC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, k_infer_queue, 256, kFwdLdsElems * sizeof(uint16_t)));
+		h->inferBlocks = std::max(1, cus * std::max(perCu, 1));
 	}
 	NRC_HIP(h->wt.alloc(sizeof(uint16_t) * kFwdElems));
 	NRC_HIP(h->wb.alloc(sizeof(uint16_t) * kBwdElems));
@@ -842,7 +857,9 @@ int krr_nrc_gradient(krr_nrc *nrc, const krr_nrc_vertex *vertices_dev, const flo
 
 int nrcInferAccumulate(krr_nrc *nrc, const float4 *queries, int strideF4, const int *count, int capacity, float4 *L, hipStream_t st) {
 	if (capacity <= 0) return KRR_OK;
-	const int blocks = std::max(1, std::min(1024, (capacity + 127) / 128));
+	// one resident wave of blocks: each block loads the weights into LDS once
+	// and then takes query tiles until the queue is done
+	const int blocks = std::max(1, std::min(nrc->inferBlocks, (capacity + 127) / 128));
 	hipLaunchKernelGGL(k_infer_queue, dim3(blocks), dim3(256), kFwdLdsElems * (int) sizeof(uint16_t), st, nrc->wt.as<uint16_t>(), queries, strideF4,
 					   count, capacity, L);
 	NRC_HIP(hipGetLastError());
