@@ -323,6 +323,10 @@ typedef struct krr_wf krr_wf;
  *                            unfinished rays resume in the next round (0: each
  *                            ray runs to its end; 1..64); images and ray
  *                            counts do not depend on it
+ *   leaf_share (35)          refill launch: an iteration of the traversal
+ *                            takes only leaves once this percentage of the
+ *                            traversing lanes holds one, else only inner
+ *                            nodes (0: both every iteration)
  *   lds_nodes (0)            refill launch with float nodes: the first
  *                            lds_nodes nodes of the tree (its top levels) are
  *                            read from a per-block LDS copy (0..1024)

@@ -118,6 +118,7 @@ struct WfParams {
 	uint32_t sampleIndex; // frameIndex * spp (as uint)
 	int refillLanes;	  // persistent launch without media: end a traversal round below this many traversing lanes (1..64)
 	int ldsNodes;		  // refill launch: tree nodes 0..ldsNodes-1 are read from a per-block copy in LDS
+	int leafShare;		  // refill launch: leaf-only iterations once this percent of traversing lanes hold a leaf (0: off)
 };
 
 // Values that change from frame to frame, read from device memory by the

@@ -372,6 +372,12 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 // 8 traversing lanes of 64 took 4.12 ms per frame against 4.57 ms for rays
 // run to their end, 4.23 ms at 16 and 4.49 ms at 32.
 constexpr int kDefaultRefillLanes = 8;
+// Default leaf share of the refill launch's traversal (pass param leaf_share):
+// on the same field, two sweeps on two GPUs: iterations that take only leaves
+// once 30-50 % of the traversing lanes hold one ran 3.86-3.99 ms per frame
+// (35 %: 3.90 and 3.93) against 4.08 and 4.12 ms for iterations that take both
+// kinds of entry (20 %: 4.02, 65 %: 4.09, 80 %: 4.40).
+constexpr int kDefaultLeafShare = 35;
 // Largest per-block LDS copy of the top tree nodes (128 KB of 128-byte nodes).
 constexpr int kMaxLdsNodes = 1024;
 
@@ -406,10 +412,16 @@ KRR_DEV void travInit(TravState &s, const TraceBvh &bvh, float tmax) {
 // nodeLds: the first nLdsNodes nodes of the tree (float nodes; the top levels
 // of a breadth-first numbered tree) copied into LDS by the block; those are
 // read from there.
+//
+// leafShare (percent, 0 = off): an iteration takes either the leaves or the
+// inner nodes of the traversing lanes, not both: leaves once at least
+// leafShare % of the traversing lanes hold one, inner nodes otherwise; the
+// other lanes wait.  A mixed iteration runs both code paths, each for part of
+// the wave.  Each ray still takes its own entries in its own order.
 template <bool COUNT, bool ALPHA>
 KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const bool any, bool &going, TravState &s,
 						   const LaneStack &stack, const int minLanes, uint32_t &nodeVisits, uint32_t &triTests,
-						   const float4 *nodeLds = nullptr, int nLdsNodes = 0) {
+						   const float4 *nodeLds = nullptr, int nLdsNodes = 0, int leafShare = 0) {
 	const RayPre rp = rayPre(o, d);
 	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
 	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
@@ -418,8 +430,16 @@ KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const 
 	const float kTie   = 1.00000095367431640625f; // 1 + 2^-20
 	while (true) {
 		const uint64_t lanes = __ballot(going);
-		if (lanes == 0 || __popcll(lanes) < minLanes) break;
-		if (!going) continue;
+		const int nGoing	 = __popcll(lanes);
+		if (lanes == 0 || nGoing < minLanes) break;
+		// wave-uniform: with leafShare 0 both kinds of entry are taken
+		bool takeLeaf = true, takeNode = true;
+		if (leafShare > 0) {
+			const int nLeaf = __popcll(__ballot(going && s.entry < 0));
+			takeLeaf		= nLeaf * 100 >= leafShare * nGoing; // all lanes at leaves: true; none: false
+			takeNode		= !takeLeaf;
+		}
+		if (!going || !(s.entry < 0 ? takeLeaf : takeNode)) continue;
 		if (--s.budget < 0) {
 			going = false;
 			continue;

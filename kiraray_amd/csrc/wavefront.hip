@@ -1141,7 +1141,8 @@ __global__ void k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQ
 			}
 		}
 		if (__ballot(active) == 0) break;
-		traverseSteps<COUNT, !PLAIN>(bvh, to, td, shadowRay, going, ts, stack, exhausted ? 1 : P.refillLanes, nv, nt, nodeLds, P.ldsNodes);
+		traverseSteps<COUNT, !PLAIN>(bvh, to, td, shadowRay, going, ts, stack, exhausted ? 1 : P.refillLanes, nv, nt, nodeLds, P.ldsNodes,
+									 P.leafShare);
 		if (!active || going) continue;
 		bool next = false; // go on with the path's next closest ray
 		if (shadowRay) {
@@ -1646,6 +1647,9 @@ struct krr_wf {
 	// (its top levels: the device builder numbers nodes level by level) are
 	// read from a per-block copy in LDS
 	int ldsNodes = 0;
+	// refill launch: iterations take leaves once this share (percent) of the
+	// traversing lanes hold one, inner nodes otherwise (0: both every iteration)
+	int leafShare = kDefaultLeafShare;
 	float rr = 0.8f, clampMax = 1e3f, sahNodeCost = kSahNodeCost;
 	bool deviceBuilder = true; // "bvh_builder": "device" | "host"
 	bool quantizedNodes = false; // "node_format": "float" (default) | "quantized"
@@ -1760,6 +1764,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
 		wf->refillLanes		= (int) j.value("refill_lanes", (double) kDefaultRefillLanes);
 		wf->ldsNodes		= (int) j.value("lds_nodes", 0.0);
+		wf->leafShare		= (int) j.value("leaf_share", (double) kDefaultLeafShare);
 		if (j.contains("nrc")) {
 			const Json &n = j.at("nrc");
 			if (n.isBool()) wf->nrcOn = n.asBool();
@@ -1801,6 +1806,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
 	if (wf->refillLanes < 0 || wf->refillLanes > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "refill_lanes must be in [0, 64]");
 	if (wf->ldsNodes < 0 || wf->ldsNodes > kMaxLdsNodes) return krr::fail(KRR_ERR_BAD_CONFIG, "lds_nodes must be in [0, %d]", kMaxLdsNodes);
+	if (wf->leafShare < 0 || wf->leafShare > 100) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_share must be in [0, 100]");
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
 	if (wf->nrcBatch < 1024 || wf->nrcBatch > 1 << 22) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.batch must be in [1024, 2^22]");
 	if (wf->nrcSteps < 1 || wf->nrcSteps > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.steps_per_frame must be in [1, 64]");
@@ -2094,6 +2100,7 @@ WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex)
 	P.inc		   = ((uint64_t) P.sampleIndex << 1u) | 1u;
 	P.refillLanes  = wf->refillLanes;
 	P.ldsNodes	   = ldsNodeCount(wf);
+	P.leafShare	   = wf->leafShare;
 	return P;
 }
 
@@ -2388,6 +2395,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("refill_lanes", Json((double) wf->refillLanes));
 	j.set("lds_nodes", Json((double) wf->ldsNodes));
+	j.set("leaf_share", Json((double) wf->leafShare));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
 	j.set("occupancy", Json((double) wf->occupancy));
 	j.set("bvh_builder", Json(std::string(wf->deviceBuilder ? "device" : "host")));

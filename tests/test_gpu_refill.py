@@ -124,3 +124,16 @@ def test_lds_node_copy_parity(krr, oracle, gpu, lds_nodes):
     o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
+
+
+@pytest.mark.parametrize("leaf_share", [0, 30, 70])
+def test_leaf_phase_parity(krr, oracle, gpu, leaf_share):
+    """Iterations that take either leaves or inner nodes ("leaf_share")
+    trace the same rays with the same hits."""
+    print(f"case: leaf_share={leaf_share} icosphere field 240x135 depth 16", flush=True)
+    s = krr.Scene(icosphere_scene(res=(240, 135)))
+    params = {"spp": 1, "max_depth": 16, "rr": 0.8, "rr_min_depth": 3, "leaf_share": leaf_share}
+    g, gs = gpu_render(krr, gpu, s, 240, 135, params)
+    o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
+    assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
+    assert_images_match(g, o)
