@@ -312,15 +312,24 @@ __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, cons
 }
 
 // Writes the B-layout bf16 operand (features 8s + 4g + j of sample m) to a
-// feature-major [feature][ld] array at column `col`.
+// feature-major [feature][ld] array at column `col` (`dst` is a layer's
+// 64 x ld block).  Buffer stores: the lane's column offset is computed once
+// and the feature row's offset, (8s + j) ld, is wave-uniform (soffset), so a
+// store takes no per-lane address arithmetic; stores past the block are
+// dropped by the buffer's range check.  The rows of features >= `features`
+// (s >= features / 8, for every g) are not written.
 __device__ __forceinline__ void storeFeatureMajor(uint16_t *dst, int64_t ld, int64_t col, int g, const short4v x[8], int features = 64) {
+	// ld <= 2^24 (max_batch): the largest row offset, 63 ld bf16, stays below
+	// 2^31 bytes; the range, 64 ld bf16 <= 2^31 bytes, is read as unsigned
+	const int ld32 = (int) ld;
+	const uint32_t range = 64u * (uint32_t) ld32 * (uint32_t) sizeof(uint16_t);
+	const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(dst, (short) 0, (int) range, 0x00020000);
+	const int voff = (4 * g * ld32 + (int) col) * (int) sizeof(uint16_t);
 #pragma unroll
 	for (int s = 0; s < 8; s++)
 #pragma unroll
-		for (int j = 0; j < 4; j++) {
-			const int f = 8 * s + 4 * g + j;
-			if (f < features) dst[(int64_t) f * ld + col] = (uint16_t) x[s][j];
-		}
+		for (int j = 0; j < 4; j++)
+			if (8 * s < features) __builtin_amdgcn_raw_buffer_store_b16((unsigned short) x[s][j], rsrc, voff, (8 * s + j) * ld32 * (int) sizeof(uint16_t), 0);
 }
 
 // Forward + backward of one tile of 32 samples per wave.  Writes X_l^T and
