@@ -5,12 +5,12 @@
 // carries two constant 1 inputs).  W_l is stored [in][out].
 //
 // MFMA formulation.  Each wave owns a tile of 32 samples and computes the
-// transposed products Z^T = W^T X^T with v_mfma_f32_32x32x8_bf16: the A
-// operand is a 32x8 slice of W^T (LDS), the B operand an 8x32 slice of X^T
-// held by the lanes (lane m + 32g holds features 8s + 4g + j, j < 4, of
-// sample m for k-step s), and the 32x32 accumulator tile leaves feature
-// 8(r/4) + 4g + r%4 of sample m in register r of lane m + 32g - exactly the
-// B-operand layout of the next layer.  Activations therefore never leave the
+// transposed products Z^T = W^T X^T in 8-deep k-steps: the A operand is a
+// 32x8 slice of W^T (LDS), the B operand an 8x32 slice of X^T held by the
+// lanes (lane m + 32g holds features 8s + 4g + j, j < 4, of sample m for
+// k-step s), and the 32x32 accumulator tile leaves feature 8(r/4) + 4g + r%4
+// of sample m in register r of lane m + 32g - exactly the B-operand layout of
+// the next layer.  Pairs of k-steps run as one v_mfma_f32_32x32x16_bf16.  Activations therefore never leave the
 // registers between layers.  Backward uses the same layout with A = W
 // (dX^T = W dZ^T).  Weight gradients dW_l = X_l dZ_l^T are a split-K GEMM over
 // the batch on X^T and dZ^T stored feature-major in HBM, reduced in a fixed
@@ -46,11 +46,14 @@ namespace nrc {
 
 // ---------------------------------------------------------------- device
 
+// Round to nearest even; inf stays inf and a nan stays a (quiet) nan.  Both
+// results are computed and one selected: a branch here would diverge once
+// per converted value.
 __device__ __forceinline__ uint16_t toBf16(float f) {
-	uint32_t u = __float_as_uint(f);
-	if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t) ((u >> 16) | ((u & 0xffff) ? 0x40 : 0)); // inf / nan
-	u += 0x7fffu + ((u >> 16) & 1u);																// round to nearest even
-	return (uint16_t) (u >> 16);
+	const uint32_t u	   = __float_as_uint(f);
+	const uint32_t nearest = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+	const uint32_t special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+	return (uint16_t) (((u & 0x7f800000u) == 0x7f800000u) ? special : nearest);
 }
 __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float((uint32_t) h << 16); }
 
@@ -128,7 +131,19 @@ __device__ __forceinline__ float16v zero16() {
 	return z;
 }
 
-__device__ __forceinline__ float16v mfma(short4v a, short4v b, float16v c) { return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, c, 0, 0, 0); }
+// Two consecutive 8-deep K-steps (4 bf16 per lane half each) in one gfx950
+// v_mfma_f32_32x32x16_bf16: lane half g passes the A row elements (a0, a1)
+// and the B column elements (b0, b1) in the same order, so the 16 products
+// summed are those of the two 32x32x8 steps (a0, b0) and (a1, b1); only the
+// FP32 order of the additions inside the instruction differs.  Half the
+// instructions of the 32x32x8 form, each at the same rate.
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+typedef short short8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ float16v mfma2(short4v a0, short4v a1, short4v b0, short4v b1, float16v c) {
+	const short8v a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+	const short8v b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+	return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, b), c, 0, 0, 0);
+}
 
 // A operand: 4 consecutive bf16 at row `row` (rows `stride` bf16 apart), column 8s + 4g.
 __device__ __forceinline__ short4v ldsA(const uint16_t *m, int row, int s, int g, int stride = kRowH) {
@@ -142,9 +157,9 @@ __device__ __forceinline__ void hiddenForward(const uint16_t *wt, int lane, int 
 	acc1 = zero16();
 	const int r0 = lane & 31;
 #pragma unroll
-	for (int s = 0; s < 8; s++) {
-		acc0 = mfma(ldsA(wt, r0, s, g), x[s], acc0);
-		acc1 = mfma(ldsA(wt, 32 + r0, s, g), x[s], acc1);
+	for (int s = 0; s < 8; s += 2) {
+		acc0 = mfma2(ldsA(wt, r0, s, g), ldsA(wt, r0, s + 1, g), x[s], x[s + 1], acc0);
+		acc1 = mfma2(ldsA(wt, 32 + r0, s, g), ldsA(wt, 32 + r0, s + 1, g), x[s], x[s + 1], acc1);
 	}
 #pragma unroll
 	for (int s = 0; s < 4; s++) {
@@ -158,7 +173,7 @@ __device__ __forceinline__ float16v outputForward(const uint16_t *wt5, int lane,
 	float16v acc = zero16();
 	const int r0 = lane & 31;
 #pragma unroll
-	for (int s = 0; s < 8; s++) acc = mfma(ldsA(wt5, r0, s, g), x[s], acc);
+	for (int s = 0; s < 8; s += 2) acc = mfma2(ldsA(wt5, r0, s, g), ldsA(wt5, r0, s + 1, g), x[s], x[s + 1], acc);
 	return acc;
 }
 
@@ -441,9 +456,9 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 			a0 = zero16();
 			a1 = zero16();
 #pragma unroll
-			for (int s = 0; s < 4; s++) {
-				a0 = mfma(ldsA(w5, r0, s, g, sO), dz[s], a0);
-				a1 = mfma(ldsA(w5, 32 + r0, s, g, sO), dz[s], a1);
+			for (int s = 0; s < 4; s += 2) {
+				a0 = mfma2(ldsA(w5, r0, s, g, sO), ldsA(w5, r0, s + 1, g, sO), dz[s], dz[s + 1], a0);
+				a1 = mfma2(ldsA(w5, 32 + r0, s, g, sO), ldsA(w5, 32 + r0, s + 1, g, sO), dz[s], dz[s + 1], a1);
 			}
 		}
 #pragma unroll 1
@@ -463,9 +478,9 @@ __global__ void __launch_bounds__(256) k_train_tile(const uint16_t *wtAll, const
 				a0				  = zero16();
 				a1				  = zero16();
 #pragma unroll
-				for (int s = 0; s < 8; s++) {
-					a0 = mfma(ldsA(w, r0, s, g, sH), dz[s], a0);
-					a1 = mfma(ldsA(w, 32 + r0, s, g, sH), dz[s], a1);
+				for (int s = 0; s < 8; s += 2) {
+					a0 = mfma2(ldsA(w, r0, s, g, sH), ldsA(w, r0, s + 1, g, sH), dz[s], dz[s + 1], a0);
+					a1 = mfma2(ldsA(w, 32 + r0, s, g, sH), ldsA(w, 32 + r0, s + 1, g, sH), dz[s], dz[s + 1], a1);
 				}
 			}
 		}
@@ -501,7 +516,7 @@ __global__ void __launch_bounds__(256) k_weight_grad(const uint16_t *xTAll, cons
 			b[u] = *reinterpret_cast<const short4v *>(brow + 8 * (s0 + u) + 4 * g);
 		}
 #pragma unroll
-		for (int u = 0; u < kU; u++) acc = mfma(a[u], b[u], acc);
+		for (int u = 0; u < kU; u += 2) acc = mfma2(a[u], a[u + 1], b[u], b[u + 1], acc);
 	}
 	// D[i = k][j = n]: lane holds column n = r0, rows k = 8(r/4) + 4g + r%4
 	float *dst = partialAll + (int64_t) layer * chunks * 4096 + (int64_t) blockIdx.y * 64 * nOut;
