@@ -316,12 +316,13 @@ typedef struct krr_wf krr_wf;
  *                            Needs wavefront_depths = 0 and no media.
  *   occupancy (1.0)          fraction of the resident blocks a persistent
  *                            launch uses (in-flight path capacity)
- *   refill_lanes (8)         persistent launch without media: a wave's
+ *   refill_lanes (-1)        persistent launch without media: a wave's
  *                            traversal round ends once fewer than this many
  *                            of its 64 lanes are still traversing; the
  *                            others shade or take their next ray, and the
  *                            unfinished rays resume in the next round (0: each
- *                            ray runs to its end; 1..64); images and ray
+ *                            ray runs to its end; 1..64; -1: 8 for trees of
+ *                            at least 4096 nodes, else 0); images and ray
  *                            counts do not depend on it
  *   leaf_share (35)          refill launch: an iteration of the traversal
  *                            takes only leaves once this percentage of the
@@ -435,8 +436,9 @@ int krr_wf_render_views(krr_wf *wf, uint64_t frame_index, void *hip_stream);
 /* Times persistent-launch shapes - trace_block_size {64, 128, 256} x
  * stack_lds_entries {8, 16, 24} x occupancy {1, 0.75, 0.5} - on the first
  * trial_paths (<= 0: 2^21) camera paths of the views (or of the single view,
- * after krr_wf_begin_frame) and keeps the fastest.  Images do not depend on
- * the choice.  Writes a JSON report (candidates with ms, best) into report
+ * after krr_wf_begin_frame), then (scenes without media, refill_lanes -1)
+ * refill_lanes {0, 4, 8, 16} at the fastest shape, and keeps the fastest.  Images do not
+ * depend on the choice.  Writes a JSON report (candidates with ms, best) into report
  * (cap bytes) and returns its full length.  No reference counterpart. */
 int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, void *hip_stream);
 

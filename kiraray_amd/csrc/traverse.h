@@ -372,6 +372,8 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 // 8 traversing lanes of 64 took 4.12 ms per frame against 4.57 ms for rays
 // run to their end, 4.23 ms at 16 and 4.49 ms at 32.
 constexpr int kDefaultRefillLanes = 8;
+// Smallest tree (nodes) for which refill_lanes -1 turns the refill launch on.
+constexpr int64_t kRefillMinNodes = 4096;
 // Default leaf share of the refill launch's traversal (pass param leaf_share):
 // on the same field, two sweeps on two GPUs: iterations that take only leaves
 // once 30-50 % of the traversing lanes hold one ran 3.86-3.99 ms per frame

@@ -1642,7 +1642,7 @@ struct krr_wf {
 	int stackLdsEntries = kShortStack;
 	// persistent launch without media: traversal rounds end when fewer than
 	// this many lanes of a wave are still traversing (0: each ray to its end)
-	int refillLanes = kDefaultRefillLanes;
+	int refillLanes = -1; // -1: by the tree's size (refillLanesOf)
 	// refill launch with float nodes: the first ldsNodes nodes of the tree
 	// (its top levels: the device builder numbers nodes level by level) are
 	// read from a per-block copy in LDS
@@ -1762,7 +1762,7 @@ int parseParams(krr_wf *wf, const char *json) {
 		wf->leafSize	   = (int) j.value("leaf_size", 4.0);
 		wf->wavefrontDepths = (int) j.value("wavefront_depths", 0.0);
 		wf->stackLdsEntries = (int) j.value("stack_lds_entries", (double) kShortStack);
-		wf->refillLanes		= (int) j.value("refill_lanes", (double) kDefaultRefillLanes);
+		wf->refillLanes		= (int) j.value("refill_lanes", -1.0);
 		wf->ldsNodes		= (int) j.value("lds_nodes", 0.0);
 		wf->leafShare		= (int) j.value("leaf_share", (double) kDefaultLeafShare);
 		if (j.contains("nrc")) {
@@ -1804,7 +1804,7 @@ int parseParams(krr_wf *wf, const char *json) {
 	if (wf->leafSize < 1 || wf->leafSize > 8) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_size must be in [1, 8]");
 	if (wf->wavefrontDepths < 0) return krr::fail(KRR_ERR_BAD_CONFIG, "wavefront_depths must be >= 0");
 	if (wf->stackLdsEntries < 0 || wf->stackLdsEntries > 96) return krr::fail(KRR_ERR_BAD_CONFIG, "stack_lds_entries must be in [0, 96]");
-	if (wf->refillLanes < 0 || wf->refillLanes > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "refill_lanes must be in [0, 64]");
+	if (wf->refillLanes < -1 || wf->refillLanes > 64) return krr::fail(KRR_ERR_BAD_CONFIG, "refill_lanes must be -1 (by scene) or in [0, 64]");
 	if (wf->ldsNodes < 0 || wf->ldsNodes > kMaxLdsNodes) return krr::fail(KRR_ERR_BAD_CONFIG, "lds_nodes must be in [0, %d]", kMaxLdsNodes);
 	if (wf->leafShare < 0 || wf->leafShare > 100) return krr::fail(KRR_ERR_BAD_CONFIG, "leaf_share must be in [0, 100]");
 	if (wf->nrcTrainMod < 1 || wf->nrcTrainMod > 1 << 16) return krr::fail(KRR_ERR_BAD_CONFIG, "nrc.train_mod must be in [1, 65536]");
@@ -2074,10 +2074,20 @@ int uploadScene(krr_wf *wf, const krr_scene_desc *s) {
 	return KRR_OK;
 }
 
+// The refill threshold in force: refill_lanes, or with -1 (the default)
+// kDefaultRefillLanes for trees of at least kRefillMinNodes nodes and 0 (each
+// ray to its end) for smaller ones, whose short traversals do not pay for the
+// rounds (36-triangle Cornell box 1024^2: 0.87 ms per frame without refill,
+// 1.20-1.32 ms with 4-16).
+int refillLanesOf(const krr_wf *wf) {
+	if (wf->refillLanes >= 0) return wf->refillLanes;
+	return wf->accel.nNodes >= kRefillMinNodes ? kDefaultRefillLanes : 0;
+}
+
 // Nodes of the tree the refill launch reads from LDS: lds_nodes, at most the
 // tree's node count, none with quantised nodes or without the refill launch.
 int ldsNodeCount(const krr_wf *wf) {
-	if (wf->ldsNodes <= 0 || wf->refillLanes <= 0 || wf->quantizedNodes) return 0;
+	if (wf->ldsNodes <= 0 || refillLanesOf(wf) <= 0 || wf->quantizedNodes) return 0;
 	return (int) std::min<int64_t>(wf->ldsNodes, wf->accel.nNodes);
 }
 
@@ -2098,7 +2108,7 @@ WfParams frameParams(const krr_wf *wf, int W, int H, int N, uint64_t frameIndex)
 	P.legacySpp	   = wf->legacySpp ? 1 : 0;
 	P.sampleIndex  = (uint32_t) (frameIndex * (uint64_t) wf->spp);
 	P.inc		   = ((uint64_t) P.sampleIndex << 1u) | 1u;
-	P.refillLanes  = wf->refillLanes;
+	P.refillLanes  = refillLanesOf(wf);
 	P.ldsNodes	   = ldsNodeCount(wf);
 	P.leafShare	   = wf->leafShare;
 	return P;
@@ -2394,6 +2404,7 @@ int krr_wf_get_params(const krr_wf *wf, char *buf, int32_t cap) {
 	j.set("wavefront_depths", Json((double) wf->wavefrontDepths));
 	j.set("stack_lds_entries", Json((double) wf->stackLdsEntries));
 	j.set("refill_lanes", Json((double) wf->refillLanes));
+	j.set("refill_lanes_in_force", Json((double) refillLanesOf(wf))); // -1 resolved by the scene set
 	j.set("lds_nodes", Json((double) wf->ldsNodes));
 	j.set("leaf_share", Json((double) wf->leafShare));
 	j.set("sah_node_cost", Json((double) wf->sahNodeCost));
@@ -2485,7 +2496,7 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 	const bool count   = wf->countTraversal;
 	const bool media   = wf->enableMedium && wf->sceneHasMedia;
 	const bool plain   = !wf->sceneTextured;
-	const bool refill  = wf->refillLanes > 0;
+	const bool refill  = refillLanesOf(wf) > 0;
 	// persistent grid sizes (resident blocks), computed once per shape
 	const uint64_t shapeKey = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) bs << 2) ^ ((uint64_t) wf->accel.stackSize << 50) ^ ((uint64_t) wf->stackLdsEntries << 43) ^
 							  (count ? 2 : 0) ^ (media ? 1 : 0) ^ (wf->nrcOn ? 4 : 0) ^ (plain ? 8 : 0) ^ (refill ? 16 : 0) ^
@@ -2663,11 +2674,11 @@ int viewsLaunch(krr_wf *wf, int N, bool nrc, ViewsLaunch &L) {
 	L.nrc			   = nrc;
 	const uint64_t key = ((uint64_t) N << 24) ^ ((uint64_t) tb << 12) ^ ((uint64_t) wf->accel.stackSize << 50) ^
 						 ((uint64_t) wf->stackLdsEntries << 43) ^ (L.count ? 2 : 0) ^ (L.media ? 1 : 0) ^ (nrc ? 4 : 0) ^ (wf->sceneTextured ? 0 : 8) ^
-						 (wf->refillLanes > 0 ? 16 : 0) ^ ((uint64_t) ldsNodeCount(wf) << 5) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
+						 (refillLanesOf(wf) > 0 ? 16 : 0) ^ ((uint64_t) ldsNodeCount(wf) << 5) ^ ((uint64_t) (wf->occupancy * 1024.f) << 32);
 	if (key != wf->viewsGridKey) {
 		const int maxBlocks = std::max(1, (N + tb - 1) / tb);
 		const size_t nodeBytes = (size_t) ldsNodeCount(wf) * sizeof(Bvh4Node);
-		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, wf->refillLanes > 0,
+		const int g = withMega<true>(L.media, L.count, nrc, !wf->sceneTextured, refillLanesOf(wf) > 0,
 									 [&](auto k) { return decltype(k)::grid(tb, L.lds, maxBlocks, nodeBytes); });
 		wf->gMegaViews		= scaledGrid(g, wf->occupancy);
 		if (int rc = reserveOverflow(wf->ovf, wf->accel, (size_t) wf->gMegaViews * tb)) return rc;
@@ -2729,7 +2740,7 @@ int enqueueViews(krr_wf *wf, const WfParams &P, const ViewsLaunch &L, hipStream_
 			hipLaunchKernelGGL(k_tile_pixels_views, dim3(nT), dim3(64), 0, st, sorted, off, V, wf->vOrder.as<int>());
 		}
 		timed(4, [&] {
-			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
+			withMega<true>(L.media, L.count, L.nrc, !wf->sceneTextured, refillLanesOf(wf) > 0, [&](auto k) {
 				decltype(k)::launch(L.grid, tb, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, px, ovf, cnt, nrc, V);
 				return 0;
 			});
@@ -2897,7 +2908,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 	const int total = views ? wf->NV : wf->W * wf->H;
 	const int N		= (int) std::min<int64_t>(total, trial_paths);
 	const bool media = wf->enableMedium && wf->sceneHasMedia;
-	const int saveTb = wf->traceBlockSize, saveLds = wf->stackLdsEntries;
+	const int saveTb = wf->traceBlockSize, saveLds = wf->stackLdsEntries, saveRefill = wf->refillLanes;
 	const float saveOcc = wf->occupancy;
 	const bool saveCount = wf->countTraversal;
 	wf->countTraversal	 = false;
@@ -2906,6 +2917,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 	struct Trial {
 		int tb, lds;
 		float occ;
+		int refill;
 		float ms;
 	};
 	std::vector<Trial> trials;
@@ -2917,6 +2929,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 		wf->traceBlockSize	= t.tb;
 		wf->stackLdsEntries = t.lds;
 		wf->occupancy		= t.occ;
+		wf->refillLanes		= t.refill;
 		t.ms				= 1e30f;
 		for (int r = 0; r < runs; r++) {
 			if (views) {
@@ -2933,7 +2946,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_begin_views, dim3((unsigned) ((N + 255) / 256)), dim3(256), 0, st, P, viewsDev(wf), tpx);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, wf->counters.as<Counters>());
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<true>(L.media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
+				withMega<true>(L.media, false, false, !wf->sceneTextured, refillLanesOf(wf) > 0, [&](auto k) {
 					decltype(k)::launch(L.grid, wf->traceBlockSize, L.lds, st, P, wf->ds, L.bvh, RayQueueDev{}, nullptr, tpx,
 										wf->ovf.as<int>(), wf->counters.as<Counters>(), NrcDev{}, viewsDev(wf));
 					return 0;
@@ -2945,7 +2958,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				const size_t lds   = stackLds(wf->accel, t.tb);
 				const int maxBlocks = std::max(1, (N + t.tb - 1) / t.tb);
 				const size_t nodeBytes = (size_t) ldsNodeCount(wf) * sizeof(Bvh4Node);
-				int g = withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0,
+				int g = withMega<false>(media, false, false, !wf->sceneTextured, refillLanesOf(wf) > 0,
 										[&](auto k) { return decltype(k)::grid(t.tb, lds, maxBlocks, nodeBytes); });
 				g	  = scaledGrid(g, t.occ);
 				if (int e = reserveOverflow(wf->ovf, wf->accel, (size_t) g * t.tb)) return e;
@@ -2958,7 +2971,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 				hipLaunchKernelGGL(k_camera, dim3(gN), dim3(256), 0, st, P, cameraDev(wf->cam), px, rayQueue(wf, 0), cnt);
 				hipLaunchKernelGGL(k_mega_begin, dim3(1), dim3(1), 0, st, cnt);
 				KRR_HIP(hipEventRecord(e0, st));
-				withMega<false>(media, false, false, !wf->sceneTextured, wf->refillLanes > 0, [&](auto k) {
+				withMega<false>(media, false, false, !wf->sceneTextured, refillLanesOf(wf) > 0, [&](auto k) {
 					decltype(k)::launch((unsigned) g, t.tb, lds, st, P, wf->ds, bvh, rayQueue(wf, 0), (const int *) &cnt->rayCount[0], px,
 										wf->ovf.as<int>(), cnt, NrcDev{}, ViewsDev{});
 					return 0;
@@ -2976,25 +2989,41 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 	const int tbs[]		= {64, 128, 256};
 	const int ldss[]	= {8, 16, 24};
 	const float occs[]	= {1.f, 0.75f, 0.5f};
-	// the first failing candidate ends the search (its error is returned)
+	// the first failing candidate ends the search (its error is returned);
+	// the launch shapes at the refill threshold in force, then, when
+	// refill_lanes is left to the scene (-1), other thresholds at the best
+	// shape (scenes without media; the launch without refill wins on small
+	// trees, whose traversals are short)
+	const int refill0 = refillLanesOf(wf);
 	for (int ti = 0; ti < 3 && rc == KRR_OK; ti++)
 		for (int li = 0; li < 3 && rc == KRR_OK; li++)
 			for (int oi = 0; oi < 3 && rc == KRR_OK; oi++) {
-				Trial t{tbs[ti], ldss[li], occs[oi], 0.f};
+				Trial t{tbs[ti], ldss[li], occs[oi], refill0, 0.f};
 				rc = measure(t, 3);
 				if (rc == KRR_OK) trials.push_back(t);
 			}
+	if (rc == KRR_OK && !media && saveRefill < 0) {
+		const Trial shape = *std::min_element(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.ms < b.ms; });
+		for (const int r : {0, 4, kDefaultRefillLanes, 16}) {
+			if (r == refill0 || rc != KRR_OK) continue;
+			Trial t{shape.tb, shape.lds, shape.occ, r, 0.f};
+			rc = measure(t, 3);
+			if (rc == KRR_OK) trials.push_back(t);
+		}
+	}
 	wf->countTraversal = saveCount;
 	if (rc) {
 		wf->traceBlockSize	= saveTb;
 		wf->stackLdsEntries = saveLds;
 		wf->occupancy		= saveOcc;
+		wf->refillLanes		= saveRefill;
 		return rc;
 	}
 	const Trial best = *std::min_element(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.ms < b.ms; });
 	wf->traceBlockSize	= best.tb;
 	wf->stackLdsEntries = best.lds;
 	wf->occupancy		= best.occ;
+	wf->refillLanes		= best.refill;
 	wf->gridKey = wf->viewsGridKey = ~0ull; // grids and the recorded frame follow the new shape
 	wf->dropGraph();
 	if (!views) wf->beginPending = true; // the trials used the frame's pixel state: seed it again
@@ -3004,6 +3033,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 		o.set("trace_block_size", Json((double) t.tb));
 		o.set("stack_lds_entries", Json((double) t.lds));
 		o.set("occupancy", Json((double) t.occ));
+		o.set("refill_lanes", Json((double) t.refill));
 		o.set("ms", Json((double) t.ms));
 		arr.push(o);
 	}
@@ -3011,6 +3041,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 	b.set("trace_block_size", Json((double) best.tb));
 	b.set("stack_lds_entries", Json((double) best.lds));
 	b.set("occupancy", Json((double) best.occ));
+	b.set("refill_lanes", Json((double) best.refill));
 	b.set("ms", Json((double) best.ms));
 	j.set("trial_paths", Json((double) N));
 	j.set("candidates", arr);

@@ -50,13 +50,13 @@ def test_pass_params_roundtrip(krr):
     assert abs(d["rr"] - 0.8) < 1e-7 and d["enable_clamp"] is False and d["clamp_max"] == 1000
     assert d["wavefront_depths"] == 0 and d["stack_lds_entries"] == 16 and d["leaf_size"] == 4
     assert d["occupancy"] == 1 and d["graph"] is True and d["bvh_builder"] == "device"
-    assert d["refill_lanes"] == 8 and d["leaf_share"] == 35 and d["lds_nodes"] == 0
+    assert d["refill_lanes"] == -1 and d["leaf_share"] == 35 and d["lds_nodes"] == 0
 
 
 @pytest.mark.parametrize("bad", ["{", "[1,2]", '{"max_depth": "ten"}', '{"spp": 0}', '{"block_size": 100}',
                                  '{"stack_lds_entries": -1}', '{"stack_lds_entries": 97}', '{"wavefront_depths": -1}',
                                  '{"occupancy": 0}', '{"occupancy": 1.5}', '{"bvh_builder": "gpu"}',
-                                 '{"nrc": {"spread": -1}}', '{"refill_lanes": -1}', '{"refill_lanes": 65}',
+                                 '{"nrc": {"spread": -1}}', '{"refill_lanes": -2}', '{"refill_lanes": 65}',
                                  '{"lds_nodes": -1}', '{"lds_nodes": 1025}',
                                  '{"leaf_share": -1}', '{"leaf_share": 101}'])
 def test_bad_params_are_reported(krr, bad):

@@ -137,3 +137,20 @@ def test_leaf_phase_parity(krr, oracle, gpu, leaf_share):
     o, os_ = oracle.render(s.desc, s.camera(0), 240, 135, 1, spp=1, max_depth=16, rr=0.8, rr_min_depth=3)
     assert gs["closest_rays"] == os_["closest_rays"] and gs["shadow_rays"] == os_["shadow_rays"]
     assert_images_match(g, o)
+
+
+def test_refill_by_scene_size(krr, gpu):
+    """refill_lanes -1 (the default): the refill launch for trees of at least
+    4096 nodes, rays run to their end for smaller ones (the 36-triangle
+    Cornell box); an explicit value is kept."""
+    print("case: refill_lanes -1 resolved by the tree size", flush=True)
+    for cfg, expected in ((CORNELL, 0), (icosphere_scene(count=64, level=2, res=(64, 64)), 8)):
+        wf = krr.WavefrontPathTracer({"spp": 1})
+        assert wf.to_json()["refill_lanes"] == -1
+        wf.set_scene(krr.Scene(cfg))
+        assert wf.to_json()["refill_lanes_in_force"] == expected
+        wf.close()
+    wf = krr.WavefrontPathTracer({"refill_lanes": 16})
+    wf.set_scene(krr.Scene(CORNELL))
+    assert wf.to_json()["refill_lanes_in_force"] == 16
+    wf.close()

@@ -185,8 +185,10 @@ def test_graph_with_radiance_cache(krr, gpu):
 
 @pytest.mark.parametrize("views", [False, True])
 def test_autotune_keeps_the_image(krr, gpu, views):
-    """The autotuner times 27 launch shapes and keeps the fastest; the
-    frame rendered afterwards equals the frame rendered before."""
+    """The autotuner times 27 launch shapes, then the three other refill
+    thresholds of {0, 4, 8, 16} at the fastest shape (refill_lanes left to
+    the scene), and keeps the fastest; the frame rendered afterwards equals
+    the frame rendered before."""
     print(f"case: autotune views={views}", flush=True)
     cfg = icosphere_scene(count=64, level=2, res=(96, 64))
     s = krr.Scene(cfg)
@@ -211,12 +213,14 @@ def test_autotune_keeps_the_image(krr, gpu, views):
     before = frame()
     rep = wf.autotune(trial_paths=96 * 64)
     print("best", rep["best"], flush=True)
-    assert len(rep["candidates"]) == 27
+    assert len(rep["candidates"]) == 30
+    assert sorted({c["refill_lanes"] for c in rep["candidates"]}) == [0, 4, 8, 16]
     assert all(c["ms"] > 0 for c in rep["candidates"])
     assert rep["best"]["ms"] == min(c["ms"] for c in rep["candidates"])
     p = wf.to_json()
     assert p["trace_block_size"] == rep["best"]["trace_block_size"]
     assert p["stack_lds_entries"] == rep["best"]["stack_lds_entries"]
+    assert p["refill_lanes"] == p["refill_lanes_in_force"] == rep["best"]["refill_lanes"]
     after = frame()
     wf.close()
     assert np.array_equal(before, after)
