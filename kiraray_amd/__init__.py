@@ -48,7 +48,10 @@ def _json_arg(cfg):
 
 
 class Scene:
-    """Procedural scene (SceneImporter counterpart), host memory only."""
+    """Procedural scene (SceneImporter counterpart), host memory only.
+
+    The arrays that `desc` points to belong to the scene and are freed with
+    it: keep the Scene alive while the descriptor is read."""
 
     def __init__(self, config):
         self._lib = load_library()

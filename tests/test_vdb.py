@@ -126,7 +126,8 @@ def test_other_maps(krr, tmp_path, kind):
     g = _tree(np.random.default_rng(1))
     g.update(map=kind, voxel=(0.5, 0.5, 0.5), translate=(3.0, 4.0, 5.0))
     path = write_vdb(str(tmp_path / "m.vdb"), [g])
-    m, dense = _grid(_scene(krr, path).desc)
+    s = _scene(krr, path)  # the descriptor's arrays live as long as the scene
+    m, dense = _grid(s.desc)
     lo, hi, exp = dense_expected(g)
     t = np.array([3.0, 4.0, 5.0]) if "Translate" in kind else np.zeros(3)
     assert np.allclose(list(m.bounds_min), t + 0.5 * lo) and np.array_equal(dense, exp)
