@@ -10,12 +10,12 @@
 // lanes (lane m + 32g holds features 8s + 4g + j, j < 4, of sample m for
 // k-step s), and the 32x32 accumulator tile leaves feature 8(r/4) + 4g + r%4
 // of sample m in register r of lane m + 32g - exactly the B-operand layout of
-// the next layer.  Pairs of k-steps run as one v_mfma_f32_32x32x16_bf16.  Activations therefore never leave the
-// registers between layers.  Backward uses the same layout with A = W
-// (dX^T = W dZ^T).  Weight gradients dW_l = X_l dZ_l^T are a split-K GEMM over
-// the batch on X^T and dZ^T stored feature-major in HBM, reduced in a fixed
-// order (deterministic), then Adam updates the FP32 master weights and
-// rewrites both BF16 copies.
+// the next layer, so activations never leave the registers between layers.
+// Pairs of k-steps run as one v_mfma_f32_32x32x16_bf16.  Backward uses the
+// same layout with A = W (dX^T = W dZ^T).  Weight gradients
+// dW_l = X_l dZ_l^T are a split-K GEMM over the batch on X^T and dZ^T stored
+// feature-major in HBM, reduced in a fixed order (deterministic), then Adam
+// updates the FP32 master weights and rewrites both BF16 copies.
 //
 // Encoding (64 features, lane half g = 0/1 computes slots q = 4s + j of
 // feature 8s + 4g + j):
@@ -46,15 +46,10 @@ namespace nrc {
 
 // ---------------------------------------------------------------- device
 
-// Round to nearest even; inf stays inf and a nan stays a (quiet) nan.  Both
-// results are computed and one selected: a branch here would diverge once
-// per converted value.
-__device__ __forceinline__ uint16_t toBf16(float f) {
-	const uint32_t u	   = __float_as_uint(f);
-	const uint32_t nearest = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-	const uint32_t special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
-	return (uint16_t) (((u & 0x7f800000u) == 0x7f800000u) ? special : nearest);
-}
+// Round to nearest even (inf stays inf, a nan becomes a quiet nan): the
+// hardware conversion, v_cvt_pk_bf16_f32 on gfx950, two values per
+// instruction where the compiler pairs them.
+__device__ __forceinline__ uint16_t toBf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16) f); }
 __device__ __forceinline__ float fromBf16(uint16_t h) { return __uint_as_float((uint32_t) h << 16); }
 
 __device__ __forceinline__ short4v pack4(float a, float b, float c, float d) {
@@ -275,7 +270,30 @@ __global__ void __launch_bounds__(256) k_infer(const uint16_t *wtAll, const floa
 }
 
 // Inference on a query queue whose length lives on the device; adds
-// thp * max(y, 0) to L[pixel].
+// thp * max(y, 0) to L[pixel].  A wave's next tile of queries is loaded while
+// it evaluates the current one, and the pixel's radiance is read as soon as
+// the query names the pixel: the network in between hides both loads.  A
+// launch answers one sample's queries, at most one per pixel, so no other
+// lane writes that pixel's radiance meanwhile.
+__device__ __forceinline__ void loadQuery(const float4 *queries, int stride, int i, int n, float v[16], float4 &pay) {
+	pay = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
+	if (i < n) {
+		const float4 *p = queries + (int64_t) i * stride;
+#pragma unroll
+		for (int q = 0; q < 4; q++) {
+			const float4 f = p[q];
+			v[4 * q]	   = f.x;
+			v[4 * q + 1]   = f.y;
+			v[4 * q + 2]   = f.z;
+			v[4 * q + 3]   = f.w;
+		}
+		pay = p[4];
+	} else {
+#pragma unroll
+		for (int q = 0; q < 16; q++) v[q] = 0.f;
+	}
+}
+
 __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, const float4 *queries, int stride, const int *countPtr, int capacity,
 													 float4 *L) {
 	extern __shared__ uint16_t ldsW[];
@@ -285,27 +303,21 @@ __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, cons
 	const int lane = threadIdx.x & 63, g = lane >> 5, m = lane & 31;
 	const int wavesPerBlock = blockDim.x >> 6;
 	const int tiles			= (n + 31) / 32;
-	for (int t = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); t < tiles; t += gridDim.x * wavesPerBlock) {
-		const int i = t * 32 + m;
-		float v[16];
-		float4 pay = make_float4(0.f, 0.f, 0.f, 0.f);
-		if (i < n) {
-			const float4 *p = queries + (int64_t) i * stride;
-#pragma unroll
-			for (int q = 0; q < 4; q++) {
-				const float4 f = p[q];
-				v[4 * q]	   = f.x;
-				v[4 * q + 1]   = f.y;
-				v[4 * q + 2]   = f.z;
-				v[4 * q + 3]   = f.w;
-			}
-			pay = p[4];
-		} else {
-#pragma unroll
-			for (int q = 0; q < 16; q++) v[q] = 0.f;
-		}
+	const int tStride		= gridDim.x * wavesPerBlock;
+	int t					= blockIdx.x * wavesPerBlock + (threadIdx.x >> 6);
+	float v[16];
+	float4 pay;
+	loadQuery(queries, stride, t * 32 + m, n, v, pay);
+	for (; t < tiles; t += tStride) {
+		const int i		= t * 32 + m;
+		const int pixel = __float_as_int(pay.w);
+		const bool mine = g == 0 && i < n;
+		const float4 thp = pay;
+		float4 acc		 = make_float4(0.f, 0.f, 0.f, 0.f);
+		if (mine) acc = L[pixel];
 		short4v x[8], y[8];
 		encodeB(v, g, x);
+		loadQuery(queries, stride, (t + tStride) * 32 + m, n, v, pay); // the next tile's queries
 		float16v a0, a1;
 		// not unrolled: an unrolled loop hoists every layer's weight loads
 #pragma unroll 1
@@ -315,12 +327,10 @@ __global__ void __launch_bounds__(256) k_infer_queue(const uint16_t *wtAll, cons
 			for (int s = 0; s < 8; s++) x[s] = y[s];
 		}
 		const float16v o = outputForward(ldsW + kHidden * 64 * kRowH, lane, g, x);
-		if (g == 0 && i < n) {
-			const int pixel = __float_as_int(pay.w);
-			float4 acc		= L[pixel];
-			acc.x += pay.x * fmaxf(o[0], 0.f);
-			acc.y += pay.y * fmaxf(o[1], 0.f);
-			acc.z += pay.z * fmaxf(o[2], 0.f);
+		if (mine) {
+			acc.x += thp.x * fmaxf(o[0], 0.f);
+			acc.y += thp.y * fmaxf(o[1], 0.f);
+			acc.z += thp.z * fmaxf(o[2], 0.f);
 			L[pixel] = acc;
 		}
 	}
