@@ -193,19 +193,21 @@ struct TraceResult {
 // entered) branch to global memory.  A pointer chosen between LDS and
 // global memory would compile to flat loads and stores on every access.
 constexpr int kShortStack = 16;
-
 struct LaneStack {
 	int *lds;
 	int stride;
 	int *ovf;
 	int ovStride;
 	int nLds;
+	// LDS slot min(sp, nLds) * stride: both factors are below 2^23, so the
+	// full-rate 24-bit multiply gives the product
+	KRR_DEV int slot(int sp) const { return __mul24(min(sp, nLds), stride); }
 	KRR_DEV void put(int sp, int v) const {
-		lds[min(sp, nLds) * stride] = v;
+		lds[slot(sp)] = v;
 		if (sp >= nLds) ovf[(size_t) (sp - nLds) * ovStride] = v;
 	}
 	KRR_DEV int get(int sp) const {
-		int v = lds[min(sp, nLds) * stride];
+		int v = lds[slot(sp)];
 		// keeps the LDS read a ds_read: without the barrier the compiler folds
 		// both reads into one flat load of a selected address
 		asm volatile("" : "+v"(v));
@@ -213,6 +215,163 @@ struct LaneStack {
 		return v;
 	}
 };
+
+// Per-ray constants of the slab test.  The distance to plane p along axis a,
+// (p - o_a) / d_a, is one fma p * i_a + b_a with i_a = 1/d_a (rounded) and
+// b_a = -(o_a i_a) - w_a for near planes, -(o_a i_a) + w_a for far planes,
+// w_a = 2^-22 |o_a i_a| + 2^-40 |i_a|.  The roundings of o_a i_a and of b_a
+// are each at most 2^-24 |o_a i_a|, so near distances stay at or below
+// (p - o_a) i_a and far distances at or above it before the fma's own
+// rounding; what remains is relative error (the rounding of i_a and of the
+// fma, gamma(2)), which the far distance's 1 + 2 gamma(3) widening covers as
+// it covers the subtract-then-multiply form (Ize, "Robust BVH Ray
+// Traversal", JCGT 2(2), 2013).  In world units w_a is a margin of
+// 2^-22 |o_a| + 2^-40 on each side of a slab.
+//
+// |i_a| is capped at 2^100: a zero (or tiny) direction component is taken
+// as +-2^-100, a ray that drifts from the true one by at most 2^-100 t along
+// that axis, less than the 2^-40 margin for every t below 2^60, so the test
+// stays conservative and the axis still culls.  (An infinite i_a would make
+// o_a i_a infinite and the axis' distances NaN: no culling on that axis,
+// and a ray along a coordinate plane would visit every box its projection
+// crosses.)  Only an o_a beyond 2^27 overflows o_a i_a; that axis then stops
+// culling, which is still conservative.
+constexpr float kSlabMaxInv = 0x1p100f;
+struct SlabRay {
+	float ix, iy, iz;
+	float bnx, bny, bnz; // near planes
+	float bfx, bfy, bfz; // far planes
+	// byte offsets of the near planes inside a node (lo of axis a at 32a, hi
+	// at 32a + 16, by direction sign); the far plane is at offset ^ 16
+	uint32_t nX, nY, nZ;
+};
+
+KRR_DEV SlabRay slabRay(const f3 &o, const f3 &d) {
+	SlabRay s;
+	const float rx = 1.0f / d.x, ry = 1.0f / d.y, rz = 1.0f / d.z;
+	s.ix		   = fabsf(rx) <= kSlabMaxInv ? rx : copysignf(kSlabMaxInv, d.x);
+	s.iy		   = fabsf(ry) <= kSlabMaxInv ? ry : copysignf(kSlabMaxInv, d.y);
+	s.iz		   = fabsf(rz) <= kSlabMaxInv ? rz : copysignf(kSlabMaxInv, d.z);
+	const float qx = o.x * s.ix, qy = o.y * s.iy, qz = o.z * s.iz;
+	const float wx = fmaf(fabsf(s.ix), 0x1p-40f, fabsf(qx) * 0x1p-22f);
+	const float wy = fmaf(fabsf(s.iy), 0x1p-40f, fabsf(qy) * 0x1p-22f);
+	const float wz = fmaf(fabsf(s.iz), 0x1p-40f, fabsf(qz) * 0x1p-22f);
+	s.bnx		   = -qx - wx;
+	s.bny		   = -qy - wy;
+	s.bnz		   = -qz - wz;
+	s.bfx		   = -qx + wx;
+	s.bfy		   = -qy + wy;
+	s.bfz		   = -qz + wz;
+	s.nX		   = __builtin_signbit(d.x) ? 16u : 0u;
+	s.nY		   = __builtin_signbit(d.y) ? 48u : 32u;
+	s.nZ		   = __builtin_signbit(d.z) ? 80u : 64u;
+	return s;
+}
+
+constexpr float kWiden = 1.00000036f;			 // 1 + 2*gamma(3)
+constexpr float kTie   = 1.00000095367431640625f; // 1 + 2^-20
+
+// Slab test of the four children of inner node `entry` against [0, tlim]:
+// k[i] = entry distance of child i, +inf when missed; c[i] = its reference.
+// Float nodes are read from LDS for entries below nLdsNodes (the block's copy
+// of the top of the tree), from global memory otherwise: seven 16-byte loads
+// of the node's 128-byte line (the six planes the ray needs and the children).
+KRR_DEV void nodeTest(const TraceBvh &bvh, const SlabRay &sr, int entry, float tlim, const float4 *nodeLds, int nLdsNodes, float &k0,
+					  float &k1, float &k2, float &k3, int &c0, int &c1, int &c2, int &c3) {
+#define KRR_SLAB(kc, px, qx, py, qy, pz, qz)                                                                                                   \
+	{                                                                                                                                          \
+		const float t0 = fmaxf(fmaxf(fmaf(px, sr.ix, sr.bnx), fmaf(py, sr.iy, sr.bny)), fmaxf(fmaf(pz, sr.iz, sr.bnz), 0.0f));                 \
+		const float t1 = fminf(fminf(fminf(fmaf(qx, sr.ix, sr.bfx), fmaf(qy, sr.iy, sr.bfy)), fmaf(qz, sr.iz, sr.bfz)) * kWiden, tlim);         \
+		kc			   = t0 <= t1 ? t0 : __builtin_inff();                                                                                      \
+	}
+	if (bvh.qnodes) {
+		// One 64-byte quantised node: {org, e, valid}, planes lox hix loy
+		// hiy, planes loz hiz + children 0 1, children 2 3.  Plane c of a
+		// dword is byte c; org + q 2^e is exact (one fma, no rounding).
+		const uint4 *np = reinterpret_cast<const uint4 *>(bvh.qnodes + entry);
+		const uint4 A = np[0], B = np[1], C = np[2], D = np[3];
+		const float sx = __int_as_float(((int) (int8_t) (A.w & 0xffu) + 127) << 23);
+		const float sy = __int_as_float(((int) (int8_t) ((A.w >> 8) & 0xffu) + 127) << 23);
+		const float sz = __int_as_float(((int) (int8_t) ((A.w >> 16) & 0xffu) + 127) << 23);
+		const float gx = __uint_as_float(A.x), gy = __uint_as_float(A.y), gz = __uint_as_float(A.z);
+		const uint32_t valid = A.w >> 24;
+		const uint32_t qnx = sr.nX ? B.y : B.x, qfx = sr.nX ? B.x : B.y;
+		const uint32_t qny = sr.nY == 48u ? B.w : B.z, qfy = sr.nY == 48u ? B.z : B.w;
+		const uint32_t qnz = sr.nZ == 80u ? C.y : C.x, qfz = sr.nZ == 80u ? C.x : C.y;
+#define KRR_DEQ(w, c, sc, g) fmaf((float) (((w) >> (8 * (c))) & 0xffu), sc, g)
+#define KRR_QSLAB(kc, c)                                                                                                                       \
+	KRR_SLAB(kc, KRR_DEQ(qnx, c, sx, gx), KRR_DEQ(qfx, c, sx, gx), KRR_DEQ(qny, c, sy, gy), KRR_DEQ(qfy, c, sy, gy), KRR_DEQ(qnz, c, sz, gz),  \
+			 KRR_DEQ(qfz, c, sz, gz))                                                                                                          \
+	if (!((valid >> (c)) & 1u)) kc = __builtin_inff();
+		KRR_QSLAB(k0, 0)
+		KRR_QSLAB(k1, 1)
+		KRR_QSLAB(k2, 2)
+		KRR_QSLAB(k3, 3)
+#undef KRR_QSLAB
+#undef KRR_DEQ
+		c0 = (int) C.z;
+		c1 = (int) C.w;
+		c2 = (int) D.x;
+		c3 = (int) D.y;
+	} else {
+		float4 nx, fx, ny, fy, nz, fz;
+		int4 ch;
+		if (entry < nLdsNodes) {
+			const float4 *np = nodeLds + (uint32_t) entry * 8u;
+			nx				 = np[sr.nX >> 4];
+			fx				 = np[(sr.nX ^ 16u) >> 4];
+			ny				 = np[sr.nY >> 4];
+			fy				 = np[(sr.nY ^ 16u) >> 4];
+			nz				 = np[sr.nZ >> 4];
+			fz				 = np[(sr.nZ ^ 16u) >> 4];
+			const float4 c	 = np[6];
+			// keeps these LDS reads ds_reads: without the barrier the compiler
+			// may merge both branches into flat loads of a selected address
+			asm volatile("" : "+v"(nx.x));
+			ch = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
+		} else {
+			const char *nodeBase = reinterpret_cast<const char *>(bvh.nodes);
+			const uint32_t nb	 = (uint32_t) entry * 128u; // node array < 4 GiB (checked at build)
+#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
+			nx = KRR_PLANE(sr.nX);
+			fx = KRR_PLANE(sr.nX ^ 16u);
+			ny = KRR_PLANE(sr.nY);
+			fy = KRR_PLANE(sr.nY ^ 16u);
+			nz = KRR_PLANE(sr.nZ);
+			fz = KRR_PLANE(sr.nZ ^ 16u);
+#undef KRR_PLANE
+			ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
+		}
+		KRR_SLAB(k0, nx.x, fx.x, ny.x, fy.x, nz.x, fz.x)
+		KRR_SLAB(k1, nx.y, fx.y, ny.y, fy.y, nz.y, fz.y)
+		KRR_SLAB(k2, nx.z, fx.z, ny.z, fy.z, nz.z, fz.z)
+		KRR_SLAB(k3, nx.w, fx.w, ny.w, fy.w, nz.w, fz.w)
+		c0 = ch.x;
+		c1 = ch.y;
+		c2 = ch.z;
+		c3 = ch.w;
+	}
+#undef KRR_SLAB
+	// Hit children nearest first (a five-exchange sorting network on
+	// selects; equal distances keep their slot order).  Missed children
+	// (+inf) end up last.
+#define KRR_CSWAP(ka, ca, kb, cb)                                                                                                              \
+	{                                                                                                                                          \
+		const bool sw = kb < ka;                                                                                                               \
+		const float lo = sw ? kb : ka, hi = sw ? ka : kb;                                                                                      \
+		const int clo = sw ? cb : ca, chi = sw ? ca : cb;                                                                                      \
+		ka			  = lo;                                                                                                                     \
+		kb			  = hi;                                                                                                                     \
+		ca			  = clo;                                                                                                                    \
+		cb			  = chi;                                                                                                                    \
+	}
+	KRR_CSWAP(k0, c0, k1, c1)
+	KRR_CSWAP(k2, c2, k3, c3)
+	KRR_CSWAP(k0, c0, k2, c2)
+	KRR_CSWAP(k1, c1, k3, c3)
+	KRR_CSWAP(k1, c1, k2, c2)
+#undef KRR_CSWAP
+}
 
 // Traverses one ray.  ANY: stop at the first hit of a non-null-material
 // triangle (shadow rays, device.cu:83-100).  ALPHA: hits on triangles flagged
@@ -232,17 +391,10 @@ template <bool ANY, bool COUNT, bool ALPHA = false>
 KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, float tmax, const LaneStack &stack,
 							 uint32_t &nodeVisits, uint32_t &triTests) {
 	TraceResult res{tmax, -1, 0.f, 0.f};
-	const RayPre rp = rayPre(o, d);
-	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-	// Byte offsets of the near/far planes inside a node, by direction sign
-	// (lo of axis a at 32a, hi at 32a + 16): the loads fetch the planes the
-	// ray needs directly, with 32-bit offsets from the node array base.
-	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
-	const char *nodeBase = reinterpret_cast<const char *>(bvh.nodes);
-	const float kWiden = 1.00000036f;					// 1 + 2*gamma(3)
-	const float kTie   = 1.00000095367431640625f;		// 1 + 2^-20
-	int sp			   = 0;
-	int entry		   = 0; // >= 0: inner node; < 0: leaf code ~(first * 8 + count - 1)
+	const RayPre rp	 = rayPre(o, d);
+	const SlabRay sr = slabRay(o, d);
+	int sp			 = 0;
+	int entry		 = 0; // >= 0: inner node; < 0: leaf code ~(first * 8 + count - 1)
 	// Each node and leaf is taken at most once; more steps can only come from
 	// a corrupt stack or tree, and the budget turns that into a finished
 	// (wrong) traversal instead of a wave that never ends.
@@ -275,93 +427,17 @@ KRR_DEV TraceResult traverse(const TraceBvh &bvh, const f3 &o, const f3 &d, floa
 			continue;
 		}
 		if (COUNT) nodeVisits++;
-		const float tlim = res.t * kTie;
 		float k0, k1, k2, k3;
 		int c0, c1, c2, c3;
-		if (bvh.qnodes) {
-			// One 64-byte quantised node: {org, e, valid}, planes lox hix loy
-			// hiy, planes loz hiz + children 0 1, children 2 3.  Plane c of a
-			// dword is byte c; org + q 2^e is exact (one fma, no rounding).
-			const uint4 *np = reinterpret_cast<const uint4 *>(bvh.qnodes + entry);
-			const uint4 A = np[0], B = np[1], C = np[2], D = np[3];
-			const float sx = __int_as_float(((int) (int8_t) (A.w & 0xffu) + 127) << 23);
-			const float sy = __int_as_float(((int) (int8_t) ((A.w >> 8) & 0xffu) + 127) << 23);
-			const float sz = __int_as_float(((int) (int8_t) ((A.w >> 16) & 0xffu) + 127) << 23);
-			const float gx = __uint_as_float(A.x), gy = __uint_as_float(A.y), gz = __uint_as_float(A.z);
-			const uint32_t valid = A.w >> 24;
-			const uint32_t qnx = nX ? B.y : B.x, qfx = nX ? B.x : B.y;
-			const uint32_t qny = nY == 48u ? B.w : B.z, qfy = nY == 48u ? B.z : B.w;
-			const uint32_t qnz = nZ == 80u ? C.y : C.x, qfz = nZ == 80u ? C.x : C.y;
-#define KRR_DEQ(w, c, sc, g) fmaf((float) (((w) >> (8 * (c))) & 0xffu), sc, g)
-#define KRR_QSLAB(kc, c)                                                                                             \
-	{                                                                                                                \
-		float t0 = fmaxf(fmaxf((KRR_DEQ(qnx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qny, c, sy, gy) - o.y) * iy),           \
-						 fmaxf((KRR_DEQ(qnz, c, sz, gz) - o.z) * iz, 0.0f));                                         \
-		float t1 = fminf(fminf((KRR_DEQ(qfx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qfy, c, sy, gy) - o.y) * iy),           \
-						 (KRR_DEQ(qfz, c, sz, gz) - o.z) * iz) *                                                     \
-				   kWiden;                                                                                           \
-		t1 = fminf(t1, tlim);                                                                                        \
-		kc = (t0 <= t1 && ((valid >> (c)) & 1u)) ? t0 : __builtin_inff();                                            \
-	}
-			KRR_QSLAB(k0, 0)
-			KRR_QSLAB(k1, 1)
-			KRR_QSLAB(k2, 2)
-			KRR_QSLAB(k3, 3)
-#undef KRR_QSLAB
-#undef KRR_DEQ
-			c0 = (int) C.z;
-			c1 = (int) C.w;
-			c2 = (int) D.x;
-			c3 = (int) D.y;
-		} else {
-			const uint32_t nb = (uint32_t) entry * 128u; // node array < 4 GiB (checked at build)
-#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
-			const float4 nx = KRR_PLANE(nX), fx = KRR_PLANE(nX ^ 16u), ny = KRR_PLANE(nY), fy = KRR_PLANE(nY ^ 16u);
-			const float4 nz = KRR_PLANE(nZ), fz = KRR_PLANE(nZ ^ 16u);
-#undef KRR_PLANE
-			const int4 ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
-#define KRR_SLAB(kc, comp)                                                                                           \
-	{                                                                                                                \
-		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
-		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
-		t1		 = fminf(t1, tlim);                                                                                  \
-		kc		 = t0 <= t1 ? t0 : __builtin_inff();                                                                 \
-	}
-			KRR_SLAB(k0, x)
-			KRR_SLAB(k1, y)
-			KRR_SLAB(k2, z)
-			KRR_SLAB(k3, w)
-#undef KRR_SLAB
-			c0 = ch.x;
-			c1 = ch.y;
-			c2 = ch.z;
-			c3 = ch.w;
-		}
-		// Hit children, nearest first.
-		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
-#define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
-	if (kb < ka) {                                                                                                   \
-		float tk = ka;                                                                                               \
-		ka		 = kb;                                                                                               \
-		kb		 = tk;                                                                                               \
-		int tc	 = ca;                                                                                               \
-		ca		 = cb;                                                                                               \
-		cb		 = tc;                                                                                               \
-	}
-		KRR_CSWAP(k0, c0, k1, c1)
-		KRR_CSWAP(k2, c2, k3, c3)
-		KRR_CSWAP(k0, c0, k2, c2)
-		KRR_CSWAP(k1, c1, k3, c3)
-		KRR_CSWAP(k1, c1, k2, c2)
-#undef KRR_CSWAP
-		if (n == 0) {
+		nodeTest(bvh, sr, entry, res.t * kTie, nullptr, 0, k0, k1, k2, k3, c0, c1, c2, c3);
+		if (k0 == __builtin_inff()) {
 			if (sp == 0) break;
 			entry = stack.get(--sp);
 			continue;
 		}
-		if (n >= 4) stack.put(sp++, c3);
-		if (n >= 3) stack.put(sp++, c2);
-		if (n >= 2) stack.put(sp++, c1);
+		if (k3 != __builtin_inff()) stack.put(sp++, c3);
+		if (k2 != __builtin_inff()) stack.put(sp++, c2);
+		if (k1 != __builtin_inff()) stack.put(sp++, c1);
 		entry = c0;
 	}
 	return res;
@@ -424,12 +500,8 @@ template <bool COUNT, bool ALPHA>
 KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const bool any, bool &going, TravState &s,
 						   const LaneStack &stack, const int minLanes, uint32_t &nodeVisits, uint32_t &triTests,
 						   const float4 *nodeLds = nullptr, int nLdsNodes = 0, int leafShare = 0) {
-	const RayPre rp = rayPre(o, d);
-	const float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
-	const uint32_t nX = __builtin_signbit(d.x) ? 16u : 0u, nY = __builtin_signbit(d.y) ? 48u : 32u, nZ = __builtin_signbit(d.z) ? 80u : 64u;
-	const char *nodeBase = reinterpret_cast<const char *>(bvh.nodes);
-	const float kWiden = 1.00000036f;			  // 1 + 2*gamma(3)
-	const float kTie   = 1.00000095367431640625f; // 1 + 2^-20
+	const RayPre rp	 = rayPre(o, d);
+	const SlabRay sr = slabRay(o, d);
 	while (true) {
 		const uint64_t lanes = __ballot(going);
 		const int nGoing	 = __popcll(lanes);
@@ -480,103 +552,10 @@ KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const 
 			continue;
 		}
 		if (COUNT) nodeVisits++;
-		const float tlim = s.t * kTie;
 		float k0, k1, k2, k3;
 		int c0, c1, c2, c3;
-		if (bvh.qnodes) {
-			const uint4 *np = reinterpret_cast<const uint4 *>(bvh.qnodes + s.entry);
-			const uint4 A = np[0], B = np[1], C = np[2], D = np[3];
-			const float sx = __int_as_float(((int) (int8_t) (A.w & 0xffu) + 127) << 23);
-			const float sy = __int_as_float(((int) (int8_t) ((A.w >> 8) & 0xffu) + 127) << 23);
-			const float sz = __int_as_float(((int) (int8_t) ((A.w >> 16) & 0xffu) + 127) << 23);
-			const float gx = __uint_as_float(A.x), gy = __uint_as_float(A.y), gz = __uint_as_float(A.z);
-			const uint32_t valid = A.w >> 24;
-			const uint32_t qnx = nX ? B.y : B.x, qfx = nX ? B.x : B.y;
-			const uint32_t qny = nY == 48u ? B.w : B.z, qfy = nY == 48u ? B.z : B.w;
-			const uint32_t qnz = nZ == 80u ? C.y : C.x, qfz = nZ == 80u ? C.x : C.y;
-#define KRR_DEQ(w, c, sc, g) fmaf((float) (((w) >> (8 * (c))) & 0xffu), sc, g)
-#define KRR_QSLAB(kc, c)                                                                                             \
-	{                                                                                                                \
-		float t0 = fmaxf(fmaxf((KRR_DEQ(qnx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qny, c, sy, gy) - o.y) * iy),           \
-						 fmaxf((KRR_DEQ(qnz, c, sz, gz) - o.z) * iz, 0.0f));                                         \
-		float t1 = fminf(fminf((KRR_DEQ(qfx, c, sx, gx) - o.x) * ix, (KRR_DEQ(qfy, c, sy, gy) - o.y) * iy),           \
-						 (KRR_DEQ(qfz, c, sz, gz) - o.z) * iz) *                                                     \
-				   kWiden;                                                                                           \
-		t1 = fminf(t1, tlim);                                                                                        \
-		kc = (t0 <= t1 && ((valid >> (c)) & 1u)) ? t0 : __builtin_inff();                                            \
-	}
-			KRR_QSLAB(k0, 0)
-			KRR_QSLAB(k1, 1)
-			KRR_QSLAB(k2, 2)
-			KRR_QSLAB(k3, 3)
-#undef KRR_QSLAB
-#undef KRR_DEQ
-			c0 = (int) C.z;
-			c1 = (int) C.w;
-			c2 = (int) D.x;
-			c3 = (int) D.y;
-		} else {
-			float4 nx, fx, ny, fy, nz, fz;
-			int4 ch;
-			if (s.entry < nLdsNodes) {
-				const float4 *np = nodeLds + (uint32_t) s.entry * 8u;
-				nx				 = np[nX >> 4];
-				fx				 = np[(nX ^ 16u) >> 4];
-				ny				 = np[nY >> 4];
-				fy				 = np[(nY ^ 16u) >> 4];
-				nz				 = np[nZ >> 4];
-				fz				 = np[(nZ ^ 16u) >> 4];
-				const float4 c	 = np[6];
-				// keeps these LDS reads ds_reads: without the barrier the compiler
-				// may merge both branches into flat loads of a selected address
-				asm volatile("" : "+v"(nx.x));
-				ch = make_int4(__float_as_int(c.x), __float_as_int(c.y), __float_as_int(c.z), __float_as_int(c.w));
-			} else {
-				const uint32_t nb = (uint32_t) s.entry * 128u; // node array < 4 GiB (checked at build)
-#define KRR_PLANE(off) (*reinterpret_cast<const float4 *>(nodeBase + (nb + (off))))
-				nx = KRR_PLANE(nX);
-				fx = KRR_PLANE(nX ^ 16u);
-				ny = KRR_PLANE(nY);
-				fy = KRR_PLANE(nY ^ 16u);
-				nz = KRR_PLANE(nZ);
-				fz = KRR_PLANE(nZ ^ 16u);
-#undef KRR_PLANE
-				ch = *reinterpret_cast<const int4 *>(nodeBase + (nb + 96u));
-			}
-#define KRR_SLAB(kc, comp)                                                                                           \
-	{                                                                                                                \
-		float t0 = fmaxf(fmaxf((nx.comp - o.x) * ix, (ny.comp - o.y) * iy), fmaxf((nz.comp - o.z) * iz, 0.0f));       \
-		float t1 = fminf(fminf((fx.comp - o.x) * ix, (fy.comp - o.y) * iy), (fz.comp - o.z) * iz) * kWiden;           \
-		t1		 = fminf(t1, tlim);                                                                                  \
-		kc		 = t0 <= t1 ? t0 : __builtin_inff();                                                                 \
-	}
-			KRR_SLAB(k0, x)
-			KRR_SLAB(k1, y)
-			KRR_SLAB(k2, z)
-			KRR_SLAB(k3, w)
-#undef KRR_SLAB
-			c0 = ch.x;
-			c1 = ch.y;
-			c2 = ch.z;
-			c3 = ch.w;
-		}
-		int n = (k0 != __builtin_inff()) + (k1 != __builtin_inff()) + (k2 != __builtin_inff()) + (k3 != __builtin_inff());
-#define KRR_CSWAP(ka, ca, kb, cb)                                                                                    \
-	if (kb < ka) {                                                                                                   \
-		float tk = ka;                                                                                               \
-		ka		 = kb;                                                                                               \
-		kb		 = tk;                                                                                               \
-		int tc	 = ca;                                                                                               \
-		ca		 = cb;                                                                                               \
-		cb		 = tc;                                                                                               \
-	}
-		KRR_CSWAP(k0, c0, k1, c1)
-		KRR_CSWAP(k2, c2, k3, c3)
-		KRR_CSWAP(k0, c0, k2, c2)
-		KRR_CSWAP(k1, c1, k3, c3)
-		KRR_CSWAP(k1, c1, k2, c2)
-#undef KRR_CSWAP
-		if (n == 0) {
+		nodeTest(bvh, sr, s.entry, s.t * kTie, nodeLds, nLdsNodes, k0, k1, k2, k3, c0, c1, c2, c3);
+		if (k0 == __builtin_inff()) {
 			if (s.sp == 0) {
 				going = false;
 				continue;
@@ -584,9 +563,9 @@ KRR_DEV void traverseSteps(const TraceBvh &bvh, const f3 &o, const f3 &d, const 
 			s.entry = stack.get(--s.sp);
 			continue;
 		}
-		if (n >= 4) stack.put(s.sp++, c3);
-		if (n >= 3) stack.put(s.sp++, c2);
-		if (n >= 2) stack.put(s.sp++, c1);
+		if (k3 != __builtin_inff()) stack.put(s.sp++, c3);
+		if (k2 != __builtin_inff()) stack.put(s.sp++, c2);
+		if (k1 != __builtin_inff()) stack.put(s.sp++, c1);
 		s.entry = c0;
 	}
 }
