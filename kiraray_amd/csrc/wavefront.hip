@@ -1082,8 +1082,16 @@ __global__ void k_megakernel(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev
 // next depth in this order, and each ray takes the same traversal steps, so
 // the radiance additions, RNG draws and hits are those of k_megakernel.  Once
 // no path is left to take, the rounds run every traversal to its end.
+// Waves per SIMD the refill launch is compiled for: 3 (168 VGPRs, no
+// scratch) for the single-view launch, where it runs as fast as 4 waves with
+// 120 bytes per lane of spills (3.80 against 3.79-3.83 ms per frame on the
+// 1.31 M-triangle 1080p field); 4 for the multi-view launch, which 3 waves
+// slow by 7 % (259 against 243 ms for 256 views).
+constexpr int refillWaves(bool views) { return views ? 4 : 3; }
+
 template <bool COUNT, bool NRC = false, bool VIEWS = false, bool PLAIN = false>
-__global__ void k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(refillWaves(VIEWS), refillWaves(VIEWS))))
+k_megakernel_refill(WfParams P0, DevScene sc, TraceBvh bvh, RayQueueDev q, const int *countPtr, int startDepth,
 									PixelStateDev px, int *ovf, Counters *cnt, NrcDev nrc = NrcDev{}, ViewsDev V = ViewsDev{},
 									CameraDev cam = CameraDev{}, int camFetch = 0) {
 	extern __shared__ int ldsStack[];
