@@ -167,8 +167,10 @@ def test_repeated_calls_of_varying_size(krr, oracle, gpu):
 def test_node_formats_on_hard_rays(krr, oracle, gpu, node_format):
     """Both node formats (128-byte float nodes, the default, and 64-byte
     quantised nodes) give the brute-force hits on rays that stress the box test:
-    axis-aligned directions (zero direction components), origins 10^4 away,
-    and origins on the triangles' own planes (t near 0)."""
+    axis-aligned directions (zero direction components), directions with
+    signed zeros and components of 1e-32 / 1e-38 (reciprocals beyond the
+    slab test's 2^100 cap, or infinite), origins 10^4 away, and origins on the
+    triangles' own planes (t near 0)."""
     print(f"case: hard rays, node_format={node_format}", flush=True)
     s = krr.Scene(icosphere_scene(count=64, level=2, res=(16, 16)))
     pos, idx, _ = s.arrays()
@@ -189,6 +191,17 @@ def test_node_formats_on_hard_rays(krr, oracle, gpu, node_format):
     b /= b.sum(1, keepdims=True)
     rays[2 * n:, 0:3] = pos[idx[tri, 0]] * b[:, :1] + pos[idx[tri, 1]] * b[:, 1:2] + pos[idx[tri, 2]] * b[:, 2:3]
     rays[2 * n:, 4:7] = rng.standard_normal((n, 3))
+    # nearly axis-aligned: an axis direction with the other components
+    # +-0.0, +-1e-32 or +-1e-38 (reciprocals above the 2^100 cap; 1e-38 is
+    # subnormal and its reciprocal may overflow)
+    m = n // 2
+    tiny = rng.choice(np.array([0.0, -0.0, 1e-32, -1e-32, 1e-38, -1e-38], np.float32), (m, 3))
+    near = np.zeros((m, 8), np.float32)
+    near[:, 0:3] = rng.random((m, 3)) * np.array([1.0, 0.3, 1.0])
+    near[:, 4:7] = tiny
+    ax2 = rng.integers(0, 3, m)
+    near[np.arange(m), 4 + ax2] = rng.choice([-1.0, 1.0], m)
+    rays = np.concatenate([rays, near])
     rays[:, 3] = np.inf
     g, vis, _ = _gpu_closest(krr, gpu, s, rays, {"node_format": node_format})
     o = oracle.trace_closest(s.desc, rays)
