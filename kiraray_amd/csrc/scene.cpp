@@ -1266,9 +1266,17 @@ VdbGrid readVdbGrid(const VdbFile &f, const std::string &key) {
 		bool active;
 	};
 	std::vector<Block> leaves, tiles;
+	// node origins are added to spans of up to 4096 voxels below: far ones
+	// would overflow int (no real grid comes near 2^28 voxels from the origin)
+	auto origin = [&](int *o) {
+		for (int a = 0; a < 3; a++) {
+			o[a] = in.get<int32_t>();
+			if (o[a] < -(1 << 28) || o[a] > (1 << 28)) throw std::runtime_error(path + ": node origin out of range");
+		}
+	};
 	for (uint32_t t = 0; t < nTiles; t++) {
 		Block b;
-		for (int &c : b.o) c = in.get<int32_t>();
+		origin(b.o);
 		b.v		 = in.get<float>();
 		b.active = in.get<uint8_t>() != 0;
 		b.span	 = 4096;
@@ -1294,7 +1302,7 @@ VdbGrid readVdbGrid(const VdbFile &f, const std::string &key) {
 	};
 	for (uint32_t c = 0; c < nChildren; c++) {
 		int o[3];
-		for (int &x : o) x = in.get<int32_t>();
+		origin(o);
 		internal(internal, 5, o);
 	}
 	if ((int64_t) in.pos != desc->blockPos) throw std::runtime_error(path + ": tree topology does not end where the buffers start");

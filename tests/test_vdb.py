@@ -164,7 +164,8 @@ def _corrupt(path, offset, data):
     ("rotate", "not rotated"), ("mirror", "not rotated"), ("temperature", "temperature grid"),
     ("albedo", "albedo grid"), ("vec3", "float grids only"), ("map", "transform AffineMap"),
     ("truncated", "truncated|bad grid offsets"), ("magic", "not an OpenVDB file"), ("key", "no grid named"),
-    ("instance", "instanced grid"), ("negative", "negative"), ("version", "older than 222")])
+    ("instance", "instanced grid"), ("negative", "negative"), ("version", "older than 222"),
+    ("far", "origin out of range")])
 def test_unsupported_files_are_errors(krr, tmp_path, case, match):
     g = _tree(np.random.default_rng(3))
     grids, comp, node, params = [g], 2, {}, {}
@@ -191,6 +192,8 @@ def test_unsupported_files_are_errors(krr, tmp_path, case, match):
         params = {"key_density": "smoke"}
     elif case == "instance":
         g["instance_parent"] = "density"
+    elif case == "far":  # a leaf 2^29 voxels out: origins that far would overflow the reader's ints
+        g["leaves"][(1 << 29, 0, 0)] = g["leaves"][next(iter(g["leaves"]))]
     elif case == "negative":
         g["leaves"][(0, 0, 0)][0][np.nonzero(g["leaves"][(0, 0, 0)][1])[0][0]] = -1.0
     path = write_vdb(str(tmp_path / "e.vdb"), grids, compression=comp, version=221 if case == "version" else 224)
