@@ -900,17 +900,32 @@ __global__ void __launch_bounds__(1024) k_order_scan(int *a, int n) {
 }
 
 // Sorted slot of each tile (stable: equal keys keep tile order), and its
-// pixel count at that slot.
+// pixel count at that slot.  A tile's rank among the block's tiles of its key:
+// the lanes of its wave below it with the same key (six ballots, one per key
+// bit), plus the tiles of that key in the lower waves (per-wave key counts in
+// LDS).
+static_assert(kTileKeys == 64, "six key bits");
 __global__ void __launch_bounds__(kTileBlock) k_tile_rank(const uint8_t *key, const int *cnt, int nTiles, int nBlk, const int *hist, int *sorted,
 														  int *sortedCnt) {
-	__shared__ uint8_t k[kTileBlock];
-	const int t		= blockIdx.x * kTileBlock + threadIdx.x;
-	k[threadIdx.x] = t < nTiles ? key[t] : 0;
+	constexpr int kWaves = kTileBlock / 64;
+	__shared__ int wcnt[kWaves][kTileKeys];
+	const int t = blockIdx.x * kTileBlock + threadIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const bool valid = t < nTiles;
+	const int my	 = valid ? key[t] : 0;
+	uint64_t eq		 = __ballot(valid);
+#pragma unroll
+	for (int bit = 0; bit < 6; bit++) {
+		const uint64_t m = __ballot(valid && ((my >> bit) & 1));
+		eq &= ((my >> bit) & 1) ? m : ~m;
+	}
+	const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+	int r				 = __popcll(eq & below);
+	for (int i = (int) threadIdx.x; i < kWaves * kTileKeys; i += (int) blockDim.x) (&wcnt[0][0])[i] = 0;
 	__syncthreads();
-	if (t >= nTiles) return;
-	const uint8_t my = k[threadIdx.x];
-	int r			 = 0;
-	for (int j = 0; j < (int) threadIdx.x; j++) r += k[j] == my;
+	if (valid && r == 0) wcnt[w][my] = __popcll(eq); // the key's first lane in the wave
+	__syncthreads();
+	if (!valid) return;
+	for (int v = 0; v < w; v++) r += wcnt[v][my];
 	const int slot	= hist[(kTileKeys - 1 - my) * nBlk + blockIdx.x] + r;
 	sorted[slot]	= t;
 	sortedCnt[slot] = cnt[t];
@@ -1250,13 +1265,20 @@ __global__ void k_accumulate(int N, PixelStateDev px) {
 }
 
 // Film write (integrator.cpp:262-266, CudaRenderTarget::write,
-// src/core/device/cuda.h:33-45): RGBA(pixel / spp, 1) at row H-1-y.
+// src/core/device/cuda.h:33-45): RGBA(pixel / spp, 1) at row H-1-y.  The
+// frame's last sample is added to the pixel sum here (k_accumulate's
+// pixel += L, the same sum), saving a pass over the pixels.
 __global__ void k_film(WfParams P, PixelStateDev px, float4 *film) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= P.N) return;
 	const int x = i % P.W, y = i / P.W;
-	const float4 p = px.pixel[i];
-	float r = p.x / (float) P.spp, g = p.y / (float) P.spp, b = p.z / (float) P.spp;
+	float4 p	   = px.pixel[i];
+	const float4 L = px.L[i];
+	p.x			   = p.x + L.x;
+	p.y			   = p.y + L.y;
+	p.z			   = p.z + L.z;
+	px.pixel[i]	   = p;
+	float r		   = p.x / (float) P.spp, g = p.y / (float) P.spp, b = p.z / (float) P.spp;
 	if (P.enableClamp) {
 		r = clampf(r, 0.f, P.clampMax);
 		g = clampf(g, 0.f, P.clampMax);
@@ -2633,7 +2655,8 @@ int krr_wf_render(krr_wf *wf, float *film_dev, void *hip_stream) {
 			});
 			if (nrcRc) return nrcRc;
 		}
-		hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
+		// the last sample is added to the pixel sums by the film write
+		if (s + 1 < wf->spp) hipLaunchKernelGGL(k_accumulate, dim3(gN), dim3(bs), 0, st, N, px);
 	}
 	if (wf->nrcOn) { // one training step per frame on the recorded vertices
 		timed(5, [&] { nrcRc = nrcTrainDevice(wf->nrc, wf->nrcBatchV.as<float>(), wf->nrcBatchT.as<float>(), &cnt->nrcTrainDone, wf->nrcBatch, wf->nrcSteps, st);
