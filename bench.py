@@ -784,8 +784,8 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on exactly one GPU; the multi-GPU sharding is described in DESIGN.md "
-              "and not launched (use --gpus 1)", file=sys.stderr)
+        print("bench.py: runs on exactly one GPU (the path shards by image rows or views with one film exchange per "
+              "frame; that sharding is not launched here); use --gpus 1", file=sys.stderr)
         raise SystemExit(2)
     cfg = CONFIGS[args.config]
     if args.wavefront_depths is not None:

@@ -1,5 +1,5 @@
 """Pins of the CPU checker against known answers, closed forms, invariants and
-brute force (the reference ships no tests or golden data, SURVEY 8c).
+brute force (the reference ships no tests or golden data).
 
 Each check is chosen so that a plausible mistake - a dropped term, a wrong
 sign or index, a swapped operand - fails it."""
