@@ -437,7 +437,7 @@ int krr_wf_render_views(krr_wf *wf, uint64_t frame_index, void *hip_stream);
  * stack_lds_entries {8, 16, 24} x occupancy {1, 0.75, 0.5} - on the first
  * trial_paths (<= 0: 2^21) camera paths of the views (or of the single view,
  * after krr_wf_begin_frame), then (scenes without media, refill_lanes -1)
- * refill_lanes {0, 4, 8, 16} at the fastest shape, and keeps the fastest.  Images do not
+ * refill_lanes {0, 4, 8, 12, 16} at the fastest shape, and keeps the fastest.  Images do not
  * depend on the choice.  Writes a JSON report (candidates with ms, best) into report
  * (cap bytes) and returns its full length.  No reference counterpart. */
 int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, void *hip_stream);

@@ -3035,7 +3035,7 @@ int krr_wf_autotune(krr_wf *wf, int64_t trial_paths, char *report, int32_t cap, 
 			}
 	if (rc == KRR_OK && !media && saveRefill < 0) {
 		const Trial shape = *std::min_element(trials.begin(), trials.end(), [](const Trial &a, const Trial &b) { return a.ms < b.ms; });
-		for (const int r : {0, 4, kDefaultRefillLanes, 16}) {
+		for (const int r : {0, 4, kDefaultRefillLanes, 12, 16}) {
 			if (r == refill0 || rc != KRR_OK) continue;
 			Trial t{shape.tb, shape.lds, shape.occ, r, 0.f};
 			rc = measure(t, 3);

@@ -185,8 +185,8 @@ def test_graph_with_radiance_cache(krr, gpu):
 
 @pytest.mark.parametrize("views", [False, True])
 def test_autotune_keeps_the_image(krr, gpu, views):
-    """The autotuner times 27 launch shapes, then the three other refill
-    thresholds of {0, 4, 8, 16} at the fastest shape (refill_lanes left to
+    """The autotuner times 27 launch shapes, then the four other refill
+    thresholds of {0, 4, 8, 12, 16} at the fastest shape (refill_lanes left to
     the scene), and keeps the fastest; the frame rendered afterwards equals
     the frame rendered before."""
     print(f"case: autotune views={views}", flush=True)
@@ -213,8 +213,8 @@ def test_autotune_keeps_the_image(krr, gpu, views):
     before = frame()
     rep = wf.autotune(trial_paths=96 * 64)
     print("best", rep["best"], flush=True)
-    assert len(rep["candidates"]) == 30
-    assert sorted({c["refill_lanes"] for c in rep["candidates"]}) == [0, 4, 8, 16]
+    assert len(rep["candidates"]) == 31
+    assert sorted({c["refill_lanes"] for c in rep["candidates"]}) == [0, 4, 8, 12, 16]
     assert all(c["ms"] > 0 for c in rep["candidates"])
     assert rep["best"]["ms"] == min(c["ms"] for c in rep["candidates"])
     p = wf.to_json()
